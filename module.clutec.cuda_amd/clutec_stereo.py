@@ -1,0 +1,335 @@
+"""ctypes binding of the gfx950 stereo engine's C ABI (include/clutec_stereo.h).
+
+The engine is the HIP library lib/libclutec_stereo.so next to this file; there is
+no Python or CPU compute path.  Importing this module on a machine without the
+built library raises immediately, and every compute call goes through the HIP
+kernels.
+
+    import clutec_stereo as cs
+    ctx = cs.Context(0)
+    m = cs.Matcher(ctx, W, H, algorithm=cs.ALGO_CENSUS, window=(9, 9), census=(9, 7), num_disp=256)
+    disp = m.run(left_u8, right_u8)          # numpy in, numpy int16 out
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libclutec_stereo.so")
+
+OK = 0
+ERROR_INVALID_ARGUMENT = 1
+ERROR_INVALID_CONFIG = 2
+ERROR_UNSUPPORTED_PIXEL_TYPE = 3
+ERROR_FORMAT_MISMATCH = 4
+ERROR_INVALID_DEVICE = 5
+ERROR_INVALID_FORMAT = 6
+ERROR_HIP = 7
+ERROR_OUT_OF_MEMORY = 8
+ERROR_NOT_READY = 9
+
+PIXEL_LUM_U8 = 1
+PIXEL_LUM_U16 = 2
+PIXEL_LUM_I16 = 3
+PIXEL_LUM_U32 = 4
+PIXEL_LUM_U64 = 5
+PIXEL_LUM_F32 = 6
+PIXEL_RGBA_U16 = 7
+
+ALGO_SAD = 0
+ALGO_CENSUS = 1
+ALGO_NCC = 2
+ALGO_AW1 = 3
+ALGO_AW2 = 4
+
+_NP_OF = {
+    PIXEL_LUM_U8: (np.uint8, 1),
+    PIXEL_LUM_U16: (np.uint16, 1),
+    PIXEL_LUM_I16: (np.int16, 1),
+    PIXEL_LUM_U32: (np.uint32, 1),
+    PIXEL_LUM_U64: (np.uint64, 1),
+    PIXEL_LUM_F32: (np.float32, 1),
+    PIXEL_RGBA_U16: (np.uint16, 4),
+}
+
+
+class ImageFormat(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("pixel_type", C.c_int32)]
+
+
+class Image(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("pitch", C.c_size_t), ("format", ImageFormat)]
+
+
+class CensusParams(C.Structure):
+    _fields_ = [("window_w", C.c_int32), ("window_h", C.c_int32), ("reference_3x3", C.c_int32)]
+
+
+class MatcherParams(C.Structure):
+    _fields_ = [("algorithm", C.c_int32), ("window_w", C.c_int32), ("window_h", C.c_int32),
+                ("census_w", C.c_int32), ("census_h", C.c_int32), ("num_disparities", C.c_int32),
+                ("lr_tolerance", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
+class CrossCheckParams(C.Structure):
+    _fields_ = [("disp_delta_thresh", C.c_int32), ("left_to_right", C.c_int32),
+                ("disp_offset", C.c_int32), ("reserved", C.c_int32)]
+
+
+# every exported entry point: (name, restype, argtypes)
+_P = C.c_void_p
+_PROTOS = [
+    ("stereo_abi_version", C.c_int, []),
+    ("stereo_create", C.c_int, [C.POINTER(_P), C.c_int]),
+    ("stereo_destroy", None, [_P]),
+    ("stereo_last_error", C.c_char_p, [_P]),
+    ("stereo_status_string", C.c_char_p, [C.c_int]),
+    ("stereo_set_timing", C.c_int, [_P, C.c_int]),
+    ("stereo_synchronize", C.c_int, [_P, _P]),
+    ("stereo_pixel_size", C.c_size_t, [C.c_int32]),
+    ("stereo_image_alloc", C.c_int, [_P, C.POINTER(Image), C.c_int32, C.c_int32, C.c_int32]),
+    ("stereo_image_free", C.c_int, [_P, C.POINTER(Image)]),
+    ("stereo_image_upload", C.c_int, [_P, C.POINTER(Image), _P, C.c_size_t, _P]),
+    ("stereo_image_download", C.c_int, [_P, C.POINTER(Image), _P, C.c_size_t, _P]),
+    ("stereo_driver_destroy", C.c_int, [_P]),
+    ("stereo_driver_name", C.c_char_p, [_P]),
+    ("stereo_driver_last_time_ms", C.c_int, [_P, C.POINTER(C.c_double)]),
+    ("stereo_census_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(CensusParams)]),
+    ("stereo_census_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), _P]),
+    ("stereo_matcher_validate", C.c_int, [C.POINTER(ImageFormat), C.POINTER(MatcherParams), C.c_char_p, C.c_size_t]),
+    ("stereo_matcher_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(MatcherParams)]),
+    ("stereo_matcher_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image),
+                                          C.POINTER(Image), C.POINTER(Image), _P]),
+    ("stereo_matcher_stage_times", C.c_int, [_P, C.POINTER(C.c_double), C.c_int32]),
+    ("stereo_crosscheck_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(CrossCheckParams)]),
+    ("stereo_crosscheck_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
+    ("stereo_crosscheck_range", C.c_int, [_P, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16)]),
+    ("stereo_synth_pair", C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    _P, _P, _P]),
+]
+
+EXPORTED = [p[0] for p in _PROTOS]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run module.clutec.cuda_amd/build.py "
+                          "(the engine has no fallback path)")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in _PROTOS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class StereoError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+def _check(st, ctx=None):
+    if st != OK:
+        msg = lib.stereo_last_error(ctx.handle if ctx is not None else None)
+        raise StereoError(st, (msg or b"").decode() or lib.stereo_status_string(st).decode())
+
+
+def synth_pair(seed, W, H, kind=0, nregions=8, dmax=63, dbg=8, with_gt=False):
+    """Seeded synthetic pair (host generator of the engine)."""
+    L = np.empty((H, W), np.uint8)
+    R = np.empty((H, W), np.uint8)
+    gt = np.empty((H, W), np.int16) if with_gt else None
+    rc = lib.stereo_synth_pair(seed, W, H, kind, nregions, dmax, dbg, L.ctypes.data, R.ctypes.data,
+                               gt.ctypes.data if with_gt else None)
+    if rc != 0:
+        raise ValueError(f"stereo_synth_pair failed ({rc})")
+    return (L, R, gt) if with_gt else (L, R)
+
+
+def validate_matcher(W, H, params):
+    fmt = ImageFormat(W, H, PIXEL_LUM_U8)
+    buf = C.create_string_buffer(256)
+    st = lib.stereo_matcher_validate(C.byref(fmt), C.byref(params), buf, 256)
+    return st, buf.value.decode()
+
+
+def matcher_params(algorithm, window=(9, 9), census=(9, 7), num_disp=64, tol=1):
+    p = MatcherParams()
+    p.algorithm = algorithm
+    p.window_w, p.window_h = window
+    p.census_w, p.census_h = census
+    p.num_disparities = num_disp
+    p.lr_tolerance = tol
+    return p
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        st = lib.stereo_create(C.byref(h), device)
+        if st != OK:
+            raise StereoError(st, lib.stereo_last_error(None).decode())
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib.stereo_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, on=True):
+        _check(lib.stereo_set_timing(self.handle, 1 if on else 0), self)
+
+    def synchronize(self, stream=None):
+        _check(lib.stereo_synchronize(self.handle, stream), self)
+
+
+class DeviceImage:
+    """Owning pitched device image (CDeviceImage)."""
+
+    def __init__(self, ctx, W, H, pixel_type):
+        self.ctx = ctx
+        self.img = Image()
+        _check(lib.stereo_image_alloc(ctx.handle, C.byref(self.img), W, H, pixel_type), ctx)
+
+    @property
+    def ref(self):
+        return C.byref(self.img)
+
+    def free(self):
+        if self.img.data:
+            lib.stereo_image_free(self.ctx.handle, C.byref(self.img))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def upload(self, arr, stream=None):
+        dt, ch = _NP_OF[self.img.format.pixel_type]
+        a = np.ascontiguousarray(arr, dtype=dt)
+        assert a.shape[0] == self.img.format.height and a.shape[1] == self.img.format.width
+        _check(lib.stereo_image_upload(self.ctx.handle, C.byref(self.img), a.ctypes.data, a.strides[0], stream), self.ctx)
+
+    def download(self, stream=None):
+        dt, ch = _NP_OF[self.img.format.pixel_type]
+        shape = (self.img.format.height, self.img.format.width) + ((ch,) if ch > 1 else ())
+        a = np.empty(shape, dt)
+        _check(lib.stereo_image_download(self.ctx.handle, C.byref(self.img), a.ctypes.data, a.strides[0], stream), self.ctx)
+        return a
+
+    @classmethod
+    def from_array(cls, ctx, arr, pixel_type):
+        img = cls(ctx, arr.shape[1], arr.shape[0], pixel_type)
+        img.upload(arr)
+        return img
+
+
+class _Driver:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.handle = C.c_void_p()
+
+    def close(self):
+        if self.handle:
+            lib.stereo_driver_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_time_ms(self):
+        v = C.c_double()
+        _check(lib.stereo_driver_last_time_ms(self.handle, C.byref(v)), self.ctx)
+        return v.value
+
+    @property
+    def name(self):
+        return lib.stereo_driver_name(self.handle).decode()
+
+
+class Census(_Driver):
+    def __init__(self, ctx, W, H, window=(9, 7), reference_3x3=False):
+        super().__init__(ctx)
+        self.W, self.H = W, H
+        self.ref3 = bool(reference_3x3)
+        fmt = ImageFormat(W, H, PIXEL_LUM_U8)
+        p = CensusParams(window[0], window[1], 1 if reference_3x3 else 0)
+        _check(lib.stereo_census_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(p)), ctx)
+
+    def process(self, trg, src, stream=None):
+        _check(lib.stereo_census_process(self.handle, trg.ref, src.ref, stream), self.ctx)
+
+    def run(self, img):
+        src = DeviceImage.from_array(self.ctx, img, PIXEL_LUM_U8)
+        trg = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_U8 if self.ref3 else PIXEL_LUM_U64)
+        self.process(trg, src)
+        return trg.download()
+
+
+class Matcher(_Driver):
+    def __init__(self, ctx, W, H, algorithm=ALGO_CENSUS, window=(9, 9), census=(9, 7), num_disp=64, tol=1):
+        super().__init__(ctx)
+        self.W, self.H = W, H
+        fmt = ImageFormat(W, H, PIXEL_LUM_U8)
+        self.params = matcher_params(algorithm, window, census, num_disp, tol)
+        _check(lib.stereo_matcher_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(self.params)), ctx)
+
+    def process(self, disp, disp_lr, disp_rl, left, right, stream=None):
+        _check(lib.stereo_matcher_process(self.handle, disp.ref if disp else None, disp_lr.ref if disp_lr else None,
+                                          disp_rl.ref if disp_rl else None, left.ref, right.ref, stream), self.ctx)
+
+    def stage_times_ms(self):
+        v = (C.c_double * 3)()
+        _check(lib.stereo_matcher_stage_times(self.handle, v, 3), self.ctx)
+        return list(v)
+
+    def run(self, L, R, maps=False, dispid=False):
+        """numpy in, numpy out: the checked map (and the two one-sided maps)."""
+        dl = DeviceImage.from_array(self.ctx, L, PIXEL_LUM_U8)
+        dr = DeviceImage.from_array(self.ctx, R, PIXEL_LUM_U8)
+        out = DeviceImage(self.ctx, self.W, self.H, PIXEL_RGBA_U16 if dispid else PIXEL_LUM_I16)
+        lr = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16) if maps else None
+        rl = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16) if maps else None
+        self.process(out, lr, rl, dl, dr)
+        if maps:
+            return out.download(), lr.download(), rl.download()
+        return out.download()
+
+
+class CrossCheck(_Driver):
+    def __init__(self, ctx, W, H, thresh=1, left_to_right=True, offset=0):
+        super().__init__(ctx)
+        self.W, self.H = W, H
+        fmt = ImageFormat(W, H, PIXEL_RGBA_U16)
+        p = CrossCheckParams(thresh, 1 if left_to_right else 0, offset, 0)
+        _check(lib.stereo_crosscheck_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(p)), ctx)
+
+    def process(self, result, lr, rl, stream=None):
+        _check(lib.stereo_crosscheck_process(self.handle, result.ref, lr.ref, rl.ref, stream), self.ctx)
+
+    def range(self):
+        a, b = C.c_uint16(), C.c_uint16()
+        _check(lib.stereo_crosscheck_range(self.handle, C.byref(a), C.byref(b)), self.ctx)
+        return a.value, b.value
+
+    def run(self, lr, rl):
+        dlr = DeviceImage.from_array(self.ctx, lr, PIXEL_RGBA_U16)
+        drl = DeviceImage.from_array(self.ctx, rl, PIXEL_RGBA_U16)
+        out = DeviceImage(self.ctx, self.W, self.H, PIXEL_RGBA_U16)
+        self.process(out, dlr, drl)
+        return out.download()

@@ -1,0 +1,302 @@
+// Fused SAD / census-Hamming block matcher for gfx950: cost, separable box
+// aggregation, left-to-right and right-to-left winner-take-all in one pass over
+// the cost volume, followed by a small kernel that merges the right-view
+// partials and applies the left-right check.
+//
+// Reference operators restated here (plain specification, see the oracle):
+//   per-pixel costs      AbsoluteDifference  BlockMatcherNCC.cu:147-151
+//                        CensusDifference    BlockMatcherNCC.cu:153-158
+//   box aggregation      AlgoArraySum::SumCache (Kernel.Algo.ArraySum.h:149-189):
+//                        separable running sums, slide-in / slide-out
+//   winner-take-all      strict comparison scanning d upward (BlockMatcherNCC.cu:298)
+//   left-right check     CrossCheckDisparityMaps::Kernel::Transform (:62-109)
+//
+// Inputs are padded views: columns [-BM_PAD_LEFT, W + BM_PAD_RIGHT) of every row
+// are readable memory (values there only meet masked candidates), so the loads
+// need no clamping.  Rows are clamped.
+//
+// Workgroup = 256 lanes = 4 waves; tile = 64 output columns x 56 output rows; all D
+// disparities are streamed through the tile two at a time:
+//   row phase     lane = (H row 0..63, run of 18 columns).  Costs of the pair
+//                 (d, d+1) are packed as two uint16 halves of one register; the
+//                 right-image window of the lane lives in registers and slides by
+//                 register renaming across 8 disparities.  The horizontal running
+//                 sum over 2*AX+1 columns takes its halo from the neighbouring runs
+//                 with DPP row shifts, and the sums H go to LDS (double buffered).
+//   column phase  lane = output column, wave = 14-row group.  Vertical running
+//                 sums of H give the aggregated cost A of 14 pixels; the key
+//                 ((65535-A) << 8 | 255-d) is built with one v_perm_b32.  The left
+//                 winner is a v_max_u32 per pixel.  The right winner for right
+//                 pixel x-d is carried along the wave: each step shifts the running
+//                 maxima one lane up with a DPP wave shift fused into v_max_u32, so
+//                 lane x always holds right pixel x-d.  What leaves lane 63 is a
+//                 partial result for the tile to the right; lane 63 parks it in
+//                 LDS and the wave stores 64 steps at a time, coalesced.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace stereo {
+namespace bm {
+
+constexpr int TW = 64;
+constexpr int TH = 56;
+constexpr int HROWS = 64;
+constexpr int RUN = 18;
+constexpr int HSTRIDE = 73;
+constexpr int RG = TH / 4;
+
+struct CostCensus {
+    static constexpr int DC = 4;  // disparities per right-window load
+    struct T { uint32_t lo, hi; };
+    static __device__ __forceinline__ T load(const View& v, int y, int x)
+    {
+        const uint64_t q = row_ptr<const uint64_t>(v, y)[x];
+        return T{(uint32_t)q, (uint32_t)(q >> 32)};
+    }
+    // two Hamming costs packed as (c(b0) | c(b1) << 16)
+    static __device__ __forceinline__ uint32_t pair(T a, T b0, T b1)
+    {
+        uint32_t c1 = __builtin_popcount(a.lo ^ b1.lo) + __builtin_popcount(a.hi ^ b1.hi);
+        return __builtin_popcount(a.lo ^ b0.lo) + __builtin_popcount(a.hi ^ b0.hi) + (c1 << 16);
+    }
+};
+
+struct CostSad {
+    static constexpr int DC = 8;
+    using T = uint32_t;
+    static __device__ __forceinline__ T load(const View& v, int y, int x) { return row_ptr<const uint8_t>(v, y)[x]; }
+    static __device__ __forceinline__ uint32_t pair(T a, T b0, T b1)
+    {
+        const uint32_t c1 = __builtin_amdgcn_sad_u8(a, b1, 0u);
+        return __builtin_amdgcn_sad_u8(a, b0, c1 << 16);
+    }
+};
+
+template <class Cost, int AX, int AY, bool EDGE>
+__device__ __forceinline__ void bm_tile(const BmParams& p, uint32_t (*hbuf)[HROWS * HSTRIDE],
+                                        uint32_t (*ebuf)[RG][64])
+{
+    static_assert(TH + 2 * AY <= HROWS, "tile too tall for the aggregation window");
+    static_assert(AX <= 4, "horizontal radius limited by the run halo");
+    constexpr int DC = Cost::DC;
+    const int tid = threadIdx.x;
+    const int tx = blockIdx.x, ty = blockIdx.y;
+    const int x0 = tx * TW, y0 = ty * TH;
+    const int W = p.W, H = p.H, D = p.D;
+
+    // row-phase identity
+    const int run = tid & 3;
+    const int hr = tid >> 2;
+    const int gyr = clampi(y0 - AY + hr, 0, H - 1);
+    const int xs = x0 - 4 + RUN * run;
+    typename Cost::T cl[RUN];
+#pragma unroll
+    for (int i = 0; i < RUN; ++i) cl[i] = Cost::load(p.L, gyr, xs + i);
+
+    // column-phase identity
+    const int lane = tid & 63;
+    const int grp = tid >> 6;
+    const int x = x0 + lane;
+    const bool lane_ok = x >= p.bx && x < W - p.bx;
+    const int dlim = x - p.bx;
+    uint32_t lacc[RG], racc[RG];
+#pragma unroll
+    for (int i = 0; i < RG; ++i) { lacc[i] = 0; racc[i] = 0; }
+
+    int s = 0;
+    for (int d0 = 0; d0 < D; d0 += DC) {
+        typename Cost::T cr[RUN + DC - 1];
+#pragma unroll
+        for (int k = 0; k < RUN + DC - 1; ++k) cr[k] = Cost::load(p.R, gyr, xs - d0 - (DC - 1) + k);
+
+#pragma unroll
+        for (int q = 0; q < DC / 2; ++q, ++s) {
+            const int d = d0 + 2 * q;
+            uint32_t* hb = hbuf[s & 1];
+
+            // ---- row phase: packed costs of (d, d+1) and horizontal sums
+            uint32_t e[RUN + 2 * AX];
+#pragma unroll
+            for (int i = 0; i < RUN; ++i)
+                e[AX + i] = Cost::pair(cl[i], cr[i - 2 * q + DC - 1], cr[i - 2 * q + DC - 2]);
+#pragma unroll
+            for (int i = 0; i < AX; ++i) {
+                e[i] = dpp_mov<DPP_ROW_SHR1>(e[AX + RUN - AX + i]);
+                e[AX + RUN + i] = dpp_mov<DPP_ROW_SHL1>(e[AX + i]);
+            }
+            uint32_t h = 0;
+#pragma unroll
+            for (int i = 0; i <= 2 * AX; ++i) h += e[i];
+            uint32_t* hrow = hb + hr * HSTRIDE + RUN * run;
+            hrow[0] = h;
+#pragma unroll
+            for (int i = 1; i < RUN; ++i) {
+                h = h + e[i + 2 * AX] - e[i - 1];
+                hrow[i] = h;
+            }
+            __syncthreads();
+
+            // ---- column phase: vertical sums, keys, both winner-take-alls
+            uint32_t hv[RG + 2 * AY];
+#pragma unroll
+            for (int i = 0; i < RG + 2 * AY; ++i) hv[i] = hb[(grp * RG + i) * HSTRIDE + lane + 4];
+            uint32_t abar = 0xFFFFFFFFu;
+#pragma unroll
+            for (int i = 0; i <= 2 * AY; ++i) abar -= hv[i];
+            const uint32_t ddv = (uint32_t)(255 - d) | ((uint32_t)(254 - d) << 8);
+            bool ok0 = true, ok1 = true;
+            if (EDGE) { ok0 = lane_ok && d <= dlim; ok1 = lane_ok && d + 1 <= dlim; }
+#pragma unroll
+            for (int i = 0; i < RG; ++i) {
+                if (i > 0) abar = abar + hv[i - 1] - hv[i + 2 * AY];
+                uint32_t k0 = __builtin_amdgcn_perm(ddv, abar, 0x0C010004u);
+                uint32_t k1 = __builtin_amdgcn_perm(ddv, abar, 0x0C030205u);
+                if (EDGE) { k0 = ok0 ? k0 : 0u; k1 = ok1 ? k1 : 0u; }
+                const uint32_t kk = k0 > k1 ? k0 : k1;
+                lacc[i] = lacc[i] > kk ? lacc[i] : kk;
+                racc[i] = max_shift_in(racc[i], k0);
+                if (lane == 63) ebuf[grp][i][d & 63] = racc[i];
+                racc[i] = max_shift_in(racc[i], k1);
+                if (lane == 63) ebuf[grp][i][(d + 1) & 63] = racc[i];
+            }
+            // flush the lane-63 emissions every 64 disparities (and at the end)
+            const int dl = d + 1;
+            if ((dl & 63) == 63 || dl == D - 1) {
+                const int base = dl & ~63;
+                if (lane <= (dl & 63)) {
+                    const int k = 318 - (base + lane);
+#pragma unroll
+                    for (int i = 0; i < RG; ++i) {
+                        const int y = y0 + grp * RG + i;
+                        if (y < H) p.rpart[((size_t)tx * H + y) * BM_NPART + k] = ebuf[grp][i][lane];
+                    }
+                }
+            }
+        }
+    }
+
+    // what is still in flight belongs to right pixels x0+lane-(D-1)
+    {
+        const int k = lane + 255 - (D - 1);
+#pragma unroll
+        for (int i = 0; i < RG; ++i) {
+            const int y = y0 + grp * RG + i;
+            if (y < H) p.rpart[((size_t)tx * H + y) * BM_NPART + k] = racc[i];
+        }
+    }
+    if (x < W) {
+#pragma unroll
+        for (int i = 0; i < RG; ++i) {
+            const int y = y0 + grp * RG + i;
+            if (y < H) p.lkey[(size_t)y * W + x] = lacc[i];
+        }
+    }
+}
+
+template <class Cost, int AX, int AY>
+__global__ __launch_bounds__(256, 2) void bm_fused_kernel(BmParams p)
+{
+    __shared__ uint32_t hbuf[2][HROWS * HSTRIDE];
+    __shared__ uint32_t ebuf[4][RG][64];
+    const int x0 = blockIdx.x * TW;
+    const bool edge = (x0 < p.bx + p.D - 1) || (x0 + TW > p.W - p.bx);
+    if (edge)
+        bm_tile<Cost, AX, AY, true>(p, hbuf, ebuf);
+    else
+        bm_tile<Cost, AX, AY, false>(p, hbuf, ebuf);
+}
+
+__device__ __forceinline__ uint32_t rkey_at(const BmFinishParams& f, int xr, int y)
+{
+    int tlo = xr - 63 <= 0 ? 0 : (xr - 63 + 63) / 64;
+    int thi = (xr + f.D - 1) / 64;
+    if (thi > f.ntx - 1) thi = f.ntx - 1;
+    uint32_t best = 0;
+    for (int t = tlo; t <= thi; ++t) {
+        const uint32_t v = f.rpart[((size_t)t * f.H + y) * BM_NPART + (xr - 64 * t + 255)];
+        best = v > best ? v : best;
+    }
+    return best;
+}
+
+__global__ __launch_bounds__(256) void bm_finish_kernel(BmFinishParams f)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= f.W || y >= f.H) return;
+    const bool row_ok = y >= f.by && y < f.H - f.by;
+    const bool col_ok = x >= f.bx && x < f.W - f.bx;
+    int dl = -1, dr = -1, outd = -1;
+    uint32_t lk = 0;
+    if (row_ok && col_ok) {
+        lk = f.lkey[(size_t)y * f.W + x];
+        dl = key_disp(lk);
+        if (dl >= 0) {
+            const int xr = x - dl;
+            const int dxr = key_disp(rkey_at(f, xr, y));
+            outd = (dxr >= 0 && abs(dxr - dl) <= f.tol) ? dl : -1;
+        }
+        if (f.disp_r.ptr) dr = key_disp(rkey_at(f, x, y));
+    }
+    if (f.disp.ptr) row_ptr<int16_t>(f.disp, y)[x] = (int16_t)outd;
+    if (f.disp_l.ptr) row_ptr<int16_t>(f.disp_l, y)[x] = (int16_t)dl;
+    if (f.disp_r.ptr) row_ptr<int16_t>(f.disp_r, y)[x] = (int16_t)dr;
+    if (f.dispid.ptr) {
+        // reference pixel layout (EDisparityId in channel 0, DisparityId.h:33-45)
+        uint16_t id;
+        if (!(row_ok && col_ok)) id = 0;                       // Unknown
+        else if (outd >= 0) id = (uint16_t)(outd + 1);        // First + d
+        else if (dl >= 0) id = 0xFFFFu - 5u;                   // Inconsistent
+        else id = 0xFFFFu - 2u;                                // NotFound
+        const int cost = lk ? key_cost(lk) : 0;
+        ushort4 v;
+        v.x = id;
+        v.y = (uint16_t)(cost > 65535 ? 65535 : cost);
+        v.z = 0;
+        v.w = 0;
+        row_ptr<ushort4>(f.dispid, y)[x] = v;
+    }
+}
+
+}  // namespace bm
+
+bool bm_supported(int kind, int ww, int wh, int D)
+{
+    if (D < 8 || D > 256 || (D % 8) != 0) return false;
+    if (kind == BM_COST_CENSUS) return (ww == 9 && wh == 9) || (ww == 7 && wh == 7) || (ww == 5 && wh == 5);
+    if (kind == BM_COST_SAD) return (ww == 9 && wh == 9) || (ww == 7 && wh == 7) || (ww == 5 && wh == 5);
+    return false;
+}
+
+void bm_grid(int W, int H, int* ntx, int* nty)
+{
+    *ntx = (W + bm::TW - 1) / bm::TW;
+    *nty = (H + bm::TH - 1) / bm::TH;
+}
+
+template <class Cost>
+static hipError_t launch_cost(const BmParams& p, int ww, hipStream_t s)
+{
+    int ntx, nty;
+    bm_grid(p.W, p.H, &ntx, &nty);
+    dim3 grid(ntx, nty);
+    if (ww == 9) hipLaunchKernelGGL((bm::bm_fused_kernel<Cost, 4, 4>), grid, dim3(256), 0, s, p);
+    else if (ww == 7) hipLaunchKernelGGL((bm::bm_fused_kernel<Cost, 3, 3>), grid, dim3(256), 0, s, p);
+    else hipLaunchKernelGGL((bm::bm_fused_kernel<Cost, 2, 2>), grid, dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_bm(int kind, int ww, const BmParams& p, hipStream_t s)
+{
+    if (kind == BM_COST_CENSUS) return launch_cost<bm::CostCensus>(p, ww, s);
+    return launch_cost<bm::CostSad>(p, ww, s);
+}
+
+hipError_t launch_bm_finish(const BmFinishParams& f, hipStream_t s)
+{
+    dim3 grid((f.W + 63) / 64, (f.H + 3) / 4);
+    hipLaunchKernelGGL(bm::bm_finish_kernel, grid, dim3(256), 0, s, f);
+    return hipGetLastError();
+}
+
+}  // namespace stereo

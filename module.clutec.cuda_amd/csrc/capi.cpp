@@ -1,0 +1,647 @@
+// C ABI of the gfx950 stereo engine: contexts, device images and operator drivers.
+// The declarations (and the reference interface each entry point replaces) are in
+// include/clutec_stereo.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "clutec_stereo.h"
+#include "kernels.hpp"
+#include "synth.h"
+
+using stereo::View;
+
+// ------------------------------------------------------------------------------------
+// error plumbing and fault injection
+// ------------------------------------------------------------------------------------
+namespace {
+
+std::mutex g_err_mutex;
+std::string g_create_error;
+
+// STEREO_FAIL_AT=<n>: the n-th guarded operation (device allocation or kernel launch
+// sequence) of the process fails with STEREO_ERROR_HIP, so error paths can be tested.
+long g_fail_at = -2;
+long g_guard_count = 0;
+
+bool injected_failure()
+{
+    std::lock_guard<std::mutex> lk(g_err_mutex);
+    if (g_fail_at == -2) {
+        const char* e = std::getenv("STEREO_FAIL_AT");
+        g_fail_at = e ? std::atol(e) : -1;
+    }
+    ++g_guard_count;
+    return g_fail_at > 0 && g_guard_count == g_fail_at;
+}
+
+}  // namespace
+
+struct stereo_context {
+    int device = 0;
+    int timing = 0;
+    std::string last_error;
+};
+
+namespace {
+
+stereo_status fail(stereo_context* ctx, stereo_status st, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->last_error = buf;
+    else {
+        std::lock_guard<std::mutex> lk(g_err_mutex);
+        g_create_error = buf;
+    }
+    return st;
+}
+
+stereo_status hip_fail(stereo_context* ctx, hipError_t e, const char* what)
+{
+    if (e == hipErrorOutOfMemory) return fail(ctx, STEREO_ERROR_OUT_OF_MEMORY, "%s: %s", what, hipGetErrorString(e));
+    return fail(ctx, STEREO_ERROR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr, what)                          \
+    do {                                                  \
+        hipError_t _e = (expr);                           \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, what); \
+    } while (0)
+
+#define GUARD(ctx, what)                                                       \
+    do {                                                                       \
+        if (injected_failure())                                                \
+            return fail(ctx, STEREO_ERROR_HIP, "%s: injected failure", what);  \
+    } while (0)
+
+View view_of(const stereo_image* img)
+{
+    View v;
+    v.ptr = static_cast<uint8_t*>(img->data);
+    v.pitch = img->pitch;
+    v.w = img->format.width;
+    v.h = img->format.height;
+    return v;
+}
+
+View null_view()
+{
+    View v;
+    v.ptr = nullptr;
+    v.pitch = 0;
+    v.w = 0;
+    v.h = 0;
+    return v;
+}
+
+bool format_valid(const stereo_image_format* f)
+{
+    return f && f->width > 0 && f->height > 0 && f->width <= 32768 && f->height <= 32768 &&
+           stereo_pixel_size(f->pixel_type) > 0;
+}
+
+stereo_status check_image(stereo_context* ctx, const stereo_image* img, int w, int h, int type, const char* what)
+{
+    if (!img || !img->data) return fail(ctx, STEREO_ERROR_INVALID_ARGUMENT, "%s: null image", what);
+    if (img->format.pixel_type != type)
+        return fail(ctx, STEREO_ERROR_UNSUPPORTED_PIXEL_TYPE, "%s: pixel types of given images not supported", what);
+    if (img->format.width != w || img->format.height != h)
+        return fail(ctx, STEREO_ERROR_FORMAT_MISMATCH, "%s: image is %dx%d, driver configured for %dx%d", what,
+                    img->format.width, img->format.height, w, h);
+    if (img->pitch < (size_t)w * stereo_pixel_size(type))
+        return fail(ctx, STEREO_ERROR_INVALID_FORMAT, "%s: pitch %zu too small", what, img->pitch);
+    return STEREO_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+// drivers
+// ------------------------------------------------------------------------------------
+struct stereo_driver {
+    enum Kind { CENSUS, MATCHER, CROSSCHECK } kind;
+    stereo_context* ctx = nullptr;
+    const char* name = "";
+    stereo_image_format fmt{};
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipEvent_t ev_stage[2] = {nullptr, nullptr};
+    bool timed = false;
+
+    // census
+    stereo_census_params cen{};
+    // matcher
+    stereo_matcher_params mp{};
+    int bx = 0, by = 0, ntx = 0, nty = 0;
+    uint8_t* padL = nullptr;  // padded matcher inputs (grey bytes or census words)
+    uint8_t* padR = nullptr;
+    size_t pad_pitch = 0;
+    size_t pad_elem = 0;
+    uint32_t* lkey = nullptr;
+    uint32_t* rpart = nullptr;
+    // crosscheck
+    stereo_crosscheck_params cc{};
+    uint32_t* range_dev = nullptr;
+    uint32_t* range_host = nullptr;
+    hipEvent_t ev_range = nullptr;
+    bool range_pending = false;
+
+    explicit stereo_driver(Kind k) : kind(k) {}
+};
+
+namespace {
+
+stereo_status driver_events(stereo_driver* d)
+{
+    HIP_TRY(d->ctx, hipEventCreate(&d->ev_start), "hipEventCreate");
+    HIP_TRY(d->ctx, hipEventCreate(&d->ev_stop), "hipEventCreate");
+    HIP_TRY(d->ctx, hipEventCreate(&d->ev_stage[0]), "hipEventCreate");
+    HIP_TRY(d->ctx, hipEventCreate(&d->ev_stage[1]), "hipEventCreate");
+    return STEREO_OK;
+}
+
+void driver_release(stereo_driver* d)
+{
+    if (d->ev_start) (void)hipEventDestroy(d->ev_start);
+    if (d->ev_stop) (void)hipEventDestroy(d->ev_stop);
+    for (hipEvent_t e : d->ev_stage)
+        if (e) (void)hipEventDestroy(e);
+    if (d->ev_range) (void)hipEventDestroy(d->ev_range);
+    if (d->padL) (void)hipFree(d->padL);
+    if (d->padR) (void)hipFree(d->padR);
+    if (d->lkey) (void)hipFree(d->lkey);
+    if (d->rpart) (void)hipFree(d->rpart);
+    if (d->range_dev) (void)hipFree(d->range_dev);
+    if (d->range_host) (void)hipHostFree(d->range_host);
+}
+
+stereo_status begin(stereo_driver* d, hipStream_t s)
+{
+    d->timed = false;
+    if (d->ctx->timing) HIP_TRY(d->ctx, hipEventRecord(d->ev_start, s), "hipEventRecord");
+    return STEREO_OK;
+}
+
+stereo_status end(stereo_driver* d, hipStream_t s)
+{
+    if (d->ctx->timing) {
+        HIP_TRY(d->ctx, hipEventRecord(d->ev_stop, s), "hipEventRecord");
+        d->timed = true;
+    }
+    return STEREO_OK;
+}
+
+stereo_status set_device(stereo_context* ctx)
+{
+    HIP_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+    return STEREO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int stereo_abi_version(void) { return STEREO_ABI_VERSION; }
+
+const char* stereo_status_string(stereo_status s)
+{
+    switch (s) {
+    case STEREO_OK: return "ok";
+    case STEREO_ERROR_INVALID_ARGUMENT: return "invalid argument";
+    case STEREO_ERROR_INVALID_CONFIG: return "invalid algorithm configuration";
+    case STEREO_ERROR_UNSUPPORTED_PIXEL_TYPE: return "pixel types of given images not supported";
+    case STEREO_ERROR_FORMAT_MISMATCH: return "image formats do not match";
+    case STEREO_ERROR_INVALID_DEVICE: return "invalid device";
+    case STEREO_ERROR_INVALID_FORMAT: return "invalid image format";
+    case STEREO_ERROR_HIP: return "HIP runtime error";
+    case STEREO_ERROR_OUT_OF_MEMORY: return "out of device memory";
+    case STEREO_ERROR_NOT_READY: return "not ready";
+    }
+    return "unknown status";
+}
+
+stereo_status stereo_create(stereo_context** out, int device)
+{
+    if (!out) return fail(nullptr, STEREO_ERROR_INVALID_ARGUMENT, "stereo_create: null output");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return fail(nullptr, STEREO_ERROR_INVALID_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= n)
+        return fail(nullptr, STEREO_ERROR_INVALID_DEVICE, "device index %d out of range [0, %d)", device, n);
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return fail(nullptr, STEREO_ERROR_INVALID_DEVICE, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, STEREO_ERROR_INVALID_DEVICE, "device %d is %s; this library is built for gfx950", device,
+                    prop.gcnArchName);
+    stereo_context* ctx = new stereo_context();
+    ctx->device = device;
+    e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, STEREO_ERROR_INVALID_DEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return STEREO_OK;
+}
+
+void stereo_destroy(stereo_context* ctx) { delete ctx; }
+
+const char* stereo_last_error(const stereo_context* ctx)
+{
+    if (ctx) return ctx->last_error.c_str();
+    std::lock_guard<std::mutex> lk(g_err_mutex);
+    return g_create_error.c_str();
+}
+
+stereo_status stereo_set_timing(stereo_context* ctx, int enable)
+{
+    if (!ctx) return STEREO_ERROR_INVALID_ARGUMENT;
+    ctx->timing = enable ? 1 : 0;
+    return STEREO_OK;
+}
+
+stereo_status stereo_synchronize(stereo_context* ctx, stereo_stream stream)
+{
+    if (!ctx) return STEREO_ERROR_INVALID_ARGUMENT;
+    HIP_TRY(ctx, hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+    return STEREO_OK;
+}
+
+size_t stereo_pixel_size(int32_t t)
+{
+    switch (t) {
+    case STEREO_PIXEL_LUM_U8: return 1;
+    case STEREO_PIXEL_LUM_U16: return 2;
+    case STEREO_PIXEL_LUM_I16: return 2;
+    case STEREO_PIXEL_LUM_U32: return 4;
+    case STEREO_PIXEL_LUM_U64: return 8;
+    case STEREO_PIXEL_LUM_F32: return 4;
+    case STEREO_PIXEL_RGBA_U16: return 8;
+    default: return 0;
+    }
+}
+
+stereo_status stereo_image_alloc(stereo_context* ctx, stereo_image* img, int32_t w, int32_t h, int32_t type)
+{
+    if (!ctx || !img) return STEREO_ERROR_INVALID_ARGUMENT;
+    stereo_image_format f{w, h, type};
+    if (!format_valid(&f)) return fail(ctx, STEREO_ERROR_INVALID_FORMAT, "invalid image format %dx%d type %d", w, h, type);
+    stereo_status st = set_device(ctx);
+    if (st) return st;
+    GUARD(ctx, "stereo_image_alloc");
+    const size_t row = (size_t)w * stereo_pixel_size(type);
+    const size_t pitch = (row + 255) & ~(size_t)255;  // 256-byte rows: whole cache lines
+    void* p = nullptr;
+    HIP_TRY(ctx, hipMalloc(&p, pitch * (size_t)h), "hipMalloc");
+    img->data = p;
+    img->pitch = pitch;
+    img->format = f;
+    return STEREO_OK;
+}
+
+stereo_status stereo_image_free(stereo_context* ctx, stereo_image* img)
+{
+    if (!ctx || !img) return STEREO_ERROR_INVALID_ARGUMENT;
+    if (img->data) HIP_TRY(ctx, hipFree(img->data), "hipFree");
+    img->data = nullptr;
+    return STEREO_OK;
+}
+
+stereo_status stereo_image_upload(stereo_context* ctx, stereo_image* img, const void* host, size_t host_pitch,
+                                  stereo_stream stream)
+{
+    if (!ctx || !img || !img->data || !host) return STEREO_ERROR_INVALID_ARGUMENT;
+    const size_t row = (size_t)img->format.width * stereo_pixel_size(img->format.pixel_type);
+    if (host_pitch < row) return fail(ctx, STEREO_ERROR_INVALID_ARGUMENT, "host pitch %zu < row bytes %zu", host_pitch, row);
+    HIP_TRY(ctx,
+            hipMemcpy2DAsync(img->data, img->pitch, host, host_pitch, row, img->format.height, hipMemcpyHostToDevice,
+                             (hipStream_t)stream),
+            "hipMemcpy2DAsync");
+    HIP_TRY(ctx, hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+    return STEREO_OK;
+}
+
+stereo_status stereo_image_download(stereo_context* ctx, const stereo_image* img, void* host, size_t host_pitch,
+                                    stereo_stream stream)
+{
+    if (!ctx || !img || !img->data || !host) return STEREO_ERROR_INVALID_ARGUMENT;
+    const size_t row = (size_t)img->format.width * stereo_pixel_size(img->format.pixel_type);
+    if (host_pitch < row) return fail(ctx, STEREO_ERROR_INVALID_ARGUMENT, "host pitch %zu < row bytes %zu", host_pitch, row);
+    HIP_TRY(ctx,
+            hipMemcpy2DAsync(host, host_pitch, img->data, img->pitch, row, img->format.height, hipMemcpyDeviceToHost,
+                             (hipStream_t)stream),
+            "hipMemcpy2DAsync");
+    HIP_TRY(ctx, hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+    return STEREO_OK;
+}
+
+stereo_status stereo_driver_destroy(stereo_driver* d)
+{
+    if (!d) return STEREO_ERROR_INVALID_ARGUMENT;
+    driver_release(d);
+    delete d;
+    return STEREO_OK;
+}
+
+const char* stereo_driver_name(const stereo_driver* d) { return d ? d->name : ""; }
+
+stereo_status stereo_driver_last_time_ms(stereo_driver* d, double* ms)
+{
+    if (!d || !ms) return STEREO_ERROR_INVALID_ARGUMENT;
+    if (!d->timed) return fail(d->ctx, STEREO_ERROR_NOT_READY, "%s: no timed process call (enable stereo_set_timing)", d->name);
+    HIP_TRY(d->ctx, hipEventSynchronize(d->ev_stop), "hipEventSynchronize");
+    float t = 0.f;
+    HIP_TRY(d->ctx, hipEventElapsedTime(&t, d->ev_start, d->ev_stop), "hipEventElapsedTime");
+    *ms = t;
+    return STEREO_OK;
+}
+
+// ---- census -----------------------------------------------------------------------
+
+stereo_status stereo_census_configure(stereo_context* ctx, stereo_driver** out, const stereo_image_format* fmt,
+                                      const stereo_census_params* p)
+{
+    if (!ctx || !out || !fmt || !p) return STEREO_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (!format_valid(fmt)) return fail(ctx, STEREO_ERROR_INVALID_FORMAT, "Invalid image format");
+    if (!stereo::census_supported(p->window_w, p->window_h, p->reference_3x3))
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "census window %dx%d (reference_3x3=%d) not supported", p->window_w,
+                    p->window_h, p->reference_3x3);
+    stereo_status st = set_device(ctx);
+    if (st) return st;
+    stereo_driver* d = new stereo_driver(stereo_driver::CENSUS);
+    d->ctx = ctx;
+    d->name = "Census Transform";
+    d->fmt = *fmt;
+    d->cen = *p;
+    if ((st = driver_events(d))) { driver_release(d); delete d; return st; }
+    *out = d;
+    return STEREO_OK;
+}
+
+stereo_status stereo_census_process(stereo_driver* d, stereo_image* trg, const stereo_image* src, stereo_stream stream)
+{
+    if (!d || d->kind != stereo_driver::CENSUS || !trg || !src) return STEREO_ERROR_INVALID_ARGUMENT;
+    stereo_context* ctx = d->ctx;
+    if (trg->format.width != src->format.width || trg->format.height != src->format.height)
+        return fail(ctx, STEREO_ERROR_FORMAT_MISMATCH, "Input and output images do not have the same format");
+    stereo_status st;
+    if ((st = check_image(ctx, src, d->fmt.width, d->fmt.height, STEREO_PIXEL_LUM_U8, "census source"))) return st;
+    const int out_type = d->cen.reference_3x3 ? STEREO_PIXEL_LUM_U8 : STEREO_PIXEL_LUM_U64;
+    if ((st = check_image(ctx, trg, d->fmt.width, d->fmt.height, out_type, "census target"))) return st;
+    if ((st = set_device(ctx))) return st;
+    GUARD(ctx, "stereo_census_process");
+    hipStream_t s = (hipStream_t)stream;
+    if ((st = begin(d, s))) return st;
+    HIP_TRY(ctx, stereo::launch_census(view_of(src), view_of(trg), d->cen.window_w, d->cen.window_h, d->cen.reference_3x3, s),
+            "census launch");
+    return end(d, s);
+}
+
+// ---- matchers -----------------------------------------------------------------------
+
+stereo_status stereo_matcher_validate(const stereo_image_format* fmt, const stereo_matcher_params* p, char* msg,
+                                      size_t msg_len)
+{
+    auto say = [&](stereo_status st, const char* text) {
+        if (msg && msg_len) std::snprintf(msg, msg_len, "%s", text);
+        return st;
+    };
+    if (!fmt || !p) return say(STEREO_ERROR_INVALID_ARGUMENT, "null argument");
+    if (!format_valid(fmt)) return say(STEREO_ERROR_INVALID_FORMAT, "Invalid image format");
+    if (fmt->pixel_type != STEREO_PIXEL_LUM_U8)
+        return say(STEREO_ERROR_UNSUPPORTED_PIXEL_TYPE, "matchers take LUM_U8 images");
+    for (int i = 0; i < 5; ++i)
+        if (p->reserved[i]) return say(STEREO_ERROR_INVALID_CONFIG, "reserved parameters must be 0");
+    if (p->lr_tolerance < 0) return say(STEREO_ERROR_INVALID_CONFIG, "lr_tolerance must be >= 0");
+    if (p->num_disparities < 8 || p->num_disparities > 256 || p->num_disparities % 8)
+        return say(STEREO_ERROR_INVALID_CONFIG, "num_disparities must be a multiple of 8 in [8, 256]");
+    switch (p->algorithm) {
+    case STEREO_ALGO_SAD:
+        if (!stereo::bm_supported(stereo::BM_COST_SAD, p->window_w, p->window_h, p->num_disparities))
+            return say(STEREO_ERROR_INVALID_CONFIG, "SAD window must be 5x5, 7x7 or 9x9");
+        break;
+    case STEREO_ALGO_CENSUS:
+        if (!stereo::bm_supported(stereo::BM_COST_CENSUS, p->window_w, p->window_h, p->num_disparities))
+            return say(STEREO_ERROR_INVALID_CONFIG, "census aggregation window must be 5x5, 7x7 or 9x9");
+        if (!stereo::census_supported(p->census_w, p->census_h, 0))
+            return say(STEREO_ERROR_INVALID_CONFIG, "census window must be 9x7, 7x9, 7x7, 5x5 or 3x3");
+        break;
+    default:
+        return say(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
+    }
+    const int bx = p->window_w / 2 + (p->algorithm == STEREO_ALGO_CENSUS ? p->census_w / 2 : 0);
+    const int by = p->window_h / 2 + (p->algorithm == STEREO_ALGO_CENSUS ? p->census_h / 2 : 0);
+    if (fmt->width <= 2 * bx || fmt->height <= 2 * by) return say(STEREO_ERROR_INVALID_FORMAT, "image smaller than the window");
+    return say(STEREO_OK, "");
+}
+
+stereo_status stereo_matcher_configure(stereo_context* ctx, stereo_driver** out, const stereo_image_format* fmt,
+                                       const stereo_matcher_params* p)
+{
+    if (!ctx || !out || !fmt || !p) return STEREO_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    char msg[256];
+    stereo_status st = stereo_matcher_validate(fmt, p, msg, sizeof(msg));
+    if (st) return fail(ctx, st, "%s", msg);
+    if ((st = set_device(ctx))) return st;
+    stereo_driver* d = new stereo_driver(stereo_driver::MATCHER);
+    d->ctx = ctx;
+    d->fmt = *fmt;
+    d->mp = *p;
+    const bool census = p->algorithm == STEREO_ALGO_CENSUS;
+    d->name = census ? "Block Matcher Census" : "Block Matcher SAD";
+    d->bx = p->window_w / 2 + (census ? p->census_w / 2 : 0);
+    d->by = p->window_h / 2 + (census ? p->census_h / 2 : 0);
+    stereo::bm_grid(fmt->width, fmt->height, &d->ntx, &d->nty);
+    d->pad_elem = census ? 8 : 1;
+    const size_t cols = (size_t)stereo::BM_PAD_LEFT + fmt->width + stereo::BM_PAD_RIGHT;
+    d->pad_pitch = (cols * d->pad_elem + 255) & ~(size_t)255;
+    const size_t W = fmt->width, H = fmt->height;
+    auto cleanup = [&](stereo_status s2) { driver_release(d); delete d; return s2; };
+    if ((st = driver_events(d))) return cleanup(st);
+    if (injected_failure()) return cleanup(fail(ctx, STEREO_ERROR_HIP, "stereo_matcher_configure: injected failure"));
+    hipError_t e;
+    if ((e = hipMalloc((void**)&d->padL, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
+    if ((e = hipMalloc((void**)&d->padR, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
+    if ((e = hipMalloc((void**)&d->lkey, W * H * 4)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
+    if ((e = hipMalloc((void**)&d->rpart, (size_t)d->ntx * H * stereo::BM_NPART * 4)) != hipSuccess)
+        return cleanup(hip_fail(ctx, e, "hipMalloc"));
+    // the pad columns only ever meet masked candidates; clear them once so that no
+    // uninitialised memory is read
+    if ((e = hipMemset(d->padL, 0, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMemset"));
+    if ((e = hipMemset(d->padR, 0, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMemset"));
+    *out = d;
+    return STEREO_OK;
+}
+
+stereo_status stereo_matcher_process(stereo_driver* d, stereo_image* disp, stereo_image* disp_lr, stereo_image* disp_rl,
+                                     const stereo_image* left, const stereo_image* right, stereo_stream stream)
+{
+    if (!d || d->kind != stereo_driver::MATCHER || !left || !right) return STEREO_ERROR_INVALID_ARGUMENT;
+    stereo_context* ctx = d->ctx;
+    const int W = d->fmt.width, H = d->fmt.height;
+    stereo_status st;
+    if ((st = check_image(ctx, left, W, H, STEREO_PIXEL_LUM_U8, "left image"))) return st;
+    if ((st = check_image(ctx, right, W, H, STEREO_PIXEL_LUM_U8, "right image"))) return st;
+    bool dispid = false;
+    if (disp) {
+        if (disp->format.pixel_type == STEREO_PIXEL_RGBA_U16) dispid = true;
+        if ((st = check_image(ctx, disp, W, H, dispid ? STEREO_PIXEL_RGBA_U16 : STEREO_PIXEL_LUM_I16, "disparity output")))
+            return st;
+    }
+    if (disp_lr && (st = check_image(ctx, disp_lr, W, H, STEREO_PIXEL_LUM_I16, "left-to-right output"))) return st;
+    if (disp_rl && (st = check_image(ctx, disp_rl, W, H, STEREO_PIXEL_LUM_I16, "right-to-left output"))) return st;
+    if ((st = set_device(ctx))) return st;
+    GUARD(ctx, "stereo_matcher_process");
+    hipStream_t s = (hipStream_t)stream;
+    if ((st = begin(d, s))) return st;
+
+    View pl, pr;
+    pl.pitch = pr.pitch = d->pad_pitch;
+    pl.w = pr.w = W;
+    pl.h = pr.h = H;
+    pl.ptr = d->padL + (size_t)stereo::BM_PAD_LEFT * d->pad_elem;
+    pr.ptr = d->padR + (size_t)stereo::BM_PAD_LEFT * d->pad_elem;
+    const bool census = d->mp.algorithm == STEREO_ALGO_CENSUS;
+    if (census) {
+        HIP_TRY(ctx, stereo::launch_census(view_of(left), pl, d->mp.census_w, d->mp.census_h, 0, s), "census launch");
+        HIP_TRY(ctx, stereo::launch_census(view_of(right), pr, d->mp.census_w, d->mp.census_h, 0, s), "census launch");
+    } else {
+        HIP_TRY(ctx, hipMemcpy2DAsync(pl.ptr, pl.pitch, left->data, left->pitch, W, H, hipMemcpyDeviceToDevice, s),
+                "hipMemcpy2DAsync");
+        HIP_TRY(ctx, hipMemcpy2DAsync(pr.ptr, pr.pitch, right->data, right->pitch, W, H, hipMemcpyDeviceToDevice, s),
+                "hipMemcpy2DAsync");
+    }
+    if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[0], s), "hipEventRecord");
+    stereo::BmParams bp;
+    bp.L = pl;
+    bp.R = pr;
+    bp.lkey = d->lkey;
+    bp.rpart = d->rpart;
+    bp.W = W;
+    bp.H = H;
+    bp.D = d->mp.num_disparities;
+    bp.bx = d->bx;
+    bp.by = d->by;
+    HIP_TRY(ctx, stereo::launch_bm(census ? stereo::BM_COST_CENSUS : stereo::BM_COST_SAD, d->mp.window_w, bp, s),
+            "matcher launch");
+    if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[1], s), "hipEventRecord");
+    stereo::BmFinishParams f;
+    f.lkey = d->lkey;
+    f.rpart = d->rpart;
+    f.disp = (disp && !dispid) ? view_of(disp) : null_view();
+    f.dispid = (disp && dispid) ? view_of(disp) : null_view();
+    f.disp_l = disp_lr ? view_of(disp_lr) : null_view();
+    f.disp_r = disp_rl ? view_of(disp_rl) : null_view();
+    f.W = W;
+    f.H = H;
+    f.D = bp.D;
+    f.ntx = d->ntx;
+    f.bx = d->bx;
+    f.by = d->by;
+    f.tol = d->mp.lr_tolerance;
+    HIP_TRY(ctx, stereo::launch_bm_finish(f, s), "finish launch");
+    return end(d, s);
+}
+
+stereo_status stereo_matcher_stage_times(stereo_driver* d, double* ms, int32_t n)
+{
+    if (!d || d->kind != stereo_driver::MATCHER || !ms || n < 1) return STEREO_ERROR_INVALID_ARGUMENT;
+    if (!d->timed) return fail(d->ctx, STEREO_ERROR_NOT_READY, "%s: no timed process call (enable stereo_set_timing)", d->name);
+    HIP_TRY(d->ctx, hipEventSynchronize(d->ev_stop), "hipEventSynchronize");
+    hipEvent_t marks[4] = {d->ev_start, d->ev_stage[0], d->ev_stage[1], d->ev_stop};
+    for (int i = 0; i < 3 && i < n; ++i) {
+        float t = 0.f;
+        HIP_TRY(d->ctx, hipEventElapsedTime(&t, marks[i], marks[i + 1]), "hipEventElapsedTime");
+        ms[i] = t;
+    }
+    return STEREO_OK;
+}
+
+// ---- reference-format cross check ----------------------------------------------------
+
+stereo_status stereo_crosscheck_configure(stereo_context* ctx, stereo_driver** out, const stereo_image_format* fmt,
+                                          const stereo_crosscheck_params* p)
+{
+    if (!ctx || !out || !fmt || !p) return STEREO_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (!format_valid(fmt)) return fail(ctx, STEREO_ERROR_INVALID_FORMAT, "Invalid image format");
+    if (p->disp_delta_thresh < 0) return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "disp_delta_thresh must be >= 0");
+    stereo_status st;
+    if ((st = set_device(ctx))) return st;
+    stereo_driver* d = new stereo_driver(stereo_driver::CROSSCHECK);
+    d->ctx = ctx;
+    d->name = "Cross Check Disparity Maps";
+    d->fmt = *fmt;
+    d->cc = *p;
+    auto cleanup = [&](stereo_status s2) { driver_release(d); delete d; return s2; };
+    if ((st = driver_events(d))) return cleanup(st);
+    if (injected_failure()) return cleanup(fail(ctx, STEREO_ERROR_HIP, "stereo_crosscheck_configure: injected failure"));
+    hipError_t e;
+    if ((e = hipMalloc((void**)&d->range_dev, 2 * sizeof(uint32_t))) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
+    if ((e = hipHostMalloc((void**)&d->range_host, 2 * sizeof(uint32_t), 0)) != hipSuccess)
+        return cleanup(hip_fail(ctx, e, "hipHostMalloc"));
+    if ((e = hipEventCreateWithFlags(&d->ev_range, hipEventDisableTiming)) != hipSuccess)
+        return cleanup(hip_fail(ctx, e, "hipEventCreate"));
+    d->range_host[0] = 0xFFF0 - 1;
+    d->range_host[1] = 1;
+    *out = d;
+    return STEREO_OK;
+}
+
+stereo_status stereo_crosscheck_process(stereo_driver* d, stereo_image* result, const stereo_image* lr,
+                                        const stereo_image* rl, stereo_stream stream)
+{
+    if (!d || d->kind != stereo_driver::CROSSCHECK || !result || !lr || !rl) return STEREO_ERROR_INVALID_ARGUMENT;
+    stereo_context* ctx = d->ctx;
+    if (lr->format.width != rl->format.width || lr->format.height != rl->format.height ||
+        lr->format.pixel_type != rl->format.pixel_type)
+        return fail(ctx, STEREO_ERROR_FORMAT_MISMATCH, "Input disparity images have different formats");
+    if (lr->format.width != result->format.width || lr->format.height != result->format.height ||
+        lr->format.pixel_type != result->format.pixel_type)
+        return fail(ctx, STEREO_ERROR_FORMAT_MISMATCH, "Input and output disparity images have different formats");
+    stereo_status st;
+    const int W = d->fmt.width, H = d->fmt.height;
+    if ((st = check_image(ctx, lr, W, H, STEREO_PIXEL_RGBA_U16, "left-to-right map"))) return st;
+    if ((st = check_image(ctx, rl, W, H, STEREO_PIXEL_RGBA_U16, "right-to-left map"))) return st;
+    if ((st = check_image(ctx, result, W, H, STEREO_PIXEL_RGBA_U16, "result map"))) return st;
+    if ((st = set_device(ctx))) return st;
+    GUARD(ctx, "stereo_crosscheck_process");
+    hipStream_t s = (hipStream_t)stream;
+    if ((st = begin(d, s))) return st;
+    HIP_TRY(ctx,
+            stereo::launch_crosscheck_ref(view_of(result), view_of(lr), view_of(rl), d->cc.disp_delta_thresh,
+                                          d->cc.left_to_right, d->cc.disp_offset, d->range_dev, s),
+            "crosscheck launch");
+    HIP_TRY(ctx, hipMemcpyAsync(d->range_host, d->range_dev, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync");
+    HIP_TRY(ctx, hipEventRecord(d->ev_range, s), "hipEventRecord");
+    d->range_pending = true;
+    return end(d, s);
+}
+
+stereo_status stereo_crosscheck_range(stereo_driver* d, uint16_t* mn, uint16_t* mx)
+{
+    if (!d || d->kind != stereo_driver::CROSSCHECK || !mn || !mx) return STEREO_ERROR_INVALID_ARGUMENT;
+    if (d->range_pending) {
+        HIP_TRY(d->ctx, hipEventSynchronize(d->ev_range), "hipEventSynchronize");
+        d->range_pending = false;
+    }
+    *mn = (uint16_t)d->range_host[0];
+    *mx = (uint16_t)d->range_host[1];
+    return STEREO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// Census transform kernels for gfx950.
+//
+// Reference: CensusTransform::Kernel::Transform (CluTec.Cuda.ImgProc/CensusTransform.cu:59-120)
+// compares the 8 neighbours of a 3x3 window against the centre with strict '>' and
+// packs them raster-ordered, most significant bit first, into a uint8.  The wide
+// variant keeps those rules for a CW x CH window packed into uint64: raster position
+// k maps to bit CW*CH-1-k, the centre bit is always 0.  Pixels whose window leaves
+// the image are written as 0 (the reference leaves them untouched).
+//
+// Layout: a 64 x 32 output tile per 256-thread workgroup; the input tile plus halo
+// is staged in LDS once; each thread produces an 8-row column strip, keeping the
+// (8 + CH - 1) x CW neighbourhood in registers so that each LDS byte is read once
+// per thread.  Each descriptor bit costs one compare and one add-with-carry.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace stereo {
+
+namespace {
+
+constexpr int CT_W = 64;
+constexpr int CT_H = 32;
+constexpr int CT_STRIP = 8;
+
+template <int CW, int CH>
+__global__ __launch_bounds__(256) void census_wide_kernel(View src, View dst)
+{
+    constexpr int RX = CW / 2, RY = CH / 2, NB = CW * CH;
+    constexpr int LW = CT_W + 2 * RX, LH = CT_H + 2 * RY;
+    constexpr int NHI = NB > 32 ? NB - 32 : 0;  // bits landing in the high word
+    __shared__ uint8_t tile[LH][LW + 4];
+
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * CT_W, y0 = blockIdx.y * CT_H;
+    for (int i = tid; i < LH * LW; i += 256) {
+        const int r = i / LW, c = i - r * LW;
+        const int gy = clampi(y0 - RY + r, 0, src.h - 1);
+        const int gx = clampi(x0 - RX + c, 0, src.w - 1);
+        tile[r][c] = row_ptr<const uint8_t>(src, gy)[gx];
+    }
+    __syncthreads();
+
+    const int cx = tid & 63;
+    const int strip = tid >> 6;
+    const int x = x0 + cx;
+    uint8_t win[CT_STRIP + 2 * RY][CW];
+#pragma unroll
+    for (int r = 0; r < CT_STRIP + 2 * RY; ++r)
+#pragma unroll
+        for (int c = 0; c < CW; ++c) win[r][c] = tile[strip * CT_STRIP + r][cx + c];
+
+#pragma unroll
+    for (int k = 0; k < CT_STRIP; ++k) {
+        const int y = y0 + strip * CT_STRIP + k;
+        const uint32_t centre = win[k + RY][RX];
+        uint32_t hi = 0, lo = 0;
+        int bit = 0;
+#pragma unroll
+        for (int dy = 0; dy < CH; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < CW; ++dx, ++bit) {
+                const uint32_t g = (uint32_t)win[k + dy][dx] > centre ? 1u : 0u;
+                if (bit < NHI)
+                    hi = hi + hi + g;
+                else
+                    lo = lo + lo + g;
+            }
+        const bool inside = x >= RX && x < src.w - RX && y >= RY && y < src.h - RY;
+        if (x < dst.w && y < dst.h) {
+            const uint64_t v = inside ? (((uint64_t)hi << 32) | lo) : 0ull;
+            row_ptr<uint64_t>(dst, y)[x] = v;
+        }
+    }
+}
+
+// Reference 3x3 census into uint8 (CensusTransform.cu:103-119 bit layout).
+__global__ __launch_bounds__(256) void census3x3_ref_kernel(View src, View dst)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= dst.w || y >= dst.h) return;
+    uint32_t v = 0;
+    if (x >= 1 && x < src.w - 1 && y >= 1 && y < src.h - 1) {
+        const uint8_t* r0 = row_ptr<const uint8_t>(src, y - 1);
+        const uint8_t* r1 = row_ptr<const uint8_t>(src, y);
+        const uint8_t* r2 = row_ptr<const uint8_t>(src, y + 1);
+        const uint32_t c = r1[x];
+        v |= (r0[x - 1] > c) ? 0x80u : 0u;
+        v |= (r0[x] > c) ? 0x40u : 0u;
+        v |= (r0[x + 1] > c) ? 0x20u : 0u;
+        v |= (r1[x - 1] > c) ? 0x10u : 0u;
+        v |= (r1[x + 1] > c) ? 0x08u : 0u;
+        v |= (r2[x - 1] > c) ? 0x04u : 0u;
+        v |= (r2[x] > c) ? 0x02u : 0u;
+        v |= (r2[x + 1] > c) ? 0x01u : 0u;
+    }
+    row_ptr<uint8_t>(dst, y)[x] = (uint8_t)v;
+}
+
+template <int CW, int CH>
+hipError_t launch_wide(const View& src, const View& dst, hipStream_t s)
+{
+    dim3 grid((src.w + CT_W - 1) / CT_W, (src.h + CT_H - 1) / CT_H);
+    hipLaunchKernelGGL((census_wide_kernel<CW, CH>), grid, dim3(256), 0, s, src, dst);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+bool census_supported(int cw, int ch, int ref3x3)
+{
+    if (ref3x3) return cw == 3 && ch == 3;
+    return (cw == 9 && ch == 7) || (cw == 7 && ch == 9) || (cw == 5 && ch == 5) || (cw == 7 && ch == 7) ||
+           (cw == 3 && ch == 3);
+}
+
+hipError_t launch_census(const View& src, const View& dst, int cw, int ch, int ref3x3, hipStream_t s)
+{
+    if (ref3x3) {
+        dim3 grid((src.w + 63) / 64, (src.h + 3) / 4);
+        hipLaunchKernelGGL(census3x3_ref_kernel, grid, dim3(256), 0, s, src, dst);
+        return hipGetLastError();
+    }
+    if (cw == 9 && ch == 7) return launch_wide<9, 7>(src, dst, s);
+    if (cw == 7 && ch == 9) return launch_wide<7, 9>(src, dst, s);
+    if (cw == 5 && ch == 5) return launch_wide<5, 5>(src, dst, s);
+    if (cw == 7 && ch == 7) return launch_wide<7, 7>(src, dst, s);
+    if (cw == 3 && ch == 3) return launch_wide<3, 3>(src, dst, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace stereo

@@ -1,0 +1,49 @@
+// Launch interfaces of the HIP kernels (host side of each .hip translation unit).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace stereo {
+
+// ---- census ----------------------------------------------------------------
+bool census_supported(int cw, int ch, int ref3x3);
+hipError_t launch_census(const View& src, const View& dst, int cw, int ch, int ref3x3, hipStream_t s);
+
+// ---- fused SAD / census matcher -------------------------------------------------
+enum { BM_COST_SAD = 0, BM_COST_CENSUS = 1 };
+constexpr int BM_NPART = 320;     // right-view partial keys per tile row
+constexpr int BM_PAD_LEFT = 320;  // readable columns left of x = 0 in matcher inputs
+constexpr int BM_PAD_RIGHT = 128; // readable columns right of x = W-1
+
+struct BmParams {
+    View L, R;         // uint8 grey (SAD) or uint64 census (CENSUS)
+    uint32_t* lkey;    // W*H best left keys (pitch W)
+    uint32_t* rpart;   // ntx*H*BM_NPART right-view partial keys
+    int W, H, D;
+    int bx, by;        // matchable border
+};
+
+struct BmFinishParams {
+    const uint32_t* lkey;
+    const uint32_t* rpart;
+    View disp;     // int16 checked map (optional: ptr == nullptr)
+    View disp_l;   // int16 left map (optional)
+    View disp_r;   // int16 right map (optional)
+    View dispid;   // 4 x uint16 reference pixels (optional)
+    int W, H, D, ntx;
+    int bx, by, tol;
+};
+
+bool bm_supported(int kind, int ww, int wh, int D);
+void bm_grid(int W, int H, int* ntx, int* nty);
+hipError_t launch_bm(int kind, int ww, const BmParams& p, hipStream_t s);
+hipError_t launch_bm_finish(const BmFinishParams& f, hipStream_t s);
+
+// ---- reference-format cross check ------------------------------------------------
+// range[0..1] receive min / max of the kept ids (device memory, reset by the launch).
+hipError_t launch_crosscheck_ref(const View& out, const View& lr, const View& rl, int thresh, int ltor, int offset,
+                                 uint32_t* range, hipStream_t s);
+
+}  // namespace stereo

@@ -1,0 +1,161 @@
+"""ctypes wrapper of the CPU oracle (oracle/build/liborc_stereo.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker / CPU baseline.  The engine never
+uses it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liborc_stereo.so")
+
+COST_SAD = 0
+COST_CENSUS = 1
+
+DISP_UNKNOWN = 0
+DISP_FIRST = 1
+DISP_LAST = 0xFFF0 - 1
+DISP_CANNOT_EVALUATE = 0xFFFF
+DISP_SATURATED = 0xFFFF - 1
+DISP_NOT_FOUND = 0xFFFF - 2
+DISP_NOT_SPECIFIC = 0xFFFF - 3
+DISP_NOT_UNIQUE = 0xFFFF - 4
+DISP_INCONSISTENT = 0xFFFF - 5
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, no GPU)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return LIB_PATH
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    i = C.c_int
+    lib.orc_version.restype = i
+    lib.orc_census.argtypes = [P, i, i, i, i, P]
+    lib.orc_census3x3_ref.argtypes = [P, i, i, P]
+    lib.orc_bm.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P, i]
+    lib.orc_bm_brute.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P]
+    lib.orc_ncc.argtypes = [P, P, i, i, i, i, i, i, P, P, P, P, P, i]
+    lib.orc_aw.argtypes = [i, P, P, i, i, i, i, i, P, P, P, P, P, i]
+    lib.orc_crosscheck_plain.argtypes = [P, P, i, i, i, P]
+    lib.orc_crosscheck_ref.argtypes = [P, P, i, i, i, i, i, P, P, P]
+    for name in ["orc_census", "orc_census3x3_ref", "orc_bm", "orc_bm_brute", "orc_ncc", "orc_aw",
+                 "orc_crosscheck_plain", "orc_crosscheck_ref"]:
+        getattr(lib, name).restype = i
+    return lib
+
+
+lib = _load()
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _chk(rc, name):
+    if rc != 0:
+        raise RuntimeError(f"{name} failed ({rc})")
+
+
+def census(img, cw=9, ch=7):
+    img = np.ascontiguousarray(img, np.uint8)
+    H, W = img.shape
+    out = np.empty((H, W), np.uint64)
+    _chk(lib.orc_census(_p(img), W, H, cw, ch, _p(out)), "orc_census")
+    return out
+
+
+def census3x3_ref(img):
+    img = np.ascontiguousarray(img, np.uint8)
+    H, W = img.shape
+    out = np.empty((H, W), np.uint8)
+    _chk(lib.orc_census3x3_ref(_p(img), W, H, _p(out)), "orc_census3x3_ref")
+    return out
+
+
+def bm(kind, L, R, D, window=(9, 9), census_win=(1, 1), tol=1, nthreads=1, with_cost=False):
+    """Plain-spec matcher.  SAD: L, R uint8.  Census: L, R uint64 descriptors and
+    census_win the window that produced them (sets the border)."""
+    dt = np.uint8 if kind == COST_SAD else np.uint64
+    L = np.ascontiguousarray(L, dt)
+    R = np.ascontiguousarray(R, dt)
+    H, W = L.shape
+    dL = np.empty((H, W), np.int16)
+    dR = np.empty((H, W), np.int16)
+    out = np.empty((H, W), np.int16)
+    cost = np.empty((H, W), np.int32) if with_cost else None
+    _chk(lib.orc_bm(kind, _p(L), _p(R), W, H, D, window[0], window[1], census_win[0], census_win[1], tol,
+                    _p(dL), _p(dR), _p(out), _p(cost), nthreads), "orc_bm")
+    return (out, dL, dR, cost) if with_cost else (out, dL, dR)
+
+
+def bm_brute(kind, L, R, D, window=(9, 9), census_win=(1, 1), tol=1, with_volume=False):
+    dt = np.uint8 if kind == COST_SAD else np.uint64
+    L = np.ascontiguousarray(L, dt)
+    R = np.ascontiguousarray(R, dt)
+    H, W = L.shape
+    dL = np.empty((H, W), np.int16)
+    dR = np.empty((H, W), np.int16)
+    out = np.empty((H, W), np.int16)
+    vol = np.empty((H, W, D), np.int32) if with_volume else None
+    _chk(lib.orc_bm_brute(kind, _p(L), _p(R), W, H, D, window[0], window[1], census_win[0], census_win[1], tol,
+                          _p(dL), _p(dR), _p(out), _p(vol)), "orc_bm_brute")
+    return (out, dL, dR, vol) if with_volume else (out, dL, dR)
+
+
+def ncc(L, R, D, window=(11, 11), tol=1, nthreads=1, with_volume=False):
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    dL = np.empty((H, W), np.int16)
+    dR = np.empty((H, W), np.int16)
+    out = np.empty((H, W), np.int16)
+    sc = np.empty((H, W), np.float64)
+    vol = np.empty((H, W, D), np.float64) if with_volume else None
+    _chk(lib.orc_ncc(_p(L), _p(R), W, H, D, window[0], window[1], tol, _p(dL), _p(dR), _p(out), _p(sc), _p(vol),
+                     nthreads), "orc_ncc")
+    return (out, dL, dR, sc, vol) if with_volume else (out, dL, dR, sc)
+
+
+def aw(mode, L, R, D, sub=5, tol=1, nthreads=1, with_volume=False):
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    dL = np.empty((H, W), np.int16)
+    dR = np.empty((H, W), np.int16)
+    out = np.empty((H, W), np.int16)
+    val = np.empty((H, W), np.int32)
+    vol = np.empty((H, W, D), np.int32) if with_volume else None
+    _chk(lib.orc_aw(mode, _p(L), _p(R), W, H, D, sub, tol, _p(dL), _p(dR), _p(out), _p(val), _p(vol), nthreads),
+         "orc_aw")
+    return (out, dL, dR, val, vol) if with_volume else (out, dL, dR, val)
+
+
+def crosscheck_plain(dL, dR, tol=1):
+    dL = np.ascontiguousarray(dL, np.int16)
+    dR = np.ascontiguousarray(dR, np.int16)
+    H, W = dL.shape
+    out = np.empty((H, W), np.int16)
+    _chk(lib.orc_crosscheck_plain(_p(dL), _p(dR), W, H, tol, _p(out)), "orc_crosscheck_plain")
+    return out
+
+
+def crosscheck_ref(lr, rl, thresh=1, ltor=1, offset=0):
+    lr = np.ascontiguousarray(lr, np.uint16)
+    rl = np.ascontiguousarray(rl, np.uint16)
+    H, W = lr.shape[:2]
+    out = np.empty_like(lr)
+    mn = np.zeros(1, np.uint16)
+    mx = np.zeros(1, np.uint16)
+    _chk(lib.orc_crosscheck_ref(_p(lr), _p(rl), W, H, thresh, ltor, offset, _p(out), _p(mn), _p(mx)),
+         "orc_crosscheck_ref")
+    return out, int(mn[0]), int(mx[0])

@@ -1,0 +1,462 @@
+/*
+ * CPU oracle for the stereo disparity hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * It is the checker for the HIP engine (tests/) and the CPU baseline timed by
+ * bench.py.  The engine never links or calls it.  Arrays are row-major with
+ * pitch == width.
+ *
+ * Plain specification shared by every matcher here (the reference fixes none of
+ * these rules for its own kernels because its results depend on block tiling):
+ *   - a left pixel (x, y) is matchable iff bx <= x < W-bx and by <= y < H-by,
+ *     where bx/by are the aggregation radius plus the census radius;
+ *   - candidate disparity d is valid iff 0 <= d < D and x-d >= bx, the right
+ *     window being centred on (x-d, y);
+ *   - winner-take-all keeps the best cost, the lowest d on ties (the reference
+ *     scans d upward with a strict comparison: BlockMatcherNCC.cu:298,
+ *     BlockMatcherAW1.cu:294, BlockMatcherAW2.cu:167);
+ *   - the right-to-left map takes, for right pixel xr, the best of the costs
+ *     C(xr+d, y, d) of the same cost volume, lowest d on ties;
+ *   - the checked map keeps dL where |dR(x-dL) - dL| <= tol, else -1.
+ */
+#include "stereo_oracle.h"
+
+#include <limits.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int orc_version(void) { return 3; }
+
+static inline int popc64(uint64_t v) { return __builtin_popcountll(v); }
+
+/* ---------------------------------------------------------------------------------
+ * Census.  Reference: CensusTransform.cu:103-119 compares the 8 neighbours of a 3x3
+ * window with strict '>' against the centre and packs them raster-ordered, MSB
+ * first, into a uint8; only pixels with IsInside(x, y, 1, 1) are written
+ * (CensusTransform.cu:64-68).  The generalisation keeps strict '>', raster order
+ * and MSB-first: raster position k of an n = cw*ch window maps to bit n-1-k (the
+ * centre bit is always 0).  Pixels whose window leaves the image are 0.
+ * --------------------------------------------------------------------------------- */
+int orc_census(const uint8_t* img, int W, int H, int cw, int ch, uint64_t* out)
+{
+    if (!img || !out || cw < 1 || ch < 1 || (cw & 1) == 0 || (ch & 1) == 0 || cw * ch > 64) return -1;
+    const int rx = cw / 2, ry = ch / 2, n = cw * ch;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            uint64_t v = 0;
+            if (x >= rx && x < W - rx && y >= ry && y < H - ry) {
+                const int c = img[(size_t)y * W + x];
+                int k = 0;
+                for (int dy = -ry; dy <= ry; ++dy)
+                    for (int dx = -rx; dx <= rx; ++dx, ++k)
+                        if (img[(size_t)(y + dy) * W + (x + dx)] > c) v |= 1ull << (n - 1 - k);
+            }
+            out[(size_t)y * W + x] = v;
+        }
+    return 0;
+}
+
+int orc_census3x3_ref(const uint8_t* img, int W, int H, uint8_t* out)
+{
+    static const int bits[3][3] = {{0x80, 0x40, 0x20}, {0x10, 0, 0x08}, {0x04, 0x02, 0x01}};
+    if (!img || !out) return -1;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            int v = 0;
+            if (x >= 1 && x < W - 1 && y >= 1 && y < H - 1) {
+                const int c = img[(size_t)y * W + x];
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx)
+                        if (bits[dy + 1][dx + 1] && img[(size_t)(y + dy) * W + (x + dx)] > c) v |= bits[dy + 1][dx + 1];
+            }
+            out[(size_t)y * W + x] = (uint8_t)v;
+        }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------
+ * Per-pixel matching costs.  AbsoluteDifference: BlockMatcherNCC.cu:147-151,
+ * BlockMatcherAW2.cu:129-133.  CensusDifference (popcount of XOR):
+ * BlockMatcherNCC.cu:153-158 (here over 64-bit descriptors).
+ * --------------------------------------------------------------------------------- */
+static inline int cost_at(int kind, const void* L, const void* R, size_t iL, size_t iR)
+{
+    if (kind == ORC_COST_SAD) {
+        const int a = ((const uint8_t*)L)[iL], b = ((const uint8_t*)R)[iR];
+        return a > b ? a - b : b - a;
+    }
+    return popc64(((const uint64_t*)L)[iL] ^ ((const uint64_t*)R)[iR]);
+}
+
+/* Horizontal running box sum of one cost row for disparity d over output columns
+ * [xs, xe); h[x] receives sum_{i=-ax..ax} c(x+i, y, d).  Follows the slide-in /
+ * slide-out update of AlgoArraySum (Kernel.Algo.ArraySum.h:149-189). */
+static void hsum_row(int kind, const void* L, const void* R, int W, int y, int d, int ax, int xs,
+                     int xe, int32_t* h)
+{
+    const size_t row = (size_t)y * W;
+    int s = 0;
+    for (int i = -ax; i <= ax; ++i) s += cost_at(kind, L, R, row + xs + i, row + xs + i - d);
+    h[xs] = s;
+    for (int x = xs + 1; x < xe; ++x) {
+        s += cost_at(kind, L, R, row + x + ax, row + x + ax - d) -
+             cost_at(kind, L, R, row + x - ax - 1, row + x - ax - 1 - d);
+        h[x] = s;
+    }
+}
+
+static void crosscheck_rows(const int16_t* dL, const int16_t* dR, int W, int y0, int y1, int tol,
+                            int16_t* out)
+{
+    for (int y = y0; y < y1; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t i = (size_t)y * W + x;
+            const int d = dL[i];
+            int16_t v = -1;
+            if (d >= 0 && x - d >= 0) {
+                const int r = dR[i - d];
+                if (r >= 0 && abs(r - d) <= tol) v = (int16_t)d;
+            }
+            out[i] = v;
+        }
+}
+
+int orc_crosscheck_plain(const int16_t* dL, const int16_t* dR, int W, int H, int tol, int16_t* out)
+{
+    if (!dL || !dR || !out) return -1;
+    crosscheck_rows(dL, dR, W, 0, H, tol, out);
+    return 0;
+}
+
+static int bm_args_ok(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh,
+                      int cw, int ch)
+{
+    if (!L || !R || W <= 0 || H <= 0 || D <= 0 || D > 32767) return 0;
+    if (kind != ORC_COST_SAD && kind != ORC_COST_CENSUS) return 0;
+    if (ww < 1 || wh < 1 || (ww & 1) == 0 || (wh & 1) == 0) return 0;
+    if (cw < 1 || ch < 1 || (cw & 1) == 0 || (ch & 1) == 0) return 0;
+    return 1;
+}
+
+int orc_bm(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
+           int ch, int tol, int16_t* dL, int16_t* dR, int16_t* out, int32_t* cost_out, int nthreads)
+{
+    if (!bm_args_ok(kind, L, R, W, H, D, ww, wh, cw, ch) || !dL || !dR) return -1;
+    const int ax = ww / 2, ay = wh / 2;
+    const int bx = ax + cw / 2, by = ay + ch / 2;
+    const size_t N = (size_t)W * H;
+    int32_t* bestL = (int32_t*)malloc(N * sizeof(int32_t));
+    int32_t* bestR = (int32_t*)malloc(N * sizeof(int32_t));
+    if (!bestL || !bestR) { free(bestL); free(bestR); return -2; }
+    for (size_t i = 0; i < N; ++i) { bestL[i] = INT_MAX; bestR[i] = INT_MAX; dL[i] = -1; dR[i] = -1; }
+
+    const int ylo = by, yhi = H - by;
+    if (nthreads < 1) nthreads = 1;
+    int rc = 0;
+    if (yhi > ylo && W - bx > bx) {
+        const int rows = yhi - ylo;
+        int nb = nthreads * 4;
+        if (nb > rows) nb = rows;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+        for (int b = 0; b < nb; ++b) {
+            const int ya = ylo + (int)((long)rows * b / nb);
+            const int yb = ylo + (int)((long)rows * (b + 1) / nb);
+            const int nr = 2 * ay + 1;
+            int32_t* ring = (int32_t*)malloc((size_t)nr * W * sizeof(int32_t));
+            int32_t* V = (int32_t*)malloc((size_t)W * sizeof(int32_t));
+            if (!ring || !V) { rc = -2; free(ring); free(V); continue; }
+            for (int d = 0; d < D; ++d) {
+                const int xs = bx + d, xe = W - bx;
+                if (xs >= xe) break;
+                for (int j = 0; j < nr; ++j) hsum_row(kind, L, R, W, ya - ay + j, d, ax, xs, xe, ring + (size_t)j * W);
+                for (int x = xs; x < xe; ++x) {
+                    int s = 0;
+                    for (int j = 0; j < nr; ++j) s += ring[(size_t)j * W + x];
+                    V[x] = s;
+                }
+                for (int y = ya; y < yb; ++y) {
+                    if (y > ya) {
+                        int32_t* slot = ring + (size_t)((y - ya - 1) % nr) * W; /* holds row y-ay-1 */
+                        /* subtract the row leaving the window, add the row entering it */
+                        for (int x = xs; x < xe; ++x) V[x] -= slot[x];
+                        hsum_row(kind, L, R, W, y + ay, d, ax, xs, xe, slot);
+                        for (int x = xs; x < xe; ++x) V[x] += slot[x];
+                    }
+                    const size_t row = (size_t)y * W;
+                    for (int x = xs; x < xe; ++x) {
+                        const int A = V[x];
+                        if (A < bestL[row + x]) { bestL[row + x] = A; dL[row + x] = (int16_t)d; }
+                        if (A < bestR[row + x - d]) { bestR[row + x - d] = A; dR[row + x - d] = (int16_t)d; }
+                    }
+                }
+            }
+            free(ring);
+            free(V);
+        }
+    }
+    if (out) crosscheck_rows(dL, dR, W, 0, H, tol, out);
+    if (cost_out)
+        for (size_t i = 0; i < N; ++i) cost_out[i] = dL[i] >= 0 ? bestL[i] : -1;
+    free(bestL);
+    free(bestR);
+    return rc;
+}
+
+int orc_bm_brute(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh,
+                 int cw, int ch, int tol, int16_t* dL, int16_t* dR, int16_t* out, int32_t* volume)
+{
+    if (!bm_args_ok(kind, L, R, W, H, D, ww, wh, cw, ch) || !dL || !dR) return -1;
+    const int ax = ww / 2, ay = wh / 2;
+    const int bx = ax + cw / 2, by = ay + ch / 2;
+    const size_t N = (size_t)W * H;
+    if (volume)
+        for (size_t i = 0; i < N * (size_t)D; ++i) volume[i] = -1;
+    for (size_t i = 0; i < N; ++i) { dL[i] = -1; dR[i] = -1; }
+    int32_t* bestR = (int32_t*)malloc(N * sizeof(int32_t));
+    if (!bestR) return -2;
+    for (size_t i = 0; i < N; ++i) bestR[i] = INT_MAX;
+    for (int y = by; y < H - by; ++y)
+        for (int x = bx; x < W - bx; ++x) {
+            int best = INT_MAX;
+            for (int d = 0; d < D && x - d >= bx; ++d) {
+                int A = 0;
+                for (int j = -ay; j <= ay; ++j)
+                    for (int i = -ax; i <= ax; ++i) {
+                        const size_t p = (size_t)(y + j) * W + (x + i);
+                        A += cost_at(kind, L, R, p, p - d);
+                    }
+                if (volume) volume[((size_t)y * W + x) * D + d] = A;
+                if (A < best) { best = A; dL[(size_t)y * W + x] = (int16_t)d; }
+                const size_t r = (size_t)y * W + (x - d);
+                if (A < bestR[r] || (A == bestR[r] && d < dR[r])) { bestR[r] = A; dR[r] = (int16_t)d; }
+            }
+        }
+    free(bestR);
+    if (out) crosscheck_rows(dL, dR, W, 0, H, tol, out);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------
+ * NCC.  Reference: BlockMatcherNCC.cu EvalMsd :338-373 (base mean / std-dev),
+ * EvalSim :180-336 (ncc = (E[IJ] - muI muJ) / (sdI sdJ), 0 when the norm is small,
+ * clamped at 0 before quantisation).  Restated exactly on integer box sums:
+ *   cov = N*SIJ - SI*SJ, varI = N*SII - SI^2, varJ = N*SJJ - SJ^2,
+ *   score = cov / sqrt(varI*varJ), and score = 0 when varI < N^2 or varJ < N^2
+ *   (standard deviation below one grey level on either side).
+ * Winner-take-all maximises q = floor(max(score, 0) * 2^23), lowest d on ties.
+ * --------------------------------------------------------------------------------- */
+static void box_stats(const uint8_t* img, int W, int H, int ax, int ay, int64_t* S, int64_t* SS)
+{
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            int64_t s = 0, ss = 0;
+            if (x >= ax && x < W - ax && y >= ay && y < H - ay)
+                for (int j = -ay; j <= ay; ++j)
+                    for (int i = -ax; i <= ax; ++i) {
+                        const int v = img[(size_t)(y + j) * W + x + i];
+                        s += v;
+                        ss += v * v;
+                    }
+            S[(size_t)y * W + x] = s;
+            SS[(size_t)y * W + x] = ss;
+        }
+}
+
+int orc_ncc(const uint8_t* L, const uint8_t* R, int W, int H, int D, int ww, int wh, int tol,
+            int16_t* dL, int16_t* dR, int16_t* out, double* score_out, double* volume, int nthreads)
+{
+    if (!L || !R || !dL || !dR || W <= 0 || H <= 0 || D <= 0 || ww < 1 || wh < 1 || !(ww & 1) || !(wh & 1)) return -1;
+    const int ax = ww / 2, ay = wh / 2, bx = ax, by = ay;
+    const int64_t Nw = (int64_t)ww * wh;
+    const size_t N = (size_t)W * H;
+    int64_t* SI = (int64_t*)malloc(N * sizeof(int64_t));
+    int64_t* SII = (int64_t*)malloc(N * sizeof(int64_t));
+    int64_t* SJ = (int64_t*)malloc(N * sizeof(int64_t));
+    int64_t* SJJ = (int64_t*)malloc(N * sizeof(int64_t));
+    int64_t* bestL = (int64_t*)malloc(N * sizeof(int64_t));
+    int64_t* bestR = (int64_t*)malloc(N * sizeof(int64_t));
+    double* sc = (double*)malloc(N * sizeof(double));
+    if (!SI || !SII || !SJ || !SJJ || !bestL || !bestR || !sc) {
+        free(SI); free(SII); free(SJ); free(SJJ); free(bestL); free(bestR); free(sc);
+        return -2;
+    }
+    box_stats(L, W, H, ax, ay, SI, SII);
+    box_stats(R, W, H, ax, ay, SJ, SJJ);
+    for (size_t i = 0; i < N; ++i) { bestL[i] = -1; bestR[i] = -1; dL[i] = -1; dR[i] = -1; sc[i] = NAN; }
+    if (volume)
+        for (size_t i = 0; i < N * (size_t)D; ++i) volume[i] = NAN;
+    if (nthreads < 1) nthreads = 1;
+    const int ylo = by, yhi = H - by;
+    if (yhi > ylo) {
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+        for (int y = ylo; y < yhi; ++y) {
+            int64_t* V = (int64_t*)malloc((size_t)W * sizeof(int64_t));
+            for (int d = 0; d < D; ++d) {
+                const int xs = bx + d, xe = W - bx;
+                if (xs >= xe) break;
+                /* column sums of I*J over the window rows, then a horizontal running sum */
+                for (int x = xs - ax; x < xe + ax; ++x) {
+                    int64_t s = 0;
+                    for (int j = -ay; j <= ay; ++j) {
+                        const size_t p = (size_t)(y + j) * W + x;
+                        s += (int64_t)L[p] * R[p - d];
+                    }
+                    V[x] = s;
+                }
+                int64_t sij = 0;
+                for (int i = -ax; i <= ax; ++i) sij += V[xs + i];
+                for (int x = xs; x < xe; ++x) {
+                    if (x > xs) sij += V[x + ax] - V[x - ax - 1];
+                    const size_t pl = (size_t)y * W + x, pr = pl - d;
+                    const int64_t vI = Nw * SII[pl] - SI[pl] * SI[pl];
+                    const int64_t vJ = Nw * SJJ[pr] - SJ[pr] * SJ[pr];
+                    double s = 0.0;
+                    if (vI >= Nw * Nw && vJ >= Nw * Nw)
+                        s = (double)(Nw * sij - SI[pl] * SJ[pr]) / sqrt((double)vI * (double)vJ);
+                    if (volume) volume[pl * D + d] = s;
+                    const int64_t q = (int64_t)floor((s > 0.0 ? s : 0.0) * 8388608.0);
+                    if (q > bestL[pl]) { bestL[pl] = q; dL[pl] = (int16_t)d; sc[pl] = s; }
+                    if (q > bestR[pr]) { bestR[pr] = q; dR[pr] = (int16_t)d; }
+                }
+            }
+            free(V);
+        }
+    }
+    if (out) crosscheck_rows(dL, dR, W, 0, H, tol, out);
+    if (score_out) memcpy(score_out, sc, N * sizeof(double));
+    free(SI); free(SII); free(SJ); free(SJJ); free(bestL); free(bestR); free(sc);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------
+ * Adaptive windows.  Sub-patch sums S over s x s windows; the window value is the
+ * centre sub-patch plus the two best of the four edge neighbours at distance s
+ * (top, left, bottom, right) -- AlgoAdaptWnd_W32H16::SNeighborhood<Two_OutOf_Four>
+ * (Kernel.Algo.AdaptWnd_W32H16.h:240-285, OrderedInsertMax :122-127) for AW1 with
+ * similarity 255-|a-b| (BlockMatcherAW1.cu:191-195) and the maximum kept with
+ * strict '>' (:294); AlgoAdaptWnd3D Two_OutOf_Four (Kernel.Algo.AdaptWnd3D.h:244-300,
+ * OrderedInsert :127-131) for AW2 with |a-b| (BlockMatcherAW2.cu:129-133) and the
+ * minimum kept with strict '<' (:167).  Border: bx = by = s + s/2.
+ * --------------------------------------------------------------------------------- */
+static inline void ins_max(int* f, int* s, int v)
+{
+    const int lo = *f < v ? *f : v;
+    *s = *s > lo ? *s : lo;
+    *f = *f > v ? *f : v;
+}
+
+static inline void ins_min(int* f, int* s, int v)
+{
+    const int hi = *f > v ? *f : v;
+    *s = *s < hi ? *s : hi;
+    *f = *f < v ? *f : v;
+}
+
+int orc_aw(int mode, const uint8_t* L, const uint8_t* R, int W, int H, int D, int s, int tol,
+           int16_t* dL, int16_t* dR, int16_t* out, int32_t* value_out, int32_t* volume, int nthreads)
+{
+    if ((mode != 1 && mode != 2) || !L || !R || !dL || !dR || s < 1 || (s & 1) == 0 || D <= 0) return -1;
+    const int h = s / 2, b = s + h;
+    const size_t N = (size_t)W * H;
+    int32_t* bestL = (int32_t*)malloc(N * sizeof(int32_t));
+    int32_t* bestR = (int32_t*)malloc(N * sizeof(int32_t));
+    if (!bestL || !bestR) { free(bestL); free(bestR); return -2; }
+    const int init = (mode == 1) ? -32767 : 32767;
+    for (size_t i = 0; i < N; ++i) { bestL[i] = init; bestR[i] = init; dL[i] = -1; dR[i] = -1; }
+    if (volume)
+        for (size_t i = 0; i < N * (size_t)D; ++i) volume[i] = -1;
+    if (nthreads < 1) nthreads = 1;
+    if (H - b > b && W - b > b) {
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+        for (int y = b; y < H - b; ++y) {
+            /* S rows y-s, y, y+s over all columns that a valid window touches */
+            int32_t* S = (int32_t*)malloc((size_t)3 * W * sizeof(int32_t));
+            for (int d = 0; d < D; ++d) {
+                const int xs = b + d, xe = W - b;
+                if (xs >= xe) break;
+                for (int r = 0; r < 3; ++r) {
+                    const int yc = y + (r - 1) * s;
+                    for (int x = xs - s; x < xe + s; ++x) {
+                        int acc = 0;
+                        for (int j = -h; j <= h; ++j)
+                            for (int i = -h; i <= h; ++i) {
+                                const size_t p = (size_t)(yc + j) * W + x + i;
+                                const int a = L[p], c = R[p - d];
+                                const int ad = a > c ? a - c : c - a;
+                                acc += (mode == 1) ? 255 - ad : ad;
+                            }
+                        S[(size_t)r * W + x] = acc;
+                    }
+                }
+                for (int x = xs; x < xe; ++x) {
+                    int v;
+                    if (mode == 1) {
+                        int m1 = S[x], m2 = -32767;              /* top    (x,   y-s) */
+                        ins_max(&m1, &m2, S[(size_t)W + x - s]);  /* left   (x-s, y)   */
+                        ins_max(&m1, &m2, S[(size_t)2 * W + x]);  /* bottom (x,   y+s) */
+                        ins_max(&m1, &m2, S[(size_t)W + x + s]);  /* right  (x+s, y)   */
+                        v = m1 + m2 + S[(size_t)W + x];
+                    } else {
+                        int m1 = S[x], m2 = 32767;
+                        ins_min(&m1, &m2, S[(size_t)W + x - s]);
+                        ins_min(&m1, &m2, S[(size_t)2 * W + x]);
+                        ins_min(&m1, &m2, S[(size_t)W + x + s]);
+                        v = m1 + m2 + S[(size_t)W + x];
+                    }
+                    const size_t pl = (size_t)y * W + x, pr = pl - d;
+                    if (volume) volume[pl * D + d] = v;
+                    const int betterL = (mode == 1) ? (v > bestL[pl]) : (v < bestL[pl]);
+                    if (betterL) { bestL[pl] = v; dL[pl] = (int16_t)d; }
+                    const int betterR = (mode == 1) ? (v > bestR[pr]) : (v < bestR[pr]);
+                    if (betterR) { bestR[pr] = v; dR[pr] = (int16_t)d; }
+                }
+            }
+            free(S);
+        }
+    }
+    if (out) crosscheck_rows(dL, dR, W, 0, H, tol, out);
+    if (value_out)
+        for (size_t i = 0; i < N; ++i) value_out[i] = dL[i] >= 0 ? bestL[i] : -1;
+    free(bestL);
+    free(bestR);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------
+ * Reference cross-check: CrossCheckDisparityMaps.cu:62-109 (per pixel) and the
+ * host range reduction :135-192.  Codes outside [First, Last] pass through; the
+ * right position is MapPixelPos(x, id-First, LtoR) = LtoR ? x-d-off : x+d+off
+ * (inferred from the test-patch addressing of BlockMatcherNCC.cu:792-823);
+ * outside the image or |id_R - id_L| > thresh gives Inconsistent; kept ids update
+ * the global min/max that start at Last/First.
+ * --------------------------------------------------------------------------------- */
+int orc_crosscheck_ref(const uint16_t* lr, const uint16_t* rl, int W, int H, int thresh, int ltor,
+                       int offset, uint16_t* out, uint16_t* dmin, uint16_t* dmax)
+{
+    if (!lr || !rl || !out) return -1;
+    unsigned mn = ORC_DISP_LAST, mx = ORC_DISP_FIRST;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t i = ((size_t)y * W + x) * 4;
+            const unsigned id = lr[i];
+            memcpy(out + i, lr + i, 4 * sizeof(uint16_t));
+            if (id < ORC_DISP_FIRST || id > ORC_DISP_LAST) continue;
+            const int d = (int)id - ORC_DISP_FIRST;
+            const int xr = ltor ? x - d - offset : x + d + offset;
+            if (xr < 0 || xr >= W) { out[i] = ORC_DISP_INCONSISTENT; continue; }
+            const int idr = rl[((size_t)y * W + xr) * 4];
+            if (abs(idr - (int)id) <= thresh) {
+                if (id < mn) mn = id;
+                if (id > mx) mx = id;
+            } else {
+                out[i] = ORC_DISP_INCONSISTENT;
+            }
+        }
+    if (dmin) *dmin = (uint16_t)mn;
+    if (dmax) *dmax = (uint16_t)mx;
+    return 0;
+}

@@ -1,0 +1,76 @@
+/*
+ * CPU oracle for the stereo disparity hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP engine is
+ * compared against and the CPU baseline that bench.py times beside it.  Nothing
+ * in the shipped engine links, loads or calls it.
+ *
+ * Every function restates the arithmetic of the reference operator it names
+ * (file:line in the .c file) on plain row-major arrays (pitch == width).
+ */
+#ifndef CLUTEC_STEREO_ORACLE_H
+#define CLUTEC_STEREO_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_COST_SAD = 0, ORC_COST_CENSUS = 1 };
+
+/* Disparity identifiers, values of EDisparityId (CluTec.Cuda.MultiView/DisparityId.h:33-45). */
+enum {
+    ORC_DISP_UNKNOWN = 0,
+    ORC_DISP_FIRST = 1,
+    ORC_DISP_LAST = 0xFFF0 - 1,
+    ORC_DISP_CANNOT_EVALUATE = 0xFFFF,
+    ORC_DISP_SATURATED = 0xFFFF - 1,
+    ORC_DISP_NOT_FOUND = 0xFFFF - 2,
+    ORC_DISP_NOT_SPECIFIC = 0xFFFF - 3,
+    ORC_DISP_NOT_UNIQUE = 0xFFFF - 4,
+    ORC_DISP_INCONSISTENT = 0xFFFF - 5
+};
+
+int orc_version(void);
+
+/* Census with a cw x ch window (odd sizes, cw*ch <= 64) into uint64. */
+int orc_census(const uint8_t* img, int W, int H, int cw, int ch, uint64_t* out);
+/* Reference 3x3 census into uint8. */
+int orc_census3x3_ref(const uint8_t* img, int W, int H, uint8_t* out);
+
+/* SAD (L,R uint8) or census-Hamming (L,R uint64) block matcher, plain spec,
+ * separable running sums.  Outputs int16 maps (-1 invalid).  cost_out (optional)
+ * receives the winning aggregated cost of the left map (-1 where invalid). */
+int orc_bm(int cost_kind, const void* L, const void* R, int W, int H, int D, int ww, int wh,
+           int cw, int ch, int tol, int16_t* dL, int16_t* dR, int16_t* out, int32_t* cost_out,
+           int nthreads);
+
+/* Same specification evaluated by direct 2-D window sums (no running sums).
+ * volume (optional, W*H*D int32, index (y*W+x)*D+d) receives every aggregated cost,
+ * -1 for invalid (pixel, d) combinations. */
+int orc_bm_brute(int cost_kind, const void* L, const void* R, int W, int H, int D, int ww, int wh,
+                 int cw, int ch, int tol, int16_t* dL, int16_t* dR, int16_t* out, int32_t* volume);
+
+/* Normalised cross-correlation, FP64.  score_out (optional) = winning score of the
+ * left map (NaN where invalid); volume (optional, (y*W+x)*D+d) = every score. */
+int orc_ncc(const uint8_t* L, const uint8_t* R, int W, int H, int D, int ww, int wh, int tol,
+            int16_t* dL, int16_t* dR, int16_t* out, double* score_out, double* volume, int nthreads);
+
+/* Adaptive-window matchers.  mode 1 = AW1 similarity (255-|a-b|, two best of four
+ * neighbouring sub-patches, maximise); mode 2 = AW2 SAD (two smallest, minimise).
+ * s = sub-patch size (5 for the 15x15 window).  value_out (optional) = winning
+ * aggregated value of the left map (-1 invalid); volume optional as above. */
+int orc_aw(int mode, const uint8_t* L, const uint8_t* R, int W, int H, int D, int s, int tol,
+           int16_t* dL, int16_t* dR, int16_t* out, int32_t* value_out, int32_t* volume, int nthreads);
+
+/* Plain left-right check of two int16 maps. */
+int orc_crosscheck_plain(const int16_t* dL, const int16_t* dR, int W, int H, int tol, int16_t* out);
+
+/* Reference cross-check of two 4 x uint16 disparity-id maps. */
+int orc_crosscheck_ref(const uint16_t* lr, const uint16_t* rl, int W, int H, int thresh, int ltor,
+                       int offset, uint16_t* out, uint16_t* dmin, uint16_t* dmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

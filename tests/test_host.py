@@ -1,0 +1,116 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every
+entry point its header declares, configuration validation mirrors the
+reference's throw sites, and the synthetic workload generator is pinned."""
+import hashlib
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import clutec_stereo as cs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "clutec_stereo.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(stereo_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_symbols_exported():
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(cs.lib, n), f"{n} declared in clutec_stereo.h but not exported"
+    # the binding covers the whole header
+    assert sorted(cs.EXPORTED) == names
+
+
+def test_abi_version():
+    assert cs.lib.stereo_abi_version() == 1
+
+
+def test_status_strings():
+    assert cs.lib.stereo_status_string(cs.ERROR_INVALID_CONFIG).decode() == "invalid algorithm configuration"
+    assert cs.lib.stereo_pixel_size(cs.PIXEL_RGBA_U16) == 8
+    assert cs.lib.stereo_pixel_size(99) == 0
+
+
+@pytest.mark.parametrize("algo,win,cen,D,expect", [
+    (cs.ALGO_SAD, (9, 9), (1, 1), 64, cs.OK),
+    (cs.ALGO_SAD, (7, 7), (1, 1), 256, cs.OK),
+    (cs.ALGO_CENSUS, (9, 9), (9, 7), 256, cs.OK),
+    (cs.ALGO_CENSUS, (9, 9), (9, 9), 256, cs.ERROR_INVALID_CONFIG),   # 81 bits do not fit
+    (cs.ALGO_SAD, (11, 11), (1, 1), 64, cs.ERROR_INVALID_CONFIG),
+    (cs.ALGO_SAD, (9, 9), (1, 1), 300, cs.ERROR_INVALID_CONFIG),
+    (cs.ALGO_SAD, (9, 9), (1, 1), 12, cs.ERROR_INVALID_CONFIG),
+    (9, (9, 9), (1, 1), 64, cs.ERROR_INVALID_CONFIG),
+])
+def test_matcher_validate(algo, win, cen, D, expect):
+    p = cs.matcher_params(algo, win, cen, D, 1)
+    st, msg = cs.validate_matcher(640, 480, p)
+    assert st == expect, msg
+    if algo == 9:
+        assert msg == "Invalid algorithm configuration."
+
+
+def test_matcher_validate_formats():
+    p = cs.matcher_params(cs.ALGO_SAD, (9, 9), (1, 1), 64, 1)
+    assert cs.validate_matcher(8, 8, p)[0] == cs.ERROR_INVALID_FORMAT
+    assert cs.validate_matcher(0, 100, p)[0] == cs.ERROR_INVALID_FORMAT
+    p.reserved[2] = 5
+    assert cs.validate_matcher(640, 480, p)[0] == cs.ERROR_INVALID_CONFIG
+    p.reserved[2] = 0
+    p.lr_tolerance = -1
+    assert cs.validate_matcher(640, 480, p)[0] == cs.ERROR_INVALID_CONFIG
+
+
+def test_create_without_device_reports_invalid_device():
+    import ctypes as C
+    h = C.c_void_p()
+    st = cs.lib.stereo_create(C.byref(h), 0)
+    if st == cs.OK:
+        cs.lib.stereo_destroy(h)
+        pytest.skip("a gfx950 device is present")
+    assert st == cs.ERROR_INVALID_DEVICE
+    assert cs.lib.stereo_last_error(None)
+    assert cs.lib.stereo_create(C.byref(h), -1) == cs.ERROR_INVALID_DEVICE
+
+
+def test_synth_deterministic_and_seeded():
+    a = cs.synth_pair(3, 120, 80, 0, 5, 40, 8, with_gt=True)
+    b = cs.synth_pair(3, 120, 80, 0, 5, 40, 8, with_gt=True)
+    c = cs.synth_pair(4, 120, 80, 0, 5, 40, 8, with_gt=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert not np.array_equal(a[0], c[0])
+    L, R, gt = a
+    assert L.min() == 0 and L.max() == 255          # min/max rescale
+    assert gt.min() >= 0 and gt.max() <= 40
+    # forward warp: where the right pixel is not disoccluded it is L(x+d) plus small noise
+    y, x = 40, 100
+    d = int(gt[y, x])
+    assert abs(int(R[y, x - d]) - int(L[y, x])) <= 12
+
+
+def test_synth_golden_hashes():
+    """Byte-exact pin of the generator (fixture written by tests/golden/make_synth_golden.py)."""
+    path = os.path.join(ROOT, "tests", "golden", "synth_sha256.json")
+    gold = json.load(open(path))
+    for entry in gold:
+        L, R, gt = cs.synth_pair(entry["seed"], entry["W"], entry["H"], entry["kind"], entry["nregions"],
+                                 entry["dmax"], entry["dbg"], with_gt=True)
+        h = hashlib.sha256(L.tobytes() + R.tobytes() + gt.tobytes()).hexdigest()
+        assert h == entry["sha256"], entry
+
+
+def test_synth_rejects_bad_args():
+    import ctypes as C
+    buf = np.zeros(100, np.uint8)
+    assert cs.lib.stereo_synth_pair(0, 5, 5, 0, 1, 10, 1, buf.ctypes.data, buf.ctypes.data, None) != 0
+    assert cs.lib.stereo_synth_pair(0, 50, 50, 7, 1, 10, 1, buf.ctypes.data, buf.ctypes.data, None) != 0
