@@ -42,17 +42,14 @@ constexpr int TW = 64;
 constexpr int TH = 56;
 constexpr int HROWS = 64;
 constexpr int RUN = 18;
-constexpr int HSTRIDE = 73;
+constexpr int HSTRIDE = 72;  // == 8 (mod 32): the b64 H stores of a 16-lane group hit 32 distinct banks
 constexpr int RG = TH / 4;
 
 struct CostCensus {
     static constexpr int DC = 4;  // disparities per right-window load
+    using E = uint64_t;           // element type in memory
     struct T { uint32_t lo, hi; };
-    static __device__ __forceinline__ T load(const View& v, int y, int x)
-    {
-        const uint64_t q = row_ptr<const uint64_t>(v, y)[x];
-        return T{(uint32_t)q, (uint32_t)(q >> 32)};
-    }
+    static __device__ __forceinline__ T make(E q) { return T{(uint32_t)q, (uint32_t)(q >> 32)}; }
     // two Hamming costs packed as (c(b0) | c(b1) << 16)
     static __device__ __forceinline__ uint32_t pair(T a, T b0, T b1)
     {
@@ -63,8 +60,9 @@ struct CostCensus {
 
 struct CostSad {
     static constexpr int DC = 8;
+    using E = uint8_t;
     using T = uint32_t;
-    static __device__ __forceinline__ T load(const View& v, int y, int x) { return row_ptr<const uint8_t>(v, y)[x]; }
+    static __device__ __forceinline__ T make(E v) { return v; }
     static __device__ __forceinline__ uint32_t pair(T a, T b0, T b1)
     {
         const uint32_t c1 = __builtin_amdgcn_sad_u8(a, b1, 0u);
@@ -80,23 +78,32 @@ __device__ __forceinline__ void bm_tile(const BmParams& p, uint32_t (*hbuf)[HROW
     static_assert(AX <= 4, "horizontal radius limited by the run halo");
     constexpr int DC = Cost::DC;
     const int tid = threadIdx.x;
-    const int tx = blockIdx.x, ty = blockIdx.y;
+    const int tx = blockIdx.x, ty = blockIdx.y, seg = blockIdx.z;
     const int x0 = tx * TW, y0 = ty * TH;
-    const int W = p.W, H = p.H, D = p.D;
+    const int W = p.W, H = p.H;
+    // this workgroup's disparity segment [ds, de)
+    const int ds = seg * p.seg_len;
+    const int de = min(p.D, ds + p.seg_len);
 
     // row-phase identity
     const int run = tid & 3;
     const int hr = tid >> 2;
     const int gyr = clampi(y0 - AY + hr, 0, H - 1);
     const int xs = x0 - 4 + RUN * run;
+    using E = typename Cost::E;
+    const E* const rowL = row_ptr<const E>(p.L, gyr) + xs;
+    const E* rowR = row_ptr<const E>(p.R, gyr) + xs - ds - (DC - 1);  // window start for d0 = ds
     typename Cost::T cl[RUN];
 #pragma unroll
-    for (int i = 0; i < RUN; ++i) cl[i] = Cost::load(p.L, gyr, xs + i);
+    for (int i = 0; i < RUN; ++i) cl[i] = Cost::make(rowL[i]);
 
-    // column-phase identity
+    // column-phase identity (gw is the wave index, uniform across the wave)
     const int lane = tid & 63;
-    const int grp = tid >> 6;
+    const int gw = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x = x0 + lane;
+    const int rows_here = H - (y0 + gw * RG);  // output rows of this wave inside the image
+    uint32_t* const rp = p.rpart + (((size_t)seg * p.ntx + tx) * H + y0 + gw * RG) * BM_NPART;
+    uint32_t* const eb = &ebuf[gw][0][0];
     const bool lane_ok = x >= p.bx && x < W - p.bx;
     const int dlim = x - p.bx;
     uint32_t lacc[RG], racc[RG];
@@ -104,10 +111,10 @@ __device__ __forceinline__ void bm_tile(const BmParams& p, uint32_t (*hbuf)[HROW
     for (int i = 0; i < RG; ++i) { lacc[i] = 0; racc[i] = 0; }
 
     int s = 0;
-    for (int d0 = 0; d0 < D; d0 += DC) {
+    for (int d0 = ds; d0 < de; d0 += DC, rowR -= DC) {
         typename Cost::T cr[RUN + DC - 1];
 #pragma unroll
-        for (int k = 0; k < RUN + DC - 1; ++k) cr[k] = Cost::load(p.R, gyr, xs - d0 - (DC - 1) + k);
+        for (int k = 0; k < RUN + DC - 1; ++k) cr[k] = Cost::make(rowR[k]);
 
 #pragma unroll
         for (int q = 0; q < DC / 2; ++q, ++s) {
@@ -127,69 +134,76 @@ __device__ __forceinline__ void bm_tile(const BmParams& p, uint32_t (*hbuf)[HROW
             uint32_t h = 0;
 #pragma unroll
             for (int i = 0; i <= 2 * AX; ++i) h += e[i];
-            uint32_t* hrow = hb + hr * HSTRIDE + RUN * run;
-            hrow[0] = h;
+            uint2* hrow = reinterpret_cast<uint2*>(hb + hr * HSTRIDE + RUN * run);
 #pragma unroll
-            for (int i = 1; i < RUN; ++i) {
-                h = h + e[i + 2 * AX] - e[i - 1];
-                hrow[i] = h;
+            for (int i = 0; i < RUN; i += 2) {
+                if (i > 0) h = h + e[i + 2 * AX] - e[i - 1];
+                const uint32_t h0 = h;
+                h = h + e[i + 1 + 2 * AX] - e[i];
+                hrow[i / 2] = make_uint2(h0, h);
             }
             __syncthreads();
 
             // ---- column phase: vertical sums, keys, both winner-take-alls
             uint32_t hv[RG + 2 * AY];
+            const uint32_t* hcol = hb + gw * RG * HSTRIDE + lane + 4;
 #pragma unroll
-            for (int i = 0; i < RG + 2 * AY; ++i) hv[i] = hb[(grp * RG + i) * HSTRIDE + lane + 4];
+            for (int i = 0; i < RG + 2 * AY; ++i) hv[i] = hcol[i * HSTRIDE];
             uint32_t abar = 0xFFFFFFFFu;
 #pragma unroll
             for (int i = 0; i <= 2 * AY; ++i) abar -= hv[i];
             const uint32_t ddv = (uint32_t)(255 - d) | ((uint32_t)(254 - d) << 8);
             bool ok0 = true, ok1 = true;
             if (EDGE) { ok0 = lane_ok && d <= dlim; ok1 = lane_ok && d + 1 <= dlim; }
+            uint32_t k0[RG], k1[RG];
 #pragma unroll
             for (int i = 0; i < RG; ++i) {
                 if (i > 0) abar = abar + hv[i - 1] - hv[i + 2 * AY];
-                uint32_t k0 = __builtin_amdgcn_perm(ddv, abar, 0x0C010004u);
-                uint32_t k1 = __builtin_amdgcn_perm(ddv, abar, 0x0C030205u);
-                if (EDGE) { k0 = ok0 ? k0 : 0u; k1 = ok1 ? k1 : 0u; }
-                const uint32_t kk = k0 > k1 ? k0 : k1;
+                k0[i] = __builtin_amdgcn_perm(ddv, abar, 0x0C010004u);
+                k1[i] = __builtin_amdgcn_perm(ddv, abar, 0x0C030205u);
+                if (EDGE) { k0[i] = ok0 ? k0[i] : 0u; k1[i] = ok1 ? k1[i] : 0u; }
+                const uint32_t kk = k0[i] > k1[i] ? k0[i] : k1[i];
                 lacc[i] = lacc[i] > kk ? lacc[i] : kk;
-                racc[i] = max_shift_in(racc[i], k0);
-                if (lane == 63) ebuf[grp][i][d & 63] = racc[i];
-                racc[i] = max_shift_in(racc[i], k1);
-                if (lane == 63) ebuf[grp][i][(d + 1) & 63] = racc[i];
+            }
+            // right-view chain: one lane per step; all rows first so that no DPP read
+            // waits on the VALU write just before it, then lane 63 parks what leaves
+            static_assert(RG == 14, "the right-view chain block is written for 14 rows");
+            max_shift_in14(racc, k0);
+            if (lane == 63) {
+#pragma unroll
+                for (int i = 0; i < RG; ++i) eb[i * 64 + (d & 63)] = racc[i];
+            }
+            max_shift_in14(racc, k1);
+            if (lane == 63) {
+#pragma unroll
+                for (int i = 0; i < RG; ++i) eb[i * 64 + ((d + 1) & 63)] = racc[i];
             }
             // flush the lane-63 emissions every 64 disparities (and at the end)
             const int dl = d + 1;
-            if ((dl & 63) == 63 || dl == D - 1) {
+            if ((dl & 63) == 63 || dl == de - 1) {
                 const int base = dl & ~63;
-                if (lane <= (dl & 63)) {
+                if (lane <= (dl & 63) && base + lane >= ds) {
                     const int k = 318 - (base + lane);
 #pragma unroll
-                    for (int i = 0; i < RG; ++i) {
-                        const int y = y0 + grp * RG + i;
-                        if (y < H) p.rpart[((size_t)tx * H + y) * BM_NPART + k] = ebuf[grp][i][lane];
-                    }
+                    for (int i = 0; i < RG; ++i)
+                        if (i < rows_here) rp[i * BM_NPART + k] = eb[i * 64 + lane];
                 }
             }
         }
     }
 
-    // what is still in flight belongs to right pixels x0+lane-(D-1)
+    // what is still in flight belongs to right pixels x0+lane-(de-1)
     {
-        const int k = lane + 255 - (D - 1);
+        const int k = lane + 255 - (de - 1);
 #pragma unroll
-        for (int i = 0; i < RG; ++i) {
-            const int y = y0 + grp * RG + i;
-            if (y < H) p.rpart[((size_t)tx * H + y) * BM_NPART + k] = racc[i];
-        }
+        for (int i = 0; i < RG; ++i)
+            if (i < rows_here) rp[i * BM_NPART + k] = racc[i];
     }
     if (x < W) {
+        uint32_t* lk = p.lkey + (size_t)seg * W * H + (size_t)(y0 + gw * RG) * W + x;
 #pragma unroll
-        for (int i = 0; i < RG; ++i) {
-            const int y = y0 + grp * RG + i;
-            if (y < H) p.lkey[(size_t)y * W + x] = lacc[i];
-        }
+        for (int i = 0; i < RG; ++i)
+            if (i < rows_here) lk[(size_t)i * W] = lacc[i];
     }
 }
 
@@ -199,7 +213,8 @@ __global__ __launch_bounds__(256, 2) void bm_fused_kernel(BmParams p)
     __shared__ uint32_t hbuf[2][HROWS * HSTRIDE];
     __shared__ uint32_t ebuf[4][RG][64];
     const int x0 = blockIdx.x * TW;
-    const bool edge = (x0 < p.bx + p.D - 1) || (x0 + TW > p.W - p.bx);
+    const int de = min(p.D, ((int)blockIdx.z + 1) * p.seg_len);
+    const bool edge = (x0 < p.bx + de - 1) || (x0 + TW > p.W - p.bx);
     if (edge)
         bm_tile<Cost, AX, AY, true>(p, hbuf, ebuf);
     else
@@ -208,13 +223,20 @@ __global__ __launch_bounds__(256, 2) void bm_fused_kernel(BmParams p)
 
 __device__ __forceinline__ uint32_t rkey_at(const BmFinishParams& f, int xr, int y)
 {
-    int tlo = xr - 63 <= 0 ? 0 : (xr - 63 + 63) / 64;
-    int thi = (xr + f.D - 1) / 64;
-    if (thi > f.ntx - 1) thi = f.ntx - 1;
+    // right pixel xr meets left pixels xr+d; segment s covers d in [ds, de), and the
+    // tile holding left pixel x keeps the partial under index xr - x0 + 255
     uint32_t best = 0;
-    for (int t = tlo; t <= thi; ++t) {
-        const uint32_t v = f.rpart[((size_t)t * f.H + y) * BM_NPART + (xr - 64 * t + 255)];
-        best = v > best ? v : best;
+    for (int s = 0; s < f.nseg; ++s) {
+        const int ds = s * f.seg_len;
+        const int de = min(f.D, ds + f.seg_len);
+        const int tlo = (xr + ds) / 64;
+        int thi = (xr + de - 1) / 64;
+        if (thi > f.ntx - 1) thi = f.ntx - 1;
+        const uint32_t* base = f.rpart + (size_t)s * f.ntx * f.H * BM_NPART;
+        for (int t = tlo; t <= thi; ++t) {
+            const uint32_t v = base[((size_t)t * f.H + y) * BM_NPART + (xr - 64 * t + 255)];
+            best = v > best ? v : best;
+        }
     }
     return best;
 }
@@ -229,7 +251,10 @@ __global__ __launch_bounds__(256) void bm_finish_kernel(BmFinishParams f)
     int dl = -1, dr = -1, outd = -1;
     uint32_t lk = 0;
     if (row_ok && col_ok) {
-        lk = f.lkey[(size_t)y * f.W + x];
+        for (int s = 0; s < f.nseg; ++s) {
+            const uint32_t v = f.lkey[(size_t)s * f.W * f.H + (size_t)y * f.W + x];
+            lk = v > lk ? v : lk;
+        }
         dl = key_disp(lk);
         if (dl >= 0) {
             const int xr = x - dl;
@@ -268,18 +293,29 @@ bool bm_supported(int kind, int ww, int wh, int D)
     return false;
 }
 
-void bm_grid(int W, int H, int* ntx, int* nty)
+void bm_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len)
 {
     *ntx = (W + bm::TW - 1) / bm::TW;
     *nty = (H + bm::TH - 1) / bm::TH;
+    // Split the disparity range so that the launch holds enough equal workgroups for
+    // the 256 CUs (3 resident each) to finish together: about 1200.  Every extra
+    // segment costs one more set of partial keys, so no more than that.  Each
+    // segment is a multiple of 8 disparities.
+    const int tiles = *ntx * *nty;
+    int n = (1200 + tiles - 1) / tiles;
+    if (n > D / 8) n = D / 8;
+    if (n > BM_MAX_SEG) n = BM_MAX_SEG;
+    if (n < 1) n = 1;
+    int len = (D + n - 1) / n;
+    len = (len + 7) & ~7;
+    *seg_len = len;
+    *nseg = (D + len - 1) / len;
 }
 
 template <class Cost>
 static hipError_t launch_cost(const BmParams& p, int ww, hipStream_t s)
 {
-    int ntx, nty;
-    bm_grid(p.W, p.H, &ntx, &nty);
-    dim3 grid(ntx, nty);
+    dim3 grid(p.ntx, (p.H + bm::TH - 1) / bm::TH, p.nseg);
     if (ww == 9) hipLaunchKernelGGL((bm::bm_fused_kernel<Cost, 4, 4>), grid, dim3(256), 0, s, p);
     else if (ww == 7) hipLaunchKernelGGL((bm::bm_fused_kernel<Cost, 3, 3>), grid, dim3(256), 0, s, p);
     else hipLaunchKernelGGL((bm::bm_fused_kernel<Cost, 2, 2>), grid, dim3(256), 0, s, p);

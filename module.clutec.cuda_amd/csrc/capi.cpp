@@ -140,7 +140,7 @@ struct stereo_driver {
     stereo_census_params cen{};
     // matcher
     stereo_matcher_params mp{};
-    int bx = 0, by = 0, ntx = 0, nty = 0;
+    int bx = 0, by = 0, ntx = 0, nty = 0, nseg = 1, seg_len = 0;
     uint8_t* padL = nullptr;  // padded matcher inputs (grey bytes or census words)
     uint8_t* padR = nullptr;
     size_t pad_pitch = 0;
@@ -464,7 +464,7 @@ stereo_status stereo_matcher_configure(stereo_context* ctx, stereo_driver** out,
     d->name = census ? "Block Matcher Census" : "Block Matcher SAD";
     d->bx = p->window_w / 2 + (census ? p->census_w / 2 : 0);
     d->by = p->window_h / 2 + (census ? p->census_h / 2 : 0);
-    stereo::bm_grid(fmt->width, fmt->height, &d->ntx, &d->nty);
+    stereo::bm_grid(fmt->width, fmt->height, p->num_disparities, &d->ntx, &d->nty, &d->nseg, &d->seg_len);
     d->pad_elem = census ? 8 : 1;
     const size_t cols = (size_t)stereo::BM_PAD_LEFT + fmt->width + stereo::BM_PAD_RIGHT;
     d->pad_pitch = (cols * d->pad_elem + 255) & ~(size_t)255;
@@ -475,8 +475,8 @@ stereo_status stereo_matcher_configure(stereo_context* ctx, stereo_driver** out,
     hipError_t e;
     if ((e = hipMalloc((void**)&d->padL, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
     if ((e = hipMalloc((void**)&d->padR, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
-    if ((e = hipMalloc((void**)&d->lkey, W * H * 4)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
-    if ((e = hipMalloc((void**)&d->rpart, (size_t)d->ntx * H * stereo::BM_NPART * 4)) != hipSuccess)
+    if ((e = hipMalloc((void**)&d->lkey, (size_t)d->nseg * W * H * 4)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
+    if ((e = hipMalloc((void**)&d->rpart, (size_t)d->nseg * d->ntx * H * stereo::BM_NPART * 4)) != hipSuccess)
         return cleanup(hip_fail(ctx, e, "hipMalloc"));
     // the pad columns only ever meet masked candidates; clear them once so that no
     // uninitialised memory is read
@@ -535,6 +535,9 @@ stereo_status stereo_matcher_process(stereo_driver* d, stereo_image* disp, stere
     bp.D = d->mp.num_disparities;
     bp.bx = d->bx;
     bp.by = d->by;
+    bp.ntx = d->ntx;
+    bp.nseg = d->nseg;
+    bp.seg_len = d->seg_len;
     HIP_TRY(ctx, stereo::launch_bm(census ? stereo::BM_COST_CENSUS : stereo::BM_COST_SAD, d->mp.window_w, bp, s),
             "matcher launch");
     if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[1], s), "hipEventRecord");
@@ -549,6 +552,8 @@ stereo_status stereo_matcher_process(stereo_driver* d, stereo_image* disp, stere
     f.H = H;
     f.D = bp.D;
     f.ntx = d->ntx;
+    f.nseg = d->nseg;
+    f.seg_len = d->seg_len;
     f.bx = d->bx;
     f.by = d->by;
     f.tol = d->mp.lr_tolerance;

@@ -16,28 +16,31 @@ enum { BM_COST_SAD = 0, BM_COST_CENSUS = 1 };
 constexpr int BM_NPART = 320;     // right-view partial keys per tile row
 constexpr int BM_PAD_LEFT = 320;  // readable columns left of x = 0 in matcher inputs
 constexpr int BM_PAD_RIGHT = 128; // readable columns right of x = W-1
+constexpr int BM_MAX_SEG = 8;     // disparity segments per tile (one workgroup each)
 
 struct BmParams {
     View L, R;         // uint8 grey (SAD) or uint64 census (CENSUS)
-    uint32_t* lkey;    // W*H best left keys (pitch W)
-    uint32_t* rpart;   // ntx*H*BM_NPART right-view partial keys
+    uint32_t* lkey;    // nseg*W*H best left keys per disparity segment (pitch W)
+    uint32_t* rpart;   // nseg*ntx*H*BM_NPART right-view partial keys
     int W, H, D;
     int bx, by;        // matchable border
+    int ntx, nseg, seg_len;
 };
 
 struct BmFinishParams {
-    const uint32_t* lkey;
-    const uint32_t* rpart;
+    const uint32_t* lkey;   // nseg*W*H
+    const uint32_t* rpart;  // nseg*ntx*H*BM_NPART
     View disp;     // int16 checked map (optional: ptr == nullptr)
     View disp_l;   // int16 left map (optional)
     View disp_r;   // int16 right map (optional)
     View dispid;   // 4 x uint16 reference pixels (optional)
     int W, H, D, ntx;
+    int nseg, seg_len;
     int bx, by, tol;
 };
 
 bool bm_supported(int kind, int ww, int wh, int D);
-void bm_grid(int W, int H, int* ntx, int* nty);
+void bm_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len);
 hipError_t launch_bm(int kind, int ww, const BmParams& p, hipStream_t s);
 hipError_t launch_bm_finish(const BmFinishParams& f, hipStream_t s);
 
