@@ -168,6 +168,14 @@ stereo_status stereo_matcher_process(stereo_driver* drv, stereo_image* disp, ste
                                      stereo_image* disp_rl, const stereo_image* left, const stereo_image* right,
                                      stereo_stream stream);
 
+/* Winning NCC score of every left pixel for the last process call on the same
+ * images (FP32, NaN where the left map has no match).  Scores carry the exact
+ * integer covariance through one float conversion and two multiplies: relative
+ * error below 1e-6 against FP64.  The score the reference stores quantised to
+ * uint8 in channel 1 of its disparity pixel (BlockMatcherNCC.cu:321-323, :563). */
+stereo_status stereo_matcher_scores(stereo_driver* drv, stereo_image* score, const stereo_image* left,
+                                    const stereo_image* right, stereo_stream stream);
+
 /* Per-stage device time (ms) of the last timed matcher process call: stage 0 =
  * input preparation (census of both images, or the padded copy for SAD), stage 1
  * = the fused matching kernel, stage 2 = right-view merge and check.  Extends

@@ -101,6 +101,7 @@ _PROTOS = [
     ("stereo_matcher_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(MatcherParams)]),
     ("stereo_matcher_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image),
                                           C.POINTER(Image), C.POINTER(Image), _P]),
+    ("stereo_matcher_scores", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_stage_times", C.c_int, [_P, C.POINTER(C.c_double), C.c_int32]),
     ("stereo_crosscheck_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(CrossCheckParams)]),
     ("stereo_crosscheck_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
@@ -298,17 +299,23 @@ class Matcher(_Driver):
         _check(lib.stereo_matcher_stage_times(self.handle, v, 3), self.ctx)
         return list(v)
 
-    def run(self, L, R, maps=False, dispid=False):
-        """numpy in, numpy out: the checked map (and the two one-sided maps)."""
+    def run(self, L, R, maps=False, dispid=False, scores=False):
+        """numpy in, numpy out: the checked map (and the two one-sided maps, and for
+        NCC the winning scores)."""
         dl = DeviceImage.from_array(self.ctx, L, PIXEL_LUM_U8)
         dr = DeviceImage.from_array(self.ctx, R, PIXEL_LUM_U8)
         out = DeviceImage(self.ctx, self.W, self.H, PIXEL_RGBA_U16 if dispid else PIXEL_LUM_I16)
         lr = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16) if maps else None
         rl = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16) if maps else None
         self.process(out, lr, rl, dl, dr)
+        res = [out.download()]
         if maps:
-            return out.download(), lr.download(), rl.download()
-        return out.download()
+            res += [lr.download(), rl.download()]
+        if scores:
+            sc = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_F32)
+            _check(lib.stereo_matcher_scores(self.handle, sc.ref, dl.ref, dr.ref, None), self.ctx)
+            res.append(sc.download())
+        return res[0] if len(res) == 1 else tuple(res)
 
 
 class CrossCheck(_Driver):

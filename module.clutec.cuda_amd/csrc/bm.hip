@@ -224,7 +224,8 @@ __global__ __launch_bounds__(256, 2) void bm_fused_kernel(BmParams p)
 __device__ __forceinline__ uint32_t rkey_at(const BmFinishParams& f, int xr, int y)
 {
     // right pixel xr meets left pixels xr+d; segment s covers d in [ds, de), and the
-    // tile holding left pixel x keeps the partial under index xr - x0 + 255
+    // tile holding left pixel x keeps the partial under index xr - x0 + 255.  A
+    // segment spans at most 5 tiles (seg_len <= 256); the loads are independent.
     uint32_t best = 0;
     for (int s = 0; s < f.nseg; ++s) {
         const int ds = s * f.seg_len;
@@ -232,54 +233,69 @@ __device__ __forceinline__ uint32_t rkey_at(const BmFinishParams& f, int xr, int
         const int tlo = (xr + ds) / 64;
         int thi = (xr + de - 1) / 64;
         if (thi > f.ntx - 1) thi = f.ntx - 1;
-        const uint32_t* base = f.rpart + (size_t)s * f.ntx * f.H * BM_NPART;
-        for (int t = tlo; t <= thi; ++t) {
-            const uint32_t v = base[((size_t)t * f.H + y) * BM_NPART + (xr - 64 * t + 255)];
-            best = v > best ? v : best;
+        const uint32_t* base = f.rpart + ((size_t)s * f.ntx * f.H + y) * BM_NPART + (xr + 255);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const int t = tlo + j;
+            if (t <= thi) {
+                const uint32_t v = base[(size_t)t * f.H * BM_NPART - 64 * t];
+                best = v > best ? v : best;
+            }
         }
     }
     return best;
 }
 
+// One workgroup per image row: first every right pixel of the row merges its partial
+// keys (contiguous reads, one pass) into LDS, then every left pixel looks up the right
+// winner at x - dL for the check.
 __global__ __launch_bounds__(256) void bm_finish_kernel(BmFinishParams f)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= f.W || y >= f.H) return;
+    extern __shared__ uint32_t rrow[];
+    const int y = blockIdx.x;
+    const int W = f.W;
     const bool row_ok = y >= f.by && y < f.H - f.by;
-    const bool col_ok = x >= f.bx && x < f.W - f.bx;
-    int dl = -1, dr = -1, outd = -1;
-    uint32_t lk = 0;
-    if (row_ok && col_ok) {
-        for (int s = 0; s < f.nseg; ++s) {
-            const uint32_t v = f.lkey[(size_t)s * f.W * f.H + (size_t)y * f.W + x];
-            lk = v > lk ? v : lk;
-        }
-        dl = key_disp(lk);
-        if (dl >= 0) {
-            const int xr = x - dl;
-            const int dxr = key_disp(rkey_at(f, xr, y));
-            outd = (dxr >= 0 && abs(dxr - dl) <= f.tol) ? dl : -1;
-        }
-        if (f.disp_r.ptr) dr = key_disp(rkey_at(f, x, y));
+#pragma unroll 4
+    for (int x = threadIdx.x; x < W; x += 256) {
+        const bool col_ok = x >= f.bx && x < W - f.bx;
+        rrow[x] = (row_ok && col_ok) ? rkey_at(f, x, y) : 0u;
     }
-    if (f.disp.ptr) row_ptr<int16_t>(f.disp, y)[x] = (int16_t)outd;
-    if (f.disp_l.ptr) row_ptr<int16_t>(f.disp_l, y)[x] = (int16_t)dl;
-    if (f.disp_r.ptr) row_ptr<int16_t>(f.disp_r, y)[x] = (int16_t)dr;
-    if (f.dispid.ptr) {
-        // reference pixel layout (EDisparityId in channel 0, DisparityId.h:33-45)
-        uint16_t id;
-        if (!(row_ok && col_ok)) id = 0;                       // Unknown
-        else if (outd >= 0) id = (uint16_t)(outd + 1);        // First + d
-        else if (dl >= 0) id = 0xFFFFu - 5u;                   // Inconsistent
-        else id = 0xFFFFu - 2u;                                // NotFound
-        const int cost = lk ? key_cost(lk) : 0;
-        ushort4 v;
-        v.x = id;
-        v.y = (uint16_t)(cost > 65535 ? 65535 : cost);
-        v.z = 0;
-        v.w = 0;
-        row_ptr<ushort4>(f.dispid, y)[x] = v;
+    __syncthreads();
+    for (int x = threadIdx.x; x < W; x += 256) {
+        const bool col_ok = x >= f.bx && x < W - f.bx;
+        int dl = -1, dr = -1, outd = -1;
+        uint32_t lk = 0;
+        if (row_ok && col_ok) {
+            for (int s = 0; s < f.nseg; ++s) {
+                const uint32_t v = f.lkey[(size_t)s * W * f.H + (size_t)y * W + x];
+                lk = v > lk ? v : lk;
+            }
+            dl = key_disp(lk);
+            if (dl >= 0) {
+                const int dxr = key_disp(rrow[x - dl]);
+                outd = (dxr >= 0 && abs(dxr - dl) <= f.tol) ? dl : -1;
+            }
+            dr = key_disp(rrow[x]);
+        }
+        if (f.disp.ptr) row_ptr<int16_t>(f.disp, y)[x] = (int16_t)outd;
+        if (f.disp_l.ptr) row_ptr<int16_t>(f.disp_l, y)[x] = (int16_t)dl;
+        if (f.disp_r.ptr) row_ptr<int16_t>(f.disp_r, y)[x] = (int16_t)dr;
+        if (f.dispid.ptr) {
+            // reference pixel layout (EDisparityId in channel 0, DisparityId.h:33-45)
+            uint16_t id;
+            if (!(row_ok && col_ok)) id = 0;                       // Unknown
+            else if (outd >= 0) id = (uint16_t)(outd + 1);        // First + d
+            else if (dl >= 0) id = 0xFFFFu - 5u;                   // Inconsistent
+            else id = 0xFFFFu - 2u;                                // NotFound
+            int cost = lk ? key_cost(lk) : 0;
+            if (lk && f.value_base) cost = f.value_base - cost;
+            ushort4 v;
+            v.x = id;
+            v.y = (uint16_t)(cost > 65535 ? 65535 : cost);
+            v.z = 0;
+            v.w = 0;
+            row_ptr<ushort4>(f.dispid, y)[x] = v;
+        }
     }
 }
 
@@ -330,8 +346,8 @@ hipError_t launch_bm(int kind, int ww, const BmParams& p, hipStream_t s)
 
 hipError_t launch_bm_finish(const BmFinishParams& f, hipStream_t s)
 {
-    dim3 grid((f.W + 63) / 64, (f.H + 3) / 4);
-    hipLaunchKernelGGL(bm::bm_finish_kernel, grid, dim3(256), 0, s, f);
+    if (f.W > BM_MAX_WIDTH) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bm::bm_finish_kernel, dim3(f.H), dim3(256), (size_t)f.W * sizeof(uint32_t), s, f);
     return hipGetLastError();
 }
 

@@ -147,6 +147,8 @@ struct stereo_driver {
     size_t pad_elem = 0;
     uint32_t* lkey = nullptr;
     uint32_t* rpart = nullptr;
+    int2* lstats = nullptr;  // NCC window statistics
+    int2* rstats = nullptr;
     // crosscheck
     stereo_crosscheck_params cc{};
     uint32_t* range_dev = nullptr;
@@ -179,6 +181,8 @@ void driver_release(stereo_driver* d)
     if (d->padR) (void)hipFree(d->padR);
     if (d->lkey) (void)hipFree(d->lkey);
     if (d->rpart) (void)hipFree(d->rpart);
+    if (d->lstats) (void)hipFree(d->lstats);
+    if (d->rstats) (void)hipFree(d->rstats);
     if (d->range_dev) (void)hipFree(d->range_dev);
     if (d->range_host) (void)hipHostFree(d->range_host);
 }
@@ -438,9 +442,19 @@ stereo_status stereo_matcher_validate(const stereo_image_format* fmt, const ster
         if (!stereo::census_supported(p->census_w, p->census_h, 0))
             return say(STEREO_ERROR_INVALID_CONFIG, "census window must be 9x7, 7x9, 7x7, 5x5 or 3x3");
         break;
+    case STEREO_ALGO_NCC:
+        if (!stereo::ncc_supported(p->window_w, p->window_h, p->num_disparities))
+            return say(STEREO_ERROR_INVALID_CONFIG, "NCC window must be 5x5, 7x7, 9x9 or 11x11");
+        break;
+    case STEREO_ALGO_AW1:
+    case STEREO_ALGO_AW2:
+        if (p->window_w != p->window_h || !stereo::aw_supported(p->window_w, p->num_disparities))
+            return say(STEREO_ERROR_INVALID_CONFIG, "adaptive-window sub-patch must be 3x3 or 5x5 (Patch_9x9 / Patch_15x15)");
+        break;
     default:
         return say(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
     }
+    if (fmt->width > stereo::BM_MAX_WIDTH) return say(STEREO_ERROR_INVALID_FORMAT, "matcher images are limited to 16384 columns");
     const int bx = p->window_w / 2 + (p->algorithm == STEREO_ALGO_CENSUS ? p->census_w / 2 : 0);
     const int by = p->window_h / 2 + (p->algorithm == STEREO_ALGO_CENSUS ? p->census_h / 2 : 0);
     if (fmt->width <= 2 * bx || fmt->height <= 2 * by) return say(STEREO_ERROR_INVALID_FORMAT, "image smaller than the window");
@@ -461,10 +475,21 @@ stereo_status stereo_matcher_configure(stereo_context* ctx, stereo_driver** out,
     d->fmt = *fmt;
     d->mp = *p;
     const bool census = p->algorithm == STEREO_ALGO_CENSUS;
-    d->name = census ? "Block Matcher Census" : "Block Matcher SAD";
+    const bool ncc = p->algorithm == STEREO_ALGO_NCC;
+    const bool aw = p->algorithm == STEREO_ALGO_AW1 || p->algorithm == STEREO_ALGO_AW2;
+    d->name = census ? "Block Matcher Census"
+              : ncc  ? "Block Matcher NCC"
+              : aw   ? (p->algorithm == STEREO_ALGO_AW1 ? "Block Matcher Adaptive Window 1" : "Block Matcher Adaptive Window 2")
+                     : "Block Matcher SAD";
     d->bx = p->window_w / 2 + (census ? p->census_w / 2 : 0);
     d->by = p->window_h / 2 + (census ? p->census_h / 2 : 0);
-    stereo::bm_grid(fmt->width, fmt->height, p->num_disparities, &d->ntx, &d->nty, &d->nseg, &d->seg_len);
+    if (aw) d->bx = d->by = p->window_w + p->window_w / 2;
+    if (ncc)
+        stereo::ncc_grid(fmt->width, fmt->height, p->num_disparities, &d->ntx, &d->nty, &d->nseg, &d->seg_len);
+    else if (aw)
+        stereo::aw_grid(fmt->width, fmt->height, p->num_disparities, &d->ntx, &d->nty, &d->nseg, &d->seg_len);
+    else
+        stereo::bm_grid(fmt->width, fmt->height, p->num_disparities, &d->ntx, &d->nty, &d->nseg, &d->seg_len);
     d->pad_elem = census ? 8 : 1;
     const size_t cols = (size_t)stereo::BM_PAD_LEFT + fmt->width + stereo::BM_PAD_RIGHT;
     d->pad_pitch = (cols * d->pad_elem + 255) & ~(size_t)255;
@@ -478,6 +503,10 @@ stereo_status stereo_matcher_configure(stereo_context* ctx, stereo_driver** out,
     if ((e = hipMalloc((void**)&d->lkey, (size_t)d->nseg * W * H * 4)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
     if ((e = hipMalloc((void**)&d->rpart, (size_t)d->nseg * d->ntx * H * stereo::BM_NPART * 4)) != hipSuccess)
         return cleanup(hip_fail(ctx, e, "hipMalloc"));
+    if (ncc) {
+        if ((e = hipMalloc((void**)&d->lstats, W * H * sizeof(int2))) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
+        if ((e = hipMalloc((void**)&d->rstats, W * H * sizeof(int2))) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
+    }
     // the pad columns only ever meet masked candidates; clear them once so that no
     // uninitialised memory is read
     if ((e = hipMemset(d->padL, 0, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMemset"));
@@ -515,31 +544,74 @@ stereo_status stereo_matcher_process(stereo_driver* d, stereo_image* disp, stere
     pl.ptr = d->padL + (size_t)stereo::BM_PAD_LEFT * d->pad_elem;
     pr.ptr = d->padR + (size_t)stereo::BM_PAD_LEFT * d->pad_elem;
     const bool census = d->mp.algorithm == STEREO_ALGO_CENSUS;
+    const bool ncc = d->mp.algorithm == STEREO_ALGO_NCC;
+    const bool aw = d->mp.algorithm == STEREO_ALGO_AW1 || d->mp.algorithm == STEREO_ALGO_AW2;
     if (census) {
-        HIP_TRY(ctx, stereo::launch_census(view_of(left), pl, d->mp.census_w, d->mp.census_h, 0, s), "census launch");
-        HIP_TRY(ctx, stereo::launch_census(view_of(right), pr, d->mp.census_w, d->mp.census_h, 0, s), "census launch");
+        const View vl = view_of(left), vr = view_of(right);
+        HIP_TRY(ctx, stereo::launch_census_pair(vl, pl, &vr, &pr, d->mp.census_w, d->mp.census_h, s), "census launch");
     } else {
         HIP_TRY(ctx, hipMemcpy2DAsync(pl.ptr, pl.pitch, left->data, left->pitch, W, H, hipMemcpyDeviceToDevice, s),
                 "hipMemcpy2DAsync");
         HIP_TRY(ctx, hipMemcpy2DAsync(pr.ptr, pr.pitch, right->data, right->pitch, W, H, hipMemcpyDeviceToDevice, s),
                 "hipMemcpy2DAsync");
     }
+    if (ncc) {
+        HIP_TRY(ctx, stereo::launch_ncc_stats(view_of(left), d->lstats, W, d->mp.window_w, d->mp.window_h, 8388608.0f, s),
+                "ncc stats launch");
+        HIP_TRY(ctx, stereo::launch_ncc_stats(view_of(right), d->rstats, W, d->mp.window_w, d->mp.window_h, 1.0f, s),
+                "ncc stats launch");
+    }
     if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[0], s), "hipEventRecord");
-    stereo::BmParams bp;
-    bp.L = pl;
-    bp.R = pr;
-    bp.lkey = d->lkey;
-    bp.rpart = d->rpart;
-    bp.W = W;
-    bp.H = H;
-    bp.D = d->mp.num_disparities;
-    bp.bx = d->bx;
-    bp.by = d->by;
-    bp.ntx = d->ntx;
-    bp.nseg = d->nseg;
-    bp.seg_len = d->seg_len;
-    HIP_TRY(ctx, stereo::launch_bm(census ? stereo::BM_COST_CENSUS : stereo::BM_COST_SAD, d->mp.window_w, bp, s),
-            "matcher launch");
+    if (ncc) {
+        stereo::NccParams np;
+        np.L = pl;
+        np.R = pr;
+        np.lstats = d->lstats;
+        np.rstats = d->rstats;
+        np.stats_pitch = W;
+        np.lkey = d->lkey;
+        np.rpart = d->rpart;
+        np.W = W;
+        np.H = H;
+        np.D = d->mp.num_disparities;
+        np.bx = d->bx;
+        np.by = d->by;
+        np.ntx = d->ntx;
+        np.nseg = d->nseg;
+        np.seg_len = d->seg_len;
+        HIP_TRY(ctx, stereo::launch_ncc(d->mp.window_w, np, s), "ncc launch");
+    } else if (aw) {
+        stereo::AwParams ap;
+        ap.L = pl;
+        ap.R = pr;
+        ap.lkey = d->lkey;
+        ap.rpart = d->rpart;
+        ap.W = W;
+        ap.H = H;
+        ap.D = d->mp.num_disparities;
+        ap.bx = d->bx;
+        ap.by = d->by;
+        ap.ntx = d->ntx;
+        ap.nseg = d->nseg;
+        ap.seg_len = d->seg_len;
+        HIP_TRY(ctx, stereo::launch_aw(d->mp.window_w, ap, s), "adaptive-window launch");
+    } else {
+        stereo::BmParams bp;
+        bp.L = pl;
+        bp.R = pr;
+        bp.lkey = d->lkey;
+        bp.rpart = d->rpart;
+        bp.W = W;
+        bp.H = H;
+        bp.D = d->mp.num_disparities;
+        bp.bx = d->bx;
+        bp.by = d->by;
+        bp.ntx = d->ntx;
+        bp.nseg = d->nseg;
+        bp.seg_len = d->seg_len;
+        HIP_TRY(ctx, stereo::launch_bm(census ? stereo::BM_COST_CENSUS : stereo::BM_COST_SAD, d->mp.window_w, bp, s),
+                "matcher launch");
+    }
     if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[1], s), "hipEventRecord");
     stereo::BmFinishParams f;
     f.lkey = d->lkey;
@@ -550,15 +622,48 @@ stereo_status stereo_matcher_process(stereo_driver* d, stereo_image* disp, stere
     f.disp_r = disp_rl ? view_of(disp_rl) : null_view();
     f.W = W;
     f.H = H;
-    f.D = bp.D;
+    f.D = d->mp.num_disparities;
     f.ntx = d->ntx;
     f.nseg = d->nseg;
     f.seg_len = d->seg_len;
     f.bx = d->bx;
     f.by = d->by;
     f.tol = d->mp.lr_tolerance;
+    f.value_base = d->mp.algorithm == STEREO_ALGO_AW1 ? 255 * 3 * d->mp.window_w * d->mp.window_w : 0;
     HIP_TRY(ctx, stereo::launch_bm_finish(f, s), "finish launch");
     return end(d, s);
+}
+
+stereo_status stereo_matcher_scores(stereo_driver* d, stereo_image* score, const stereo_image* left,
+                                    const stereo_image* right, stereo_stream stream)
+{
+    if (!d || d->kind != stereo_driver::MATCHER || !score || !left || !right) return STEREO_ERROR_INVALID_ARGUMENT;
+    stereo_context* ctx = d->ctx;
+    if (d->mp.algorithm != STEREO_ALGO_NCC)
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "%s: winning scores are produced by the NCC matcher", d->name);
+    const int W = d->fmt.width, H = d->fmt.height;
+    stereo_status st;
+    if ((st = check_image(ctx, score, W, H, STEREO_PIXEL_LUM_F32, "score output"))) return st;
+    if ((st = check_image(ctx, left, W, H, STEREO_PIXEL_LUM_U8, "left image"))) return st;
+    if ((st = check_image(ctx, right, W, H, STEREO_PIXEL_LUM_U8, "right image"))) return st;
+    if ((st = set_device(ctx))) return st;
+    stereo::NccScoreParams f;
+    f.L = view_of(left);
+    f.R = view_of(right);
+    f.lkey = d->lkey;
+    f.score = view_of(score);
+    f.lstats = d->lstats;
+    f.rstats = d->rstats;
+    f.stats_pitch = W;
+    f.W = W;
+    f.H = H;
+    f.ax = d->mp.window_w / 2;
+    f.ay = d->mp.window_h / 2;
+    f.bx = d->bx;
+    f.by = d->by;
+    f.nseg = d->nseg;
+    HIP_TRY(ctx, stereo::launch_ncc_score(f, (hipStream_t)stream), "ncc score launch");
+    return STEREO_OK;
 }
 
 stereo_status stereo_matcher_stage_times(stereo_driver* d, double* ms, int32_t n)

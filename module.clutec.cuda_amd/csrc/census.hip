@@ -8,9 +8,11 @@
 // the image are written as 0 (the reference leaves them untouched).
 //
 // Layout: a 64 x 32 output tile per 256-thread workgroup; the input tile plus halo
-// is staged in LDS once; each thread produces an 8-row column strip, keeping the
-// (8 + CH - 1) x CW neighbourhood in registers so that each LDS byte is read once
-// per thread.  Each descriptor bit costs one compare and one add-with-carry.
+// is staged in LDS once, one pixel per dword so that every read is a conflict-free
+// ds_read_b32; each thread produces an 8-row column strip.  A descriptor bit costs
+// two VALU operations: t = centre - neighbour (negative exactly when neighbour >
+// centre), then acc = v_alignbit(acc, t, 31), which shifts the sign bit of t into
+// the accumulator.  Both images of a pair go through one launch (blockIdx.z).
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -23,13 +25,15 @@ constexpr int CT_H = 32;
 constexpr int CT_STRIP = 8;
 
 template <int CW, int CH>
-__global__ __launch_bounds__(256) void census_wide_kernel(View src, View dst)
+__global__ __launch_bounds__(256) void census_wide_kernel(View src0, View dst0, View src1, View dst1)
 {
     constexpr int RX = CW / 2, RY = CH / 2, NB = CW * CH;
     constexpr int LW = CT_W + 2 * RX, LH = CT_H + 2 * RY;
     constexpr int NHI = NB > 32 ? NB - 32 : 0;  // bits landing in the high word
-    __shared__ uint8_t tile[LH][LW + 4];
+    __shared__ uint32_t tile[LH][LW + 1];
 
+    const View src = blockIdx.z ? src1 : src0;
+    const View dst = blockIdx.z ? dst1 : dst0;
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * CT_W, y0 = blockIdx.y * CT_H;
     for (int i = tid; i < LH * LW; i += 256) {
@@ -43,27 +47,22 @@ __global__ __launch_bounds__(256) void census_wide_kernel(View src, View dst)
     const int cx = tid & 63;
     const int strip = tid >> 6;
     const int x = x0 + cx;
-    uint8_t win[CT_STRIP + 2 * RY][CW];
-#pragma unroll
-    for (int r = 0; r < CT_STRIP + 2 * RY; ++r)
-#pragma unroll
-        for (int c = 0; c < CW; ++c) win[r][c] = tile[strip * CT_STRIP + r][cx + c];
-
 #pragma unroll
     for (int k = 0; k < CT_STRIP; ++k) {
         const int y = y0 + strip * CT_STRIP + k;
-        const uint32_t centre = win[k + RY][RX];
+        const uint32_t* w = &tile[strip * CT_STRIP + k][cx];
+        const uint32_t centre = w[RY * (LW + 1) + RX];
         uint32_t hi = 0, lo = 0;
         int bit = 0;
 #pragma unroll
         for (int dy = 0; dy < CH; ++dy)
 #pragma unroll
             for (int dx = 0; dx < CW; ++dx, ++bit) {
-                const uint32_t g = (uint32_t)win[k + dy][dx] > centre ? 1u : 0u;
+                const uint32_t t = centre - w[dy * (LW + 1) + dx];
                 if (bit < NHI)
-                    hi = hi + hi + g;
+                    hi = __builtin_amdgcn_alignbit(hi, t, 31);
                 else
-                    lo = lo + lo + g;
+                    lo = __builtin_amdgcn_alignbit(lo, t, 31);
             }
         const bool inside = x >= RX && x < src.w - RX && y >= RY && y < src.h - RY;
         if (x < dst.w && y < dst.h) {
@@ -98,10 +97,11 @@ __global__ __launch_bounds__(256) void census3x3_ref_kernel(View src, View dst)
 }
 
 template <int CW, int CH>
-hipError_t launch_wide(const View& src, const View& dst, hipStream_t s)
+hipError_t launch_wide(const View& src0, const View& dst0, const View* src1, const View* dst1, hipStream_t s)
 {
-    dim3 grid((src.w + CT_W - 1) / CT_W, (src.h + CT_H - 1) / CT_H);
-    hipLaunchKernelGGL((census_wide_kernel<CW, CH>), grid, dim3(256), 0, s, src, dst);
+    dim3 grid((src0.w + CT_W - 1) / CT_W, (src0.h + CT_H - 1) / CT_H, src1 ? 2 : 1);
+    hipLaunchKernelGGL((census_wide_kernel<CW, CH>), grid, dim3(256), 0, s, src0, dst0, src1 ? *src1 : src0,
+                       dst1 ? *dst1 : dst0);
     return hipGetLastError();
 }
 
@@ -121,11 +121,18 @@ hipError_t launch_census(const View& src, const View& dst, int cw, int ch, int r
         hipLaunchKernelGGL(census3x3_ref_kernel, grid, dim3(256), 0, s, src, dst);
         return hipGetLastError();
     }
-    if (cw == 9 && ch == 7) return launch_wide<9, 7>(src, dst, s);
-    if (cw == 7 && ch == 9) return launch_wide<7, 9>(src, dst, s);
-    if (cw == 5 && ch == 5) return launch_wide<5, 5>(src, dst, s);
-    if (cw == 7 && ch == 7) return launch_wide<7, 7>(src, dst, s);
-    if (cw == 3 && ch == 3) return launch_wide<3, 3>(src, dst, s);
+    return launch_census_pair(src, dst, nullptr, nullptr, cw, ch, s);
+}
+
+hipError_t launch_census_pair(const View& src0, const View& dst0, const View* src1, const View* dst1, int cw, int ch,
+                              hipStream_t s)
+{
+    if (src1 && (src1->w != src0.w || src1->h != src0.h)) return hipErrorInvalidValue;
+    if (cw == 9 && ch == 7) return launch_wide<9, 7>(src0, dst0, src1, dst1, s);
+    if (cw == 7 && ch == 9) return launch_wide<7, 9>(src0, dst0, src1, dst1, s);
+    if (cw == 5 && ch == 5) return launch_wide<5, 5>(src0, dst0, src1, dst1, s);
+    if (cw == 7 && ch == 7) return launch_wide<7, 7>(src0, dst0, src1, dst1, s);
+    if (cw == 3 && ch == 3) return launch_wide<3, 3>(src0, dst0, src1, dst1, s);
     return hipErrorInvalidValue;
 }
 

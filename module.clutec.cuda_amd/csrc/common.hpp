@@ -48,11 +48,11 @@ __device__ __forceinline__ uint32_t max_shift_in(uint32_t acc, uint32_t key)
     return s > key ? s : key;
 }
 
-// Fourteen independent chains acc[i] = max(acc[i] shifted one lane up (lane 0 reads
-// 0), key[i]) as one block of v_max_u32_dpp, so that the compiler can neither split
-// the DPP move from the max nor schedule a VALU write between them.  The leading
-// s_nop covers the two wait states a DPP read needs after a VALU write of its
-// source; inside the block each source was written fourteen instructions earlier.
+// N independent chains acc[i] = max(acc[i] shifted one lane up (lane 0 reads 0),
+// key[i]) as one block of v_max_u32_dpp, so that the compiler can neither split the
+// DPP move from the max nor schedule a VALU write between them.  The leading s_nop
+// covers the two wait states a DPP read needs after a VALU write of its source;
+// inside the block each source was written N instructions earlier (N >= 2).
 __device__ __forceinline__ void max_shift_in14(uint32_t (&acc)[14], const uint32_t (&key)[14])
 {
     asm volatile("s_nop 1\n"
@@ -70,10 +70,47 @@ __device__ __forceinline__ void max_shift_in14(uint32_t (&acc)[14], const uint32
                  "v_max_u32_dpp %11, %11, %25 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
                  "v_max_u32_dpp %12, %12, %26 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
                  "v_max_u32_dpp %13, %13, %27 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
-                 : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]), "+v"(acc[6]),
-                   "+v"(acc[7]), "+v"(acc[8]), "+v"(acc[9]), "+v"(acc[10]), "+v"(acc[11]), "+v"(acc[12]), "+v"(acc[13])
-                 : "v"(key[0]), "v"(key[1]), "v"(key[2]), "v"(key[3]), "v"(key[4]), "v"(key[5]), "v"(key[6]),
-                   "v"(key[7]), "v"(key[8]), "v"(key[9]), "v"(key[10]), "v"(key[11]), "v"(key[12]), "v"(key[13]));
+                 : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]), "+v"(acc[6]), "+v"(acc[7]), "+v"(acc[8]), "+v"(acc[9]), "+v"(acc[10]), "+v"(acc[11]), "+v"(acc[12]), "+v"(acc[13])
+                 : "v"(key[0]), "v"(key[1]), "v"(key[2]), "v"(key[3]), "v"(key[4]), "v"(key[5]), "v"(key[6]), "v"(key[7]), "v"(key[8]), "v"(key[9]), "v"(key[10]), "v"(key[11]), "v"(key[12]), "v"(key[13]));
+}
+
+__device__ __forceinline__ void max_shift_in13(uint32_t (&acc)[13], const uint32_t (&key)[13])
+{
+    asm volatile("s_nop 1\n"
+                 "v_max_u32_dpp %0, %0, %13 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %1, %1, %14 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %2, %2, %15 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %3, %3, %16 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %4, %4, %17 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %5, %5, %18 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %6, %6, %19 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %7, %7, %20 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %8, %8, %21 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %9, %9, %22 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %10, %10, %23 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %11, %11, %24 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %12, %12, %25 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]), "+v"(acc[6]), "+v"(acc[7]), "+v"(acc[8]), "+v"(acc[9]), "+v"(acc[10]), "+v"(acc[11]), "+v"(acc[12])
+                 : "v"(key[0]), "v"(key[1]), "v"(key[2]), "v"(key[3]), "v"(key[4]), "v"(key[5]), "v"(key[6]), "v"(key[7]), "v"(key[8]), "v"(key[9]), "v"(key[10]), "v"(key[11]), "v"(key[12]));
+}
+
+__device__ __forceinline__ void max_shift_in12(uint32_t (&acc)[12], const uint32_t (&key)[12])
+{
+    asm volatile("s_nop 1\n"
+                 "v_max_u32_dpp %0, %0, %12 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %1, %1, %13 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %2, %2, %14 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %3, %3, %15 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %4, %4, %16 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %5, %5, %17 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %6, %6, %18 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %7, %7, %19 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %8, %8, %20 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %9, %9, %21 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %10, %10, %22 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_max_u32_dpp %11, %11, %23 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]), "+v"(acc[6]), "+v"(acc[7]), "+v"(acc[8]), "+v"(acc[9]), "+v"(acc[10]), "+v"(acc[11])
+                 : "v"(key[0]), "v"(key[1]), "v"(key[2]), "v"(key[3]), "v"(key[4]), "v"(key[5]), "v"(key[6]), "v"(key[7]), "v"(key[8]), "v"(key[9]), "v"(key[10]), "v"(key[11]));
 }
 
 }  // namespace stereo

@@ -1,0 +1,296 @@
+// Normalised cross-correlation block matcher for gfx950.
+//
+// Reference: BlockMatcherNCC (CluTec.Cuda.MultiView/BlockMatcherNCC.cu): EvalMsd :338-373
+// (base mean / std-dev), EvalSim :180-336 (ncc = (E[IJ] - muI muJ) / (sdI sdJ), zero when
+// the norm is small, clamped at 0 before quantisation), strict '>' winner-take-all over d
+// (:298).  The reference forms E[x^2] - mu^2 in float, which cancels catastrophically;
+// here every box sum is an exact integer:
+//     cov = N*SIJ - SI*SJ,  var = N*SS - S^2,  score = cov / sqrt(varI * varJ)
+// and only the last step is FP32: score = float(cov) * rI * rJ with rI = 1/sqrt(varI)
+// rounded once from FP64.  A window whose standard deviation is below one grey level
+// (var < N^2) scores 0.  The winner key is ((floor(max(score,0) * 2^23)) << 8 | 255-d):
+// the best score, lowest d on ties, the same key layout as the SAD / census matcher so
+// the right-view merge and the check are shared.
+//
+// Kernel layout (one disparity per step): workgroup = 4 waves, tile = 64 columns x 52
+// rows.  Row phase: lane = (H row 0..63, run of 19 columns), products I*J(x-d) and
+// their horizontal sums over 2*AX+1 columns (halo from the neighbouring runs by DPP)
+// into LDS (one buffer, two barriers per disparity).  Column phase: lane = column,
+// wave = 13 rows: vertical running sums give SIJ, the right window's (SJ, rJ) come
+// from an LDS tile of the right statistics that is refreshed every 16 disparities,
+// the left window's are registers.  Left winner in
+// a register, right winner carried along the wave by a DPP shift (see bm.hip).
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace stereo {
+namespace ncc {
+
+constexpr int TW = 64;
+constexpr int RG = 13;
+constexpr int TH = 4 * RG;  // 52
+constexpr int HROWS = 64;
+constexpr int RUN = 19;
+constexpr int COLS = 4 * RUN;  // 76 cost columns: tile columns [-6, 70)
+constexpr int HSTRIDE = 76;
+constexpr int DCH = 16;        // disparities per right-statistics tile
+constexpr int SCOLS = TW + DCH - 1;
+
+// Per-pixel window statistics: .x = window sum S (int), .y = float bits of the
+// reciprocal standard-deviation factor (0 when var < N^2).
+__global__ __launch_bounds__(256) void ncc_stats_kernel(View img, int2* out, int pitch_px, int ax, int ay, float scale)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= img.w || y >= img.h) return;
+    int2 r = make_int2(0, 0);
+    if (x >= ax && x < img.w - ax && y >= ay && y < img.h - ay) {
+        int s = 0, ss = 0;
+        for (int j = -ay; j <= ay; ++j) {
+            const uint8_t* row = row_ptr<const uint8_t>(img, y + j);
+            for (int i = -ax; i <= ax; ++i) {
+                const int v = row[x + i];
+                s += v;
+                ss += v * v;
+            }
+        }
+        const long long n = (long long)(2 * ax + 1) * (2 * ay + 1);
+        const long long var = n * (long long)ss - (long long)s * s;
+        float rf = 0.0f;
+        if (var >= n * n) rf = (float)((double)scale / sqrt((double)var));
+        r = make_int2(s, __float_as_int(rf));
+    }
+    out[(size_t)y * pitch_px + x] = r;
+}
+
+template <int AX, int AY, bool EDGE>
+__device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int2* stile, uint32_t (*ebuf)[RG][64])
+{
+    static_assert(TH + 2 * AY <= HROWS, "tile too tall for the window");
+    static_assert(AX <= 6 && 2 * AX + 64 <= COLS, "window wider than the run layout");
+    const int tid = threadIdx.x;
+    const int tx = blockIdx.x, ty = blockIdx.y, seg = blockIdx.z;
+    const int x0 = tx * TW, y0 = ty * TH;
+    const int W = p.W, H = p.H;
+    const int ds = seg * p.seg_len;
+    const int de = min(p.D, ds + p.seg_len);
+    constexpr int N = (2 * AX + 1) * (2 * AY + 1);
+
+    // row phase identity
+    const int run = tid & 3;
+    const int hr = tid >> 2;
+    const int gyr = clampi(y0 - AY + hr, 0, H - 1);
+    const int xs = x0 - 6 + RUN * run;
+    const uint8_t* const rowL = row_ptr<const uint8_t>(p.L, gyr) + xs;
+    const uint8_t* const rowR = row_ptr<const uint8_t>(p.R, gyr) + xs;
+    uint32_t il[RUN];
+#pragma unroll
+    for (int i = 0; i < RUN; ++i) il[i] = rowL[i];
+
+    // column phase identity
+    const int lane = tid & 63;
+    const int gw = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int x = x0 + lane;
+    const int rows_here = H - (y0 + gw * RG);
+    uint32_t* const rp = p.rpart + (((size_t)seg * p.ntx + tx) * H + y0 + gw * RG) * BM_NPART;
+    uint32_t* const eb = &ebuf[gw][0][0];
+    const bool lane_ok = x >= p.bx && x < W - p.bx;
+    const int dlim = x - p.bx;
+    int nsi[RG];     // -SI of the left window
+    float rfi[RG];   // 2^23 / sqrt(varI), or 0
+    uint32_t lacc[RG], racc[RG];
+#pragma unroll
+    for (int i = 0; i < RG; ++i) {
+        const int y = min(y0 + gw * RG + i, H - 1);
+        const int2 st = p.lstats[(size_t)y * p.stats_pitch + min(x, W - 1)];
+        nsi[i] = -st.x;
+        rfi[i] = __int_as_float(st.y);
+        lacc[i] = 0;
+        racc[i] = 0;
+    }
+
+    for (int dc = ds; dc < de; dc += DCH) {
+        // right statistics for right columns x0-dc-(DCH-1) .. x0+63-dc, rows of the tile
+        __syncthreads();
+        for (int k = tid; k < TH * SCOLS; k += 256) {
+            const int r = k / SCOLS, c = k - r * SCOLS;
+            const int y = min(y0 + r, H - 1);
+            const int xc = clampi(x0 - dc - (DCH - 1) + c, 0, W - 1);
+            stile[k] = p.rstats[(size_t)y * p.stats_pitch + xc];
+        }
+        const int dce = min(de, dc + DCH);
+        for (int d = dc; d < dce; ++d) {
+            // ---- row phase: products and horizontal sums for disparity d
+            uint32_t e[RUN + 2 * AX];
+#pragma unroll
+            for (int i = 0; i < RUN; ++i) e[AX + i] = il[i] * (uint32_t)rowR[i - d];
+            __syncthreads();  // the previous column phase has read the H buffer
+#pragma unroll
+            for (int i = 0; i < AX; ++i) {
+                e[i] = dpp_mov<DPP_ROW_SHR1>(e[AX + RUN - AX + i]);
+                e[AX + RUN + i] = dpp_mov<DPP_ROW_SHL1>(e[AX + i]);
+            }
+            uint32_t h = 0;
+#pragma unroll
+            for (int i = 0; i <= 2 * AX; ++i) h += e[i];
+            uint32_t* hrow = hb + hr * HSTRIDE + RUN * run;
+#pragma unroll
+            for (int i = 0; i < RUN; ++i) {
+                if (i > 0) h = h + e[i + 2 * AX] - e[i - 1];
+                hrow[i] = h;
+            }
+            __syncthreads();
+
+            // ---- column phase
+            uint32_t hv[RG + 2 * AY];
+            const uint32_t* hcol = hb + gw * RG * HSTRIDE + lane + 6;
+#pragma unroll
+            for (int i = 0; i < RG + 2 * AY; ++i) hv[i] = hcol[i * HSTRIDE];
+            uint32_t sij = 0;
+#pragma unroll
+            for (int i = 0; i <= 2 * AY; ++i) sij += hv[i];
+            const int2* srow = stile + (gw * RG) * SCOLS + lane + (DCH - 1) - (d - dc);
+            const uint32_t dd = 255u - (uint32_t)d;
+            bool ok = true;
+            if (EDGE) ok = lane_ok && d <= dlim;
+            uint32_t key[RG];
+#pragma unroll
+            for (int i = 0; i < RG; ++i) {
+                if (i > 0) sij = sij + hv[i + 2 * AY] - hv[i - 1];
+                const int2 rs = srow[i * SCOLS];
+                const int cov = __mul24(nsi[i], rs.x) + (int)__umul24((uint32_t)N, sij);
+                const float sc = (float)cov * rfi[i] * __int_as_float(rs.y);
+                const uint32_t q = sc > 0.0f ? (uint32_t)sc : 0u;
+                uint32_t k = (q << 8) | dd;
+                if (EDGE) k = ok ? k : 0u;
+                key[i] = k;
+                lacc[i] = lacc[i] > k ? lacc[i] : k;
+            }
+            static_assert(RG == 13, "right-view chain written for 13 rows");
+            max_shift_in13(racc, key);
+            if (lane == 63) {
+#pragma unroll
+                for (int i = 0; i < RG; ++i) eb[i * 64 + (d & 63)] = racc[i];
+            }
+            if ((d & 63) == 63 || d == de - 1) {
+                const int base = d & ~63;
+                if (lane <= (d & 63) && base + lane >= ds) {
+                    const int kk = 318 - (base + lane);
+#pragma unroll
+                    for (int i = 0; i < RG; ++i)
+                        if (i < rows_here) rp[i * BM_NPART + kk] = eb[i * 64 + lane];
+                }
+            }
+        }
+    }
+    {
+        const int kk = lane + 255 - (de - 1);
+#pragma unroll
+        for (int i = 0; i < RG; ++i)
+            if (i < rows_here) rp[i * BM_NPART + kk] = racc[i];
+    }
+    if (x < W) {
+        uint32_t* lk = p.lkey + (size_t)seg * W * H + (size_t)(y0 + gw * RG) * W + x;
+#pragma unroll
+        for (int i = 0; i < RG; ++i)
+            if (i < rows_here) lk[(size_t)i * W] = lacc[i];
+    }
+}
+
+template <int AX, int AY>
+__global__ __launch_bounds__(256, 2) void ncc_fused_kernel(NccParams p)
+{
+    __shared__ uint32_t hbuf[HROWS * HSTRIDE];
+    __shared__ int2 stile[TH * SCOLS];
+    __shared__ uint32_t ebuf[4][RG][64];
+    const int x0 = blockIdx.x * TW;
+    const int de = min(p.D, ((int)blockIdx.z + 1) * p.seg_len);
+    const bool edge = (x0 < p.bx + de - 1) || (x0 + TW > p.W - p.bx);
+    if (edge)
+        ncc_tile<AX, AY, true>(p, hbuf, stile, ebuf);
+    else
+        ncc_tile<AX, AY, false>(p, hbuf, stile, ebuf);
+}
+
+// Winning score of every left pixel: the same FP32 arithmetic as the matcher, on a
+// window sum computed directly for the winning disparity.
+__global__ __launch_bounds__(256) void ncc_score_kernel(NccScoreParams f)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= f.W || y >= f.H) return;
+    float out = __int_as_float(0x7fc00000);  // NaN: no match
+    uint32_t lk = 0;
+    if (x >= f.bx && x < f.W - f.bx && y >= f.by && y < f.H - f.by)
+        for (int s = 0; s < f.nseg; ++s) {
+            const uint32_t v = f.lkey[(size_t)s * f.W * f.H + (size_t)y * f.W + x];
+            lk = v > lk ? v : lk;
+        }
+    const int d = key_disp(lk);
+    if (d >= 0) {
+        int sij = 0;
+        for (int j = -f.ay; j <= f.ay; ++j) {
+            const uint8_t* rl = row_ptr<const uint8_t>(f.L, y + j);
+            const uint8_t* rr = row_ptr<const uint8_t>(f.R, y + j);
+            for (int i = -f.ax; i <= f.ax; ++i) sij += (int)rl[x + i] * (int)rr[x + i - d];
+        }
+        const int2 a = f.lstats[(size_t)y * f.stats_pitch + x];
+        const int2 b = f.rstats[(size_t)y * f.stats_pitch + x - d];
+        const int n = (2 * f.ax + 1) * (2 * f.ay + 1);
+        const int cov = n * sij - a.x * b.x;
+        // lstats carry the 2^23 key scale; undo it exactly (power of two)
+        out = (float)cov * (__int_as_float(a.y) * (1.0f / 8388608.0f)) * __int_as_float(b.y);
+    }
+    row_ptr<float>(f.score, y)[x] = out;
+}
+
+}  // namespace ncc
+
+bool ncc_supported(int ww, int wh, int D)
+{
+    if (D < 8 || D > 256 || (D % 8) != 0) return false;
+    return (ww == 11 && wh == 11) || (ww == 9 && wh == 9) || (ww == 5 && wh == 5) || (ww == 7 && wh == 7);
+}
+
+void ncc_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len)
+{
+    *ntx = (W + ncc::TW - 1) / ncc::TW;
+    *nty = (H + ncc::TH - 1) / ncc::TH;
+    const int tiles = *ntx * *nty;
+    int n = (1200 + tiles - 1) / tiles;
+    if (n > D / 16) n = D / 16;
+    if (n > BM_MAX_SEG) n = BM_MAX_SEG;
+    if (n < 1) n = 1;
+    int len = (D + n - 1) / n;
+    len = (len + 15) & ~15;
+    *seg_len = len;
+    *nseg = (D + len - 1) / len;
+}
+
+hipError_t launch_ncc_stats(const View& img, int2* out, int pitch_px, int ww, int wh, float scale, hipStream_t s)
+{
+    dim3 grid((img.w + 63) / 64, (img.h + 3) / 4);
+    hipLaunchKernelGGL(ncc::ncc_stats_kernel, grid, dim3(256), 0, s, img, out, pitch_px, ww / 2, wh / 2, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_ncc(int ww, const NccParams& p, hipStream_t s)
+{
+    dim3 grid(p.ntx, (p.H + ncc::TH - 1) / ncc::TH, p.nseg);
+    switch (ww) {
+    case 11: hipLaunchKernelGGL((ncc::ncc_fused_kernel<5, 5>), grid, dim3(256), 0, s, p); break;
+    case 9: hipLaunchKernelGGL((ncc::ncc_fused_kernel<4, 4>), grid, dim3(256), 0, s, p); break;
+    case 7: hipLaunchKernelGGL((ncc::ncc_fused_kernel<3, 3>), grid, dim3(256), 0, s, p); break;
+    default: hipLaunchKernelGGL((ncc::ncc_fused_kernel<2, 2>), grid, dim3(256), 0, s, p); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ncc_score(const NccScoreParams& f, hipStream_t s)
+{
+    dim3 grid((f.W + 63) / 64, (f.H + 3) / 4);
+    hipLaunchKernelGGL(ncc::ncc_score_kernel, grid, dim3(256), 0, s, f);
+    return hipGetLastError();
+}
+
+}  // namespace stereo
