@@ -1,0 +1,486 @@
+// C++ driver layer over the C ABI of the gfx950 stereo engine.
+//
+// Keeps the call shape of the reference's MultiView drivers: a device, device images,
+// one CDriver per operator with SParameter::Set(...), Configure(device, format, pars),
+// Process(...), and ProcessKernel(driver, args...) that times the call and returns the
+// milliseconds (KernelDriverBase.h:47-200).  Errors come back as exceptions carrying
+// the C ABI status and message (EnsureOK, Api.cpp:86-94).  Header only; link
+// libclutec_stereo.so.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../clutec_stereo.h"
+
+namespace Clu
+{
+namespace Cuda
+{
+
+class CException : public std::runtime_error
+{
+public:
+    CException(stereo_status st, const std::string& msg) : std::runtime_error(msg), m_status(st) {}
+    stereo_status Status() const { return m_status; }
+
+private:
+    stereo_status m_status;
+};
+
+inline void EnsureOK(stereo_status st, const stereo_context* ctx, const char* what)
+{
+    if (st != STEREO_OK) {
+        const char* m = stereo_last_error(ctx);
+        throw CException(st, std::string(what) + ": " + (m && *m ? m : stereo_status_string(st)));
+    }
+}
+
+// Pixel types used on the path (Clu::EPixelType x EDataType subset).
+enum class EPixelType : int32_t
+{
+    Lum_UInt8 = STEREO_PIXEL_LUM_U8,
+    Lum_UInt16 = STEREO_PIXEL_LUM_U16,
+    Lum_Int16 = STEREO_PIXEL_LUM_I16,
+    Lum_UInt32 = STEREO_PIXEL_LUM_U32,
+    Lum_UInt64 = STEREO_PIXEL_LUM_U64,
+    Lum_Single = STEREO_PIXEL_LUM_F32,
+    RGBA_UInt16 = STEREO_PIXEL_RGBA_U16,
+};
+
+// Clu::SImageFormat subset: width, height, pixel type.
+struct SImageFormat
+{
+    int32_t iWidth = 0, iHeight = 0;
+    EPixelType ePixelType = EPixelType::Lum_UInt8;
+
+    SImageFormat() = default;
+    SImageFormat(int32_t w, int32_t h, EPixelType t) : iWidth(w), iHeight(h), ePixelType(t) {}
+    bool IsValid() const { return iWidth > 0 && iHeight > 0; }
+    stereo_image_format Raw() const { return stereo_image_format{iWidth, iHeight, (int32_t)ePixelType}; }
+};
+
+// EDisparityId (DisparityId.h:33-45).
+enum class EDisparityId : uint16_t
+{
+    First = 1,
+    Last = 0xFFF0 - 1,
+    Unknown = 0,
+    CannotEvaluate = 0xFFFF,
+    Saturated = 0xFFFF - 1,
+    NotFound = 0xFFFF - 2,
+    NotSpecific = 0xFFFF - 3,
+    NotUnique = 0xFFFF - 4,
+    Inconsistent = 0xFFFF - 5,
+};
+
+// _SDisparityConfig subset (offset added by MapPixelPos).
+struct _SDisparityConfig
+{
+    int32_t iMin = 0, iMax = 0, iOffset = 0;
+    int32_t Min() const { return iMin; }
+    int32_t Max() const { return iMax; }
+    int32_t Range() const { return iMax - iMin; }
+    int32_t MapPixelPos(int32_t x, int32_t d, int32_t ltor) const { return ltor ? x - d - iOffset : x + d + iOffset; }
+};
+
+class CDevice
+{
+public:
+    CDevice() = default;
+    explicit CDevice(int iIdx) { Set(iIdx); }
+    ~CDevice()
+    {
+        if (m_ctx) stereo_destroy(m_ctx);
+    }
+    CDevice(const CDevice&) = delete;
+    CDevice& operator=(const CDevice&) = delete;
+
+    void Set(int iIdx)
+    {
+        if (m_ctx) stereo_destroy(m_ctx);
+        m_ctx = nullptr;
+        EnsureOK(stereo_create(&m_ctx, iIdx), nullptr, "CDevice::Set");
+    }
+    stereo_context* Context() const { return m_ctx; }
+    void SetTiming(bool b) const { EnsureOK(stereo_set_timing(m_ctx, b ? 1 : 0), m_ctx, "CDevice::SetTiming"); }
+    void Synchronize() const { EnsureOK(stereo_synchronize(m_ctx, nullptr), m_ctx, "CDevice::Synchronize"); }
+
+private:
+    stereo_context* m_ctx = nullptr;
+};
+
+// Owning pitched device image (CDeviceImage, DeviceImage.h).
+class CDeviceImage
+{
+public:
+    CDeviceImage() = default;
+    CDeviceImage(const CDevice& dev, const SImageFormat& f) { Create(dev, f); }
+    ~CDeviceImage() { Destroy(); }
+    CDeviceImage(const CDeviceImage&) = delete;
+    CDeviceImage& operator=(const CDeviceImage&) = delete;
+    CDeviceImage(CDeviceImage&& o) noexcept : m_ctx(o.m_ctx), m_img(o.m_img) { o.m_img.data = nullptr; }
+    CDeviceImage& operator=(CDeviceImage&& o) noexcept
+    {
+        if (this != &o) {
+            Destroy();
+            m_ctx = o.m_ctx;
+            m_img = o.m_img;
+            o.m_img.data = nullptr;
+        }
+        return *this;
+    }
+
+    void Create(const CDevice& dev, const SImageFormat& f)
+    {
+        Destroy();
+        m_ctx = dev.Context();
+        EnsureOK(stereo_image_alloc(m_ctx, &m_img, f.iWidth, f.iHeight, (int32_t)f.ePixelType), m_ctx,
+                 "CDeviceImage::Create");
+    }
+    void Destroy()
+    {
+        if (m_img.data) stereo_image_free(m_ctx, &m_img);
+        m_img.data = nullptr;
+    }
+    void CopyFrom(const void* host, size_t hostPitch)
+    {
+        EnsureOK(stereo_image_upload(m_ctx, &m_img, host, hostPitch, nullptr), m_ctx, "CDeviceImage::CopyFrom");
+    }
+    void CopyInto(void* host, size_t hostPitch) const
+    {
+        EnsureOK(stereo_image_download(m_ctx, &m_img, host, hostPitch, nullptr), m_ctx, "CDeviceImage::CopyInto");
+    }
+    SImageFormat Format() const
+    {
+        return SImageFormat(m_img.format.width, m_img.format.height, (EPixelType)m_img.format.pixel_type);
+    }
+    stereo_image* Raw() { return &m_img; }
+    const stereo_image* Raw() const { return &m_img; }
+
+private:
+    stereo_context* m_ctx = nullptr;
+    stereo_image m_img{};
+};
+
+// KernelDriverBase.h:47-177: name, last process time, timing around Process.
+class CKernelDriverBase
+{
+public:
+    explicit CKernelDriverBase(const char* name) : m_sKernelName(name) {}
+    virtual ~CKernelDriverBase()
+    {
+        if (m_drv) stereo_driver_destroy(m_drv);
+    }
+    CKernelDriverBase(const CKernelDriverBase&) = delete;
+    CKernelDriverBase& operator=(const CKernelDriverBase&) = delete;
+
+    const std::string& KernelName() const { return m_sKernelName; }
+    double LastProcessTime() const { return m_dLastProcessTime; }
+    void LogLastProcessTime() const { std::fprintf(stderr, "Kernel '%s': %gms\n", m_sKernelName.c_str(), m_dLastProcessTime); }
+
+    void Begin() { m_bTimed = true; }
+    void End()
+    {
+        if (!m_drv || !m_bTimed) return;
+        double ms = 0.0;
+        EnsureOK(stereo_driver_last_time_ms(m_drv, &ms), m_ctx, "End");
+        m_dLastProcessTime = ms;
+    }
+
+protected:
+    void Reset(stereo_context* ctx, stereo_driver* drv)
+    {
+        if (m_drv) stereo_driver_destroy(m_drv);
+        m_ctx = ctx;
+        m_drv = drv;
+    }
+
+    stereo_context* m_ctx = nullptr;
+    stereo_driver* m_drv = nullptr;
+    std::string m_sKernelName;
+    double m_dLastProcessTime = 0.0;
+    bool m_bTimed = false;
+};
+
+// ProcessKernel (KernelDriverBase.h:180-200): times one Process call and returns ms.
+// Timing needs CDevice::SetTiming(true); otherwise 0 is returned.
+template <typename TDriver, typename... TPars>
+double ProcessKernel(TDriver& xDriver, TPars&&... xPars)
+{
+    xDriver.Begin();
+    xDriver.Process(std::forward<TPars>(xPars)...);
+    try {
+        xDriver.End();
+    } catch (const CException& e) {
+        if (e.Status() != STEREO_ERROR_NOT_READY) throw;
+    }
+    return xDriver.LastProcessTime();
+}
+
+namespace CensusTransform
+{
+struct SParameter
+{
+    int32_t iWindowW = 9, iWindowH = 7;
+    bool bReference3x3 = false;
+    void Set(int32_t w, int32_t h, bool ref3x3 = false)
+    {
+        iWindowW = w;
+        iWindowH = h;
+        bReference3x3 = ref3x3;
+    }
+};
+
+class CDriver : public CKernelDriverBase
+{
+public:
+    CDriver() : CKernelDriverBase("Census Transform") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& pars = SParameter())
+    {
+        const stereo_image_format f = fmt.Raw();
+        const stereo_census_params p{pars.iWindowW, pars.iWindowH, pars.bReference3x3 ? 1 : 0};
+        stereo_driver* d = nullptr;
+        EnsureOK(stereo_census_configure(dev.Context(), &d, &f, &p), dev.Context(), "CensusTransform::Configure");
+        Reset(dev.Context(), d);
+    }
+    void Process(CDeviceImage& trg, const CDeviceImage& src)
+    {
+        EnsureOK(stereo_census_process(m_drv, trg.Raw(), src.Raw(), nullptr), m_ctx, "CensusTransform::Process");
+    }
+};
+}  // namespace CensusTransform
+
+// Shared implementation of the matcher drivers.
+class CMatcherDriver : public CKernelDriverBase
+{
+public:
+    using CKernelDriverBase::CKernelDriverBase;
+
+    // disp: LUM_I16 checked map or RGBA_UInt16 disparity-id pixels; lr / rl optional.
+    void Process(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R)
+    {
+        EnsureOK(stereo_matcher_process(m_drv, disp.Raw(), nullptr, nullptr, L.Raw(), R.Raw(), nullptr), m_ctx,
+                 "Matcher::Process");
+    }
+    void Process(CDeviceImage& disp, CDeviceImage& lr, CDeviceImage& rl, const CDeviceImage& L, const CDeviceImage& R)
+    {
+        EnsureOK(stereo_matcher_process(m_drv, disp.Raw(), lr.Raw(), rl.Raw(), L.Raw(), R.Raw(), nullptr), m_ctx,
+                 "Matcher::Process");
+    }
+
+protected:
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const stereo_matcher_params& p)
+    {
+        const stereo_image_format f = fmt.Raw();
+        stereo_driver* d = nullptr;
+        EnsureOK(stereo_matcher_configure(dev.Context(), &d, &f, &p), dev.Context(), "Matcher::Configure");
+        Reset(dev.Context(), d);
+    }
+    static stereo_matcher_params Params(int32_t algo, int32_t ww, int32_t wh, int32_t D, int32_t tol, int32_t cw = 1,
+                                        int32_t ch = 1)
+    {
+        stereo_matcher_params p{};
+        p.algorithm = algo;
+        p.window_w = ww;
+        p.window_h = wh;
+        p.census_w = cw;
+        p.census_h = ch;
+        p.num_disparities = D;
+        p.lr_tolerance = tol;
+        return p;
+    }
+};
+
+namespace BlockMatcherSAD1
+{
+struct SParameter
+{
+    _SDisparityConfig xDispConfig;
+    int32_t iPatchSize = 9;
+    int32_t iLrTolerance = 1;
+    void Set(const _SDisparityConfig& cfg, int32_t patch = 9, int32_t tol = 1)
+    {
+        xDispConfig = cfg;
+        iPatchSize = patch;
+        iLrTolerance = tol;
+    }
+};
+class CDriver : public CMatcherDriver
+{
+public:
+    CDriver() : CMatcherDriver("Block Matcher SAD 1") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    {
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_SAD, p.iPatchSize, p.iPatchSize, p.xDispConfig.Range(), p.iLrTolerance));
+    }
+};
+}  // namespace BlockMatcherSAD1
+
+namespace BlockMatcherCensus
+{
+struct SParameter
+{
+    _SDisparityConfig xDispConfig;
+    int32_t iCensusW = 9, iCensusH = 7, iPatchSize = 9, iLrTolerance = 1;
+    void Set(const _SDisparityConfig& cfg, int32_t cw = 9, int32_t ch = 7, int32_t patch = 9, int32_t tol = 1)
+    {
+        xDispConfig = cfg;
+        iCensusW = cw;
+        iCensusH = ch;
+        iPatchSize = patch;
+        iLrTolerance = tol;
+    }
+};
+class CDriver : public CMatcherDriver
+{
+public:
+    CDriver() : CMatcherDriver("Block Matcher Census") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    {
+        CMatcherDriver::Configure(dev, fmt,
+                                  Params(STEREO_ALGO_CENSUS, p.iPatchSize, p.iPatchSize, p.xDispConfig.Range(), p.iLrTolerance,
+                                         p.iCensusW, p.iCensusH));
+    }
+};
+}  // namespace BlockMatcherCensus
+
+namespace BlockMatcherNCC
+{
+enum class EConfig
+{
+    Patch_11x11 = 0,
+    Patch_9x9,
+    Patch_5x5,
+};
+struct SParameter
+{
+    _SDisparityConfig xDispConfig;
+    EConfig eConfig = EConfig::Patch_11x11;
+    int32_t iLrTolerance = 1;
+    void Set(const _SDisparityConfig& cfg, EConfig config, int32_t tol = 1)
+    {
+        xDispConfig = cfg;
+        eConfig = config;
+        iLrTolerance = tol;
+    }
+};
+class CDriver : public CMatcherDriver
+{
+public:
+    CDriver() : CMatcherDriver("Block Matcher NCC") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    {
+        const int32_t w = p.eConfig == EConfig::Patch_11x11 ? 11 : p.eConfig == EConfig::Patch_9x9 ? 9 : p.eConfig == EConfig::Patch_5x5 ? 5 : -1;
+        if (w < 0) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_NCC, w, w, p.xDispConfig.Range(), p.iLrTolerance));
+    }
+    // winning scores of the last Process (LUM_Single)
+    void Scores(CDeviceImage& score, const CDeviceImage& L, const CDeviceImage& R)
+    {
+        EnsureOK(stereo_matcher_scores(m_drv, score.Raw(), L.Raw(), R.Raw(), nullptr), m_ctx, "BlockMatcherNCC::Scores");
+    }
+};
+}  // namespace BlockMatcherNCC
+
+namespace BlockMatcherAW1
+{
+enum class EConfig
+{
+    Patch_15x15 = 0,
+    Patch_9x9,
+};
+struct SParameter
+{
+    _SDisparityConfig xDispConfig;
+    EConfig eConfig = EConfig::Patch_15x15;
+    int32_t iLrTolerance = 1;
+    void Set(const _SDisparityConfig& cfg, EConfig config, int32_t tol = 1)
+    {
+        xDispConfig = cfg;
+        eConfig = config;
+        iLrTolerance = tol;
+    }
+};
+class CDriver : public CMatcherDriver
+{
+public:
+    CDriver() : CMatcherDriver("Block Matcher Adaptive Window 1") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    {
+        const int32_t s = p.eConfig == EConfig::Patch_15x15 ? 5 : p.eConfig == EConfig::Patch_9x9 ? 3 : -1;
+        if (s < 0) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_AW1, s, s, p.xDispConfig.Range(), p.iLrTolerance));
+    }
+};
+}  // namespace BlockMatcherAW1
+
+namespace BlockMatcherAW2
+{
+using EConfig = BlockMatcherAW1::EConfig;
+using SParameter = BlockMatcherAW1::SParameter;
+class CDriver : public CMatcherDriver
+{
+public:
+    CDriver() : CMatcherDriver("Block Matcher Adaptive Window 2") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    {
+        const int32_t s = p.eConfig == EConfig::Patch_15x15 ? 5 : p.eConfig == EConfig::Patch_9x9 ? 3 : -1;
+        if (s < 0) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_AW2, s, s, p.xDispConfig.Range(), p.iLrTolerance));
+    }
+};
+}  // namespace BlockMatcherAW2
+
+namespace CrossCheckDisparityMaps
+{
+struct SParameter
+{
+    _SDisparityConfig xDispConfig;
+    int32_t iDispDeltaThresh = 1;
+    int32_t iIsLeftToRight = 1;
+    void Set(const _SDisparityConfig& cfg, int32_t thresh, int32_t ltor)
+    {
+        xDispConfig = cfg;
+        iDispDeltaThresh = thresh;
+        iIsLeftToRight = ltor;
+    }
+};
+class CDriver : public CKernelDriverBase
+{
+public:
+    CDriver() : CKernelDriverBase("Cross Check Disparity Maps") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    {
+        const stereo_image_format f = fmt.Raw();
+        const stereo_crosscheck_params cp{p.iDispDeltaThresh, p.iIsLeftToRight, p.xDispConfig.iOffset, 0};
+        stereo_driver* d = nullptr;
+        EnsureOK(stereo_crosscheck_configure(dev.Context(), &d, &f, &cp), dev.Context(), "CrossCheckDisparityMaps::Configure");
+        Reset(dev.Context(), d);
+    }
+    void Process(CDeviceImage& result, const CDeviceImage& lr, const CDeviceImage& rl)
+    {
+        EnsureOK(stereo_crosscheck_process(m_drv, result.Raw(), lr.Raw(), rl.Raw(), nullptr), m_ctx,
+                 "CrossCheckDisparityMaps::Process");
+    }
+    uint16_t DispMin()
+    {
+        uint16_t a = 0, b = 0;
+        EnsureOK(stereo_crosscheck_range(m_drv, &a, &b), m_ctx, "DispMin");
+        return a;
+    }
+    uint16_t DispMax()
+    {
+        uint16_t a = 0, b = 0;
+        EnsureOK(stereo_crosscheck_range(m_drv, &a, &b), m_ctx, "DispMax");
+        return b;
+    }
+};
+}  // namespace CrossCheckDisparityMaps
+
+}  // namespace Cuda
+}  // namespace Clu

@@ -14,6 +14,7 @@ from concurrent.futures import ThreadPoolExecutor
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
+EXAMPLES = os.path.join(PKG, "examples")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "lib")
 LIBNAME = "libclutec_stereo.so"
@@ -65,6 +66,13 @@ def build(jobs=8, verbose=False):
     tmp = os.path.join(LIB, LIBNAME + ".tmp")
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lm"], verbose)
     os.replace(tmp, os.path.join(LIB, LIBNAME))
+    # host programs over the C++ driver layer (plain g++, linked against the library)
+    for f in sorted(os.listdir(EXAMPLES)):
+        if f.endswith(".cpp"):
+            exe = os.path.join(LIB, f[:-4])
+            _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), os.path.join(EXAMPLES, f),
+                  "-o", exe + ".tmp", "-L", LIB, "-lclutec_stereo", "-Wl,-rpath,$ORIGIN"], verbose)
+            os.replace(exe + ".tmp", exe)
     return os.path.join(LIB, LIBNAME)
 
 

@@ -1,0 +1,138 @@
+// stereo_run: the reference call sequence through the C++ driver layer.
+//
+//   1. generate a synthetic pair, 2. upload into device images, 3. Configure the
+//   drivers, 4. ProcessKernel: matcher (census or SAD; L->R, R->L and check fused)
+//   and, from the two one-sided maps in disparity-id form, the reference-style
+//   cross-check, 5. download, 6. write the maps and report milliseconds.
+//
+// usage: stereo_run <algo:sad|census|ncc|aw1|aw2> <W> <H> <D> <seed> <out_prefix> [runs]
+// Writes <out_prefix>_checked.i16, _lr.i16, _rl.i16 (row-major int16, -1 invalid)
+// and prints one line "ms <median> valid <fraction> range <min> <max>".
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "clutec/stereo.hpp"
+
+using namespace Clu::Cuda;
+
+static bool write_file(const std::string& path, const void* p, size_t n)
+{
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(p, 1, n, f) == n;
+    std::fclose(f);
+    return ok;
+}
+
+int main(int argc, char** argv)
+{
+    if (argc < 7) {
+        std::fprintf(stderr, "usage: %s <sad|census|ncc|aw1|aw2> W H D seed out_prefix [runs]\n", argv[0]);
+        return 2;
+    }
+    const std::string algo = argv[1];
+    const int W = std::atoi(argv[2]), H = std::atoi(argv[3]), D = std::atoi(argv[4]);
+    const unsigned long long seed = std::strtoull(argv[5], nullptr, 10);
+    const std::string out = argv[6];
+    const int runs = argc > 7 ? std::max(1, std::atoi(argv[7])) : 5;
+
+    try {
+        std::vector<uint8_t> L((size_t)W * H), R((size_t)W * H);
+        if (stereo_synth_pair(seed, W, H, 0, 8, D - 1, std::min(8, D - 1), L.data(), R.data(), nullptr) != 0)
+            throw CException(STEREO_ERROR_INVALID_ARGUMENT, "stereo_synth_pair failed");
+
+        CDevice dev(0);
+        dev.SetTiming(true);
+        const SImageFormat fmtIn(W, H, EPixelType::Lum_UInt8);
+        const SImageFormat fmtDisp(W, H, EPixelType::Lum_Int16);
+        const SImageFormat fmtId(W, H, EPixelType::RGBA_UInt16);
+        CDeviceImage imgL(dev, fmtIn), imgR(dev, fmtIn);
+        imgL.CopyFrom(L.data(), W);
+        imgR.CopyFrom(R.data(), W);
+        CDeviceImage disp(dev, fmtDisp), lr(dev, fmtDisp), rl(dev, fmtDisp);
+
+        _SDisparityConfig dcfg;
+        dcfg.iMin = 0;
+        dcfg.iMax = D;
+        std::vector<double> ms;
+
+        auto run = [&](auto& drv) {
+            for (int i = 0; i < runs; ++i) ms.push_back(ProcessKernel(drv, disp, lr, rl, imgL, imgR));
+            drv.LogLastProcessTime();
+        };
+        if (algo == "census") {
+            BlockMatcherCensus::SParameter p;
+            p.Set(dcfg, 9, 7, 9, 1);
+            BlockMatcherCensus::CDriver drv;
+            drv.Configure(dev, fmtIn, p);
+            run(drv);
+        } else if (algo == "sad") {
+            BlockMatcherSAD1::SParameter p;
+            p.Set(dcfg, 9, 1);
+            BlockMatcherSAD1::CDriver drv;
+            drv.Configure(dev, fmtIn, p);
+            run(drv);
+        } else if (algo == "ncc") {
+            BlockMatcherNCC::SParameter p;
+            p.Set(dcfg, BlockMatcherNCC::EConfig::Patch_11x11, 1);
+            BlockMatcherNCC::CDriver drv;
+            drv.Configure(dev, fmtIn, p);
+            run(drv);
+        } else if (algo == "aw1" || algo == "aw2") {
+            BlockMatcherAW1::SParameter p;
+            p.Set(dcfg, BlockMatcherAW1::EConfig::Patch_15x15, 1);
+            if (algo == "aw1") {
+                BlockMatcherAW1::CDriver drv;
+                drv.Configure(dev, fmtIn, p);
+                run(drv);
+            } else {
+                BlockMatcherAW2::CDriver drv;
+                drv.Configure(dev, fmtIn, p);
+                run(drv);
+            }
+        } else {
+            std::fprintf(stderr, "unknown algorithm '%s'\n", algo.c_str());
+            return 2;
+        }
+
+        // the reference's separate cross-check over disparity-id maps: the matcher's
+        // one-sided maps converted to EDisparityId (First + d), checked L->R
+        std::vector<int16_t> hlr((size_t)W * H), hrl((size_t)W * H), hchk((size_t)W * H);
+        lr.CopyInto(hlr.data(), (size_t)W * 2);
+        rl.CopyInto(hrl.data(), (size_t)W * 2);
+        disp.CopyInto(hchk.data(), (size_t)W * 2);
+        std::vector<uint16_t> idl((size_t)W * H * 4, 0), idr((size_t)W * H * 4, 0);
+        for (size_t i = 0; i < (size_t)W * H; ++i) {
+            idl[4 * i] = hlr[i] >= 0 ? (uint16_t)((int)EDisparityId::First + hlr[i]) : (uint16_t)EDisparityId::NotFound;
+            idr[4 * i] = hrl[i] >= 0 ? (uint16_t)((int)EDisparityId::First + hrl[i]) : (uint16_t)EDisparityId::NotFound;
+        }
+        CDeviceImage dl(dev, fmtId), dr(dev, fmtId), dchk(dev, fmtId);
+        dl.CopyFrom(idl.data(), (size_t)W * 8);
+        dr.CopyFrom(idr.data(), (size_t)W * 8);
+        CrossCheckDisparityMaps::SParameter cp;
+        cp.Set(dcfg, 1, 1);
+        CrossCheckDisparityMaps::CDriver cc;
+        cc.Configure(dev, fmtId, cp);
+        const double cms = ProcessKernel(cc, dchk, dl, dr);
+        const int dmin = cc.DispMin(), dmax = cc.DispMax();
+
+        size_t valid = 0;
+        for (int16_t v : hchk) valid += v >= 0;
+        std::sort(ms.begin(), ms.end());
+        if (!write_file(out + "_checked.i16", hchk.data(), hchk.size() * 2) ||
+            !write_file(out + "_lr.i16", hlr.data(), hlr.size() * 2) || !write_file(out + "_rl.i16", hrl.data(), hrl.size() * 2)) {
+            std::fprintf(stderr, "cannot write %s_*.i16\n", out.c_str());
+            return 1;
+        }
+        std::printf("ms %.4f valid %.6f range %d %d crosscheck_ms %.4f\n", ms[ms.size() / 2], (double)valid / hchk.size(),
+                    dmin, dmax, cms);
+    } catch (const CException& e) {
+        std::fprintf(stderr, "error %d: %s\n", (int)e.Status(), e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -1,0 +1,39 @@
+"""The C++ driver layer (include/clutec/stereo.hpp) end to end: the stereo_run host
+program configures and runs the drivers in the reference's call sequence; its maps
+must equal the oracle's bit for bit."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import clutec_stereo as cs
+import orc
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(os.path.dirname(cs.__file__), "lib", "stereo_run")
+
+
+@pytest.mark.parametrize("algo,W,H,D", [("census", 320, 160, 64), ("sad", 300, 140, 32)])
+def test_stereo_run_matches_oracle(tmp_path, algo, W, H, D):
+    seed = 5
+    prefix = str(tmp_path / algo)
+    print(f"stereo_run {algo} {W}x{H} D={D}", flush=True)
+    r = subprocess.run([EXE, algo, str(W), str(H), str(D), str(seed), prefix, "3"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    line = r.stdout.strip().splitlines()[-1].split()
+    assert line[0] == "ms" and float(line[1]) > 0.0
+    maps = [np.fromfile(prefix + s, np.int16).reshape(H, W) for s in ("_checked.i16", "_lr.i16", "_rl.i16")]
+    L, R = cs.synth_pair(seed, W, H, 0, 8, D - 1, min(8, D - 1))
+    if algo == "census":
+        ref = orc.bm(orc.COST_CENSUS, orc.census(L, 9, 7), orc.census(R, 9, 7), D, window=(9, 9), census_win=(9, 7),
+                     tol=1)
+    else:
+        ref = orc.bm(orc.COST_SAD, L, R, D, window=(9, 9), tol=1)
+    for got, want in zip(maps, ref):
+        assert np.array_equal(got, want)
+    # the reference-style cross-check over the one-sided maps reports the kept range
+    kept = maps[0][maps[0] >= 0]
+    assert int(line[5]) == kept.min() + 1 and int(line[6]) == kept.max() + 1

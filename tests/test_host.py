@@ -114,3 +114,27 @@ def test_synth_rejects_bad_args():
     buf = np.zeros(100, np.uint8)
     assert cs.lib.stereo_synth_pair(0, 5, 5, 0, 1, 10, 1, buf.ctypes.data, buf.ctypes.data, None) != 0
     assert cs.lib.stereo_synth_pair(0, 50, 50, 7, 1, 10, 1, buf.ctypes.data, buf.ctypes.data, None) != 0
+
+
+def test_cpp_driver_layer_builds_and_reports_errors(tmp_path):
+    """The C++ driver header compiles on its own, and the host program built on it
+    prints its usage and turns a missing device into a CException exit."""
+    import subprocess
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "clutec/stereo.hpp"\nint main(){ Clu::Cuda::BlockMatcherNCC::SParameter p; '
+                   'p.Set(Clu::Cuda::_SDisparityConfig{}, Clu::Cuda::BlockMatcherNCC::EConfig::Patch_9x9); return 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    exe = os.path.join(os.path.dirname(cs.__file__), "lib", "stereo_run")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    ctx_ok = True
+    try:
+        cs.Context(0).close()
+    except cs.StereoError:
+        ctx_ok = False
+    if not ctx_ok:
+        r = subprocess.run([exe, "census", "64", "32", "16", "1", str(tmp_path / "o")], capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode == 1 and "error 5" in r.stderr
