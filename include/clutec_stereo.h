@@ -203,6 +203,114 @@ stereo_status stereo_crosscheck_process(stereo_driver* drv, stereo_image* result
 /* DispMin / DispMax (CrossCheckDisparityMaps.h:96-104); waits for the last process. */
 stereo_status stereo_crosscheck_range(stereo_driver* drv, uint16_t* disp_min, uint16_t* disp_max);
 
+/* ---- batched matcher: many pairs of uneven size in one persistent launch ---
+ *
+ * The reference processes one pair per Process call (BlockMatcherSAD1.h:121-125,
+ * one CDriver per image format, KernelDriverBase.h:47-177).  A batch driver is
+ * configured once for a list of pair formats and disparity counts; each process
+ * call runs the census / padding of every pair, one persistent matching launch
+ * over the work items of all pairs, and one merge-and-check launch, and on the
+ * second call with the same images replays them as a captured HIP graph.  Every
+ * pair's maps equal what stereo_matcher_process gives for that pair alone. */
+
+typedef struct stereo_batch_tuning {
+    int32_t chunk;       /* disparities per work item (multiple of 8); 0 = autotuned or default */
+    int32_t grid;        /* workgroups of the persistent launch; 0 = 2 per CU */
+    int32_t order;       /* 0: items longest first, 1: raster order */
+    int32_t use_graph;   /* 1: capture the launch sequence and replay it */
+} stereo_batch_tuning;
+
+/* params: algorithm SAD or CENSUS, windows and tolerance as for the matcher;
+ * params->num_disparities is ignored, num_disparities[i] gives pair i's D. */
+stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** drv, int32_t npairs,
+                                     const stereo_image_format* fmts, const int32_t* num_disparities,
+                                     const stereo_matcher_params* params, const stereo_batch_tuning* tuning);
+/* disp[i], left[i], right[i] for i < npairs (disp LUM_I16); disp_lr / disp_rl
+ * may be NULL or arrays of npairs (entries with data == NULL are skipped). */
+stereo_status stereo_batch_process(stereo_driver* drv, stereo_image* disp, stereo_image* disp_lr,
+                                   stereo_image* disp_rl, const stereo_image* left, const stereo_image* right,
+                                   stereo_stream stream);
+/* Times every candidate tuning on the given images and keeps the fastest; returns
+ * it in *best (may be NULL).  Blocks. */
+stereo_status stereo_batch_autotune(stereo_driver* drv, const stereo_image* left, const stereo_image* right,
+                                    stereo_batch_tuning* best);
+stereo_status stereo_batch_get_tuning(const stereo_driver* drv, stereo_batch_tuning* tuning);
+
+/* ---- learned disparity proposal ----------------------------------------------
+ *
+ * No counterpart in the reference; it takes the place of the initial disparity map
+ * of MatcherUseDisparity (BlockMatcherNCC.cu:894-1255, BlockMatcherAW1.cu:923-1263),
+ * which searches a band around a given disparity.  A 5-layer 3x3 convolutional
+ * network (BF16 MFMA, FP32 accumulation) sees the pair at 1/4 resolution -- the left
+ * image and the right image warped by a census match at that resolution -- and
+ * proposes a centre and a confidence per quarter pixel.  Parameters: 28930 FP32
+ * values, layer by layer: weights [cout][3][3][cin] then bias [cout], for
+ * (cin, cout, dilation) = (2,32,1) (32,32,2) (32,32,4) (32,32,2) (32,2,1). */
+
+#define STEREO_CNN_PARAMS 28930
+
+typedef struct stereo_cnn stereo_cnn;
+
+/* A network with buffers for up to max_frames inputs of W4 x H4. */
+stereo_status stereo_cnn_create(stereo_context* ctx, stereo_cnn** cnn, int32_t W4, int32_t H4, int32_t max_frames);
+void stereo_cnn_destroy(stereo_cnn* cnn);
+stereo_status stereo_cnn_set_params(stereo_cnn* cnn, const float* host, size_t n); /* resets Adam */
+stereo_status stereo_cnn_get_params(stereo_cnn* cnn, float* host, size_t n);
+/* Gradients of the loss of the last train step, same layout as the parameters. */
+stereo_status stereo_cnn_get_grads(stereo_cnn* cnn, float* host, size_t n);
+/* Input slot `index`: quarter-resolution L4, R4 (LUM_U8, W4 x H4) and the coarse
+ * disparity (LUM_I16, -1 invalid) that warps R4. */
+stereo_status stereo_cnn_set_input(stereo_cnn* cnn, int32_t index, const stereo_image* L4, const stereo_image* R4,
+                                   const stereo_image* coarse, stereo_stream stream);
+/* Labels of slot `index` from a full-resolution checked map (LUM_I16, 4 W4 x 4 H4). */
+stereo_status stereo_cnn_set_labels(stereo_cnn* cnn, int32_t index, const stereo_image* disp, stereo_stream stream);
+/* Forward over slots [0, n): out (device, may be NULL) receives n x H4 x W4 x 2 floats
+ * (centre residual in quarter pixels, confidence logit). */
+stereo_status stereo_cnn_forward(stereo_cnn* cnn, int32_t n, float* out, stereo_stream stream);
+/* One Adam step (lr, beta1 0.9, beta2 0.999, eps 1e-8) on slots [0, n).  loss
+ * (host, may be NULL) receives {mean Huber, mean BCE, labelled pixels, fraction of
+ * labelled pixels within tau}; waits for the step when loss is given. */
+stereo_status stereo_cnn_train_step(stereo_cnn* cnn, int32_t n, float lr, float* loss, stereo_stream stream);
+
+/* Kernel-level entry points of the network's convolutions (device pointers, NHWC
+ * bf16 activations with cin, cout in {16, 32}).  y = conv(x, w) + bias with
+ * w [cout][9][cin]; relu != 0 writes bf16 max(y, 0), else FP32 y. */
+stereo_status stereo_conv3x3_bf16(stereo_context* ctx, const uint16_t* x, int32_t N, int32_t H, int32_t W,
+                                  int32_t cin, const uint16_t* w, const float* bias, int32_t cout, int32_t dil,
+                                  int32_t relu, void* y, stereo_stream stream);
+/* Gradients for upstream dy [N][H][W][cout] (bf16): dx [N][H][W][cin] (FP32, may be
+ * NULL) from wt = w transposed to [cin][9][cout]; dwdb = cout*9*cin weight
+ * gradients then cout bias gradients (FP32 device buffer). */
+stereo_status stereo_conv3x3_bwd_bf16(stereo_context* ctx, const uint16_t* x, int32_t N, int32_t H, int32_t W,
+                                      int32_t cin, const uint16_t* wt, const uint16_t* dy, int32_t cout, int32_t dil,
+                                      float* dx, float* dwdb, stereo_stream stream);
+
+typedef struct stereo_learned_params {
+    stereo_matcher_params matcher; /* full-resolution census matcher (algorithm CENSUS) */
+    int32_t coarse_disparities;    /* D of the quarter-resolution matcher (default D/4) */
+    int32_t band;                  /* width of the searched band around the centre (32) */
+    int32_t chunk;                 /* disparities per work item (32) */
+    float logit_thresh;            /* tiles with a smaller confidence logit search the full range */
+    int32_t reserved[4];
+} stereo_learned_params;
+
+/* The learned-proposal census matcher for LUM_U8 pairs of fmt (width and height
+ * multiples of 4); cnn_params = STEREO_CNN_PARAMS trained values. */
+stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** drv, const stereo_image_format* fmt,
+                                       const stereo_learned_params* params, const float* cnn_params, size_t n);
+/* Outputs as stereo_matcher_process. */
+stereo_status stereo_learned_process(stereo_driver* drv, stereo_image* disp, stereo_image* disp_lr,
+                                     stereo_image* disp_rl, const stereo_image* left, const stereo_image* right,
+                                     stereo_stream stream);
+/* Of the last process call (waits for it): disparity evaluations searched at full
+ * resolution, tiles, tiles that fell back to the full range, and with timing on
+ * the stage times in ms {census, coarse match, network, band match, merge/check}. */
+stereo_status stereo_learned_stats(stereo_driver* drv, int64_t* evals, int32_t* tiles, int32_t* fallback_tiles,
+                                   double* stage_ms, int32_t n_stages);
+/* Disparity range [lo, hi) searched by each 64 x 56 tile in the last process call,
+ * row-major tiles, lohi[2 t], lohi[2 t + 1]; n >= 2 * tiles.  Waits for the call. */
+stereo_status stereo_learned_tile_ranges(stereo_driver* drv, int32_t* lohi, int32_t n);
+
 /* ---- synthetic workloads (host only, no device needed) ------------------- */
 
 /* Seeded stereo pair: L, R (W*H bytes, pitch W) and optional ground truth

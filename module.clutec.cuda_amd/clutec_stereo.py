@@ -77,6 +77,23 @@ class CrossCheckParams(C.Structure):
                 ("disp_offset", C.c_int32), ("reserved", C.c_int32)]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("chunk", C.c_int32), ("grid", C.c_int32), ("order", C.c_int32), ("use_graph", C.c_int32)]
+
+    def as_dict(self):
+        return {"chunk": self.chunk, "grid": self.grid, "order": self.order, "use_graph": self.use_graph}
+
+
+class LearnedParams(C.Structure):
+    _fields_ = [("matcher", MatcherParams), ("coarse_disparities", C.c_int32), ("band", C.c_int32),
+                ("chunk", C.c_int32), ("logit_thresh", C.c_float), ("reserved", C.c_int32 * 4)]
+
+
+CNN_PARAMS = 28930
+# (cin, cout, dilation) of the proposal network's layers
+CNN_LAYERS = [(2, 32, 1), (32, 32, 2), (32, 32, 4), (32, 32, 2), (32, 2, 1)]
+
+
 # every exported entry point: (name, restype, argtypes)
 _P = C.c_void_p
 _PROTOS = [
@@ -106,6 +123,32 @@ _PROTOS = [
     ("stereo_crosscheck_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(CrossCheckParams)]),
     ("stereo_crosscheck_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_crosscheck_range", C.c_int, [_P, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16)]),
+    ("stereo_batch_configure", C.c_int, [_P, C.POINTER(_P), C.c_int32, C.POINTER(ImageFormat), C.POINTER(C.c_int32),
+                                          C.POINTER(MatcherParams), C.POINTER(BatchTuning)]),
+    ("stereo_batch_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), C.POINTER(Image),
+                                        C.POINTER(Image), _P]),
+    ("stereo_batch_autotune", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(BatchTuning)]),
+    ("stereo_batch_get_tuning", C.c_int, [_P, C.POINTER(BatchTuning)]),
+    ("stereo_cnn_create", C.c_int, [_P, C.POINTER(_P), C.c_int32, C.c_int32, C.c_int32]),
+    ("stereo_cnn_destroy", None, [_P]),
+    ("stereo_cnn_set_params", C.c_int, [_P, _P, C.c_size_t]),
+    ("stereo_cnn_get_params", C.c_int, [_P, _P, C.c_size_t]),
+    ("stereo_cnn_get_grads", C.c_int, [_P, _P, C.c_size_t]),
+    ("stereo_cnn_set_input", C.c_int, [_P, C.c_int32, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
+    ("stereo_cnn_set_labels", C.c_int, [_P, C.c_int32, C.POINTER(Image), _P]),
+    ("stereo_cnn_forward", C.c_int, [_P, C.c_int32, _P, _P]),
+    ("stereo_cnn_train_step", C.c_int, [_P, C.c_int32, C.c_float, _P, _P]),
+    ("stereo_conv3x3_bf16", C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_int32,
+                                       C.c_int32, C.c_int32, _P, _P]),
+    ("stereo_conv3x3_bwd_bf16", C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_int32,
+                                           C.c_int32, _P, _P, _P]),
+    ("stereo_learned_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(LearnedParams), _P,
+                                            C.c_size_t]),
+    ("stereo_learned_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), C.POINTER(Image),
+                                          C.POINTER(Image), _P]),
+    ("stereo_learned_stats", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_double), C.c_int32]),
+    ("stereo_learned_tile_ranges", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
     ("stereo_synth_pair", C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     _P, _P, _P]),
 ]
@@ -339,4 +382,180 @@ class CrossCheck(_Driver):
         drl = DeviceImage.from_array(self.ctx, rl, PIXEL_RGBA_U16)
         out = DeviceImage(self.ctx, self.W, self.H, PIXEL_RGBA_U16)
         self.process(out, dlr, drl)
+        return out.download()
+
+
+class BatchMatcher(_Driver):
+    """Many pairs of uneven size in one persistent launch (stereo_batch_*)."""
+
+    def __init__(self, ctx, sizes, num_disps, algorithm=ALGO_CENSUS, window=(9, 9), census=(9, 7), tol=1,
+                 tuning=None):
+        super().__init__(ctx)
+        n = len(sizes)
+        self.sizes = list(sizes)
+        fmts = (ImageFormat * n)(*[ImageFormat(w, h, PIXEL_LUM_U8) for (w, h) in sizes])
+        ds = (C.c_int32 * n)(*num_disps)
+        self.params = matcher_params(algorithm, window, census, 8, tol)
+        t = None
+        if tuning is not None:
+            t = BatchTuning(tuning.get("chunk", 0), tuning.get("grid", 0), tuning.get("order", 0),
+                            tuning.get("use_graph", 1))
+        _check(lib.stereo_batch_configure(ctx.handle, C.byref(self.handle), n, fmts, ds, C.byref(self.params),
+                                          C.byref(t) if t is not None else None), ctx)
+        self.dev = None
+
+    def _images(self, pairs):
+        n = len(self.sizes)
+        self.left = [DeviceImage.from_array(self.ctx, L, PIXEL_LUM_U8) for (L, R) in pairs]
+        self.right = [DeviceImage.from_array(self.ctx, R, PIXEL_LUM_U8) for (L, R) in pairs]
+        self.outs = [DeviceImage(self.ctx, w, h, PIXEL_LUM_I16) for (w, h) in self.sizes]
+        self.lrs = [DeviceImage(self.ctx, w, h, PIXEL_LUM_I16) for (w, h) in self.sizes]
+        self.rls = [DeviceImage(self.ctx, w, h, PIXEL_LUM_I16) for (w, h) in self.sizes]
+        arr = lambda imgs: (Image * n)(*[i.img for i in imgs])
+        self.dev = (arr(self.outs), arr(self.lrs), arr(self.rls), arr(self.left), arr(self.right))
+
+    def upload(self, pairs):
+        self._images(pairs)
+
+    def refill(self, pairs):
+        """New content in the same device images (the captured graph stays valid)."""
+        for (L, R), dl, dr in zip(pairs, self.left, self.right):
+            dl.upload(L)
+            dr.upload(R)
+
+    def process(self, stream=None):
+        o, a, b, l, r = self.dev
+        _check(lib.stereo_batch_process(self.handle, o, a, b, l, r, stream), self.ctx)
+
+    def autotune(self):
+        t = BatchTuning()
+        _check(lib.stereo_batch_autotune(self.handle, self.dev[3], self.dev[4], C.byref(t)), self.ctx)
+        return t.as_dict()
+
+    def tuning(self):
+        t = BatchTuning()
+        _check(lib.stereo_batch_get_tuning(self.handle, C.byref(t)), self.ctx)
+        return t.as_dict()
+
+    def results(self):
+        return [(o.download(), a.download(), b.download()) for o, a, b in zip(self.outs, self.lrs, self.rls)]
+
+    def run(self, pairs):
+        self.upload(pairs)
+        self.process()
+        return self.results()
+
+
+class ProposalNet:
+    """The learned proposal network (stereo_cnn_*): parameters, training, forward."""
+
+    def __init__(self, ctx, W4, H4, max_frames=1):
+        self.ctx = ctx
+        self.W4, self.H4, self.maxn = W4, H4, max_frames
+        self.handle = C.c_void_p()
+        _check(lib.stereo_cnn_create(ctx.handle, C.byref(self.handle), W4, H4, max_frames), ctx)
+
+    def close(self):
+        if self.handle:
+            lib.stereo_cnn_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib.stereo_cnn_set_params(self.handle, p.ctypes.data, p.size), self.ctx)
+
+    def params(self):
+        p = np.empty(CNN_PARAMS, np.float32)
+        _check(lib.stereo_cnn_get_params(self.handle, p.ctypes.data, p.size), self.ctx)
+        return p
+
+    def grads(self):
+        g = np.empty(CNN_PARAMS, np.float32)
+        _check(lib.stereo_cnn_get_grads(self.handle, g.ctypes.data, g.size), self.ctx)
+        return g
+
+    def set_input(self, index, L4, R4, coarse, stream=None):
+        _check(lib.stereo_cnn_set_input(self.handle, index, L4.ref, R4.ref, coarse.ref, stream), self.ctx)
+
+    def set_labels(self, index, disp, stream=None):
+        _check(lib.stereo_cnn_set_labels(self.handle, index, disp.ref, stream), self.ctx)
+
+    def forward_ptr(self, n, out_ptr, stream=None):
+        _check(lib.stereo_cnn_forward(self.handle, n, out_ptr, stream), self.ctx)
+
+    def train_step(self, n, lr):
+        loss = np.zeros(4, np.float32)
+        _check(lib.stereo_cnn_train_step(self.handle, n, lr, loss.ctypes.data, None), self.ctx)
+        return loss
+
+
+def cnn_init_params(seed=0):
+    """He-normal weights, zero biases, in the public layer-by-layer layout."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for cin, cout, _ in CNN_LAYERS:
+        std = np.sqrt(2.0 / (9 * cin))
+        out.append((rng.standard_normal((cout, 3, 3, cin)) * std).astype(np.float32).ravel())
+        out.append(np.zeros(cout, np.float32))
+    p = np.concatenate(out)
+    assert p.size == CNN_PARAMS
+    return p
+
+
+class LearnedMatcher(_Driver):
+    """Census matcher with the learned band proposal (stereo_learned_*)."""
+
+    def __init__(self, ctx, W, H, cnn_params, window=(9, 9), census=(9, 7), num_disp=256, tol=1, band=32, chunk=32,
+                 logit_thresh=0.0, coarse_disp=0):
+        super().__init__(ctx)
+        self.W, self.H = W, H
+        fmt = ImageFormat(W, H, PIXEL_LUM_U8)
+        lp = LearnedParams()
+        lp.matcher = matcher_params(ALGO_CENSUS, window, census, num_disp, tol)
+        lp.coarse_disparities = coarse_disp
+        lp.band = band
+        lp.chunk = chunk
+        lp.logit_thresh = logit_thresh
+        p = np.ascontiguousarray(cnn_params, np.float32)
+        _check(lib.stereo_learned_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(lp),
+                                            p.ctypes.data, p.size), ctx)
+
+    def process(self, disp, disp_lr, disp_rl, left, right, stream=None):
+        _check(lib.stereo_learned_process(self.handle, disp.ref if disp else None, disp_lr.ref if disp_lr else None,
+                                          disp_rl.ref if disp_rl else None, left.ref, right.ref, stream), self.ctx)
+
+    def stats(self, timed=False):
+        ev = C.c_int64()
+        nt = C.c_int32()
+        fb = C.c_int32()
+        st = (C.c_double * 5)()
+        _check(lib.stereo_learned_stats(self.handle, C.byref(ev), C.byref(nt), C.byref(fb), st if timed else None,
+                                        5 if timed else 0), self.ctx)
+        d = {"evals": ev.value, "tiles": nt.value, "fallback_tiles": fb.value}
+        if timed:
+            d["stage_ms"] = list(st)
+        return d
+
+    def tile_ranges(self):
+        ntx = (self.W + 63) // 64
+        nty = (self.H + 55) // 56
+        buf = (C.c_int32 * (2 * ntx * nty))()
+        _check(lib.stereo_learned_tile_ranges(self.handle, buf, len(buf)), self.ctx)
+        return np.array(buf, np.int32).reshape(nty, ntx, 2)
+
+    def run(self, L, R, maps=False):
+        dl = DeviceImage.from_array(self.ctx, L, PIXEL_LUM_U8)
+        dr = DeviceImage.from_array(self.ctx, R, PIXEL_LUM_U8)
+        out = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16)
+        lr = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16) if maps else None
+        rl = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16) if maps else None
+        self.process(out, lr, rl, dl, dr)
+        if maps:
+            return out.download(), lr.download(), rl.download()
         return out.download()

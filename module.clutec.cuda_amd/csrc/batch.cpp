@@ -1,0 +1,438 @@
+// C ABI of the batched matcher: many stereo pairs of uneven size and disparity range
+// in one persistent work-list launch, captured into a HIP graph and replayed.
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "capi_internal.hpp"
+
+using namespace capi;
+
+namespace {
+
+constexpr int kMinChunk = 32;  // smallest work item the workspace is sized for
+
+struct BatchExt : DriverExt {
+    stereo_context* ctx = nullptr;
+    int npairs = 0;
+    stereo_matcher_params mp{};
+    int kind = 0;
+    size_t elem = 1;
+    stereo_batch_tuning tuning{};
+    std::vector<uint8_t*> padL, padR;
+    std::vector<stereo::BmFrame> frames;  // host copy of the device frame table
+    stereo::BmFrame* frames_dev = nullptr;
+    stereo::BmTile* tiles_dev = nullptr;
+    stereo::BmItem* items_dev = nullptr;
+    int* sched_dev = nullptr;
+    int* nitems_dev = nullptr;
+    int ntiles = 0, nitems = 0, max_items = 0;
+    uint32_t* lkey = nullptr;
+    uint32_t* rpart = nullptr;
+    size_t rpart_elems = 0;
+    int total_rows = 0, max_width = 0;
+    hipStream_t cap = nullptr;  // capture stream
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void*> graph_key;
+
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_key.clear();
+    }
+
+    ~BatchExt() override
+    {
+        drop_graph();
+        if (cap) (void)hipStreamDestroy(cap);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        for (uint8_t* p : padL)
+            if (p) (void)hipFree(p);
+        for (uint8_t* p : padR)
+            if (p) (void)hipFree(p);
+        for (void* p : {(void*)frames_dev, (void*)tiles_dev, (void*)items_dev, (void*)sched_dev, (void*)nitems_dev,
+                        (void*)lkey, (void*)rpart})
+            if (p) (void)hipFree(p);
+    }
+};
+
+size_t rpart_need(const std::vector<stereo::BmFrame>& frames, int chunk)
+{
+    size_t n = 0;
+    for (const stereo::BmFrame& f : frames)
+        n += (size_t)f.ntx * f.nty * ((f.D + chunk - 1) / chunk) * stereo::BM_TILE_H * stereo::bm_list_rp_stride(chunk);
+    return n;
+}
+
+// Items of every tile of every pair, tile by tile (the slots the merge reads), the
+// tile table, and the schedule: 0 = longest items of the largest frames first,
+// 1 = table order.
+stereo_status upload_work(BatchExt* x)
+{
+    const int chunk = x->tuning.chunk;
+    std::vector<stereo::BmItem> items;
+    std::vector<stereo::BmTile> tiles;
+    for (int f = 0; f < x->npairs; ++f) {
+        const stereo::BmFrame& fr = x->frames[f];
+        for (int ty = 0; ty < fr.nty; ++ty)
+            for (int tx = 0; tx < fr.ntx; ++tx) {
+                stereo::BmTile t;
+                t.lo = 0;
+                t.hi = fr.D;
+                t.first_item = (int)items.size();
+                t.nitems = 0;
+                for (int ds = 0; ds < fr.D; ds += chunk) {
+                    stereo::BmItem it;
+                    it.frame = f;
+                    it.tx = tx;
+                    it.ty = ty;
+                    it.ds = ds;
+                    it.de = std::min(fr.D, ds + chunk);
+                    items.push_back(it);
+                    ++t.nitems;
+                }
+                tiles.push_back(t);
+            }
+    }
+    const int n = (int)items.size();
+    if (n > x->max_items || rpart_need(x->frames, chunk) > x->rpart_elems)
+        return fail(x->ctx, STEREO_ERROR_INVALID_CONFIG, "batch: chunk %d needs more workspace than configured", chunk);
+    std::vector<int> sched(n);
+    std::iota(sched.begin(), sched.end(), 0);
+    if (x->tuning.order == 0)
+        std::stable_sort(sched.begin(), sched.end(), [&](int a, int b) {
+            const int la = items[a].de - items[a].ds, lb = items[b].de - items[b].ds;
+            if (la != lb) return la > lb;
+            const stereo::BmFrame& fa = x->frames[items[a].frame];
+            const stereo::BmFrame& fb = x->frames[items[b].frame];
+            return (long)fa.W * fa.H > (long)fb.W * fb.H;
+        });
+    x->nitems = n;
+    x->ntiles = (int)tiles.size();
+    HIP_TRY(x->ctx, hipMemcpy(x->items_dev, items.data(), n * sizeof(stereo::BmItem), hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(x->ctx, hipMemcpy(x->sched_dev, sched.data(), n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(x->ctx, hipMemcpy(x->tiles_dev, tiles.data(), tiles.size() * sizeof(stereo::BmTile), hipMemcpyHostToDevice),
+            "hipMemcpy");
+    HIP_TRY(x->ctx, hipMemcpy(x->nitems_dev, &n, sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    x->drop_graph();
+    return STEREO_OK;
+}
+
+stereo_status enqueue(stereo_driver* d, BatchExt* x, const stereo_image* left, const stereo_image* right, hipStream_t s)
+{
+    stereo_context* ctx = d->ctx;
+    for (int i = 0; i < x->npairs; ++i) {
+        const stereo::BmFrame& f = x->frames[i];
+        if (x->kind == stereo::BM_COST_CENSUS) {
+            const View vl = view_of(&left[i]), vr = view_of(&right[i]);
+            HIP_TRY(ctx, stereo::launch_census_pair(vl, f.L, &vr, &f.R, x->mp.census_w, x->mp.census_h, s), "census launch");
+        } else {
+            HIP_TRY(ctx,
+                    hipMemcpy2DAsync(f.L.ptr, f.L.pitch, left[i].data, left[i].pitch, f.W, f.H, hipMemcpyDeviceToDevice, s),
+                    "hipMemcpy2DAsync");
+            HIP_TRY(ctx,
+                    hipMemcpy2DAsync(f.R.ptr, f.R.pitch, right[i].data, right[i].pitch, f.W, f.H, hipMemcpyDeviceToDevice, s),
+                    "hipMemcpy2DAsync");
+        }
+    }
+    stereo::BmListParams lp;
+    lp.frames = x->frames_dev;
+    lp.tiles = x->tiles_dev;
+    lp.items = x->items_dev;
+    lp.nitems = x->nitems_dev;
+    lp.sched = x->sched_dev;
+    lp.lkey = x->lkey;
+    lp.rpart = x->rpart;
+    lp.rp_stride = stereo::bm_list_rp_stride(x->tuning.chunk);
+    lp.nframes = x->npairs;
+    lp.total_rows = x->total_rows;
+    const int grid = std::max(1, std::min(x->tuning.grid, x->nitems));
+    HIP_TRY(ctx, stereo::launch_bm_list(x->kind, x->mp.window_w, lp, grid, s), "list matcher launch");
+    HIP_TRY(ctx, stereo::launch_bm_list_finish(lp, x->max_width, s), "list finish launch");
+    return STEREO_OK;
+}
+
+// Outputs are part of the frame table; returns true when it changed.
+bool set_outputs(BatchExt* x, stereo_image* disp, stereo_image* disp_lr, stereo_image* disp_rl)
+{
+    bool changed = false;
+    for (int i = 0; i < x->npairs; ++i) {
+        stereo::BmFrame& f = x->frames[i];
+        const View a = view_of(&disp[i]);
+        const View b = (disp_lr && disp_lr[i].data) ? view_of(&disp_lr[i]) : null_view();
+        const View c = (disp_rl && disp_rl[i].data) ? view_of(&disp_rl[i]) : null_view();
+        if (f.disp.ptr != a.ptr || f.disp.pitch != a.pitch || f.disp_l.ptr != b.ptr || f.disp_l.pitch != b.pitch ||
+            f.disp_r.ptr != c.ptr || f.disp_r.pitch != c.pitch) {
+            f.disp = a;
+            f.disp_l = b;
+            f.disp_r = c;
+            changed = true;
+        }
+    }
+    return changed;
+}
+
+}  // namespace
+
+extern "C" {
+
+stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** out, int32_t npairs,
+                                     const stereo_image_format* fmts, const int32_t* num_disparities,
+                                     const stereo_matcher_params* params, const stereo_batch_tuning* tuning)
+{
+    if (!ctx || !out || !fmts || !num_disparities || !params || npairs < 1 || npairs > 4096)
+        return STEREO_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (params->algorithm != STEREO_ALGO_SAD && params->algorithm != STEREO_ALGO_CENSUS)
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "batch: SAD or census matcher only");
+    stereo_batch_tuning tn{64, 512, 0, 1};
+    if (tuning) {
+        tn = *tuning;
+        if (tn.chunk == 0) tn.chunk = 64;
+        if (tn.grid == 0) tn.grid = 512;
+    }
+    if (tn.chunk < kMinChunk || tn.chunk > 256 || tn.chunk % 8 || tn.grid < 1 || tn.grid > 65535 || tn.order < 0 ||
+        tn.order > 1)
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "batch tuning: chunk %d (32..256, multiple of 8), grid %d, order %d",
+                    tn.chunk, tn.grid, tn.order);
+    char msg[256];
+    for (int i = 0; i < npairs; ++i) {
+        stereo_matcher_params p = *params;
+        p.num_disparities = num_disparities[i];
+        stereo_status st = stereo_matcher_validate(&fmts[i], &p, msg, sizeof(msg));
+        if (st) return fail(ctx, st, "pair %d: %s", i, msg);
+    }
+    stereo_status st = set_device(ctx);
+    if (st) return st;
+    stereo_driver* d = new stereo_driver(stereo_driver::BATCH);
+    d->ctx = ctx;
+    d->mp = *params;
+    d->fmt = fmts[0];
+    d->name = params->algorithm == STEREO_ALGO_CENSUS ? "Batched Block Matcher Census" : "Batched Block Matcher SAD";
+    BatchExt* x = new BatchExt();
+    d->ext = x;
+    x->ctx = ctx;
+    x->npairs = npairs;
+    x->mp = *params;
+    x->kind = params->algorithm == STEREO_ALGO_CENSUS ? stereo::BM_COST_CENSUS : stereo::BM_COST_SAD;
+    x->elem = x->kind == stereo::BM_COST_CENSUS ? 8 : 1;
+    x->tuning = tn;
+    auto cleanup = [&](stereo_status s2) {
+        driver_release(d);
+        delete d;
+        return s2;
+    };
+    if ((st = driver_events(d))) return cleanup(st);
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&x->cap, hipStreamNonBlocking)) || (e = hipEventCreate(&x->ev_fork)) ||
+        (e = hipEventCreate(&x->ev_join)))
+        return cleanup(hip_fail(ctx, e, "hipStreamCreate"));
+    d->bx = params->window_w / 2 + (x->kind == stereo::BM_COST_CENSUS ? params->census_w / 2 : 0);
+    d->by = params->window_h / 2 + (x->kind == stereo::BM_COST_CENSUS ? params->census_h / 2 : 0);
+    x->padL.assign(npairs, nullptr);
+    x->padR.assign(npairs, nullptr);
+    int tiles = 0, rows = 0;
+    size_t max_items = 0;
+    for (int i = 0; i < npairs; ++i) {
+        const int W = fmts[i].width, H = fmts[i].height;
+        const size_t cols = (size_t)stereo::BM_PAD_LEFT + W + stereo::BM_PAD_RIGHT;
+        const size_t pitch = (cols * x->elem + 255) & ~(size_t)255;
+        if ((e = hipMalloc((void**)&x->padL[i], pitch * H)) || (e = hipMalloc((void**)&x->padR[i], pitch * H)) ||
+            (e = hipMemset(x->padL[i], 0, pitch * H)) || (e = hipMemset(x->padR[i], 0, pitch * H)))
+            return cleanup(hip_fail(ctx, e, "hipMalloc"));
+        stereo::BmFrame f;
+        std::memset(&f, 0, sizeof(f));
+        f.L.ptr = x->padL[i] + (size_t)stereo::BM_PAD_LEFT * x->elem;
+        f.R.ptr = x->padR[i] + (size_t)stereo::BM_PAD_LEFT * x->elem;
+        f.L.pitch = f.R.pitch = pitch;
+        f.L.w = f.R.w = W;
+        f.L.h = f.R.h = H;
+        f.W = W;
+        f.H = H;
+        f.D = num_disparities[i];
+        f.bx = d->bx;
+        f.by = d->by;
+        f.tol = params->lr_tolerance;
+        f.ntx = (W + stereo::BM_TILE_W - 1) / stereo::BM_TILE_W;
+        f.nty = (H + stereo::BM_TILE_H - 1) / stereo::BM_TILE_H;
+        f.tile_base = tiles;
+        f.row_base = rows;
+        tiles += f.ntx * f.nty;
+        rows += H;
+        max_items += (size_t)f.ntx * f.nty * ((f.D + kMinChunk - 1) / kMinChunk);
+        x->max_width = std::max(x->max_width, W);
+        x->frames.push_back(f);
+    }
+    if (x->max_width > stereo::BM_MAX_WIDTH) return cleanup(fail(ctx, STEREO_ERROR_INVALID_FORMAT, "batch: width above %d",
+                                                                 stereo::BM_MAX_WIDTH));
+    x->total_rows = rows;
+    x->max_items = (int)max_items;
+    for (int c = kMinChunk; c <= 256; c += 8) x->rpart_elems = std::max(x->rpart_elems, rpart_need(x->frames, c));
+    if ((e = hipMalloc((void**)&x->frames_dev, npairs * sizeof(stereo::BmFrame))) ||
+        (e = hipMalloc((void**)&x->tiles_dev, tiles * sizeof(stereo::BmTile))) ||
+        (e = hipMalloc((void**)&x->items_dev, max_items * sizeof(stereo::BmItem))) ||
+        (e = hipMalloc((void**)&x->sched_dev, max_items * sizeof(int))) ||
+        (e = hipMalloc((void**)&x->nitems_dev, sizeof(int))) ||
+        (e = hipMalloc((void**)&x->lkey, stereo::bm_list_lkey_elems((int)max_items) * 4)) ||
+        (e = hipMalloc((void**)&x->rpart, x->rpart_elems * 4)))
+        return cleanup(hip_fail(ctx, e, "hipMalloc"));
+    if ((e = hipMemcpy(x->frames_dev, x->frames.data(), npairs * sizeof(stereo::BmFrame), hipMemcpyHostToDevice)))
+        return cleanup(hip_fail(ctx, e, "hipMemcpy"));
+    if ((st = upload_work(x))) return cleanup(st);
+    *out = d;
+    return STEREO_OK;
+}
+
+stereo_status stereo_batch_process(stereo_driver* d, stereo_image* disp, stereo_image* disp_lr, stereo_image* disp_rl,
+                                   const stereo_image* left, const stereo_image* right, stereo_stream stream)
+{
+    if (!d || d->kind != stereo_driver::BATCH || !disp || !left || !right) return STEREO_ERROR_INVALID_ARGUMENT;
+    BatchExt* x = static_cast<BatchExt*>(d->ext);
+    stereo_context* ctx = d->ctx;
+    stereo_status st;
+    for (int i = 0; i < x->npairs; ++i) {
+        const int W = x->frames[i].W, H = x->frames[i].H;
+        if ((st = check_image(ctx, &left[i], W, H, STEREO_PIXEL_LUM_U8, "left image"))) return st;
+        if ((st = check_image(ctx, &right[i], W, H, STEREO_PIXEL_LUM_U8, "right image"))) return st;
+        if ((st = check_image(ctx, &disp[i], W, H, STEREO_PIXEL_LUM_I16, "disparity output"))) return st;
+        if (disp_lr && disp_lr[i].data && (st = check_image(ctx, &disp_lr[i], W, H, STEREO_PIXEL_LUM_I16, "left-to-right output")))
+            return st;
+        if (disp_rl && disp_rl[i].data && (st = check_image(ctx, &disp_rl[i], W, H, STEREO_PIXEL_LUM_I16, "right-to-left output")))
+            return st;
+    }
+    if ((st = set_device(ctx))) return st;
+    GUARD(ctx, "stereo_batch_process");
+    const hipStream_t s = (hipStream_t)stream;
+    if (set_outputs(x, disp, disp_lr, disp_rl)) {
+        // the device frame table is read by earlier work still in flight: wait for it
+        HIP_TRY(ctx, hipStreamSynchronize(s), "hipStreamSynchronize");
+        HIP_TRY(ctx, hipMemcpy(x->frames_dev, x->frames.data(), x->npairs * sizeof(stereo::BmFrame), hipMemcpyHostToDevice),
+                "hipMemcpy");
+        x->drop_graph();
+    }
+    if ((st = begin(d, s))) return st;
+    if (!x->tuning.use_graph) {
+        if ((st = enqueue(d, x, left, right, s))) return st;
+        return end(d, s);
+    }
+    std::vector<const void*> key;
+    key.reserve(2 * x->npairs);
+    for (int i = 0; i < x->npairs; ++i) {
+        key.push_back(left[i].data);
+        key.push_back(right[i].data);
+    }
+    if (!x->exec || key != x->graph_key) {
+        x->drop_graph();
+        // capture on the driver's own stream (the default stream cannot be captured),
+        // then launch the graph on the caller's stream
+        HIP_TRY(ctx, hipStreamBeginCapture(x->cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        const stereo_status cst = enqueue(d, x, left, right, x->cap);
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(x->cap, &g);
+        if (cst) {
+            if (g) (void)hipGraphDestroy(g);
+            return cst;
+        }
+        HIP_TRY(ctx, ee, "hipStreamEndCapture");
+        x->graph = g;
+        HIP_TRY(ctx, hipGraphInstantiate(&x->exec, x->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        x->graph_key = key;
+    }
+    HIP_TRY(ctx, hipGraphLaunch(x->exec, s), "hipGraphLaunch");
+    return end(d, s);
+}
+
+stereo_status stereo_batch_get_tuning(const stereo_driver* d, stereo_batch_tuning* t)
+{
+    if (!d || d->kind != stereo_driver::BATCH || !t) return STEREO_ERROR_INVALID_ARGUMENT;
+    *t = static_cast<const BatchExt*>(d->ext)->tuning;
+    return STEREO_OK;
+}
+
+stereo_status stereo_batch_autotune(stereo_driver* d, const stereo_image* left, const stereo_image* right,
+                                    stereo_batch_tuning* best)
+{
+    if (!d || d->kind != stereo_driver::BATCH || !left || !right) return STEREO_ERROR_INVALID_ARGUMENT;
+    BatchExt* x = static_cast<BatchExt*>(d->ext);
+    stereo_context* ctx = d->ctx;
+    stereo_status st;
+    for (int i = 0; i < x->npairs; ++i) {
+        if ((st = check_image(ctx, &left[i], x->frames[i].W, x->frames[i].H, STEREO_PIXEL_LUM_U8, "left image"))) return st;
+        if ((st = check_image(ctx, &right[i], x->frames[i].W, x->frames[i].H, STEREO_PIXEL_LUM_U8, "right image"))) return st;
+    }
+    if ((st = set_device(ctx))) return st;
+    // the timed runs need no outputs beyond what the frame table holds; point every
+    // frame at a scratch map so that no caller image is written
+    std::vector<stereo::BmFrame> saved = x->frames;
+    int16_t* scratch = nullptr;
+    size_t spitch = ((size_t)x->max_width * 2 + 255) & ~(size_t)255;
+    int maxh = 0;
+    for (const stereo::BmFrame& f : x->frames) maxh = std::max(maxh, f.H);
+    HIP_TRY(ctx, hipMalloc((void**)&scratch, spitch * maxh), "hipMalloc");
+    for (stereo::BmFrame& f : x->frames) {
+        f.disp.ptr = reinterpret_cast<uint8_t*>(scratch);
+        f.disp.pitch = spitch;
+        f.disp.w = f.W;
+        f.disp.h = f.H;
+        f.disp_l = null_view();
+        f.disp_r = null_view();
+    }
+    HIP_TRY(ctx, hipMemcpy(x->frames_dev, x->frames.data(), x->npairs * sizeof(stereo::BmFrame), hipMemcpyHostToDevice),
+            "hipMemcpy");
+    const stereo_batch_tuning keep = x->tuning;
+    stereo_batch_tuning win = keep;
+    float win_ms = 1e30f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(ctx, hipEventCreate(&e0), "hipEventCreate");
+    HIP_TRY(ctx, hipEventCreate(&e1), "hipEventCreate");
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+        cus = prop.multiProcessorCount;
+    const int chunks[] = {32, 64, 128, 256};
+    const int grids[] = {cus, 2 * cus, 3 * cus};
+    st = STEREO_OK;
+    for (int c : chunks)
+        for (int g : grids)
+            for (int o = 0; o < 2 && !st; ++o) {
+                x->tuning.chunk = c;
+                x->tuning.grid = g;
+                x->tuning.order = o;
+                x->tuning.use_graph = 0;
+                if ((st = upload_work(x))) break;
+                if ((st = enqueue(d, x, left, right, nullptr))) break;  // warm-up
+                hipError_t e = hipEventRecord(e0, nullptr);
+                for (int r = 0; r < 3 && e == hipSuccess && !st; ++r) st = enqueue(d, x, left, right, nullptr);
+                if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e != hipSuccess) st = hip_fail(ctx, e, "autotune timing");
+                if (!st && ms < win_ms) {
+                    win_ms = ms;
+                    win = x->tuning;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(scratch);
+    x->frames = saved;
+    x->tuning = st ? keep : win;
+    x->tuning.use_graph = keep.use_graph;
+    HIP_TRY(ctx, hipMemcpy(x->frames_dev, x->frames.data(), x->npairs * sizeof(stereo::BmFrame), hipMemcpyHostToDevice),
+            "hipMemcpy");
+    const stereo_status st2 = upload_work(x);
+    if (st) return st;
+    if (st2) return st2;
+    if (best) *best = x->tuning;
+    return STEREO_OK;
+}
+
+}  // extern "C"

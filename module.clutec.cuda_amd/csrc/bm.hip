@@ -70,20 +70,32 @@ struct CostSad {
     }
 };
 
+// One workgroup's work: a 64 x 56 tile of one frame and a disparity range [ds, de).
+struct Job {
+    View L, R;
+    int W, H, bx;
+    int x0, y0, ds, de;
+    uint32_t* rp;   // partial keys of tile row 0: right pixel xr at rp[row * rp_stride + xr - x0 + 255 - rp_kofs]
+    int rp_stride, rp_kofs;
+    uint32_t* lk;   // left key of tile pixel (0, 0) (row stride lk_stride)
+    int lk_stride;
+};
+
+__device__ __forceinline__ bool job_is_edge(const Job& j)
+{
+    return (j.x0 < j.bx + j.de - 1) || (j.x0 + TW > j.W - j.bx);
+}
+
 template <class Cost, int AX, int AY, bool EDGE>
-__device__ __forceinline__ void bm_tile(const BmParams& p, uint32_t (*hbuf)[HROWS * HSTRIDE],
-                                        uint32_t (*ebuf)[RG][64])
+__device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64])
 {
     static_assert(TH + 2 * AY <= HROWS, "tile too tall for the aggregation window");
     static_assert(AX <= 4, "horizontal radius limited by the run halo");
     constexpr int DC = Cost::DC;
     const int tid = threadIdx.x;
-    const int tx = blockIdx.x, ty = blockIdx.y, seg = blockIdx.z;
-    const int x0 = tx * TW, y0 = ty * TH;
+    const int x0 = p.x0, y0 = p.y0;
     const int W = p.W, H = p.H;
-    // this workgroup's disparity segment [ds, de)
-    const int ds = seg * p.seg_len;
-    const int de = min(p.D, ds + p.seg_len);
+    const int ds = p.ds, de = p.de;
 
     // row-phase identity
     const int run = tid & 3;
@@ -102,7 +114,7 @@ __device__ __forceinline__ void bm_tile(const BmParams& p, uint32_t (*hbuf)[HROW
     const int gw = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x = x0 + lane;
     const int rows_here = H - (y0 + gw * RG);  // output rows of this wave inside the image
-    uint32_t* const rp = p.rpart + (((size_t)seg * p.ntx + tx) * H + y0 + gw * RG) * BM_NPART;
+    uint32_t* const rp = p.rp + (size_t)gw * RG * p.rp_stride - p.rp_kofs;
     uint32_t* const eb = &ebuf[gw][0][0];
     const bool lane_ok = x >= p.bx && x < W - p.bx;
     const int dlim = x - p.bx;
@@ -186,7 +198,7 @@ __device__ __forceinline__ void bm_tile(const BmParams& p, uint32_t (*hbuf)[HROW
                     const int k = 318 - (base + lane);
 #pragma unroll
                     for (int i = 0; i < RG; ++i)
-                        if (i < rows_here) rp[i * BM_NPART + k] = eb[i * 64 + lane];
+                        if (i < rows_here) rp[i * p.rp_stride + k] = eb[i * 64 + lane];
                 }
             }
         }
@@ -197,28 +209,81 @@ __device__ __forceinline__ void bm_tile(const BmParams& p, uint32_t (*hbuf)[HROW
         const int k = lane + 255 - (de - 1);
 #pragma unroll
         for (int i = 0; i < RG; ++i)
-            if (i < rows_here) rp[i * BM_NPART + k] = racc[i];
+            if (i < rows_here) rp[i * p.rp_stride + k] = racc[i];
     }
     if (x < W) {
-        uint32_t* lk = p.lkey + (size_t)seg * W * H + (size_t)(y0 + gw * RG) * W + x;
+        uint32_t* lk = p.lk + (size_t)gw * RG * p.lk_stride + lane;
 #pragma unroll
         for (int i = 0; i < RG; ++i)
-            if (i < rows_here) lk[(size_t)i * W] = lacc[i];
+            if (i < rows_here) lk[(size_t)i * p.lk_stride] = lacc[i];
     }
 }
 
+template <class Cost, int AX, int AY>
+__device__ __forceinline__ void run_job(const Job& j, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64])
+{
+    if (job_is_edge(j))
+        bm_tile<Cost, AX, AY, true>(j, hbuf, ebuf);
+    else
+        bm_tile<Cost, AX, AY, false>(j, hbuf, ebuf);
+}
+
+// Grid launch over one frame: blockIdx = (tile x, tile y, disparity segment).
 template <class Cost, int AX, int AY>
 __global__ __launch_bounds__(256, 2) void bm_fused_kernel(BmParams p)
 {
     __shared__ uint32_t hbuf[2][HROWS * HSTRIDE];
     __shared__ uint32_t ebuf[4][RG][64];
-    const int x0 = blockIdx.x * TW;
-    const int de = min(p.D, ((int)blockIdx.z + 1) * p.seg_len);
-    const bool edge = (x0 < p.bx + de - 1) || (x0 + TW > p.W - p.bx);
-    if (edge)
-        bm_tile<Cost, AX, AY, true>(p, hbuf, ebuf);
-    else
-        bm_tile<Cost, AX, AY, false>(p, hbuf, ebuf);
+    const int seg = blockIdx.z;
+    Job j;
+    j.L = p.L;
+    j.R = p.R;
+    j.W = p.W;
+    j.H = p.H;
+    j.bx = p.bx;
+    j.x0 = blockIdx.x * TW;
+    j.y0 = blockIdx.y * TH;
+    j.ds = seg * p.seg_len;
+    j.de = min(p.D, j.ds + p.seg_len);
+    j.rp = p.rpart + (((size_t)seg * p.ntx + blockIdx.x) * p.H + j.y0) * BM_NPART;
+    j.rp_stride = BM_NPART;
+    j.rp_kofs = 0;
+    j.lk = p.lkey + (size_t)seg * p.W * p.H + (size_t)j.y0 * p.W + j.x0;
+    j.lk_stride = p.W;
+    run_job<Cost, AX, AY>(j, hbuf, ebuf);
+}
+
+// Persistent launch over a work list: each workgroup takes items blockIdx.x,
+// blockIdx.x + gridDim.x, ... up to the device-side count.
+template <class Cost, int AX, int AY>
+__global__ __launch_bounds__(256, 2) void bm_list_kernel(BmListParams p)
+{
+    __shared__ uint32_t hbuf[2][HROWS * HSTRIDE];
+    __shared__ uint32_t ebuf[4][RG][64];
+    const int n = *p.nitems;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const int i = p.sched ? p.sched[k] : k;
+        const BmItem it = p.items[i];
+        const BmFrame& f = p.frames[it.frame];
+        Job j;
+        j.L = f.L;
+        j.R = f.R;
+        j.W = f.W;
+        j.H = f.H;
+        j.bx = f.bx;
+        j.x0 = it.tx * TW;
+        j.y0 = it.ty * TH;
+        j.ds = it.ds;
+        j.de = it.de;
+        // the item keeps only its partial columns k in [256 - de, 318 - ds]
+        j.rp = p.rpart + (size_t)i * TH * p.rp_stride;
+        j.rp_stride = p.rp_stride;
+        j.rp_kofs = 256 - it.de;
+        j.lk = p.lkey + (size_t)i * TH * TW;
+        j.lk_stride = TW;
+        __syncthreads();  // the previous item is done with the LDS buffers
+        run_job<Cost, AX, AY>(j, hbuf, ebuf);
+    }
 }
 
 __device__ __forceinline__ uint32_t rkey_at(const BmFinishParams& f, int xr, int y)
@@ -299,7 +364,123 @@ __global__ __launch_bounds__(256) void bm_finish_kernel(BmFinishParams f)
     }
 }
 
+// Output of one left pixel: the checked disparity and the maps, in every requested
+// format.  lk = merged left key, rrow = merged right keys of the row.
+__device__ __forceinline__ void write_pixel(const View& disp, const View& disp_l, const View& disp_r,
+                                            const View& dispid, int x, int y, bool inside, uint32_t lk,
+                                            const uint32_t* rrow, int tol, int value_base)
+{
+    int dl = -1, dr = -1, outd = -1;
+    if (inside) {
+        dl = key_disp(lk);
+        if (dl >= 0) {
+            const int dxr = key_disp(rrow[x - dl]);
+            outd = (dxr >= 0 && abs(dxr - dl) <= tol) ? dl : -1;
+        }
+        dr = key_disp(rrow[x]);
+    }
+    if (disp.ptr) row_ptr<int16_t>(disp, y)[x] = (int16_t)outd;
+    if (disp_l.ptr) row_ptr<int16_t>(disp_l, y)[x] = (int16_t)dl;
+    if (disp_r.ptr) row_ptr<int16_t>(disp_r, y)[x] = (int16_t)dr;
+    if (dispid.ptr) {
+        uint16_t id;
+        if (!inside) id = 0;                                // Unknown
+        else if (outd >= 0) id = (uint16_t)(outd + 1);     // First + d
+        else if (dl >= 0) id = 0xFFFFu - 5u;                // Inconsistent
+        else id = 0xFFFFu - 2u;                             // NotFound
+        int cost = lk ? key_cost(lk) : 0;
+        if (lk && value_base) cost = value_base - cost;
+        ushort4 v;
+        v.x = id;
+        v.y = (uint16_t)(cost > 65535 ? 65535 : cost);
+        v.z = 0;
+        v.w = 0;
+        row_ptr<ushort4>(dispid, y)[x] = v;
+    }
+}
+
+// Merge and check for the work-list matcher: one workgroup per frame row.  A right
+// pixel xr of tile row ty collects the partial keys of every item whose tile and
+// disparity range reach it (x0 - (de-1) <= xr <= x0 + 63 - ds); a left pixel merges
+// the left keys of its tile's items.
+__global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p)
+{
+    extern __shared__ uint32_t rrow[];
+    const int r = blockIdx.x;
+    int fi = 0;
+    while (fi + 1 < p.nframes && p.frames[fi + 1].row_base <= r) ++fi;
+    const BmFrame& f = p.frames[fi];
+    const int y = r - f.row_base;
+    const int W = f.W;
+    const int ty = y / TH, ry = y - ty * TH;
+    const BmTile* trow = p.tiles + f.tile_base + ty * f.ntx;
+    const bool row_ok = y >= f.by && y < f.H - f.by;
+    for (int x = threadIdx.x; x < W; x += 256) {
+        uint32_t best = 0;
+        if (row_ok && x >= f.bx && x < W - f.bx) {
+            const int tlo = x / TW;  // the first tile whose pixels reach x
+            const int thi = min(f.ntx - 1, (x + f.D - 1) / 64);
+            for (int tx = tlo; tx <= thi; ++tx) {
+                const BmTile t = trow[tx];
+                const int x0 = tx * TW;
+                for (int j = t.first_item; j < t.first_item + t.nitems; ++j) {
+                    const int ds = p.items[j].ds, de = p.items[j].de;
+                    if (x >= x0 - (de - 1) && x <= x0 + 63 - ds) {
+                        const uint32_t v = p.rpart[((size_t)j * TH + ry) * p.rp_stride + (x - x0 + de - 1)];
+                        best = v > best ? v : best;
+                    }
+                }
+            }
+        }
+        rrow[x] = best;
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < W; x += 256) {
+        const bool inside = row_ok && x >= f.bx && x < W - f.bx;
+        uint32_t lk = 0;
+        if (inside) {
+            const BmTile t = trow[x / TW];
+            for (int j = t.first_item; j < t.first_item + t.nitems; ++j) {
+                const uint32_t v = p.lkey[((size_t)j * TH + ry) * TW + (x & (TW - 1))];
+                lk = v > lk ? v : lk;
+            }
+        }
+        write_pixel(f.disp, f.disp_l, f.disp_r, f.dispid, x, y, inside, lk, rrow, f.tol, f.value_base);
+    }
+}
+
 }  // namespace bm
+
+size_t bm_list_lkey_elems(int max_items) { return (size_t)max_items * bm::TH * bm::TW; }
+int bm_list_rp_stride(int chunk_max) { return 64 + chunk_max; }
+size_t bm_list_rpart_elems(int max_items, int chunk_max)
+{
+    return (size_t)max_items * bm::TH * bm_list_rp_stride(chunk_max);
+}
+
+template <class Cost>
+static hipError_t launch_list_cost(const BmListParams& p, int ww, int grid, hipStream_t s)
+{
+    if (ww == 9) hipLaunchKernelGGL((bm::bm_list_kernel<Cost, 4, 4>), dim3(grid), dim3(256), 0, s, p);
+    else if (ww == 7) hipLaunchKernelGGL((bm::bm_list_kernel<Cost, 3, 3>), dim3(grid), dim3(256), 0, s, p);
+    else hipLaunchKernelGGL((bm::bm_list_kernel<Cost, 2, 2>), dim3(grid), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_bm_list(int kind, int ww, const BmListParams& p, int grid, hipStream_t s)
+{
+    if (grid < 1) return hipErrorInvalidValue;
+    if (kind == BM_COST_CENSUS) return launch_list_cost<bm::CostCensus>(p, ww, grid, s);
+    return launch_list_cost<bm::CostSad>(p, ww, grid, s);
+}
+
+hipError_t launch_bm_list_finish(const BmListParams& p, int max_width, hipStream_t s)
+{
+    if (max_width > BM_MAX_WIDTH || p.total_rows < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bm::bm_list_finish_kernel, dim3(p.total_rows), dim3(256), (size_t)max_width * sizeof(uint32_t), s,
+                       p);
+    return hipGetLastError();
+}
 
 bool bm_supported(int kind, int ww, int wh, int D)
 {

@@ -49,6 +49,57 @@ void bm_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len);
 hipError_t launch_bm(int kind, int ww, const BmParams& p, hipStream_t s);
 hipError_t launch_bm_finish(const BmFinishParams& f, hipStream_t s);
 
+// ---- work-list matcher: many frames, per-tile disparity ranges ---------------------
+// A frame is one stereo pair; its tiles are 64 x BM_TILE_H pixels; every tile has a
+// disparity range [lo, hi) cut into items (one workgroup's work each).  Item i keeps
+// its left keys at lkey + i*BM_TILE_H*64 (row stride 64) and its right-view partial
+// keys at rpart + i*BM_TILE_H*rp_stride (row stride rp_stride = 64 + the longest
+// item; right pixel xr of tile x0 at column xr - x0 + de - 1).  The matcher launch is
+// persistent: a fixed grid walks the item list, whose length may be produced on the
+// device.
+constexpr int BM_TILE_W = 64;
+constexpr int BM_TILE_H = 56;
+
+struct BmFrame {
+    View L, R;          // padded inputs (uint8 grey or uint64 census)
+    View disp, disp_l, disp_r, dispid;  // outputs (ptr == nullptr: not written)
+    int W, H, D;
+    int bx, by, tol;
+    int value_base;
+    int ntx, nty;
+    int tile_base;      // index of tile (0, 0) in the tile table
+    int row_base;       // first finish row of this frame
+};
+
+struct BmTile {
+    int lo, hi;         // disparity range searched for the tile's left pixels
+    int first_item, nitems;
+};
+
+struct BmItem {
+    int frame, tx, ty;
+    int ds, de;
+};
+
+struct BmListParams {
+    const BmFrame* frames;
+    const BmTile* tiles;
+    const BmItem* items;
+    const int* nitems;  // device-side item count
+    const int* sched;   // order in which the persistent grid takes the items (nullptr: 0, 1, ...)
+    uint32_t* lkey;
+    uint32_t* rpart;
+    int rp_stride;      // bm_list_rp_stride(longest item)
+    int nframes;
+    int total_rows;     // sum of the frames' heights
+};
+
+hipError_t launch_bm_list(int kind, int ww, const BmListParams& p, int grid, hipStream_t s);
+hipError_t launch_bm_list_finish(const BmListParams& p, int max_width, hipStream_t s);
+size_t bm_list_lkey_elems(int max_items);
+int bm_list_rp_stride(int chunk_max);
+size_t bm_list_rpart_elems(int max_items, int chunk_max);
+
 // ---- reference-format cross check ------------------------------------------------
 // range[0..1] receive min / max of the kept ids (device memory, reset by the launch).
 hipError_t launch_crosscheck_ref(const View& out, const View& lr, const View& rl, int thresh, int ltor, int offset,
@@ -92,6 +143,50 @@ struct AwParams {
     int bx, by;
     int ntx, nseg, seg_len;
 };
+
+// ---- learned proposal network -------------------------------------------------------
+enum { CNN_EPI_RELU_BF16 = 0, CNN_EPI_F32 = 1, CNN_EPI_MASK_BF16 = 2 };
+
+struct CnnConvArgs {
+    const uint16_t* x;     // [N][H][W][cin] bf16
+    const uint16_t* w;     // [cout][9][cin] bf16
+    const float* bias;     // [cout] or nullptr
+    const uint16_t* mask;  // [N][H][W][cout] bf16 (CNN_EPI_MASK_BF16)
+    void* y;               // [N][H][W][cout] bf16 or float
+    int N, H, W, dil, sign;  // sign -1: tap offsets negated (data gradient)
+};
+
+struct CnnDwArgs {
+    const uint16_t* x;     // [N][H][W][cin] layer input
+    const uint16_t* dy;    // [N][H][W][cout] gradient of the layer output
+    float* part;           // cnn_dw_parts() x (cout*9*cin + cout) partial sums
+    int N, H, W, dil;
+};
+
+struct CnnBandArgs {
+    const float* out;      // network output [H4][W4][16]
+    const float* dfill;    // filled coarse disparity [H4][W4]
+    BmTile* tiles;         // [nty][ntx]
+    BmItem* items;
+    int* nitems;
+    int* tile_evals;       // [nty][ntx] disparities searched per tile | fallback << 16
+    int W4, H4, ntx, nty, D;
+    int band, chunk;
+    float logit_thresh;
+};
+
+hipError_t launch_cnn_conv(int cin, int cout, int epi, const CnnConvArgs& a, hipStream_t s);
+int cnn_dw_parts(const CnnDwArgs& a);
+hipError_t launch_cnn_dw(int cin, int cout, const CnnDwArgs& a, float* out, hipStream_t s);
+hipError_t launch_cnn_loss(const float* out, const float* label, const float* dfill, uint16_t* g, float* part, int P,
+                           float scale, float lambda, float tau, hipStream_t s);
+hipError_t launch_cnn_labels(const View& disp, int n, int W4, int H4, float* labels, hipStream_t s);
+hipError_t launch_cnn_adam(float* param, const float* grad, float* m, float* v, int n, int cin, int cout, float lr,
+                           float b1, float b2, float eps, float c1, float c2, uint16_t* wf, uint16_t* wb, hipStream_t s);
+hipError_t launch_downsample4(const View& in, const View& out, hipStream_t s);
+hipError_t launch_cnn_input(const View& L4, const View& R4, const View& dc, int n, uint16_t* x, float* dfill,
+                            hipStream_t s);
+hipError_t launch_band_items(const CnnBandArgs& a, hipStream_t s);
 
 bool aw_supported(int sub, int D);
 void aw_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len);

@@ -44,6 +44,8 @@ def _load():
     lib.orc_census3x3_ref.argtypes = [P, i, i, P]
     lib.orc_bm.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P, i]
     lib.orc_bm_brute.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P]
+    lib.orc_bm_ranges.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P, P, i]
+    lib.orc_bm_ranges.restype = i
     lib.orc_ncc.argtypes = [P, P, i, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_aw.argtypes = [i, P, P, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_crosscheck_plain.argtypes = [P, P, i, i, i, P]
@@ -96,6 +98,21 @@ def bm(kind, L, R, D, window=(9, 9), census_win=(1, 1), tol=1, nthreads=1, with_
     _chk(lib.orc_bm(kind, _p(L), _p(R), W, H, D, window[0], window[1], census_win[0], census_win[1], tol,
                     _p(dL), _p(dR), _p(out), _p(cost), nthreads), "orc_bm")
     return (out, dL, dR, cost) if with_cost else (out, dL, dR)
+
+
+def bm_ranges(kind, L, R, D, lo, hi, window=(9, 9), census_win=(1, 1), tol=1, nthreads=1):
+    """orc_bm restricted to candidates lo <= d < hi per left pixel (int16 arrays)."""
+    L = np.ascontiguousarray(L)
+    R = np.ascontiguousarray(R)
+    H, W = L.shape
+    lo = np.ascontiguousarray(lo, np.int16)
+    hi = np.ascontiguousarray(hi, np.int16)
+    dL = np.empty((H, W), np.int16)
+    dR = np.empty((H, W), np.int16)
+    out = np.empty((H, W), np.int16)
+    _chk(lib.orc_bm_ranges(kind, _p(L), _p(R), W, H, D, window[0], window[1], census_win[0], census_win[1], tol,
+                           _p(lo), _p(hi), _p(dL), _p(dR), _p(out), nthreads), "orc_bm_ranges")
+    return out, dL, dR
 
 
 def bm_brute(kind, L, R, D, window=(9, 9), census_win=(1, 1), tol=1, with_volume=False):
@@ -159,3 +176,53 @@ def crosscheck_ref(lr, rl, thresh=1, ltor=1, offset=0):
     _chk(lib.orc_crosscheck_ref(_p(lr), _p(rl), W, H, thresh, ltor, offset, _p(out), _p(mn), _p(mx)),
          "orc_crosscheck_ref")
     return out, int(mn[0]), int(mx[0])
+
+
+# ---- learned proposal network (FP64 restatement) --------------------------------
+
+lib.orc_conv3x3_fwd.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                C.c_int, C.c_void_p]
+lib.orc_conv3x3_bwd.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+lib.orc_conv3x3_fwd.restype = C.c_int
+lib.orc_conv3x3_bwd.restype = C.c_int
+
+
+def bf16_round(a):
+    """Round float32 values to bfloat16 (nearest, ties to even) and back to float32."""
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    out = r.astype(np.uint32).view(np.float32)
+    nan = np.isnan(a)
+    if nan.any():
+        out = out.copy()
+        out[nan] = np.nan
+    return out
+
+
+def conv3x3(x, w, b=None, dil=1):
+    """x [N,H,W,Cin] float32, w [Cout,3,3,Cin] -> y [N,H,W,Cout] float32 (FP64 accumulation)."""
+    x = np.ascontiguousarray(x, np.float32)
+    w = np.ascontiguousarray(w, np.float32)
+    N, H, W, Ci = x.shape
+    Co = w.shape[0]
+    assert w.shape == (Co, 3, 3, Ci)
+    bb = None if b is None else np.ascontiguousarray(b, np.float32)
+    y = np.empty((N, H, W, Co), np.float32)
+    _chk(lib.orc_conv3x3_fwd(_p(x), N, H, W, Ci, _p(w), _p(bb), Co, dil, _p(y)), "orc_conv3x3_fwd")
+    return y
+
+
+def conv3x3_bwd(x, w, dy, dil=1, want_dx=True):
+    """Gradients (dx float32 or None, dw float64 [Cout,3,3,Cin], db float64 [Cout])."""
+    x = np.ascontiguousarray(x, np.float32)
+    w = np.ascontiguousarray(w, np.float32)
+    dy = np.ascontiguousarray(dy, np.float32)
+    N, H, W, Ci = x.shape
+    Co = w.shape[0]
+    dx = np.empty_like(x) if want_dx else None
+    dw = np.empty((Co, 3, 3, Ci), np.float64)
+    db = np.empty(Co, np.float64)
+    _chk(lib.orc_conv3x3_bwd(_p(x), N, H, W, Ci, _p(w), _p(dy), Co, dil, _p(dx), _p(dw), _p(db)), "orc_conv3x3_bwd")
+    return dx, dw, db

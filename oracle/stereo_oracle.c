@@ -142,8 +142,12 @@ static int bm_args_ok(int kind, const void* L, const void* R, int W, int H, int 
     return 1;
 }
 
-int orc_bm(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
-           int ch, int tol, int16_t* dL, int16_t* dR, int16_t* out, int32_t* cost_out, int nthreads)
+/* lo / hi (optional, per left pixel): only candidates lo <= d < hi take part, in
+ * both winner-take-alls -- the band search of the learned proposal, whose ranges
+ * are per 64 x 56 tile and expanded to pixels by the caller. */
+static int bm_impl(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
+                   int ch, int tol, const int16_t* lo, const int16_t* hi, int16_t* dL, int16_t* dR,
+                   int16_t* out, int32_t* cost_out, int nthreads)
 {
     if (!bm_args_ok(kind, L, R, W, H, D, ww, wh, cw, ch) || !dL || !dR) return -1;
     const int ax = ww / 2, ay = wh / 2;
@@ -189,6 +193,7 @@ int orc_bm(int kind, const void* L, const void* R, int W, int H, int D, int ww, 
                     const size_t row = (size_t)y * W;
                     for (int x = xs; x < xe; ++x) {
                         const int A = V[x];
+                        if (lo && (d < lo[row + x] || d >= hi[row + x])) continue;
                         if (A < bestL[row + x]) { bestL[row + x] = A; dL[row + x] = (int16_t)d; }
                         if (A < bestR[row + x - d]) { bestR[row + x - d] = A; dR[row + x - d] = (int16_t)d; }
                     }
@@ -204,6 +209,20 @@ int orc_bm(int kind, const void* L, const void* R, int W, int H, int D, int ww, 
     free(bestL);
     free(bestR);
     return rc;
+}
+
+int orc_bm(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
+           int ch, int tol, int16_t* dL, int16_t* dR, int16_t* out, int32_t* cost_out, int nthreads)
+{
+    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, NULL, NULL, dL, dR, out, cost_out, nthreads);
+}
+
+int orc_bm_ranges(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
+                  int ch, int tol, const int16_t* lo, const int16_t* hi, int16_t* dL, int16_t* dR,
+                  int16_t* out, int nthreads)
+{
+    if (!lo || !hi) return -1;
+    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, lo, hi, dL, dR, out, NULL, nthreads);
 }
 
 int orc_bm_brute(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh,

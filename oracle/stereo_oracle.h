@@ -48,6 +48,10 @@ int orc_bm(int cost_kind, const void* L, const void* R, int W, int H, int D, int
 /* Same specification evaluated by direct 2-D window sums (no running sums).
  * volume (optional, W*H*D int32, index (y*W+x)*D+d) receives every aggregated cost,
  * -1 for invalid (pixel, d) combinations. */
+/* orc_bm restricted to candidates lo[p] <= d < hi[p] of the left pixel p. */
+int orc_bm_ranges(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
+                  int ch, int tol, const int16_t* lo, const int16_t* hi, int16_t* dL, int16_t* dR,
+                  int16_t* out, int nthreads);
 int orc_bm_brute(int cost_kind, const void* L, const void* R, int W, int H, int D, int ww, int wh,
                  int cw, int ch, int tol, int16_t* dL, int16_t* dR, int16_t* out, int32_t* volume);
 
