@@ -98,6 +98,13 @@ stereo_status stereo_image_upload(stereo_context* ctx, stereo_image* img, const 
 stereo_status stereo_image_download(stereo_context* ctx, const stereo_image* img, void* host, size_t host_pitch,
                                     stereo_stream stream);
 
+/* Clu::Cuda::Malloc / Free / MemCpy (Api.h:107-229): raw device buffers, e.g. for
+ * the network's kernel-level entry points.  kind: 0 host->device, 1 device->host,
+ * 2 device->device.  The copy is synchronous. */
+stereo_status stereo_device_alloc(stereo_context* ctx, void** ptr, size_t bytes);
+stereo_status stereo_device_free(stereo_context* ctx, void* ptr);
+stereo_status stereo_memcpy(stereo_context* ctx, void* dst, const void* src, size_t bytes, int32_t kind);
+
 /* ---- drivers (CKernelDriverBase, KernelDriverBase.h:47-177) ---------------- */
 
 stereo_status stereo_driver_destroy(stereo_driver* drv);

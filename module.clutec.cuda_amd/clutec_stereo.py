@@ -109,6 +109,9 @@ _PROTOS = [
     ("stereo_image_free", C.c_int, [_P, C.POINTER(Image)]),
     ("stereo_image_upload", C.c_int, [_P, C.POINTER(Image), _P, C.c_size_t, _P]),
     ("stereo_image_download", C.c_int, [_P, C.POINTER(Image), _P, C.c_size_t, _P]),
+    ("stereo_device_alloc", C.c_int, [_P, C.POINTER(_P), C.c_size_t]),
+    ("stereo_device_free", C.c_int, [_P, _P]),
+    ("stereo_memcpy", C.c_int, [_P, _P, _P, C.c_size_t, C.c_int32]),
     ("stereo_driver_destroy", C.c_int, [_P]),
     ("stereo_driver_name", C.c_char_p, [_P]),
     ("stereo_driver_last_time_ms", C.c_int, [_P, C.POINTER(C.c_double)]),
@@ -278,6 +281,40 @@ class DeviceImage:
         img = cls(ctx, arr.shape[1], arr.shape[0], pixel_type)
         img.upload(arr)
         return img
+
+
+class DeviceBuffer:
+    """Raw device allocation (stereo_device_alloc) holding a numpy array's bytes."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        _check(lib.stereo_device_alloc(ctx.handle, C.byref(self.ptr), self.nbytes), ctx)
+
+    @classmethod
+    def from_array(cls, ctx, arr):
+        a = np.ascontiguousarray(arr)
+        b = cls(ctx, a.nbytes)
+        _check(lib.stereo_memcpy(ctx.handle, b.ptr, a.ctypes.data, a.nbytes, 0), ctx)
+        return b
+
+    def download(self, dtype, shape):
+        a = np.empty(shape, dtype)
+        assert a.nbytes <= self.nbytes
+        _check(lib.stereo_memcpy(self.ctx.handle, a.ctypes.data, self.ptr, a.nbytes, 1), self.ctx)
+        return a
+
+    def free(self):
+        if self.ptr:
+            lib.stereo_device_free(self.ctx.handle, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class _Driver:

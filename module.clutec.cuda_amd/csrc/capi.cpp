@@ -145,6 +145,32 @@ stereo_status stereo_image_download(stereo_context* ctx, const stereo_image* img
     return STEREO_OK;
 }
 
+stereo_status stereo_device_alloc(stereo_context* ctx, void** ptr, size_t bytes)
+{
+    if (!ctx || !ptr || bytes == 0) return STEREO_ERROR_INVALID_ARGUMENT;
+    *ptr = nullptr;
+    stereo_status st = set_device(ctx);
+    if (st) return st;
+    GUARD(ctx, "stereo_device_alloc");
+    HIP_TRY(ctx, hipMalloc(ptr, bytes), "hipMalloc");
+    return STEREO_OK;
+}
+
+stereo_status stereo_device_free(stereo_context* ctx, void* ptr)
+{
+    if (!ctx) return STEREO_ERROR_INVALID_ARGUMENT;
+    if (ptr) HIP_TRY(ctx, hipFree(ptr), "hipFree");
+    return STEREO_OK;
+}
+
+stereo_status stereo_memcpy(stereo_context* ctx, void* dst, const void* src, size_t bytes, int32_t kind)
+{
+    if (!ctx || !dst || !src || kind < 0 || kind > 2) return STEREO_ERROR_INVALID_ARGUMENT;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    HIP_TRY(ctx, hipMemcpy(dst, src, bytes, k), "hipMemcpy");
+    return STEREO_OK;
+}
+
 stereo_status stereo_driver_destroy(stereo_driver* d)
 {
     if (!d) return STEREO_ERROR_INVALID_ARGUMENT;

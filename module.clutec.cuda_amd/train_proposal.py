@@ -1,0 +1,135 @@
+"""Trains the learned disparity-proposal network on the GPU with the engine's own
+kernels (BF16 MFMA forward and backward, Adam) and writes its parameters.
+
+Data: seeded synthetic pairs (the engine's generator, constant and slanted
+regions) at 960x540.  Labels: the full-range census matcher's checked disparity
+map (9x7 census, 9x9 window, D=128) of the same pair.  Network input: the pair at
+1/4 resolution, the right image warped by the census match at that resolution
+(D=32); the network learns the residual of the centre and a confidence.
+
+    python module.clutec.cuda_amd/train_proposal.py --frames 10000 --steps 3000 \
+        --out module.clutec.cuda_amd/weights/proposal_cnn.bin
+
+Writes <out> (28930 float32, little endian) and <out>.json (settings and loss).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import clutec_stereo as cs  # noqa: E402
+
+
+def downsample4(img):
+    H, W = img.shape
+    return ((img.reshape(H // 4, 4, W // 4, 4).astype(np.int32).sum(axis=(1, 3)) + 8) >> 4).astype(np.uint8)
+
+
+class FramePool:
+    """Prepared training frames resident in device memory: quarter-resolution L4,
+    R4, the quarter-resolution census match and the full-resolution label map."""
+
+    def __init__(self, ctx, W, H, D, frames, seed0, threads=16, log=print):
+        self.ctx, self.W, self.H = ctx, W, H
+        Dc = max(8, (D // 4 + 7) // 8 * 8)
+        full = cs.Matcher(ctx, W, H, algorithm=cs.ALGO_CENSUS, window=(9, 9), census=(9, 7), num_disp=D, tol=1)
+        coarse = cs.Matcher(ctx, W // 4, H // 4, algorithm=cs.ALGO_CENSUS, window=(9, 9), census=(9, 7), num_disp=Dc,
+                            tol=1)
+        dl = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_U8)
+        dr = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_U8)
+        self.items = []
+        t0 = time.time()
+
+        def make(seed):
+            kind = 1 if seed % 2 else 0  # slanted and constant regions
+            return cs.synth_pair(seed, W, H, kind, 12, D - 1, 8)
+
+        def pairs():
+            # generate in chunks so that at most a few hundred host pairs are held
+            with ThreadPoolExecutor(max_workers=threads) as ex:
+                for c in range(seed0, seed0 + frames, 256):
+                    yield from ex.map(make, range(c, min(seed0 + frames, c + 256)))
+
+        for i, (L, R) in enumerate(pairs()):
+            dl.upload(L)
+            dr.upload(R)
+            disp = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_I16)
+            full.process(disp, None, None, dl, dr)
+            L4 = cs.DeviceImage.from_array(ctx, downsample4(L), cs.PIXEL_LUM_U8)
+            R4 = cs.DeviceImage.from_array(ctx, downsample4(R), cs.PIXEL_LUM_U8)
+            dc = cs.DeviceImage(ctx, W // 4, H // 4, cs.PIXEL_LUM_I16)
+            coarse.process(dc, None, None, L4, R4)
+            self.items.append((L4, R4, dc, disp))
+            if (i + 1) % 1000 == 0:
+                log(f"  prepared {i + 1}/{frames} frames ({time.time() - t0:.1f}s)")
+        ctx.synchronize()
+
+
+def train(ctx, pool, steps, batch, lr, seed, log=print, init=None):
+    W4, H4 = pool.W // 4, pool.H // 4
+    net = cs.ProposalNet(ctx, W4, H4, max_frames=batch)
+    net.set_params(cs.cnn_init_params(seed) if init is None else init)
+    rng = np.random.default_rng(seed)
+    history = []
+    t0 = time.time()
+    for step in range(steps):
+        idx = rng.choice(len(pool.items), size=batch, replace=False)
+        for slot, k in enumerate(idx):
+            L4, R4, dc, disp = pool.items[k]
+            net.set_input(slot, L4, R4, dc)
+            net.set_labels(slot, disp)
+        # cosine decay to 5 % of the initial rate
+        rate = lr * (0.05 + 0.95 * 0.5 * (1.0 + np.cos(np.pi * step / max(1, steps - 1))))
+        loss = net.train_step(batch, float(rate))
+        history.append([float(v) for v in loss])
+        if step % 200 == 0 or step == steps - 1:
+            log(f"  step {step:5d} lr {rate:.2e} huber {loss[0]:.4f} bce {loss[1]:.4f} "
+                f"within-tau {loss[3]:.4f} ({time.time() - t0:.1f}s)")
+    return net.params(), history
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--frames", type=int, default=10000)
+    ap.add_argument("--width", type=int, default=960)
+    ap.add_argument("--height", type=int, default=540)
+    ap.add_argument("--disparities", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--seed0", type=int, default=10000, help="first synthetic pair seed")
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args()
+    ctx = cs.Context(0)
+    log = lambda s: print(s, flush=True)
+    log(f"preparing {a.frames} frames {a.width}x{a.height} D={a.disparities}")
+    t0 = time.time()
+    pool = FramePool(ctx, a.width, a.height, a.disparities, a.frames, a.seed0, log=log)
+    t_data = time.time() - t0
+    log(f"training {a.steps} steps, batch {a.batch}")
+    t1 = time.time()
+    params, hist = train(ctx, pool, a.steps, a.batch, a.lr, a.seed, log=log)
+    t_train = time.time() - t1
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    params.astype("<f4").tofile(a.out)
+    tail = np.array(hist[-100:])
+    meta = {
+        "params": int(params.size), "frames": a.frames, "size": [a.width, a.height], "disparities": a.disparities,
+        "steps": a.steps, "batch": a.batch, "lr": a.lr, "seed": a.seed, "seed0": a.seed0,
+        "first_loss": hist[0], "final_loss_mean100": tail.mean(axis=0).tolist(),
+        "data_seconds": round(t_data, 1), "train_seconds": round(t_train, 1),
+        "loss_fields": ["huber", "bce", "labelled_pixels", "within_tau"],
+    }
+    with open(a.out + ".json", "w") as f:
+        json.dump(meta, f, indent=1)
+    log(json.dumps(meta))
+
+
+if __name__ == "__main__":
+    main()

@@ -12,7 +12,6 @@ import clutec_stereo as cs
 import cnn_ref
 import orc
 
-torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 FP32_RTOL = 2e-6  # of sum |w x|: FP32 accumulation over <= 288 products
@@ -23,27 +22,24 @@ def _bf16_bits(a):
     return r, (r.view(np.uint32) >> 16).astype(np.uint16)
 
 
-def _dev(a):
-    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
-
-
 def _conv(ctx, x, w, b, dil, relu):
     N, H, W, ci = x.shape
     co = w.shape[0]
     xq, xb = _bf16_bits(x)
     wq, wb = _bf16_bits(w)
-    tx, tw = _dev(xb), _dev(wb.reshape(co, 9, ci))
-    tb = _dev(b.astype(np.float32)) if b is not None else None
-    ty = torch.empty((N, H, W, co), dtype=torch.int16 if relu else torch.float32, device="cuda")
-    torch.cuda.synchronize()
-    st = cs.lib.stereo_conv3x3_bf16(ctx.handle, tx.data_ptr(), N, H, W, ci, tw.data_ptr(),
-                                    tb.data_ptr() if tb is not None else None, co, dil, 1 if relu else 0,
-                                    ty.data_ptr(), None)
+    tx = cs.DeviceBuffer.from_array(ctx, xb)
+    tw = cs.DeviceBuffer.from_array(ctx, wb.reshape(co, 9, ci))
+    tb = cs.DeviceBuffer.from_array(ctx, b.astype(np.float32)) if b is not None else None
+    ty = cs.DeviceBuffer(ctx, N * H * W * co * (2 if relu else 4))
+    st = cs.lib.stereo_conv3x3_bf16(ctx.handle, tx.ptr, N, H, W, ci, tw.ptr, tb.ptr if tb is not None else None, co,
+                                    dil, 1 if relu else 0, ty.ptr, None)
     cs._check(st, ctx)
     cs._check(cs.lib.stereo_synchronize(ctx.handle, None), ctx)
-    y = ty.cpu().numpy()
     if relu:
-        y = (y.view(np.uint16).astype(np.uint32) << 16).view(np.float32)
+        y = ty.download(np.uint16, (N, H, W, co))
+        y = (y.astype(np.uint32) << 16).view(np.float32)
+    else:
+        y = ty.download(np.float32, (N, H, W, co))
     return xq, wq, y
 
 
@@ -85,14 +81,14 @@ def test_conv_backward(ctx, ci, co, dil):
     wq, wb = _bf16_bits((rng.standard_normal((co, 3, 3, ci)) * 0.1).astype(np.float32))
     gq, gb = _bf16_bits(rng.standard_normal((N, H, W, co)).astype(np.float32))
     wt = np.ascontiguousarray(wb.reshape(co, 9, ci).transpose(2, 1, 0))  # [ci][9][co]
-    tx, twt, tg = _dev(xb), _dev(wt), _dev(gb)
-    tdx = torch.empty((N, H, W, ci), dtype=torch.float32, device="cuda")
-    tdw = torch.empty(co * 9 * ci + co, dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
-    cs._check(cs.lib.stereo_conv3x3_bwd_bf16(ctx.handle, tx.data_ptr(), N, H, W, ci, twt.data_ptr(), tg.data_ptr(), co,
-                                             dil, tdx.data_ptr(), tdw.data_ptr(), None), ctx)
+    tx, twt, tg = (cs.DeviceBuffer.from_array(ctx, a) for a in (xb, wt, gb))
+    tdx = cs.DeviceBuffer(ctx, N * H * W * ci * 4)
+    tdw = cs.DeviceBuffer(ctx, (co * 9 * ci + co) * 4)
+    cs._check(cs.lib.stereo_conv3x3_bwd_bf16(ctx.handle, tx.ptr, N, H, W, ci, twt.ptr, tg.ptr, co, dil, tdx.ptr, tdw.ptr,
+                                             None), ctx)
     cs._check(cs.lib.stereo_synchronize(ctx.handle, None), ctx)
-    dx, dwdb = tdx.cpu().numpy(), tdw.cpu().numpy()
+    dx = tdx.download(np.float32, (N, H, W, ci))
+    dwdb = tdw.download(np.float32, (co * 9 * ci + co,))
     rdx, rdw, rdb = orc.conv3x3_bwd(xq, wq, gq, dil)
     mdx, mdw, mdb = orc.conv3x3_bwd(np.abs(xq), np.abs(wq), np.abs(gq), dil)
     e_dx = (np.abs(dx - rdx) / np.maximum(mdx, 1e-30)).max()
