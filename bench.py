@@ -62,12 +62,115 @@ def cpu_baseline(L, R, threads):
     }, out
 
 
+def learned_leg(cs, np, ctx, L, R, dl, dr, full_disp, full_ms, steps):
+    """The learned-proposal census matcher (c4) on the same frame: speedup over the
+    full-range pipeline and the fraction of pixels equal to its output.  Also the
+    coarse band alone (network output held at zero residual and high confidence)."""
+    wdir = os.path.join(ROOT, "module.clutec.cuda_amd", "weights")
+    wpath = os.path.join(wdir, "proposal_cnn.bin")
+    if not os.path.exists(wpath):
+        return {"status": "no trained parameters in module.clutec.cuda_amd/weights"}
+    params = np.fromfile(wpath, "<f4")
+    meta = {}
+    if os.path.exists(wpath + ".json"):
+        with open(wpath + ".json") as f:
+            meta = json.load(f)
+    thr = float(meta.get("logit_thresh", 0.0))
+    band, chunk = 32, 32
+
+    def run(p, t):
+        lm = cs.LearnedMatcher(ctx, W, H, p, num_disp=D, band=band, chunk=chunk, logit_thresh=t)
+        out = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_I16)
+        for _ in range(3):
+            lm.process(out, None, None, dl, dr)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            lm.process(out, None, None, dl, dr)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        ctx.set_timing(True)
+        lm.process(out, None, None, dl, dr)
+        st = lm.stats(timed=True)
+        ctx.set_timing(False)
+        disp = out.download()
+        return {
+            "ms_per_frame": round(ms, 4),
+            "speedup": round(full_ms / ms, 3),
+            "agreement": round(float((disp == full_disp).mean()), 5),
+            "evals_frac": round(st["evals"] / float(W * H * D), 4),
+            "fallback_tiles": st["fallback_tiles"],
+            "tiles": st["tiles"],
+            "stage_ms": dict(zip(["census_downsample", "coarse_match", "network", "band_match", "merge_check"],
+                                 [round(v, 4) for v in st["stage_ms"]])),
+        }
+
+    res = run(params, thr)
+    res.update({"band": band, "chunk": chunk, "logit_thresh": thr, "trained": {
+        k: meta.get(k) for k in ("frames", "size", "steps", "batch") if k in meta}})
+    # the same pipeline without the network: centre = coarse map, always confident
+    coarse = np.zeros_like(params)
+    coarse[-1] = 30.0
+    res["coarse_band_only"] = run(coarse, 0.0)
+    return res
+
+
+def batch_leg(cs, np, ctx, steps):
+    """Batched mode (c3): 64 pairs, W = 16 U[20,128], H = 16 U[15,96], D in
+    {64,128,192,256}; SAD 7x7 and census 9x7 / 9x9, each with the check; autotuned,
+    graph-captured; against the same pairs run one at a time."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(100)
+    sizes = [(16 * int(rng.integers(20, 129)), 16 * int(rng.integers(15, 97))) for _ in range(64)]
+    disps = [int(rng.choice([64, 128, 192, 256])) for _ in range(64)]
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        pairs = list(ex.map(lambda i: cs.synth_pair(100 + i, sizes[i][0], sizes[i][1], 0, 8, disps[i] - 1, 8),
+                            range(64)))
+    evals = sum(w * h * d for (w, h), d in zip(sizes, disps))
+    out = {"pairs": 64, "evals_per_batch": evals}
+    for name, algo, win, cen in (("sad7x7", cs.ALGO_SAD, (7, 7), (1, 1)), ("census9x7", cs.ALGO_CENSUS, (9, 9), (9, 7))):
+        b = cs.BatchMatcher(ctx, sizes, disps, algorithm=algo, window=win, census=cen)
+        b.upload(pairs)
+        tuning = b.autotune()
+        b.process()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.process()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        got = b.results()
+        # one pair at a time, same images resident
+        singles, exact = [], 0
+        t_single = 0.0
+        for i, ((w, h), d) in enumerate(zip(sizes, disps)):
+            m = cs.Matcher(ctx, w, h, algorithm=algo, window=win, census=cen, num_disp=d, tol=1)
+            o = cs.DeviceImage(ctx, w, h, cs.PIXEL_LUM_I16)
+            m.process(o, None, None, b.left[i], b.right[i])
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                m.process(o, None, None, b.left[i], b.right[i])
+            ctx.synchronize()
+            t_single += (time.perf_counter() - t0) * 1e3 / 3
+            exact += int(np.array_equal(o.download(), got[i][0]))
+        out[name] = {"ms_per_batch": round(ms, 3), "M_evals_per_s": round(evals / (ms * 1e-3) / 1e6, 1),
+                     "pairs_per_s": round(64 / (ms * 1e-3), 1), "tuning": tuning,
+                     "one_at_a_time_ms": round(t_single, 3), "speedup_vs_one_at_a_time": round(t_single / ms, 3),
+                     "pairs_bit_exact_vs_one_at_a_time": f"{exact}/64"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-mode leg")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run 5 device-to-device copies of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -76,6 +179,14 @@ def main():
 
     import numpy as np
     import clutec_stereo as cs
+
+    if args.copy_bytes:
+        ctx = cs.Context(0)
+        a, b = cs.DeviceBuffer(ctx, args.copy_bytes), cs.DeviceBuffer(ctx, args.copy_bytes)
+        for _ in range(5):
+            cs._check(cs.lib.stereo_memcpy(ctx.handle, b.ptr, a.ptr, args.copy_bytes, 2), ctx)
+        print(json.dumps({"copied_bytes": args.copy_bytes, "copies": 5}), flush=True)
+        return 0
 
     L, R = cs.synth_pair(1, W, H, 0, 24, 255, 8)
     ctx = cs.Context(0)
@@ -143,6 +254,11 @@ def main():
         "pipeline_frac_of_valu_peak": round(value * 1e6 * OPS_PER_EVAL / (PEAK_INT_TOPS * 1e12), 4),
         "valid_fraction": round(float((disp >= 0).mean()), 4),
     }
+
+    if not args.no_learned:
+        result["learned"] = learned_leg(cs, np, ctx, L, R, dl, dr, disp, ms_per_step, max(5, min(args.steps, 20)))
+    if not args.no_batch:
+        result["batch"] = batch_leg(cs, np, ctx, max(3, min(args.steps, 10)))
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

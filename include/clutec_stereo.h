@@ -265,6 +265,14 @@ stereo_status stereo_cnn_set_params(stereo_cnn* cnn, const float* host, size_t n
 stereo_status stereo_cnn_get_params(stereo_cnn* cnn, float* host, size_t n);
 /* Gradients of the loss of the last train step, same layout as the parameters. */
 stereo_status stereo_cnn_get_grads(stereo_cnn* cnn, float* host, size_t n);
+/* Training state for checkpoint / resume: a versioned blob of the FP32 parameters,
+ * the Adam moments and the step count (layout: 8-byte magic "CLSTCNN1", uint32
+ * version 1, uint32 step, then 3 x STEREO_CNN_PARAMS floats: parameters, first and
+ * second moments, each in the public parameter layout).  Loading sets the bf16
+ * weight copies used by inference. */
+size_t stereo_cnn_state_size(void);
+stereo_status stereo_cnn_save_state(stereo_cnn* cnn, void* host, size_t n);
+stereo_status stereo_cnn_load_state(stereo_cnn* cnn, const void* host, size_t n);
 /* Input slot `index`: quarter-resolution L4, R4 (LUM_U8, W4 x H4) and the coarse
  * disparity (LUM_I16, -1 invalid) that warps R4. */
 stereo_status stereo_cnn_set_input(stereo_cnn* cnn, int32_t index, const stereo_image* L4, const stereo_image* R4,

@@ -137,6 +137,9 @@ _PROTOS = [
     ("stereo_cnn_set_params", C.c_int, [_P, _P, C.c_size_t]),
     ("stereo_cnn_get_params", C.c_int, [_P, _P, C.c_size_t]),
     ("stereo_cnn_get_grads", C.c_int, [_P, _P, C.c_size_t]),
+    ("stereo_cnn_state_size", C.c_size_t, []),
+    ("stereo_cnn_save_state", C.c_int, [_P, _P, C.c_size_t]),
+    ("stereo_cnn_load_state", C.c_int, [_P, _P, C.c_size_t]),
     ("stereo_cnn_set_input", C.c_int, [_P, C.c_int32, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_cnn_set_labels", C.c_int, [_P, C.c_int32, C.POINTER(Image), _P]),
     ("stereo_cnn_forward", C.c_int, [_P, C.c_int32, _P, _P]),
@@ -516,6 +519,15 @@ class ProposalNet:
         g = np.empty(CNN_PARAMS, np.float32)
         _check(lib.stereo_cnn_get_grads(self.handle, g.ctypes.data, g.size), self.ctx)
         return g
+
+    def save_state(self):
+        b = np.empty(lib.stereo_cnn_state_size(), np.uint8)
+        _check(lib.stereo_cnn_save_state(self.handle, b.ctypes.data, b.size), self.ctx)
+        return b
+
+    def load_state(self, blob):
+        b = np.ascontiguousarray(blob, np.uint8)
+        _check(lib.stereo_cnn_load_state(self.handle, b.ctypes.data, b.size), self.ctx)
 
     def set_input(self, index, L4, R4, coarse, stream=None):
         _check(lib.stereo_cnn_set_input(self.handle, index, L4.ref, R4.ref, coarse.ref, stream), self.ctx)

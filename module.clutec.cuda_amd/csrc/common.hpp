@@ -35,10 +35,16 @@ constexpr int DPP_ROW_SHL1 = 0x101;
 constexpr int DPP_ROW_SHR1 = 0x111;
 constexpr int DPP_WAVE_SHR1 = 0x138;
 
+// The empty asm keeps the move a plain v_mov_b32_dpp: without it the DPP combiner
+// folds the move into the arithmetic that consumes it, and its v_subrev_u32_dpp
+// form (D = src1 - dpp(src0)) produced wrong sums on gfx950 while the unfused
+// sequence is exact.
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_mov(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, true);
+    uint32_t r = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, true);
+    asm volatile("" : "+v"(r));
+    return r;
 }
 
 // max(src shifted one lane up the wave (lane 0 reads 0), key): one v_max_u32_dpp.

@@ -248,6 +248,49 @@ stereo_status stereo_cnn_get_grads(stereo_cnn* c, float* host, size_t n)
     return download_params(c, c->grad, host, n);
 }
 
+size_t stereo_cnn_state_size(void) { return 16 + 3 * (size_t)STEREO_CNN_PARAMS * sizeof(float); }
+
+stereo_status stereo_cnn_save_state(stereo_cnn* c, void* host, size_t n)
+{
+    if (!c || !host) return STEREO_ERROR_INVALID_ARGUMENT;
+    if (n < stereo_cnn_state_size())
+        return fail(c->ctx, STEREO_ERROR_INVALID_ARGUMENT, "state buffer of %zu bytes, need %zu", n, stereo_cnn_state_size());
+    HIP_TRY(c->ctx, hipDeviceSynchronize(), "hipDeviceSynchronize");
+    uint8_t* b = static_cast<uint8_t*>(host);
+    std::memcpy(b, "CLSTCNN1", 8);
+    const uint32_t version = 1, step = (uint32_t)c->step;
+    std::memcpy(b + 8, &version, 4);
+    std::memcpy(b + 12, &step, 4);
+    float* f = reinterpret_cast<float*>(b + 16);  // the caller's buffer; 16-byte header keeps 4-byte alignment
+    stereo_status st;
+    if ((st = download_params(c, c->param, f, STEREO_CNN_PARAMS))) return st;
+    if ((st = download_params(c, c->m, f + STEREO_CNN_PARAMS, STEREO_CNN_PARAMS))) return st;
+    return download_params(c, c->v, f + 2 * (size_t)STEREO_CNN_PARAMS, STEREO_CNN_PARAMS);
+}
+
+stereo_status stereo_cnn_load_state(stereo_cnn* c, const void* host, size_t n)
+{
+    if (!c || !host) return STEREO_ERROR_INVALID_ARGUMENT;
+    const uint8_t* b = static_cast<const uint8_t*>(host);
+    uint32_t version = 0, step = 0;
+    if (n < stereo_cnn_state_size() || std::memcmp(b, "CLSTCNN1", 8) != 0)
+        return fail(c->ctx, STEREO_ERROR_INVALID_ARGUMENT, "not a network state blob");
+    std::memcpy(&version, b + 8, 4);
+    std::memcpy(&step, b + 12, 4);
+    if (version != 1) return fail(c->ctx, STEREO_ERROR_INVALID_ARGUMENT, "network state version %u not supported", version);
+    std::vector<float> f(3 * (size_t)STEREO_CNN_PARAMS);
+    std::memcpy(f.data(), b + 16, f.size() * sizeof(float));
+    stereo_status st = stereo_cnn_set_params(c, f.data(), STEREO_CNN_PARAMS);  // also resets the moments
+    if (st) return st;
+    std::vector<float> pad;
+    to_padded(f.data() + STEREO_CNN_PARAMS, pad);
+    HIP_TRY(c->ctx, hipMemcpy(c->m, pad.data(), pad.size() * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    to_padded(f.data() + 2 * (size_t)STEREO_CNN_PARAMS, pad);
+    HIP_TRY(c->ctx, hipMemcpy(c->v, pad.data(), pad.size() * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    c->step = (int)step;
+    return STEREO_OK;
+}
+
 stereo_status stereo_cnn_set_input(stereo_cnn* c, int32_t index, const stereo_image* L4, const stereo_image* R4,
                                    const stereo_image* coarse, stereo_stream stream)
 {

@@ -93,6 +93,39 @@ def train(ctx, pool, steps, batch, lr, seed, log=print, init=None):
     return net.params(), history
 
 
+def calibrate(ctx, params, W, H, D, seeds, thresholds, min_agree=0.996, log=print):
+    """Agreement with the full-range matcher and searched fraction of the cost volume
+    per confidence threshold, on held-out frames at the deployment size; returns
+    the smallest threshold whose mean agreement reaches min_agree (most tiles on
+    the band), and the table."""
+    full = cs.Matcher(ctx, W, H, algorithm=cs.ALGO_CENSUS, window=(9, 9), census=(9, 7), num_disp=D, tol=1)
+    frames = []
+    for seed in seeds:
+        L, R = cs.synth_pair(seed, W, H, seed % 3, 24, D - 1, 8)
+        dl = cs.DeviceImage.from_array(ctx, L, cs.PIXEL_LUM_U8)
+        dr = cs.DeviceImage.from_array(ctx, R, cs.PIXEL_LUM_U8)
+        ref = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_I16)
+        full.process(ref, None, None, dl, dr)
+        frames.append((dl, dr, ref.download()))
+    table = []
+    for t in thresholds:
+        lm = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, band=32, chunk=32, logit_thresh=t)
+        out = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_I16)
+        agree, frac = [], []
+        for dl, dr, ref in frames:
+            lm.process(out, None, None, dl, dr)
+            agree.append(float((out.download() == ref).mean()))
+            frac.append(lm.stats()["evals"] / float(W * H * D))
+        row = {"thresh": t, "agree_mean": round(float(np.mean(agree)), 5), "agree_min": round(float(np.min(agree)), 5),
+               "evals_frac": round(float(np.mean(frac)), 4)}
+        log(f"  threshold {t:+.2f}: agreement {row['agree_mean']:.5f} (min {row['agree_min']:.5f}) "
+            f"searched {row['evals_frac']:.3f}")
+        table.append(row)
+    ok = [r for r in table if r["agree_mean"] >= min_agree]
+    best = min(ok, key=lambda r: r["thresh"]) if ok else max(table, key=lambda r: r["thresh"])
+    return best["thresh"], table
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--frames", type=int, default=10000)
@@ -105,6 +138,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--seed0", type=int, default=10000, help="first synthetic pair seed")
     ap.add_argument("--out", required=True)
+    ap.add_argument("--calib-frames", type=int, default=4, help="held-out 1920x1080 D=256 frames for the threshold")
     a = ap.parse_args()
     ctx = cs.Context(0)
     log = lambda s: print(s, flush=True)
@@ -116,6 +150,9 @@ def main():
     t1 = time.time()
     params, hist = train(ctx, pool, a.steps, a.batch, a.lr, a.seed, log=log)
     t_train = time.time() - t1
+    log("calibrating the confidence threshold at 1920x1080, D=256")
+    thr, table = calibrate(ctx, params, 1920, 1080, 256, range(20000, 20000 + a.calib_frames),
+                           [-6.0, -4.0, -3.0, -2.0, -1.5, -1.0, -0.5, 0.0, 0.5, 1.0, 2.0, 3.0], log=log)
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     params.astype("<f4").tofile(a.out)
     tail = np.array(hist[-100:])
@@ -124,6 +161,7 @@ def main():
         "steps": a.steps, "batch": a.batch, "lr": a.lr, "seed": a.seed, "seed0": a.seed0,
         "first_loss": hist[0], "final_loss_mean100": tail.mean(axis=0).tolist(),
         "data_seconds": round(t_data, 1), "train_seconds": round(t_train, 1),
+        "logit_thresh": thr, "calibration": table,
         "loss_fields": ["huber", "bce", "labelled_pixels", "within_tau"],
     }
     with open(a.out + ".json", "w") as f:

@@ -138,3 +138,28 @@ def test_cpp_driver_layer_builds_and_reports_errors(tmp_path):
         r = subprocess.run([exe, "census", "64", "32", "16", "1", str(tmp_path / "o")], capture_output=True, text=True,
                            timeout=60)
         assert r.returncode == 1 and "error 5" in r.stderr
+
+
+def test_kernel_isa_has_no_fused_or_in_place_dpp(tmp_path):
+    """Row-shift DPP moves must stay standalone v_mov_b32_dpp into a fresh register:
+    folded into arithmetic (v_add/v_sub/v_subrev_u32_dpp) or done in place they gave
+    wrong window sums on gfx950.  The wave-shift maxima of the right-view chain are
+    written as one inline-asm block and are allowed."""
+    import subprocess
+    csrc = os.path.join(os.path.dirname(cs.__file__), "csrc")
+    bad = []
+    for f in ("bm.hip", "ncc.hip", "aw.hip"):
+        asm = tmp_path / (f + ".s")
+        r = subprocess.run(["/opt/rocm/bin/hipcc", "-O3", "-std=c++17", "--offload-arch=gfx950", "--cuda-device-only",
+                            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I",
+                            os.path.join(ROOT, "include"), "-I", csrc, "-x", "hip", "-S", os.path.join(csrc, f), "-o",
+                            str(asm)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-2000:]
+        for line in asm.read_text().splitlines():
+            m = re.match(r"\s*(v_\w+_dpp)\s+(v\d+),\s*(v\d+)", line)
+            if not m or "wave_shr" in line:
+                continue
+            op, dst, src = m.groups()
+            if op != "v_mov_b32_dpp" or dst == src:
+                bad.append(f"{f}: {line.strip()}")
+    assert not bad, "\n".join(bad[:10])
