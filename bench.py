@@ -76,7 +76,7 @@ def learned_leg(cs, np, ctx, L, R, dl, dr, full_disp, full_ms, steps):
         with open(wpath + ".json") as f:
             meta = json.load(f)
     thr = float(meta.get("logit_thresh", 0.0))
-    band, chunk = 32, 32
+    band, chunk = 32, 64
 
     def run(p, t):
         lm = cs.LearnedMatcher(ctx, W, H, p, num_disp=D, band=band, chunk=chunk, logit_thresh=t)

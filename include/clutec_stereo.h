@@ -319,12 +319,17 @@ stereo_status stereo_learned_process(stereo_driver* drv, stereo_image* disp, ste
                                      stereo_stream stream);
 /* Of the last process call (waits for it): disparity evaluations searched at full
  * resolution, tiles, tiles that fell back to the full range, and with timing on
- * the stage times in ms {census, coarse match, network, band match, merge/check}. */
+ * the stage times in ms {census, coarse match, network, band match, merge/check}.
+ * Untimed process calls replay a captured HIP graph of the whole sequence (one
+ * per pair of input buffers); timed calls run the stages one by one. */
 stereo_status stereo_learned_stats(stereo_driver* drv, int64_t* evals, int32_t* tiles, int32_t* fallback_tiles,
                                    double* stage_ms, int32_t n_stages);
 /* Disparity range [lo, hi) searched by each 64 x 56 tile in the last process call,
  * row-major tiles, lohi[2 t], lohi[2 t + 1]; n >= 2 * tiles.  Waits for the call. */
 stereo_status stereo_learned_tile_ranges(stereo_driver* drv, int32_t* lohi, int32_t n);
+/* The searched disparities of each tile in the last process call as a mask of
+ * 8-disparity bins (bit b: d in [8b, 8b + 8)); n >= tiles.  Waits for the call. */
+stereo_status stereo_learned_tile_masks(stereo_driver* drv, uint32_t* masks, int32_t n);
 
 /* ---- synthetic workloads (host only, no device needed) ------------------- */
 

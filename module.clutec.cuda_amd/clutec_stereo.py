@@ -155,6 +155,7 @@ _PROTOS = [
     ("stereo_learned_stats", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                         C.POINTER(C.c_double), C.c_int32]),
     ("stereo_learned_tile_ranges", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
+    ("stereo_learned_tile_masks", C.c_int, [_P, C.POINTER(C.c_uint32), C.c_int32]),
     ("stereo_synth_pair", C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     _P, _P, _P]),
 ]
@@ -560,7 +561,7 @@ def cnn_init_params(seed=0):
 class LearnedMatcher(_Driver):
     """Census matcher with the learned band proposal (stereo_learned_*)."""
 
-    def __init__(self, ctx, W, H, cnn_params, window=(9, 9), census=(9, 7), num_disp=256, tol=1, band=32, chunk=32,
+    def __init__(self, ctx, W, H, cnn_params, window=(9, 9), census=(9, 7), num_disp=256, tol=1, band=32, chunk=64,
                  logit_thresh=0.0, coarse_disp=0):
         super().__init__(ctx)
         self.W, self.H = W, H
@@ -597,6 +598,13 @@ class LearnedMatcher(_Driver):
         buf = (C.c_int32 * (2 * ntx * nty))()
         _check(lib.stereo_learned_tile_ranges(self.handle, buf, len(buf)), self.ctx)
         return np.array(buf, np.int32).reshape(nty, ntx, 2)
+
+    def tile_masks(self):
+        ntx = (self.W + 63) // 64
+        nty = (self.H + 55) // 56
+        buf = (C.c_uint32 * (ntx * nty))()
+        _check(lib.stereo_learned_tile_masks(self.handle, buf, len(buf)), self.ctx)
+        return np.array(buf, np.uint32).reshape(nty, ntx)
 
     def run(self, L, R, maps=False):
         dl = DeviceImage.from_array(self.ctx, L, PIXEL_LUM_U8)

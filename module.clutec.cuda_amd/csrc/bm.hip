@@ -402,32 +402,74 @@ __device__ __forceinline__ void write_pixel(const View& disp, const View& disp_l
 // Merge and check for the work-list matcher: one workgroup per frame row.  A right
 // pixel xr of tile row ty collects the partial keys of every item whose tile and
 // disparity range reach it (x0 - (de-1) <= xr <= x0 + 63 - ds); a left pixel merges
-// the left keys of its tile's items.
-__global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p)
+// the left keys of its tile's items.  The items of the tile row are staged in LDS
+// first (up to BM_LIST_ROW_ITEMS; beyond that they are read from memory).
+constexpr int BM_LIST_ROW_ITEMS = 1024;
+
+__global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p, int max_width)
 {
-    extern __shared__ uint32_t rrow[];
+    extern __shared__ uint32_t smem[];
+    uint32_t* rrow = smem;
     const int r = blockIdx.x;
     int fi = 0;
     while (fi + 1 < p.nframes && p.frames[fi + 1].row_base <= r) ++fi;
     const BmFrame& f = p.frames[fi];
     const int y = r - f.row_base;
     const int W = f.W;
+    const int ntx = f.ntx;
     const int ty = y / TH, ry = y - ty * TH;
-    const BmTile* trow = p.tiles + f.tile_base + ty * f.ntx;
+    const BmTile* trow = p.tiles + f.tile_base + ty * ntx;
     const bool row_ok = y >= f.by && y < f.H - f.by;
+
+    // tile -> [toff[tx], toff[tx+1]) in the staged item table (slot, ds, de)
+    int* toff = reinterpret_cast<int*>(smem + ((max_width + 1) & ~1));  // even offsets keep irng 8-byte aligned
+    int* islot = toff + (BM_MAX_WIDTH / TW + 2);
+    int2* irng = reinterpret_cast<int2*>(islot + BM_LIST_ROW_ITEMS);
+    __shared__ int staged;
+    if (threadIdx.x == 0) {
+        int n = 0;
+        for (int tx = 0; tx < ntx; ++tx) {
+            toff[tx] = n;
+            n += trow[tx].nitems;
+        }
+        toff[ntx] = n;
+        staged = n <= BM_LIST_ROW_ITEMS;
+    }
+    __syncthreads();
+    if (staged) {
+        for (int tx = threadIdx.x; tx < ntx; tx += 256) {
+            const BmTile t = trow[tx];
+            for (int k = 0; k < t.nitems; ++k) {
+                const int j = t.first_item + k;
+                islot[toff[tx] + k] = j;
+                irng[toff[tx] + k] = make_int2(p.items[j].ds, p.items[j].de);
+            }
+        }
+        __syncthreads();
+    }
     for (int x = threadIdx.x; x < W; x += 256) {
         uint32_t best = 0;
         if (row_ok && x >= f.bx && x < W - f.bx) {
             const int tlo = x / TW;  // the first tile whose pixels reach x
-            const int thi = min(f.ntx - 1, (x + f.D - 1) / 64);
+            const int thi = min(ntx - 1, (x + f.D - 1) / 64);
             for (int tx = tlo; tx <= thi; ++tx) {
-                const BmTile t = trow[tx];
                 const int x0 = tx * TW;
-                for (int j = t.first_item; j < t.first_item + t.nitems; ++j) {
-                    const int ds = p.items[j].ds, de = p.items[j].de;
-                    if (x >= x0 - (de - 1) && x <= x0 + 63 - ds) {
-                        const uint32_t v = p.rpart[((size_t)j * TH + ry) * p.rp_stride + (x - x0 + de - 1)];
-                        best = v > best ? v : best;
+                if (staged) {
+                    for (int k = toff[tx]; k < toff[tx + 1]; ++k) {
+                        const int2 rg = irng[k];
+                        if (x >= x0 - (rg.y - 1) && x <= x0 + 63 - rg.x) {
+                            const uint32_t v = p.rpart[((size_t)islot[k] * TH + ry) * p.rp_stride + (x - x0 + rg.y - 1)];
+                            best = v > best ? v : best;
+                        }
+                    }
+                } else {
+                    const BmTile t = trow[tx];
+                    for (int j = t.first_item; j < t.first_item + t.nitems; ++j) {
+                        const int ds = p.items[j].ds, de = p.items[j].de;
+                        if (x >= x0 - (de - 1) && x <= x0 + 63 - ds) {
+                            const uint32_t v = p.rpart[((size_t)j * TH + ry) * p.rp_stride + (x - x0 + de - 1)];
+                            best = v > best ? v : best;
+                        }
                     }
                 }
             }
@@ -439,8 +481,11 @@ __global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p)
         const bool inside = row_ok && x >= f.bx && x < W - f.bx;
         uint32_t lk = 0;
         if (inside) {
-            const BmTile t = trow[x / TW];
-            for (int j = t.first_item; j < t.first_item + t.nitems; ++j) {
+            const int tx = x / TW;
+            const int kb = staged ? toff[tx] : trow[tx].first_item;
+            const int ke = staged ? toff[tx + 1] : trow[tx].first_item + trow[tx].nitems;
+            for (int k = kb; k < ke; ++k) {
+                const int j = staged ? islot[k] : k;
                 const uint32_t v = p.lkey[((size_t)j * TH + ry) * TW + (x & (TW - 1))];
                 lk = v > lk ? v : lk;
             }
@@ -477,8 +522,9 @@ hipError_t launch_bm_list(int kind, int ww, const BmListParams& p, int grid, hip
 hipError_t launch_bm_list_finish(const BmListParams& p, int max_width, hipStream_t s)
 {
     if (max_width > BM_MAX_WIDTH || p.total_rows < 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(bm::bm_list_finish_kernel, dim3(p.total_rows), dim3(256), (size_t)max_width * sizeof(uint32_t), s,
-                       p);
+    const size_t lds = ((size_t)(max_width + 1) & ~(size_t)1) * 4 + (BM_MAX_WIDTH / bm::TW + 2) * 4 +
+                       (size_t)bm::BM_LIST_ROW_ITEMS * 12;
+    hipLaunchKernelGGL(bm::bm_list_finish_kernel, dim3(p.total_rows), dim3(256), lds, s, p, max_width);
     return hipGetLastError();
 }
 

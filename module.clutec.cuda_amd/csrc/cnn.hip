@@ -341,12 +341,51 @@ __global__ __launch_bounds__(256) void labels_kernel(View disp, int n, int W4, i
     labels[((size_t)n * H4 + Y) * W4 + X] = v;
 }
 
-// Per-tile disparity ranges and the work list (one workgroup of 1024 lanes).
-// Tile (tx, ty) covers quarter pixels [16 tx, 16 tx + 15] x [14 ty, 14 ty + 13]
-// (64 x 56 full-resolution pixels); with a margin of one quarter pixel it takes the
-// min / max of the proposed centres c = dfill + out0 and the smallest confidence.
-// Range (full resolution): [4 cmin - band/2 - 4, 4 cmax + band/2 + 4], aligned to 8,
-// clamped to [0, D); the full range when the confidence is below the threshold.
+// Per-tile disparity sets, one wave per tile.  Tile (tx, ty) covers quarter pixels
+// [16 tx, 16 tx + 15] x [14 ty, 14 ty + 13] (64 x 56 full-resolution pixels); with a
+// margin of one quarter pixel, every quarter pixel proposes the band
+// [4 c - band/2 - 4, 4 c + band/2 + 4] around its centre c = dfill + out0.  The
+// tile keeps the union as a mask of 8-disparity bins (bit b: [8b, 8b + 8)), or all
+// bins below D when its smallest confidence logit is below the threshold.
+__global__ __launch_bounds__(256) void band_mask_kernel(CnnBandArgs a)
+{
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int ntiles = a.ntx * a.nty;
+    if (t >= ntiles) return;
+    const int tx = t % a.ntx, ty = t / a.ntx;
+    const int qx0 = max(0, 16 * tx - 1), qx1 = min(a.W4 - 1, 16 * tx + 16);
+    const int qy0 = max(0, 14 * ty - 1), qy1 = min(a.H4 - 1, 14 * ty + 14);
+    const int nx = qx1 - qx0 + 1, ny = qy1 - qy0 + 1;
+    const int nbins = a.D >> 3;
+    uint64_t mask = 0;
+    float lmin = 1e30f;
+    for (int k = lane; k < nx * ny && nx > 0 && ny > 0; k += 64) {
+        const int qy = qy0 + k / nx, qx = qx0 + k % nx;
+        const size_t p = (size_t)qy * a.W4 + qx;
+        const float c = 4.f * (a.dfill[p] + a.out[p * 16]);
+        lmin = fminf(lmin, a.out[p * 16 + 1]);
+        const int lo = max(0, (int)floorf(c) - a.band / 2 - 4);
+        const int hi = min(a.D - 1, (int)ceilf(c) + a.band / 2 + 4);
+        if (lo <= hi) {
+            const int b0 = lo >> 3, b1 = hi >> 3;
+            mask |= ((2ull << b1) - 1ull) & ~((1ull << b0) - 1ull);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mask |= (uint64_t)__shfl_xor((unsigned long long)mask, o);
+        lmin = fminf(lmin, __shfl_xor(lmin, o));
+    }
+    const uint64_t all = (nbins >= 64) ? ~0ull : ((1ull << nbins) - 1ull);
+    const bool fallback = nx <= 0 || ny <= 0 || !(lmin >= a.logit_thresh) || mask == 0;
+    if (lane == 0) {
+        a.tile_mask[t] = (uint32_t)(fallback ? all : (mask & all));
+        a.tile_evals[t] = fallback ? (1 << 16) : 0;  // the count is filled in by the item kernel
+    }
+}
+
+// Work items of the bin masks (one workgroup of 1024 lanes): every run of set bins
+// becomes items of at most `chunk` disparities; a block-wide scan places them.
 __global__ __launch_bounds__(1024) void band_items_kernel(CnnBandArgs a)
 {
     __shared__ int scan[1024];
@@ -356,32 +395,17 @@ __global__ __launch_bounds__(1024) void band_items_kernel(CnnBandArgs a)
     const int ntiles = a.ntx * a.nty;
     for (int base = 0; base < ntiles; base += 1024) {
         const int t = base + threadIdx.x;
-        int lo = 0, hi = 0, cnt = 0, fallback = 0;
+        uint32_t m = 0;
+        int cnt = 0;
         if (t < ntiles) {
-            const int tx = t % a.ntx, ty = t / a.ntx;
-            const int qx0 = max(0, 16 * tx - 1), qx1 = min(a.W4 - 1, 16 * tx + 16);
-            const int qy0 = max(0, 14 * ty - 1), qy1 = min(a.H4 - 1, 14 * ty + 14);
-            float cmin = 1e30f, cmax = -1e30f, lmin = 1e30f;
-            for (int qy = qy0; qy <= qy1; ++qy)
-                for (int qx = qx0; qx <= qx1; ++qx) {
-                    const size_t p = (size_t)qy * a.W4 + qx;
-                    const float c = a.dfill[p] + a.out[p * 16];
-                    cmin = fminf(cmin, c);
-                    cmax = fmaxf(cmax, c);
-                    lmin = fminf(lmin, a.out[p * 16 + 1]);
-                }
-            if (qx0 > qx1 || qy0 > qy1 || !(lmin >= a.logit_thresh)) {
-                lo = 0;
-                hi = a.D;
-                fallback = 1;
-            } else {
-                lo = (int)floorf(4.f * cmin) - a.band / 2 - 4;
-                hi = (int)ceilf(4.f * cmax) + a.band / 2 + 4 + 1;
-                lo = max(0, lo) & ~7;
-                hi = min(a.D, (hi + 7) & ~7);
-                if (hi <= lo) { lo = max(0, min(lo, a.D - 8)); hi = lo + 8; }
+            m = a.tile_mask[t];
+            for (uint32_t r = m; r;) {
+                const int r0 = __builtin_ctz(r);
+                const uint32_t sh = r >> r0;
+                const int len = (~sh == 0u) ? 32 - r0 : __builtin_ctz(~sh);
+                cnt += (8 * len + a.chunk - 1) / a.chunk;
+                r &= ~((len >= 32 ? 0xFFFFFFFFu : ((1u << len) - 1u)) << r0);
             }
-            cnt = (hi - lo + a.chunk - 1) / a.chunk;
         }
         scan[threadIdx.x] = cnt;
         __syncthreads();
@@ -393,22 +417,33 @@ __global__ __launch_bounds__(1024) void band_items_kernel(CnnBandArgs a)
         }
         const int first = carry + scan[threadIdx.x] - cnt;
         if (t < ntiles) {
+            int k = first, evals = 0, lo = 1 << 30, hi = 0;
+            for (uint32_t r = m; r;) {
+                const int r0 = __builtin_ctz(r);
+                const uint32_t sh = r >> r0;
+                const int len = (~sh == 0u) ? 32 - r0 : __builtin_ctz(~sh);
+                const int ds0 = 8 * r0, de0 = 8 * (r0 + len);
+                for (int ds = ds0; ds < de0; ds += a.chunk, ++k) {
+                    BmItem it;
+                    it.frame = 0;
+                    it.tx = t % a.ntx;
+                    it.ty = t / a.ntx;
+                    it.ds = ds;
+                    it.de = min(de0, ds + a.chunk);
+                    a.items[k] = it;
+                }
+                evals += de0 - ds0;
+                lo = min(lo, ds0);
+                hi = max(hi, de0);
+                r &= ~((len >= 32 ? 0xFFFFFFFFu : ((1u << len) - 1u)) << r0);
+            }
             BmTile bt;
-            bt.lo = lo;
-            bt.hi = hi;
+            bt.lo = cnt ? lo : 0;
+            bt.hi = cnt ? hi : 0;
             bt.first_item = first;
             bt.nitems = cnt;
             a.tiles[t] = bt;
-            for (int k = 0; k < cnt; ++k) {
-                BmItem it;
-                it.frame = 0;
-                it.tx = t % a.ntx;
-                it.ty = t / a.ntx;
-                it.ds = lo + k * a.chunk;
-                it.de = min(hi, it.ds + a.chunk);
-                a.items[first + k] = it;
-            }
-            a.tile_evals[t] = (hi - lo) | (fallback << 16);
+            a.tile_evals[t] = evals | (a.tile_evals[t] & (1 << 16));
         }
         __syncthreads();
         if (threadIdx.x == 1023) carry += scan[1023];
@@ -421,11 +456,15 @@ __global__ __launch_bounds__(1024) void band_items_kernel(CnnBandArgs a)
 
 hipError_t launch_cnn_conv(int cin, int cout, int epi, const CnnConvArgs& a, hipStream_t s)
 {
+    // each wave loads the layer's weights into VGPRs once and then walks about four
+    // 16-pixel blocks, so that the weight loads are amortised
     const int nbx = (a.W + 15) / 16;
     const long total = (long)a.N * a.H * nbx;
-    int grid = (int)((total + 3) / 4);
-    if (grid > 2048) grid = 2048;
-    if (grid < 1) return hipErrorInvalidValue;
+    long g = (total + 15) / 16;
+    if (g > 1024) g = 1024;
+    if (g < 1) g = 1;
+    const int grid = (int)g;
+    if (total < 1) return hipErrorInvalidValue;
 #define CNN_CONV(CI, CO, E)                                                                             \
     if (cin == CI && cout == CO && epi == E) {                                                          \
         hipLaunchKernelGGL((cnn::conv3x3_kernel<CI, CO, E>), dim3(grid), dim3(256), 0, s, a);         \
@@ -506,6 +545,11 @@ hipError_t launch_cnn_input(const View& L4, const View& R4, const View& dc, int 
 
 hipError_t launch_band_items(const CnnBandArgs& a, hipStream_t s)
 {
+    if (a.D > 256 || a.D % 8 || a.chunk < 8 || a.chunk % 8) return hipErrorInvalidValue;
+    const int ntiles = a.ntx * a.nty;
+    hipLaunchKernelGGL(cnn::band_mask_kernel, dim3((ntiles + 3) / 4), dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(cnn::band_items_kernel, dim3(1), dim3(1024), 0, s, a);
     return hipGetLastError();
 }

@@ -170,6 +170,7 @@ struct CnnBandArgs {
     BmItem* items;
     int* nitems;
     int* tile_evals;       // [nty][ntx] disparities searched per tile | fallback << 16
+    uint32_t* tile_mask;   // [nty][ntx] searched 8-disparity bins
     int W4, H4, ntx, nty, D;
     int band, chunk;
     float logit_thresh;

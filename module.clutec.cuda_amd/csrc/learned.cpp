@@ -488,20 +488,37 @@ struct LearnedExt : DriverExt {
     stereo::BmItem* items = nullptr;
     int* nitems = nullptr;
     int* tile_evals = nullptr;
+    uint32_t* tile_mask = nullptr;
     uint32_t* lkey = nullptr;
     uint32_t* rpart = nullptr;
     int ntx = 0, nty = 0, max_items = 0;
     hipEvent_t ev[6] = {};
     bool timed = false;
+    // the launch sequence captured for a given pair of input images
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* graph_key[2] = {nullptr, nullptr};
+
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_key[0] = graph_key[1] = nullptr;
+    }
 
     ~LearnedExt() override
     {
+        drop_graph();
+        if (cap) (void)hipStreamDestroy(cap);
         if (coarse) stereo_driver_destroy(coarse);
         if (cnn) stereo_cnn_destroy(cnn);
         for (stereo_image* im : {&L4, &R4, &dc})
             if (im->data) (void)hipFree(im->data);
         for (void* p : {(void*)padL, (void*)padR, (void*)frame_dev, (void*)tiles, (void*)items, (void*)nitems,
-                        (void*)tile_evals, (void*)lkey, (void*)rpart})
+                        (void*)tile_evals, (void*)tile_mask, (void*)lkey, (void*)rpart})
             if (p) (void)hipFree(p);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -569,7 +586,8 @@ stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** out,
     d->by = p->matcher.window_h / 2 + p->matcher.census_h / 2;
     x->ntx = (W + stereo::BM_TILE_W - 1) / stereo::BM_TILE_W;
     x->nty = (H + stereo::BM_TILE_H - 1) / stereo::BM_TILE_H;
-    x->max_items = x->ntx * x->nty * ((D + p->chunk - 1) / p->chunk);
+    // at most 16 runs of bins per tile, each cut into pieces of at most `chunk`
+    x->max_items = x->ntx * x->nty * (16 + (D + p->chunk - 1) / p->chunk);
     const size_t cols = (size_t)stereo::BM_PAD_LEFT + W + stereo::BM_PAD_RIGHT;
     x->pad_pitch = (cols * 8 + 255) & ~(size_t)255;
     hipError_t e;
@@ -579,6 +597,8 @@ stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** out,
         (e = hipMalloc((void**)&x->tiles, nt * sizeof(stereo::BmTile))) ||
         (e = hipMalloc((void**)&x->items, x->max_items * sizeof(stereo::BmItem))) ||
         (e = hipMalloc((void**)&x->nitems, sizeof(int))) || (e = hipMalloc((void**)&x->tile_evals, nt * sizeof(int))) ||
+        (e = hipMalloc((void**)&x->tile_mask, nt * sizeof(uint32_t))) ||
+        (e = hipStreamCreateWithFlags(&x->cap, hipStreamNonBlocking)) ||
         (e = hipMalloc((void**)&x->lkey, stereo::bm_list_lkey_elems(x->max_items) * 4)) ||
         (e = hipMalloc((void**)&x->rpart, stereo::bm_list_rpart_elems(x->max_items, p->chunk) * 4)))
         return cleanup(hip_fail(ctx, e, "hipMalloc"));
@@ -603,6 +623,65 @@ stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** out,
     *out = d;
     return STEREO_OK;
 }
+
+namespace {
+
+// The whole learned pipeline on stream s; with `timed`, events between the stages.
+stereo_status learned_enqueue(stereo_driver* d, LearnedExt* x, const stereo_image* left, const stereo_image* right,
+                              hipStream_t s, bool timed)
+{
+    stereo_context* ctx = d->ctx;
+    const int W = d->fmt.width, H = d->fmt.height;
+    const stereo::BmFrame& f = x->frame;
+    stereo_status st;
+    if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[0], s), "hipEventRecord");
+    const View vl = view_of(left), vr = view_of(right);
+    HIP_TRY(ctx, stereo::launch_census_pair(vl, f.L, &vr, &f.R, d->mp.census_w, d->mp.census_h, s), "census launch");
+    HIP_TRY(ctx, stereo::launch_downsample4(vl, view_of(&x->L4), s), "downsample launch");
+    HIP_TRY(ctx, stereo::launch_downsample4(vr, view_of(&x->R4), s), "downsample launch");
+    if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[1], s), "hipEventRecord");
+    if ((st = stereo_matcher_process(x->coarse, &x->dc, nullptr, nullptr, &x->L4, &x->R4, s))) return st;
+    if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[2], s), "hipEventRecord");
+    if ((st = stereo_cnn_set_input(x->cnn, 0, &x->L4, &x->R4, &x->dc, s))) return st;
+    if ((st = stereo_cnn_forward(x->cnn, 1, nullptr, s))) return st;
+    if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[3], s), "hipEventRecord");
+    stereo::CnnBandArgs ba;
+    ba.out = x->cnn->out;
+    ba.dfill = x->cnn->dfill;
+    ba.tiles = x->tiles;
+    ba.items = x->items;
+    ba.nitems = x->nitems;
+    ba.tile_evals = x->tile_evals;
+    ba.tile_mask = x->tile_mask;
+    ba.W4 = W / 4;
+    ba.H4 = H / 4;
+    ba.ntx = x->ntx;
+    ba.nty = x->nty;
+    ba.D = d->mp.num_disparities;
+    ba.band = x->lp.band;
+    ba.chunk = x->lp.chunk;
+    ba.logit_thresh = x->lp.logit_thresh;
+    HIP_TRY(ctx, stereo::launch_band_items(ba, s), "band launch");
+    stereo::BmListParams lp;
+    lp.frames = x->frame_dev;
+    lp.tiles = x->tiles;
+    lp.items = x->items;
+    lp.nitems = x->nitems;
+    lp.sched = nullptr;
+    lp.lkey = x->lkey;
+    lp.rpart = x->rpart;
+    lp.rp_stride = stereo::bm_list_rp_stride(x->lp.chunk);
+    lp.nframes = 1;
+    lp.total_rows = H;
+    const int grid = x->max_items < 768 ? x->max_items : 768;
+    HIP_TRY(ctx, stereo::launch_bm_list(stereo::BM_COST_CENSUS, d->mp.window_w, lp, grid, s), "list matcher launch");
+    if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[4], s), "hipEventRecord");
+    HIP_TRY(ctx, stereo::launch_bm_list_finish(lp, W, s), "list finish launch");
+    if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[5], s), "hipEventRecord");
+    return STEREO_OK;
+}
+
+}  // namespace
 
 stereo_status stereo_learned_process(stereo_driver* d, stereo_image* disp, stereo_image* disp_lr, stereo_image* disp_rl,
                                      const stereo_image* left, const stereo_image* right, stereo_stream stream)
@@ -633,50 +712,27 @@ stereo_status stereo_learned_process(stereo_driver* d, stereo_image* disp, stere
         x->frame = f;
     }
 
+    // timed calls run the stages directly with events between them; otherwise the
+    // sequence is captured once per pair of input images and replayed as a graph
     x->timed = ctx->timing != 0;
-    if (x->timed) HIP_TRY(ctx, hipEventRecord(x->ev[0], s), "hipEventRecord");
-    const View vl = view_of(left), vr = view_of(right);
-    HIP_TRY(ctx, stereo::launch_census_pair(vl, f.L, &vr, &f.R, d->mp.census_w, d->mp.census_h, s), "census launch");
-    HIP_TRY(ctx, stereo::launch_downsample4(vl, view_of(&x->L4), s), "downsample launch");
-    HIP_TRY(ctx, stereo::launch_downsample4(vr, view_of(&x->R4), s), "downsample launch");
-    if (x->timed) HIP_TRY(ctx, hipEventRecord(x->ev[1], s), "hipEventRecord");
-    if ((st = stereo_matcher_process(x->coarse, &x->dc, nullptr, nullptr, &x->L4, &x->R4, stream))) return st;
-    if (x->timed) HIP_TRY(ctx, hipEventRecord(x->ev[2], s), "hipEventRecord");
-    if ((st = stereo_cnn_set_input(x->cnn, 0, &x->L4, &x->R4, &x->dc, stream))) return st;
-    if ((st = stereo_cnn_forward(x->cnn, 1, nullptr, stream))) return st;
-    if (x->timed) HIP_TRY(ctx, hipEventRecord(x->ev[3], s), "hipEventRecord");
-    stereo::CnnBandArgs ba;
-    ba.out = x->cnn->out;
-    ba.dfill = x->cnn->dfill;
-    ba.tiles = x->tiles;
-    ba.items = x->items;
-    ba.nitems = x->nitems;
-    ba.tile_evals = x->tile_evals;
-    ba.W4 = W / 4;
-    ba.H4 = H / 4;
-    ba.ntx = x->ntx;
-    ba.nty = x->nty;
-    ba.D = d->mp.num_disparities;
-    ba.band = x->lp.band;
-    ba.chunk = x->lp.chunk;
-    ba.logit_thresh = x->lp.logit_thresh;
-    HIP_TRY(ctx, stereo::launch_band_items(ba, s), "band launch");
-    stereo::BmListParams lp;
-    lp.frames = x->frame_dev;
-    lp.tiles = x->tiles;
-    lp.items = x->items;
-    lp.nitems = x->nitems;
-    lp.sched = nullptr;
-    lp.lkey = x->lkey;
-    lp.rpart = x->rpart;
-    lp.rp_stride = stereo::bm_list_rp_stride(x->lp.chunk);
-    lp.nframes = 1;
-    lp.total_rows = H;
-    const int grid = x->max_items < 768 ? x->max_items : 768;
-    HIP_TRY(ctx, stereo::launch_bm_list(stereo::BM_COST_CENSUS, d->mp.window_w, lp, grid, s), "list matcher launch");
-    if (x->timed) HIP_TRY(ctx, hipEventRecord(x->ev[4], s), "hipEventRecord");
-    HIP_TRY(ctx, stereo::launch_bm_list_finish(lp, W, s), "list finish launch");
-    if (x->timed) HIP_TRY(ctx, hipEventRecord(x->ev[5], s), "hipEventRecord");
+    if (x->timed) return learned_enqueue(d, x, left, right, s, true);
+    if (!x->exec || x->graph_key[0] != left->data || x->graph_key[1] != right->data) {
+        x->drop_graph();
+        HIP_TRY(ctx, hipStreamBeginCapture(x->cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        const stereo_status cst = learned_enqueue(d, x, left, right, x->cap, false);
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(x->cap, &g);
+        if (cst) {
+            if (g) (void)hipGraphDestroy(g);
+            return cst;
+        }
+        HIP_TRY(ctx, ee, "hipStreamEndCapture");
+        x->graph = g;
+        HIP_TRY(ctx, hipGraphInstantiate(&x->exec, x->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        x->graph_key[0] = left->data;
+        x->graph_key[1] = right->data;
+    }
+    HIP_TRY(ctx, hipGraphLaunch(x->exec, s), "hipGraphLaunch");
     return STEREO_OK;
 }
 
@@ -711,6 +767,17 @@ stereo_status stereo_learned_stats(stereo_driver* d, int64_t* evals, int32_t* ti
             stage_ms[i] = t;
         }
     }
+    return STEREO_OK;
+}
+
+stereo_status stereo_learned_tile_masks(stereo_driver* d, uint32_t* masks, int32_t n)
+{
+    if (!d || d->kind != stereo_driver::LEARNED || !masks) return STEREO_ERROR_INVALID_ARGUMENT;
+    LearnedExt* x = static_cast<LearnedExt*>(d->ext);
+    const int nt = x->ntx * x->nty;
+    if (n < nt) return fail(d->ctx, STEREO_ERROR_INVALID_ARGUMENT, "need %d entries", nt);
+    HIP_TRY(d->ctx, hipDeviceSynchronize(), "hipDeviceSynchronize");
+    HIP_TRY(d->ctx, hipMemcpy(masks, x->tile_mask, nt * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy");
     return STEREO_OK;
 }
 

@@ -46,6 +46,8 @@ def _load():
     lib.orc_bm_brute.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P]
     lib.orc_bm_ranges.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_bm_ranges.restype = i
+    lib.orc_bm_bins.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P, i]
+    lib.orc_bm_bins.restype = i
     lib.orc_ncc.argtypes = [P, P, i, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_aw.argtypes = [i, P, P, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_crosscheck_plain.argtypes = [P, P, i, i, i, P]
@@ -112,6 +114,20 @@ def bm_ranges(kind, L, R, D, lo, hi, window=(9, 9), census_win=(1, 1), tol=1, nt
     out = np.empty((H, W), np.int16)
     _chk(lib.orc_bm_ranges(kind, _p(L), _p(R), W, H, D, window[0], window[1], census_win[0], census_win[1], tol,
                            _p(lo), _p(hi), _p(dL), _p(dR), _p(out), nthreads), "orc_bm_ranges")
+    return out, dL, dR
+
+
+def bm_bins(kind, L, R, D, bins, window=(9, 9), census_win=(1, 1), tol=1, nthreads=1):
+    """orc_bm restricted per left pixel to the 8-disparity bins set in bins (uint32)."""
+    L = np.ascontiguousarray(L)
+    R = np.ascontiguousarray(R)
+    H, W = L.shape
+    bins = np.ascontiguousarray(bins, np.uint32)
+    dL = np.empty((H, W), np.int16)
+    dR = np.empty((H, W), np.int16)
+    out = np.empty((H, W), np.int16)
+    _chk(lib.orc_bm_bins(kind, _p(L), _p(R), W, H, D, window[0], window[1], census_win[0], census_win[1], tol,
+                         _p(bins), _p(dL), _p(dR), _p(out), nthreads), "orc_bm_bins")
     return out, dL, dR
 
 

@@ -52,6 +52,10 @@ static void run_matchers(int W, int H, int D)
     EXPECT(orc_bm(ORC_COST_SAD, L, R, W, H, D, 9, 9, 1, 1, 1, dL, dR, out, cost, 3) == 0);
     EXPECT(orc_bm(ORC_COST_CENSUS, cl, cr, W, H, D, 5, 5, 9, 7, 1, dL, dR, out, NULL, 2) == 0);
     EXPECT(orc_bm_ranges(ORC_COST_CENSUS, cl, cr, W, H, D, 9, 9, 9, 7, 1, lo, hi, dL, dR, out, 2) == 0);
+    uint32_t* bins = malloc(N * 4);
+    for (size_t i = 0; i < N; ++i) bins[i] = next();
+    EXPECT(orc_bm_bins(ORC_COST_SAD, L, R, W, H, D, 5, 5, 1, 1, 1, bins, dL, dR, out, 2) == 0);
+    free(bins);
     EXPECT(orc_bm_brute(ORC_COST_SAD, L, R, W, H, D, 7, 7, 1, 1, 1, dL, dR, out, vol) == 0);
     EXPECT(orc_ncc(L, R, W, H, D, 11, 11, 1, dL, dR, out, sc, vd, 2) == 0);
     EXPECT(orc_aw(2, L, R, W, H, D, 5, 1, dL, dR, out, cost, vol, 2) == 0);

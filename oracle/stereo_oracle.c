@@ -146,8 +146,8 @@ static int bm_args_ok(int kind, const void* L, const void* R, int W, int H, int 
  * both winner-take-alls -- the band search of the learned proposal, whose ranges
  * are per 64 x 56 tile and expanded to pixels by the caller. */
 static int bm_impl(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
-                   int ch, int tol, const int16_t* lo, const int16_t* hi, int16_t* dL, int16_t* dR,
-                   int16_t* out, int32_t* cost_out, int nthreads)
+                   int ch, int tol, const int16_t* lo, const int16_t* hi, const uint32_t* bins, int16_t* dL,
+                   int16_t* dR, int16_t* out, int32_t* cost_out, int nthreads)
 {
     if (!bm_args_ok(kind, L, R, W, H, D, ww, wh, cw, ch) || !dL || !dR) return -1;
     const int ax = ww / 2, ay = wh / 2;
@@ -194,6 +194,7 @@ static int bm_impl(int kind, const void* L, const void* R, int W, int H, int D, 
                     for (int x = xs; x < xe; ++x) {
                         const int A = V[x];
                         if (lo && (d < lo[row + x] || d >= hi[row + x])) continue;
+                        if (bins && !((bins[row + x] >> (d >> 3)) & 1u)) continue;
                         if (A < bestL[row + x]) { bestL[row + x] = A; dL[row + x] = (int16_t)d; }
                         if (A < bestR[row + x - d]) { bestR[row + x - d] = A; dR[row + x - d] = (int16_t)d; }
                     }
@@ -214,7 +215,7 @@ static int bm_impl(int kind, const void* L, const void* R, int W, int H, int D, 
 int orc_bm(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
            int ch, int tol, int16_t* dL, int16_t* dR, int16_t* out, int32_t* cost_out, int nthreads)
 {
-    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, NULL, NULL, dL, dR, out, cost_out, nthreads);
+    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, NULL, NULL, NULL, dL, dR, out, cost_out, nthreads);
 }
 
 int orc_bm_ranges(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
@@ -222,7 +223,14 @@ int orc_bm_ranges(int kind, const void* L, const void* R, int W, int H, int D, i
                   int16_t* out, int nthreads)
 {
     if (!lo || !hi) return -1;
-    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, lo, hi, dL, dR, out, NULL, nthreads);
+    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, lo, hi, NULL, dL, dR, out, NULL, nthreads);
+}
+
+int orc_bm_bins(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
+                int ch, int tol, const uint32_t* bins, int16_t* dL, int16_t* dR, int16_t* out, int nthreads)
+{
+    if (!bins || D > 256) return -1;
+    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, NULL, NULL, bins, dL, dR, out, NULL, nthreads);
 }
 
 int orc_bm_brute(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh,

@@ -52,6 +52,10 @@ int orc_bm(int cost_kind, const void* L, const void* R, int W, int H, int D, int
 int orc_bm_ranges(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
                   int ch, int tol, const int16_t* lo, const int16_t* hi, int16_t* dL, int16_t* dR,
                   int16_t* out, int nthreads);
+/* orc_bm restricted to candidates whose 8-disparity bin d >> 3 is set in bins[p]
+ * of the left pixel p (D <= 256). */
+int orc_bm_bins(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
+                int ch, int tol, const uint32_t* bins, int16_t* dL, int16_t* dR, int16_t* out, int nthreads);
 int orc_bm_brute(int cost_kind, const void* L, const void* R, int W, int H, int D, int ww, int wh,
                  int cw, int ch, int tol, int16_t* dL, int16_t* dR, int16_t* out, int32_t* volume);
 

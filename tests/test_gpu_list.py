@@ -124,20 +124,42 @@ def test_learned_full_fallback_equals_full_range(ctx):
 
 
 def test_learned_band_matches_oracle(ctx):
-    """Band search (no fallback): every tile searches only its proposed range; the
-    maps equal the oracle restricted to the same per-tile ranges."""
+    """Band search (no fallback): every tile searches only its proposed disparity
+    bins; the maps equal the oracle restricted to the same per-tile bins."""
     W, H, D = 384, 224, 128
     L, R = cs.synth_pair(78, W, H, 2, 8, D - 1, 8)
-    m = cs.LearnedMatcher(ctx, W, H, cs.cnn_init_params(0), num_disp=D, logit_thresh=-1e30)
+    m = cs.LearnedMatcher(ctx, W, H, cs.cnn_init_params(0), num_disp=D, logit_thresh=-1e30, chunk=32)
     got = m.run(L, R, maps=True)
-    rng = m.tile_ranges()
+    masks = m.tile_masks()
     st = m.stats()
     assert st["fallback_tiles"] == 0
-    lo = np.repeat(np.repeat(rng[:, :, 0], 56, axis=0), 64, axis=1)[:H, :W].astype(np.int16)
-    hi = np.repeat(np.repeat(rng[:, :, 1], 56, axis=0), 64, axis=1)[:H, :W].astype(np.int16)
-    assert (lo % 8 == 0).all() and (hi % 8 == 0).all() and (lo < hi).all() and (hi <= D).all()
-    assert st["evals"] == int(((hi - lo).astype(np.int64)).sum())
-    ref = orc.bm_ranges(orc.COST_CENSUS, orc.census(L, 9, 7), orc.census(R, 9, 7), D, lo, hi, window=(9, 9),
-                        census_win=(9, 7), tol=1, nthreads=8)
+    bins = np.repeat(np.repeat(masks, 56, axis=0), 64, axis=1)[:H, :W]
+    assert (masks != 0).all() and (masks < (1 << (D // 8))).all()
+    nbits = np.array([bin(int(v)).count("1") for v in bins.ravel()]).reshape(H, W)
+    assert st["evals"] == int((8 * nbits).astype(np.int64).sum())
+    ref = orc.bm_bins(orc.COST_CENSUS, orc.census(L, 9, 7), orc.census(R, 9, 7), D, bins, window=(9, 9),
+                      census_win=(9, 7), tol=1, nthreads=8)
     for a, c in zip(got, ref):
         assert np.array_equal(a, c)
+
+
+def test_learned_graph_replays(ctx):
+    """Untimed calls replay a captured graph; replays with new content in the same
+    input buffers, and a timed direct run, all give the same maps as fresh runs."""
+    W, H, D = 384, 224, 128
+    params = cs.cnn_init_params(3)
+    m = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, logit_thresh=-2.0)
+    dl = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_U8)
+    dr = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_U8)
+    out = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_I16)
+    for rep, seed in enumerate((80, 81, 80)):
+        L, R = cs.synth_pair(seed, W, H, 2, 8, D - 1, 8)
+        dl.upload(L)
+        dr.upload(R)
+        m.process(out, None, None, dl, dr)
+        got = out.download()
+        fresh = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, logit_thresh=-2.0)
+        ctx.set_timing(True)
+        want = fresh.run(L, R)
+        ctx.set_timing(False)
+        assert np.array_equal(got, want), f"replay {rep}"
