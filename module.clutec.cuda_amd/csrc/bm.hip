@@ -228,24 +228,29 @@ __device__ __forceinline__ void run_job(const Job& j, uint32_t (*hbuf)[HROWS * H
         bm_tile<Cost, AX, AY, false>(j, hbuf, ebuf);
 }
 
-// Grid launch over one frame: blockIdx = (tile x, tile y, disparity segment).
+// Grid launch over one frame: one workgroup per (tile x, tile y, disparity segment),
+// renumbered so that every XCD works on a contiguous band of tiles.
 template <class Cost, int AX, int AY>
 __global__ __launch_bounds__(256, 2) void bm_fused_kernel(BmParams p)
 {
     __shared__ uint32_t hbuf[2][HROWS * HSTRIDE];
     __shared__ uint32_t ebuf[4][RG][64];
-    const int seg = blockIdx.z;
+    // one-dimensional grid of ntx * nty * nseg workgroups, tile x fastest
+    const int nty = (p.H + TH - 1) / TH;
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = w % p.ntx, rest = w / p.ntx;
+    const int ty = rest % nty, seg = rest / nty;
     Job j;
     j.L = p.L;
     j.R = p.R;
     j.W = p.W;
     j.H = p.H;
     j.bx = p.bx;
-    j.x0 = blockIdx.x * TW;
-    j.y0 = blockIdx.y * TH;
+    j.x0 = tx * TW;
+    j.y0 = ty * TH;
     j.ds = seg * p.seg_len;
     j.de = min(p.D, j.ds + p.seg_len);
-    j.rp = p.rpart + (((size_t)seg * p.ntx + blockIdx.x) * p.H + j.y0) * BM_NPART;
+    j.rp = p.rpart + (((size_t)seg * p.ntx + tx) * p.H + j.y0) * BM_NPART;
     j.rp_stride = BM_NPART;
     j.rp_kofs = 0;
     j.lk = p.lkey + (size_t)seg * p.W * p.H + (size_t)j.y0 * p.W + j.x0;
@@ -558,7 +563,7 @@ void bm_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len)
 template <class Cost>
 static hipError_t launch_cost(const BmParams& p, int ww, hipStream_t s)
 {
-    dim3 grid(p.ntx, (p.H + bm::TH - 1) / bm::TH, p.nseg);
+    dim3 grid(p.ntx * ((p.H + bm::TH - 1) / bm::TH) * p.nseg);
     if (ww == 9) hipLaunchKernelGGL((bm::bm_fused_kernel<Cost, 4, 4>), grid, dim3(256), 0, s, p);
     else if (ww == 7) hipLaunchKernelGGL((bm::bm_fused_kernel<Cost, 3, 3>), grid, dim3(256), 0, s, p);
     else hipLaunchKernelGGL((bm::bm_fused_kernel<Cost, 2, 2>), grid, dim3(256), 0, s, p);

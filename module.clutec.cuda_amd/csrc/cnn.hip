@@ -62,15 +62,11 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(CnnConvArgs a)
     const int nbx = (a.W + 15) >> 4;
     const int total = a.N * a.H * nbx;
     const int step = a.dil * a.sign;
-    for (int b = wave; b < total; b += nwaves) {
-        const int bxi = b % nbx;
+    // the input vectors of block b (9 taps x KC channel blocks), zero outside the image
+    auto load = [&](int b, v4s (&in)[9][KC]) {
+        const int px = (b % nbx) * 16 + col;
         const int r = b / nbx;
-        const int yy = r % a.H;
-        const int n = r / a.H;
-        const int px = bxi * 16 + col;
-        v4f acc[MB];
-#pragma unroll
-        for (int mb = 0; mb < MB; ++mb) acc[mb] = v4f{0.f, 0.f, 0.f, 0.f};
+        const int yy = r % a.H, n = r / a.H;
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int sy = yy + (t / 3 - 1) * step;
@@ -80,12 +76,35 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(CnnConvArgs a)
 #pragma unroll
             for (int kc = 0; kc < KC; ++kc) {
                 v4s bv = *reinterpret_cast<const v4s*>(src + kc * 16);
-                if (!ok) bv = v4s{0, 0, 0, 0};
-#pragma unroll
-                for (int mb = 0; mb < MB; ++mb)
-                    acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[mb][t][kc], bv, acc[mb], 0, 0, 0);
+                in[t][kc] = ok ? bv : v4s{0, 0, 0, 0};
             }
         }
+    };
+    // software pipeline: the loads of the wave's next block are in flight while
+    // the MFMAs of the current one run
+    v4s cur[9][KC], nxt[9][KC];
+    if (wave < total) load(wave, cur);
+    for (int b = wave; b < total; b += nwaves) {
+        if (b + nwaves < total) load(b + nwaves, nxt);
+        const int bxi = b % nbx;
+        const int r = b / nbx;
+        const int yy = r % a.H;
+        const int n = r / a.H;
+        const int px = bxi * 16 + col;
+        v4f acc[MB];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) acc[mb] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc)
+#pragma unroll
+                for (int mb = 0; mb < MB; ++mb)
+                    acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[mb][t][kc], cur[t][kc], acc[mb], 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) cur[t][kc] = nxt[t][kc];
         if (px < a.W) {
             const size_t o = (((size_t)n * a.H + yy) * a.W + px) * COUT;
 #pragma unroll
@@ -289,31 +308,37 @@ __global__ __launch_bounds__(256) void downsample4_kernel(View in, View out)
 // Network input of frame n: ch0 = L/255, ch1 = R(x - dfill)/255 (bf16), ch2..15 = 0,
 // where dfill is the coarse disparity with holes filled by the smaller of the
 // nearest valid values to the left and right within 64 pixels (0 when none).
-// dfill is written as float for the range kernel.
+// dfill is written as float for the range kernel.  One workgroup per row; the
+// row of coarse disparities is searched in LDS.
+constexpr int CNN_MAX_W4 = 4096;
+
 __global__ __launch_bounds__(256) void cnn_input_kernel(View L4, View R4, View dc, int n, uint16_t* x, float* dfill)
 {
-    const int X = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    __shared__ int16_t drow[CNN_MAX_W4];
+    const int Y = blockIdx.x;
     const int W = L4.w, H = L4.h;
-    if (X >= W || Y >= H) return;
-    const int16_t* drow = row_ptr<const int16_t>(dc, Y);
-    int d = drow[X];
-    if (d < 0) {
-        int a = -1, b = -1;
-        for (int k = 1; k <= 64 && (a < 0 || b < 0); ++k) {
-            if (a < 0 && X - k >= 0 && drow[X - k] >= 0) a = drow[X - k];
-            if (b < 0 && X + k < W && drow[X + k] >= 0) b = drow[X + k];
+    const int16_t* src = row_ptr<const int16_t>(dc, Y);
+    for (int X = threadIdx.x; X < W; X += 256) drow[X] = src[X];
+    __syncthreads();
+    for (int X = threadIdx.x; X < W; X += 256) {
+        int d = drow[X];
+        if (d < 0) {
+            int a = -1, b = -1;
+            for (int k = 1; k <= 64 && (a < 0 || b < 0); ++k) {
+                if (a < 0 && X - k >= 0 && drow[X - k] >= 0) a = drow[X - k];
+                if (b < 0 && X + k < W && drow[X + k] >= 0) b = drow[X + k];
+            }
+            d = a < 0 ? (b < 0 ? 0 : b) : (b < 0 ? a : min(a, b));
         }
-        d = a < 0 ? (b < 0 ? 0 : b) : (b < 0 ? a : min(a, b));
+        const size_t p = ((size_t)n * H + Y) * W + X;
+        dfill[p] = (float)d;
+        const int xr = clampi(X - d, 0, W - 1);
+        const float l = row_ptr<const uint8_t>(L4, Y)[X] * (1.0f / 255.0f);
+        const float r = row_ptr<const uint8_t>(R4, Y)[xr] * (1.0f / 255.0f);
+        uint4* o = reinterpret_cast<uint4*>(x + p * 16);
+        o[0] = make_uint4((uint32_t)f2bf(l) | ((uint32_t)f2bf(r) << 16), 0, 0, 0);
+        o[1] = make_uint4(0, 0, 0, 0);
     }
-    const size_t p = ((size_t)n * H + Y) * W + X;
-    dfill[p] = (float)d;
-    const int xr = clampi(X - d, 0, W - 1);
-    const float l = row_ptr<const uint8_t>(L4, Y)[X] * (1.0f / 255.0f);
-    const float r = row_ptr<const uint8_t>(R4, Y)[xr] * (1.0f / 255.0f);
-    uint4* o = reinterpret_cast<uint4*>(x + p * 16);
-    o[0] = make_uint4((uint32_t)f2bf(l) | ((uint32_t)f2bf(r) << 16), 0, 0, 0);
-    o[1] = make_uint4(0, 0, 0, 0);
 }
 
 // Training label of quarter pixel (X, Y) of frame n from the full-resolution checked
@@ -538,8 +563,8 @@ hipError_t launch_downsample4(const View& in, const View& out, hipStream_t s)
 hipError_t launch_cnn_input(const View& L4, const View& R4, const View& dc, int n, uint16_t* x, float* dfill,
                             hipStream_t s)
 {
-    dim3 grid((L4.w + 63) / 64, (L4.h + 3) / 4);
-    hipLaunchKernelGGL(cnn::cnn_input_kernel, grid, dim3(256), 0, s, L4, R4, dc, n, x, dfill);
+    if (L4.w > cnn::CNN_MAX_W4) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cnn::cnn_input_kernel, dim3(L4.h), dim3(256), 0, s, L4, R4, dc, n, x, dfill);
     return hipGetLastError();
 }
 

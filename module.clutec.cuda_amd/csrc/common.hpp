@@ -30,6 +30,18 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 __host__ __device__ __forceinline__ int key_disp(uint32_t key) { return key ? 255 - (int)(key & 255u) : -1; }
 __host__ __device__ __forceinline__ int key_cost(uint32_t key) { return 65535 - (int)(key >> 8); }
 
+// Workgroups of a launch are dealt to the 8 XCDs of MI355X round-robin (workgroup b
+// runs on XCD b % 8) and every XCD has its own L2.  xcd_remap renumbers them so that
+// XCD k works through the contiguous range [k q, (k + 1) q) of the work items
+// (q = n / 8), which keeps neighbouring tiles -- and the image rows they share --
+// in one L2; the last n % 8 workgroups keep their number.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_remap(int b, int n)
+{
+    const int q = n >> 3;
+    if (b >= 8 * q) return b;
+    return (b & 7) * q + (b >> 3);
+}
+
 // DPP controls (GFX9 encoding).
 constexpr int DPP_ROW_SHL1 = 0x101;
 constexpr int DPP_ROW_SHR1 = 0x111;

@@ -157,7 +157,7 @@ stereo_status stereo_cnn_create(stereo_context* ctx, stereo_cnn** out, int32_t W
 {
     if (!ctx || !out) return STEREO_ERROR_INVALID_ARGUMENT;
     *out = nullptr;
-    if (W4 < 1 || H4 < 1 || max_frames < 1 || (size_t)W4 * H4 * max_frames > ((size_t)1 << 28))
+    if (W4 < 1 || H4 < 1 || W4 > 4096 || max_frames < 1 || (size_t)W4 * H4 * max_frames > ((size_t)1 << 28))
         return fail(ctx, STEREO_ERROR_INVALID_ARGUMENT, "stereo_cnn_create: bad size %dx%d x %d", W4, H4, max_frames);
     stereo_status st = set_device(ctx);
     if (st) return st;
