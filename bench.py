@@ -27,6 +27,10 @@ sys.path.insert(0, os.path.join(ROOT, "module.clutec.cuda_amd"))
 
 W, H, D = 1920, 1080, 256
 OPS_PER_EVAL = 12
+# HBM bytes per launch of the fused census kernel on this workload, from two
+# rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 256 MiB device copy
+# (profiles/bm_census_1080p_d256.md): 42,876 KiB / 0.5 + 66,785 KiB / 1.0
+TRAFFIC_BYTES = (42876 * 2 + 66785) * 1024
 # MI355X: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T int32 lane-ops/s
 PEAK_INT_TOPS = 256 * 64 * 2.4e9 / 1e12
 PEAK_HBM_GBS = 8000.0
@@ -161,6 +165,46 @@ def batch_leg(cs, np, ctx, steps):
     return out
 
 
+def other_matchers_leg(cs, np, ctx, steps):
+    """The other matchers on their workloads: SAD 9x9 on the 640x480 pair (D=64),
+    NCC 11x11 on the 1080p pair (D=256), AW1 / AW2 with 5x5 sub-patches on the
+    1280x720 slanted-plane pair (D=128).  Whole-pipeline time per frame and the
+    fused kernel's share (HIP events on its stream)."""
+    res = {}
+    cases = [("sad9x9_640x480_d64", cs.ALGO_SAD, (9, 9), 640, 480, 64, (0, 0, 8, 63, 8), 10),
+             ("ncc11x11_1920x1080_d256", cs.ALGO_NCC, (11, 11), 1920, 1080, 256, (1, 0, 24, 255, 8), 11),
+             ("aw1_15x15_1280x720_d128", cs.ALGO_AW1, (5, 5), 1280, 720, 128, (2, 1, 6, 127, 8), None),
+             ("aw2_15x15_1280x720_d128", cs.ALGO_AW2, (5, 5), 1280, 720, 128, (2, 1, 6, 127, 8), None)]
+    for name, algo, win, w, h, d, (seed, kind, nreg, dmax, dbg), ops in cases:
+        L, R = cs.synth_pair(seed, w, h, kind, nreg, dmax, dbg)
+        m = cs.Matcher(ctx, w, h, algorithm=algo, window=win, num_disp=d, tol=1)
+        dl = cs.DeviceImage.from_array(ctx, L, cs.PIXEL_LUM_U8)
+        dr = cs.DeviceImage.from_array(ctx, R, cs.PIXEL_LUM_U8)
+        out = cs.DeviceImage(ctx, w, h, cs.PIXEL_LUM_I16)
+        for _ in range(3):
+            m.process(out, None, None, dl, dr)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.process(out, None, None, dl, dr)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        ctx.set_timing(True)
+        st = []
+        for _ in range(5):
+            m.process(out, None, None, dl, dr)
+            st.append(m.stage_times_ms())
+        ctx.set_timing(False)
+        kms = statistics.median(x[1] for x in st)
+        r = {"ms_per_frame": round(ms, 4), "M_evals_per_s": round(w * h * d / (ms * 1e-3) / 1e6, 1),
+             "kernel_ms": round(kms, 4), "stages_ms": [round(statistics.median(x[i] for x in st), 4) for i in range(3)]}
+        if ops:
+            r["kernel_valu_frac"] = round(w * h * d * ops / (kms * 1e-3) / (PEAK_INT_TOPS * 1e12), 4)
+            r["ops_per_eval"] = ops
+        res[name] = r
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +213,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-mode leg")
+    ap.add_argument("--no-others", action="store_true", help="skip the SAD / NCC / AW matcher leg")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run 5 device-to-device copies of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -244,7 +289,7 @@ def main():
             "peak": round(PEAK_INT_TOPS, 2),
             "unit": "TOP/s",
             "frac": round(achieved / PEAK_INT_TOPS, 4),
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES,
             "kernel": "bm_fused_kernel<CostCensus,4,4>",
             "kernel_ms": round(match_ms, 4),
             "ops_per_eval": OPS_PER_EVAL,
@@ -259,6 +304,8 @@ def main():
         result["learned"] = learned_leg(cs, np, ctx, L, R, dl, dr, disp, ms_per_step, max(5, min(args.steps, 20)))
     if not args.no_batch:
         result["batch"] = batch_leg(cs, np, ctx, max(3, min(args.steps, 10)))
+    if not args.no_others:
+        result["other_matchers"] = other_matchers_leg(cs, np, ctx, max(5, min(args.steps, 20)))
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

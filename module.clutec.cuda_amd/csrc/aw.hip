@@ -46,7 +46,7 @@ __device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b)
 }
 
 template <int SP, bool EDGE>
-__device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64])
+__device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64], int tx, int ty, int seg)
 {
     constexpr int R = SP / 2;        // sub-patch radius
     constexpr int B = SP + R;        // window radius
@@ -54,7 +54,6 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
     static_assert(B + 8 <= 16, "runs cover the window's columns");
     constexpr int NC = RG + 2 * SP;  // centre-column sub-patch sums per wave (rows o-s .. o+s)
     const int tid = threadIdx.x;
-    const int tx = blockIdx.x, ty = blockIdx.y, seg = blockIdx.z;
     const int x0 = tx * TW, y0 = ty * TH;
     const int W = p.W, H = p.H;
     const int ds = seg * p.seg_len;
@@ -211,13 +210,18 @@ __global__ __launch_bounds__(256, 2) void aw_fused_kernel(AwParams p)
 {
     __shared__ uint32_t hbuf[2][HROWS * HSTRIDE];
     __shared__ uint32_t ebuf[4][RG][64];
-    const int x0 = blockIdx.x * TW;
-    const int de = min(p.D, ((int)blockIdx.z + 1) * p.seg_len);
+    // one-dimensional grid of ntx * nty * nseg workgroups, renumbered per XCD
+    const int nty = (p.H + TH - 1) / TH;
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = w % p.ntx, rest = w / p.ntx;
+    const int ty = rest % nty, seg = rest / nty;
+    const int x0 = tx * TW;
+    const int de = min(p.D, (seg + 1) * p.seg_len);
     const bool edge = (x0 < p.bx + de - 1) || (x0 + TW > p.W - p.bx);
     if (edge)
-        aw_tile<SP, true>(p, hbuf, ebuf);
+        aw_tile<SP, true>(p, hbuf, ebuf, tx, ty, seg);
     else
-        aw_tile<SP, false>(p, hbuf, ebuf);
+        aw_tile<SP, false>(p, hbuf, ebuf, tx, ty, seg);
 }
 
 }  // namespace aw
@@ -241,7 +245,7 @@ void aw_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len)
 
 hipError_t launch_aw(int sub, const AwParams& p, hipStream_t s)
 {
-    dim3 grid(p.ntx, (p.H + aw::TH - 1) / aw::TH, p.nseg);
+    dim3 grid(p.ntx * ((p.H + aw::TH - 1) / aw::TH) * p.nseg);
     if (sub == 5)
         hipLaunchKernelGGL((aw::aw_fused_kernel<5>), grid, dim3(256), 0, s, p);
     else

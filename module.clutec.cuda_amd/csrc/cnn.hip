@@ -314,20 +314,41 @@ constexpr int CNN_MAX_W4 = 4096;
 
 __global__ __launch_bounds__(256) void cnn_input_kernel(View L4, View R4, View dc, int n, uint16_t* x, float* dfill)
 {
+    // nearest valid index to the left (max-scan) and to the right (min-scan) of
+    // every column, one row per workgroup
     __shared__ int16_t drow[CNN_MAX_W4];
+    __shared__ int16_t lft[CNN_MAX_W4], rgt[CNN_MAX_W4];
     const int Y = blockIdx.x;
     const int W = L4.w, H = L4.h;
     const int16_t* src = row_ptr<const int16_t>(dc, Y);
-    for (int X = threadIdx.x; X < W; X += 256) drow[X] = src[X];
+    for (int X = threadIdx.x; X < W; X += 256) {
+        const int16_t v = src[X];
+        drow[X] = v;
+        lft[X] = v >= 0 ? (int16_t)X : (int16_t)-1;
+        rgt[X] = v >= 0 ? (int16_t)X : (int16_t)CNN_MAX_W4;
+    }
     __syncthreads();
+    for (int off = 1; off < W; off <<= 1) {
+        int16_t a[CNN_MAX_W4 / 256], b[CNN_MAX_W4 / 256];
+        int k = 0;
+        for (int X = threadIdx.x; X < W; X += 256, ++k) {
+            a[k] = X >= off ? max(lft[X], lft[X - off]) : lft[X];
+            b[k] = X + off < W ? min(rgt[X], rgt[X + off]) : rgt[X];
+        }
+        __syncthreads();
+        k = 0;
+        for (int X = threadIdx.x; X < W; X += 256, ++k) {
+            lft[X] = a[k];
+            rgt[X] = b[k];
+        }
+        __syncthreads();
+    }
     for (int X = threadIdx.x; X < W; X += 256) {
         int d = drow[X];
         if (d < 0) {
-            int a = -1, b = -1;
-            for (int k = 1; k <= 64 && (a < 0 || b < 0); ++k) {
-                if (a < 0 && X - k >= 0 && drow[X - k] >= 0) a = drow[X - k];
-                if (b < 0 && X + k < W && drow[X + k] >= 0) b = drow[X + k];
-            }
+            const int l = lft[X], r = rgt[X];
+            const int a = (l >= 0 && X - l <= 64) ? drow[l] : -1;
+            const int b = (r < W && r - X <= 64) ? drow[r] : -1;
             d = a < 0 ? (b < 0 ? 0 : b) : (b < 0 ? a : min(a, b));
         }
         const size_t p = ((size_t)n * H + Y) * W + X;

@@ -37,39 +37,72 @@ constexpr int DCH = 16;        // disparities per right-statistics tile
 constexpr int SCOLS = TW + DCH - 1;
 
 // Per-pixel window statistics: .x = window sum S (int), .y = float bits of the
-// reciprocal standard-deviation factor (0 when var < N^2).
+// reciprocal standard-deviation factor scale / sqrt(N SS - S^2) (0 when the
+// variance is below N^2, and 0 outside the matchable area).  A workgroup does a
+// 64 x 32 block: the input rows it needs go to LDS, horizontal sums of v and v^2
+// over 2 ax + 1 columns go to LDS, and each lane slides the vertical sums down 8
+// rows of one column.
+constexpr int ST_W = 64, ST_H = 32, ST_MAXA = 6;
+
 __global__ __launch_bounds__(256) void ncc_stats_kernel(View img, int2* out, int pitch_px, int ax, int ay, float scale)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= img.w || y >= img.h) return;
-    int2 r = make_int2(0, 0);
-    if (x >= ax && x < img.w - ax && y >= ay && y < img.h - ay) {
-        int s = 0, ss = 0;
-        for (int j = -ay; j <= ay; ++j) {
-            const uint8_t* row = row_ptr<const uint8_t>(img, y + j);
-            for (int i = -ax; i <= ax; ++i) {
-                const int v = row[x + i];
-                s += v;
-                ss += v * v;
-            }
-        }
-        const long long n = (long long)(2 * ax + 1) * (2 * ay + 1);
-        const long long var = n * (long long)ss - (long long)s * s;
-        float rf = 0.0f;
-        if (var >= n * n) rf = (float)((double)scale / sqrt((double)var));
-        r = make_int2(s, __float_as_int(rf));
+    __shared__ uint8_t pix[ST_H + 2 * ST_MAXA][ST_W + 2 * ST_MAXA];
+    __shared__ int hs[ST_H + 2 * ST_MAXA][ST_W];
+    __shared__ int hss[ST_H + 2 * ST_MAXA][ST_W];
+    const int x0 = blockIdx.x * ST_W, y0 = blockIdx.y * ST_H;
+    const int rows = ST_H + 2 * ay, cols = ST_W + 2 * ax;
+    for (int k = threadIdx.x; k < rows * cols; k += 256) {
+        const int r = k / cols, c = k - r * cols;
+        const int y = clampi(y0 - ay + r, 0, img.h - 1), x = clampi(x0 - ax + c, 0, img.w - 1);
+        pix[r][c] = row_ptr<const uint8_t>(img, y)[x];
     }
-    out[(size_t)y * pitch_px + x] = r;
+    __syncthreads();
+    for (int k = threadIdx.x; k < rows * ST_W; k += 256) {
+        const int r = k / ST_W, c = k - r * ST_W;
+        int a = 0, b = 0;
+        for (int i = 0; i <= 2 * ax; ++i) {
+            const int v = pix[r][c + i];
+            a += v;
+            b += v * v;
+        }
+        hs[r][c] = a;
+        hss[r][c] = b;
+    }
+    __syncthreads();
+    const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;  // 4 groups of 8 rows
+    const int x = x0 + c;
+    const long long n = (long long)(2 * ax + 1) * (2 * ay + 1);
+    int sv = 0, ssv = 0;
+    for (int j = 0; j < 2 * ay; ++j) {
+        sv += hs[rg * 8 + j][c];
+        ssv += hss[rg * 8 + j][c];
+    }
+    for (int i = 0; i < 8; ++i) {
+        const int r = rg * 8 + i;  // output row y0 + r uses LDS rows r .. r + 2 ay
+        sv += hs[r + 2 * ay][c];
+        ssv += hss[r + 2 * ay][c];
+        const int y = y0 + r;
+        if (x < img.w && y < img.h) {
+            int2 o = make_int2(0, 0);
+            if (x >= ax && x < img.w - ax && y >= ay && y < img.h - ay) {
+                const long long var = n * (long long)ssv - (long long)sv * sv;
+                float rf = 0.0f;
+                if (var >= n * n) rf = (float)((double)scale / sqrt((double)var));
+                o = make_int2(sv, __float_as_int(rf));
+            }
+            out[(size_t)y * pitch_px + x] = o;
+        }
+        sv -= hs[r][c];
+        ssv -= hss[r][c];
+    }
 }
 
 template <int AX, int AY, bool EDGE>
-__device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int2* stile, uint32_t (*ebuf)[RG][64])
+__device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int2* stile, uint32_t (*ebuf)[RG][64], int tx, int ty, int seg)
 {
     static_assert(TH + 2 * AY <= HROWS, "tile too tall for the window");
     static_assert(AX <= 6 && 2 * AX + 64 <= COLS, "window wider than the run layout");
     const int tid = threadIdx.x;
-    const int tx = blockIdx.x, ty = blockIdx.y, seg = blockIdx.z;
     const int x0 = tx * TW, y0 = ty * TH;
     const int W = p.W, H = p.H;
     const int ds = seg * p.seg_len;
@@ -203,13 +236,18 @@ __global__ __launch_bounds__(256, 2) void ncc_fused_kernel(NccParams p)
     __shared__ uint32_t hbuf[HROWS * HSTRIDE];
     __shared__ int2 stile[TH * SCOLS];
     __shared__ uint32_t ebuf[4][RG][64];
-    const int x0 = blockIdx.x * TW;
-    const int de = min(p.D, ((int)blockIdx.z + 1) * p.seg_len);
+    // one-dimensional grid of ntx * nty * nseg workgroups, renumbered per XCD
+    const int nty = (p.H + TH - 1) / TH;
+    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = w % p.ntx, rest = w / p.ntx;
+    const int ty = rest % nty, seg = rest / nty;
+    const int x0 = tx * TW;
+    const int de = min(p.D, (seg + 1) * p.seg_len);
     const bool edge = (x0 < p.bx + de - 1) || (x0 + TW > p.W - p.bx);
     if (edge)
-        ncc_tile<AX, AY, true>(p, hbuf, stile, ebuf);
+        ncc_tile<AX, AY, true>(p, hbuf, stile, ebuf, tx, ty, seg);
     else
-        ncc_tile<AX, AY, false>(p, hbuf, stile, ebuf);
+        ncc_tile<AX, AY, false>(p, hbuf, stile, ebuf, tx, ty, seg);
 }
 
 // Winning score of every left pixel: the same FP32 arithmetic as the matcher, on a
@@ -269,14 +307,15 @@ void ncc_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len)
 
 hipError_t launch_ncc_stats(const View& img, int2* out, int pitch_px, int ww, int wh, float scale, hipStream_t s)
 {
-    dim3 grid((img.w + 63) / 64, (img.h + 3) / 4);
+    if (ww / 2 > ncc::ST_MAXA || wh / 2 > ncc::ST_MAXA) return hipErrorInvalidValue;
+    dim3 grid((img.w + ncc::ST_W - 1) / ncc::ST_W, (img.h + ncc::ST_H - 1) / ncc::ST_H);
     hipLaunchKernelGGL(ncc::ncc_stats_kernel, grid, dim3(256), 0, s, img, out, pitch_px, ww / 2, wh / 2, scale);
     return hipGetLastError();
 }
 
 hipError_t launch_ncc(int ww, const NccParams& p, hipStream_t s)
 {
-    dim3 grid(p.ntx, (p.H + ncc::TH - 1) / ncc::TH, p.nseg);
+    dim3 grid(p.ntx * ((p.H + ncc::TH - 1) / ncc::TH) * p.nseg);
     switch (ww) {
     case 11: hipLaunchKernelGGL((ncc::ncc_fused_kernel<5, 5>), grid, dim3(256), 0, s, p); break;
     case 9: hipLaunchKernelGGL((ncc::ncc_fused_kernel<4, 4>), grid, dim3(256), 0, s, p); break;
