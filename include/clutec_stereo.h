@@ -228,7 +228,8 @@ typedef struct stereo_batch_tuning {
 } stereo_batch_tuning;
 
 /* params: algorithm SAD or CENSUS, windows and tolerance as for the matcher;
- * params->num_disparities is ignored, num_disparities[i] gives pair i's D. */
+ * params->num_disparities is ignored, num_disparities[i] gives pair i's D.  Widths up
+ * to 11264 (STEREO_ERROR_INVALID_FORMAT above). */
 stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** drv, int32_t npairs,
                                      const stereo_image_format* fmts, const int32_t* num_disparities,
                                      const stereo_matcher_params* params, const stereo_batch_tuning* tuning);
@@ -288,8 +289,9 @@ stereo_status stereo_cnn_forward(stereo_cnn* cnn, int32_t n, float* out, stereo_
 stereo_status stereo_cnn_train_step(stereo_cnn* cnn, int32_t n, float lr, float* loss, stereo_stream stream);
 
 /* Kernel-level entry points of the network's convolutions (device pointers, NHWC
- * bf16 activations with cin, cout in {16, 32}).  y = conv(x, w) + bias with
- * w [cout][9][cin]; relu != 0 writes bf16 max(y, 0), else FP32 y. */
+ * bf16 activations with cin, cout in {16, 32}, 16-byte aligned; dilation 1 to 4, larger ones are
+ * STEREO_ERROR_INVALID_CONFIG).  y = conv(x, w) + bias with w [cout][9][cin];
+ * relu != 0 writes bf16 max(y, 0), else FP32 y. */
 stereo_status stereo_conv3x3_bf16(stereo_context* ctx, const uint16_t* x, int32_t N, int32_t H, int32_t W,
                                   int32_t cin, const uint16_t* w, const float* bias, int32_t cout, int32_t dil,
                                   int32_t relu, void* y, stereo_stream stream);
@@ -303,14 +305,17 @@ stereo_status stereo_conv3x3_bwd_bf16(stereo_context* ctx, const uint16_t* x, in
 typedef struct stereo_learned_params {
     stereo_matcher_params matcher; /* full-resolution census matcher (algorithm CENSUS) */
     int32_t coarse_disparities;    /* D of the quarter-resolution matcher (default D/4) */
-    int32_t band;                  /* width of the searched band around the centre (32) */
-    int32_t chunk;                 /* disparities per work item (32) */
+    int32_t band;                  /* the searched band reaches band/2 + 4 beyond each centre (8) */
+    int32_t chunk;                 /* disparities per work item (64) */
     float logit_thresh;            /* tiles with a smaller confidence logit search the full range */
-    int32_t reserved[4];
+    int32_t centre_residual;       /* 0: centres from the quarter-resolution census map (default);
+                                      1: plus the network's centre residual */
+    int32_t direct_launch;         /* 1: launch the stages directly instead of replaying a graph */
+    int32_t reserved[2];           /* must be 0 */
 } stereo_learned_params;
 
 /* The learned-proposal census matcher for LUM_U8 pairs of fmt (width and height
- * multiples of 4); cnn_params = STEREO_CNN_PARAMS trained values. */
+ * multiples of 4, width up to 11264); cnn_params = STEREO_CNN_PARAMS trained values. */
 stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** drv, const stereo_image_format* fmt,
                                        const stereo_learned_params* params, const float* cnn_params, size_t n);
 /* Outputs as stereo_matcher_process. */
@@ -330,6 +335,14 @@ stereo_status stereo_learned_tile_ranges(stereo_driver* drv, int32_t* lohi, int3
 /* The searched disparities of each tile in the last process call as a mask of
  * 8-disparity bins (bit b: d in [8b, 8b + 8)); n >= tiles.  Waits for the call. */
 stereo_status stereo_learned_tile_masks(stereo_driver* drv, uint32_t* masks, int32_t n);
+
+/* ---- device calibration ------------------------------------------------------ */
+
+/* INT32 VALU issue rate of the device in tera lane-operations per second, measured
+ * with independent chains of one instruction on every SIMD (op 0: v_add_u32, op 1:
+ * v_bcnt_u32_b32); the roofline of the integer matchers uses the lower of this and
+ * the specified peak.  No counterpart in the reference.  Synchronous. */
+stereo_status stereo_valu_peak(stereo_context* ctx, int32_t op, double* tera_ops);
 
 /* ---- synthetic workloads (host only, no device needed) ------------------- */
 

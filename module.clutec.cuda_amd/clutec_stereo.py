@@ -86,7 +86,8 @@ class BatchTuning(C.Structure):
 
 class LearnedParams(C.Structure):
     _fields_ = [("matcher", MatcherParams), ("coarse_disparities", C.c_int32), ("band", C.c_int32),
-                ("chunk", C.c_int32), ("logit_thresh", C.c_float), ("reserved", C.c_int32 * 4)]
+                ("chunk", C.c_int32), ("logit_thresh", C.c_float), ("centre_residual", C.c_int32),
+                ("direct_launch", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 CNN_PARAMS = 28930
@@ -156,6 +157,7 @@ _PROTOS = [
                                         C.POINTER(C.c_double), C.c_int32]),
     ("stereo_learned_tile_ranges", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
     ("stereo_learned_tile_masks", C.c_int, [_P, C.POINTER(C.c_uint32), C.c_int32]),
+    ("stereo_valu_peak", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
     ("stereo_synth_pair", C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     _P, _P, _P]),
 ]
@@ -243,6 +245,12 @@ class Context:
 
     def synchronize(self, stream=None):
         _check(lib.stereo_synchronize(self.handle, stream), self)
+
+    def valu_peak(self, op=0):
+        """Measured INT32 VALU rate in tera lane-ops/s (op 0: v_add_u32, 1: v_bcnt_u32_b32)."""
+        v = C.c_double()
+        _check(lib.stereo_valu_peak(self.handle, op, C.byref(v)), self)
+        return v.value
 
 
 class DeviceImage:
@@ -561,8 +569,8 @@ def cnn_init_params(seed=0):
 class LearnedMatcher(_Driver):
     """Census matcher with the learned band proposal (stereo_learned_*)."""
 
-    def __init__(self, ctx, W, H, cnn_params, window=(9, 9), census=(9, 7), num_disp=256, tol=1, band=32, chunk=64,
-                 logit_thresh=0.0, coarse_disp=0):
+    def __init__(self, ctx, W, H, cnn_params, window=(9, 9), census=(9, 7), num_disp=256, tol=1, band=8, chunk=64,
+                 logit_thresh=0.0, coarse_disp=0, centre_residual=False, direct_launch=False):
         super().__init__(ctx)
         self.W, self.H = W, H
         fmt = ImageFormat(W, H, PIXEL_LUM_U8)
@@ -572,6 +580,8 @@ class LearnedMatcher(_Driver):
         lp.band = band
         lp.chunk = chunk
         lp.logit_thresh = logit_thresh
+        lp.centre_residual = 1 if centre_residual else 0
+        lp.direct_launch = 1 if direct_launch else 0
         p = np.ascontiguousarray(cnn_params, np.float32)
         _check(lib.stereo_learned_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(lp),
                                             p.ctypes.data, p.size), ctx)

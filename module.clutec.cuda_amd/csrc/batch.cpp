@@ -25,6 +25,7 @@ struct BatchExt : DriverExt {
     stereo::BmTile* tiles_dev = nullptr;
     stereo::BmItem* items_dev = nullptr;
     int* sched_dev = nullptr;
+    int* counter_dev = nullptr;  // item counter of the list matcher (reset by its finish)
     int* nitems_dev = nullptr;
     int ntiles = 0, nitems = 0, max_items = 0;
     uint32_t* lkey = nullptr;
@@ -56,7 +57,7 @@ struct BatchExt : DriverExt {
             if (p) (void)hipFree(p);
         for (uint8_t* p : padR)
             if (p) (void)hipFree(p);
-        for (void* p : {(void*)frames_dev, (void*)tiles_dev, (void*)items_dev, (void*)sched_dev, (void*)nitems_dev,
+        for (void* p : {(void*)frames_dev, (void*)tiles_dev, (void*)items_dev, (void*)sched_dev, (void*)counter_dev, (void*)nitems_dev,
                         (void*)lkey, (void*)rpart})
             if (p) (void)hipFree(p);
     }
@@ -75,6 +76,7 @@ size_t rpart_need(const std::vector<stereo::BmFrame>& frames, int chunk)
 // 1 = table order.
 stereo_status upload_work(BatchExt* x)
 {
+    HIP_TRY(x->ctx, hipDeviceSynchronize(), "hipDeviceSynchronize");  // no launch of the old tables in flight
     const int chunk = x->tuning.chunk;
     std::vector<stereo::BmItem> items;
     std::vector<stereo::BmTile> tiles;
@@ -120,6 +122,8 @@ stereo_status upload_work(BatchExt* x)
     HIP_TRY(x->ctx, hipMemcpy(x->tiles_dev, tiles.data(), tiles.size() * sizeof(stereo::BmTile), hipMemcpyHostToDevice),
             "hipMemcpy");
     HIP_TRY(x->ctx, hipMemcpy(x->nitems_dev, &n, sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    const int zero = 0;  // the item counter starts every matcher launch at 0
+    HIP_TRY(x->ctx, hipMemcpy(x->counter_dev, &zero, sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
     x->drop_graph();
     return STEREO_OK;
 }
@@ -147,6 +151,7 @@ stereo_status enqueue(stereo_driver* d, BatchExt* x, const stereo_image* left, c
     lp.items = x->items_dev;
     lp.nitems = x->nitems_dev;
     lp.sched = x->sched_dev;
+    lp.counter = x->counter_dev;
     lp.lkey = x->lkey;
     lp.rpart = x->rpart;
     lp.rp_stride = stereo::bm_list_rp_stride(x->tuning.chunk);
@@ -269,8 +274,8 @@ stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** out, i
         x->max_width = std::max(x->max_width, W);
         x->frames.push_back(f);
     }
-    if (x->max_width > stereo::BM_MAX_WIDTH) return cleanup(fail(ctx, STEREO_ERROR_INVALID_FORMAT, "batch: width above %d",
-                                                                 stereo::BM_MAX_WIDTH));
+    if (x->max_width > stereo::BM_LIST_MAX_WIDTH)
+        return cleanup(fail(ctx, STEREO_ERROR_INVALID_FORMAT, "batch: width above %d", stereo::BM_LIST_MAX_WIDTH));
     x->total_rows = rows;
     x->max_items = (int)max_items;
     for (int c = kMinChunk; c <= 256; c += 8) x->rpart_elems = std::max(x->rpart_elems, rpart_need(x->frames, c));
@@ -279,8 +284,10 @@ stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** out, i
         (e = hipMalloc((void**)&x->items_dev, max_items * sizeof(stereo::BmItem))) ||
         (e = hipMalloc((void**)&x->sched_dev, max_items * sizeof(int))) ||
         (e = hipMalloc((void**)&x->nitems_dev, sizeof(int))) ||
+        (e = hipMalloc((void**)&x->counter_dev, sizeof(int))) ||
         (e = hipMalloc((void**)&x->lkey, stereo::bm_list_lkey_elems((int)max_items) * 4)) ||
-        (e = hipMalloc((void**)&x->rpart, x->rpart_elems * 4)))
+        (e = hipMalloc((void**)&x->rpart, x->rpart_elems * 4)) || (e = hipMemset(x->counter_dev, 0, sizeof(int))) ||
+        (e = hipDeviceSynchronize()))
         return cleanup(hip_fail(ctx, e, "hipMalloc"));
     if ((e = hipMemcpy(x->frames_dev, x->frames.data(), npairs * sizeof(stereo::BmFrame), hipMemcpyHostToDevice)))
         return cleanup(hip_fail(ctx, e, "hipMemcpy"));

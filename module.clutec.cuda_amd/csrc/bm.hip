@@ -265,8 +265,16 @@ __global__ __launch_bounds__(256, 2) void bm_list_kernel(BmListParams p)
 {
     __shared__ uint32_t hbuf[2][HROWS * HSTRIDE];
     __shared__ uint32_t ebuf[4][RG][64];
+    __shared__ int next;
     const int n = *p.nitems;
-    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    for (;;) {
+        // items are handed out in schedule order by a device counter (reset to 0 by
+        // the finish kernel), so that a workgroup takes the next item when it is free
+        __syncthreads();  // the previous item is done with the LDS buffers and `next`
+        if (threadIdx.x == 0) next = atomicAdd(p.counter, 1);
+        __syncthreads();
+        const int k = next;
+        if (k >= n) break;
         const int i = p.sched ? p.sched[k] : k;
         const BmItem it = p.items[i];
         const BmFrame& f = p.frames[it.frame];
@@ -286,7 +294,6 @@ __global__ __launch_bounds__(256, 2) void bm_list_kernel(BmListParams p)
         j.rp_kofs = 256 - it.de;
         j.lk = p.lkey + (size_t)i * TH * TW;
         j.lk_stride = TW;
-        __syncthreads();  // the previous item is done with the LDS buffers
         run_job<Cost, AX, AY>(j, hbuf, ebuf);
     }
 }
@@ -408,14 +415,20 @@ __device__ __forceinline__ void write_pixel(const View& disp, const View& disp_l
 // pixel xr of tile row ty collects the partial keys of every item whose tile and
 // disparity range reach it (x0 - (de-1) <= xr <= x0 + 63 - ds); a left pixel merges
 // the left keys of its tile's items.  The items of the tile row are staged in LDS
-// first (up to BM_LIST_ROW_ITEMS; beyond that they are read from memory).
+// first (up to BM_LIST_ROW_ITEMS).  The right-view merge then walks the row's item
+// segments as one flat range of elements: each thread issues the loads of eight
+// elements together and folds them into the LDS row with ds_max_u32, so that the
+// loads of a row are in flight at once rather than one per loop step.  Rows with
+// more items than fit read the item table from memory, pixel by pixel.
 constexpr int BM_LIST_ROW_ITEMS = 1024;
+constexpr int BM_LIST_LEFT_UNROLL = 8;  // left keys of a tile's first items loaded together
 
 __global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p, int max_width)
 {
     extern __shared__ uint32_t smem[];
     uint32_t* rrow = smem;
     const int r = blockIdx.x;
+    if (r == 0 && threadIdx.x == 0) *p.counter = 0;  // the matcher launch before this one has ended
     int fi = 0;
     while (fi + 1 < p.nframes && p.frames[fi + 1].row_base <= r) ++fi;
     const BmFrame& f = p.frames[fi];
@@ -426,10 +439,11 @@ __global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p, int
     const BmTile* trow = p.tiles + f.tile_base + ty * ntx;
     const bool row_ok = y >= f.by && y < f.H - f.by;
 
-    // tile -> [toff[tx], toff[tx+1]) in the staged item table (slot, ds, de)
+    // tile -> [toff[tx], toff[tx+1]) in the staged item table (slot, ds, de, tile x)
     int* toff = reinterpret_cast<int*>(smem + ((max_width + 1) & ~1));  // even offsets keep irng 8-byte aligned
     int* islot = toff + (BM_MAX_WIDTH / TW + 2);
     int2* irng = reinterpret_cast<int2*>(islot + BM_LIST_ROW_ITEMS);
+    int* itx = reinterpret_cast<int*>(irng + BM_LIST_ROW_ITEMS);
     __shared__ int staged;
     if (threadIdx.x == 0) {
         int n = 0;
@@ -440,6 +454,7 @@ __global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p, int
         toff[ntx] = n;
         staged = n <= BM_LIST_ROW_ITEMS;
     }
+    for (int x = threadIdx.x; x < W; x += 256) rrow[x] = 0u;
     __syncthreads();
     if (staged) {
         for (int tx = threadIdx.x; tx < ntx; tx += 256) {
@@ -448,26 +463,44 @@ __global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p, int
                 const int j = t.first_item + k;
                 islot[toff[tx] + k] = j;
                 irng[toff[tx] + k] = make_int2(p.items[j].ds, p.items[j].de);
+                itx[toff[tx] + k] = tx;
             }
         }
         __syncthreads();
-    }
-    for (int x = threadIdx.x; x < W; x += 256) {
-        uint32_t best = 0;
-        if (row_ok && x >= f.bx && x < W - f.bx) {
-            const int tlo = x / TW;  // the first tile whose pixels reach x
-            const int thi = min(ntx - 1, (x + f.D - 1) / 64);
-            for (int tx = tlo; tx <= thi; ++tx) {
-                const int x0 = tx * TW;
-                if (staged) {
-                    for (int k = toff[tx]; k < toff[tx + 1]; ++k) {
+        if (row_ok) {
+            // element (k, c): column c of staged item k, right pixel x0 - (de - 1) + c
+            const int tot = toff[ntx] * p.rp_stride;
+            for (int f0 = threadIdx.x; f0 < tot; f0 += 8 * 256) {
+                uint32_t v[8];
+                int xr[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int e = f0 + u * 256;
+                    v[u] = 0u;
+                    xr[u] = -1;
+                    if (e < tot) {
+                        const int k = e / p.rp_stride, c = e - k * p.rp_stride;
                         const int2 rg = irng[k];
-                        if (x >= x0 - (rg.y - 1) && x <= x0 + 63 - rg.x) {
-                            const uint32_t v = p.rpart[((size_t)islot[k] * TH + ry) * p.rp_stride + (x - x0 + rg.y - 1)];
-                            best = v > best ? v : best;
+                        const int x = itx[k] * TW - (rg.y - 1) + c;
+                        if (c < TW - 1 + rg.y - rg.x && x >= f.bx && x < W - f.bx) {
+                            v[u] = p.rpart[((size_t)islot[k] * TH + ry) * p.rp_stride + c];
+                            xr[u] = x;
                         }
                     }
-                } else {
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (xr[u] >= 0) atomicMax(&rrow[xr[u]], v[u]);
+            }
+        }
+    } else {
+        for (int x = threadIdx.x; x < W; x += 256) {
+            uint32_t best = 0;
+            if (row_ok && x >= f.bx && x < W - f.bx) {
+                const int tlo = x / TW;  // the first tile whose pixels reach x
+                const int thi = min(ntx - 1, (x + f.D - 1) / 64);
+                for (int tx = tlo; tx <= thi; ++tx) {
+                    const int x0 = tx * TW;
                     const BmTile t = trow[tx];
                     for (int j = t.first_item; j < t.first_item + t.nitems; ++j) {
                         const int ds = p.items[j].ds, de = p.items[j].de;
@@ -478,8 +511,8 @@ __global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p, int
                     }
                 }
             }
+            rrow[x] = best;
         }
-        rrow[x] = best;
     }
     __syncthreads();
     for (int x = threadIdx.x; x < W; x += 256) {
@@ -489,7 +522,16 @@ __global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p, int
             const int tx = x / TW;
             const int kb = staged ? toff[tx] : trow[tx].first_item;
             const int ke = staged ? toff[tx + 1] : trow[tx].first_item + trow[tx].nitems;
-            for (int k = kb; k < ke; ++k) {
+            uint32_t lv[BM_LIST_LEFT_UNROLL];
+#pragma unroll
+            for (int q = 0; q < BM_LIST_LEFT_UNROLL; ++q) {
+                const int k = kb + q;
+                const int j = k < ke ? (staged ? islot[k] : k) : 0;
+                lv[q] = k < ke ? p.lkey[((size_t)j * TH + ry) * TW + (x & (TW - 1))] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < BM_LIST_LEFT_UNROLL; ++q) lk = lv[q] > lk ? lv[q] : lk;
+            for (int k = kb + BM_LIST_LEFT_UNROLL; k < ke; ++k) {
                 const int j = staged ? islot[k] : k;
                 const uint32_t v = p.lkey[((size_t)j * TH + ry) * TW + (x & (TW - 1))];
                 lk = v > lk ? v : lk;
@@ -519,16 +561,17 @@ static hipError_t launch_list_cost(const BmListParams& p, int ww, int grid, hipS
 
 hipError_t launch_bm_list(int kind, int ww, const BmListParams& p, int grid, hipStream_t s)
 {
-    if (grid < 1) return hipErrorInvalidValue;
+    if (grid < 1 || !p.counter) return hipErrorInvalidValue;
     if (kind == BM_COST_CENSUS) return launch_list_cost<bm::CostCensus>(p, ww, grid, s);
     return launch_list_cost<bm::CostSad>(p, ww, grid, s);
 }
 
 hipError_t launch_bm_list_finish(const BmListParams& p, int max_width, hipStream_t s)
 {
-    if (max_width > BM_MAX_WIDTH || p.total_rows < 1) return hipErrorInvalidValue;
+    if (max_width > BM_LIST_MAX_WIDTH || p.total_rows < 1 || !p.counter) return hipErrorInvalidValue;
     const size_t lds = ((size_t)(max_width + 1) & ~(size_t)1) * 4 + (BM_MAX_WIDTH / bm::TW + 2) * 4 +
-                       (size_t)bm::BM_LIST_ROW_ITEMS * 12;
+                       (size_t)bm::BM_LIST_ROW_ITEMS * 16;
+    if (lds > 65536) return hipErrorInvalidValue;
     hipLaunchKernelGGL(bm::bm_list_finish_kernel, dim3(p.total_rows), dim3(256), lds, s, p, max_width);
     return hipGetLastError();
 }

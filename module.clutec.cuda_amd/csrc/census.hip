@@ -9,10 +9,15 @@
 //
 // Layout: a 64 x 32 output tile per 256-thread workgroup; the input tile plus halo
 // is staged in LDS once, one pixel per dword so that every read is a conflict-free
-// ds_read_b32; each thread produces an 8-row column strip.  A descriptor bit costs
-// two VALU operations: t = centre - neighbour (negative exactly when neighbour >
-// centre), then acc = v_alignbit(acc, t, 31), which shifts the sign bit of t into
-// the accumulator.  Both images of a pair go through one launch (blockIdx.z).
+// ds_read_b32.  Each thread produces an 8-row column strip and walks the strip's
+// 8 + CH - 1 input rows once, feeding every output row whose window covers the row
+// (the rows arrive in window order, so the raster bit order is kept): about 16 LDS
+// reads per pixel instead of CW x CH.  A descriptor bit costs two VALU operations:
+// t = centre - neighbour (negative exactly when neighbour > centre), then
+// acc = v_alignbit(acc, t, 31), which shifts the sign bit of t into the accumulator.
+// Both images of a pair go through one launch (blockIdx.z).  On request the same
+// launch writes the 4 x 4 box means (sum + 8) >> 4 of both images (the quarter-
+// resolution pair of the learned proposal), read from the staged tile.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -25,7 +30,7 @@ constexpr int CT_H = 32;
 constexpr int CT_STRIP = 8;
 
 template <int CW, int CH>
-__global__ __launch_bounds__(256) void census_wide_kernel(View src0, View dst0, View src1, View dst1)
+__global__ __launch_bounds__(256) void census_wide_kernel(View src0, View dst0, View src1, View dst1, View q0, View q1)
 {
     constexpr int RX = CW / 2, RY = CH / 2, NB = CW * CH;
     constexpr int LW = CT_W + 2 * RX, LH = CT_H + 2 * RY;
@@ -47,27 +52,71 @@ __global__ __launch_bounds__(256) void census_wide_kernel(View src0, View dst0, 
     const int cx = tid & 63;
     const int strip = tid >> 6;
     const int x = x0 + cx;
+    const uint32_t* col = &tile[strip * CT_STRIP][cx];
+    uint32_t hi[CT_STRIP], lo[CT_STRIP], cen[CT_STRIP];
+#pragma unroll
+    for (int k = 0; k < CT_STRIP; ++k) {
+        cen[k] = col[(k + RY) * (LW + 1) + RX];
+        hi[k] = 0;
+        lo[k] = 0;
+    }
+    // one input row in flight ahead of the one being used; the compiler barrier at
+    // the end of each row keeps the LDS loads of later rows from being hoisted
+    // (which would hold all 14 rows in registers and cost most of the occupancy)
+    constexpr int NR = CT_STRIP + CH - 1;
+    uint32_t v[CW], vn[CW];
+#pragma unroll
+    for (int dx = 0; dx < CW; ++dx) v[dx] = col[dx];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        if (r + 1 < NR) {
+#pragma unroll
+            for (int dx = 0; dx < CW; ++dx) vn[dx] = col[(r + 1) * (LW + 1) + dx];
+        }
+#pragma unroll
+        for (int k = 0; k < CT_STRIP; ++k) {
+            const int dy = r - k;
+            if (dy < 0 || dy >= CH) continue;
+#pragma unroll
+            for (int dx = 0; dx < CW; ++dx) {
+                const int bit = dy * CW + dx;
+                const uint32_t t = cen[k] - v[dx];
+                if (bit < NHI)
+                    hi[k] = __builtin_amdgcn_alignbit(hi[k], t, 31);
+                else
+                    lo[k] = __builtin_amdgcn_alignbit(lo[k], t, 31);
+            }
+        }
+#pragma unroll
+        for (int dx = 0; dx < CW; ++dx) v[dx] = vn[dx];
+        asm volatile("" ::: "memory");
+    }
+    // the descriptors are complete here: without this the compiler sinks each
+    // pixel's bit chain into its bounds-checked store and keeps every row live
+#pragma unroll
+    for (int k = 0; k < CT_STRIP; ++k) asm volatile("" : "+v"(hi[k]), "+v"(lo[k]));
 #pragma unroll
     for (int k = 0; k < CT_STRIP; ++k) {
         const int y = y0 + strip * CT_STRIP + k;
-        const uint32_t* w = &tile[strip * CT_STRIP + k][cx];
-        const uint32_t centre = w[RY * (LW + 1) + RX];
-        uint32_t hi = 0, lo = 0;
-        int bit = 0;
-#pragma unroll
-        for (int dy = 0; dy < CH; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < CW; ++dx, ++bit) {
-                const uint32_t t = centre - w[dy * (LW + 1) + dx];
-                if (bit < NHI)
-                    hi = __builtin_amdgcn_alignbit(hi, t, 31);
-                else
-                    lo = __builtin_amdgcn_alignbit(lo, t, 31);
-            }
         const bool inside = x >= RX && x < src.w - RX && y >= RY && y < src.h - RY;
         if (x < dst.w && y < dst.h) {
-            const uint64_t v = inside ? (((uint64_t)hi << 32) | lo) : 0ull;
+            const uint64_t v = inside ? (((uint64_t)hi[k] << 32) | lo[k]) : 0ull;
             row_ptr<uint64_t>(dst, y)[x] = v;
+        }
+    }
+    // quarter resolution: 16 x 8 box means of the tile (the image size is a
+    // multiple of 4, so every 4 x 4 block lies inside the image)
+    const View q = blockIdx.z ? q1 : q0;
+    if (q.ptr && tid < (CT_W / 4) * (CT_H / 4)) {
+        const int qx = tid & 15, qy = tid >> 4;
+        const int X = x0 / 4 + qx, Y = y0 / 4 + qy;
+        if (X < q.w && Y < q.h) {
+            uint32_t sum = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sum += tile[RY + 4 * qy + j][RX + 4 * qx + i];
+            row_ptr<uint8_t>(q, Y)[X] = (uint8_t)((sum + 8) >> 4);
         }
     }
 }
@@ -97,11 +146,13 @@ __global__ __launch_bounds__(256) void census3x3_ref_kernel(View src, View dst)
 }
 
 template <int CW, int CH>
-hipError_t launch_wide(const View& src0, const View& dst0, const View* src1, const View* dst1, hipStream_t s)
+hipError_t launch_wide(const View& src0, const View& dst0, const View* src1, const View* dst1, const View* q0,
+                       const View* q1, hipStream_t s)
 {
     dim3 grid((src0.w + CT_W - 1) / CT_W, (src0.h + CT_H - 1) / CT_H, src1 ? 2 : 1);
+    const View none{nullptr, 0, 0, 0};
     hipLaunchKernelGGL((census_wide_kernel<CW, CH>), grid, dim3(256), 0, s, src0, dst0, src1 ? *src1 : src0,
-                       dst1 ? *dst1 : dst0);
+                       dst1 ? *dst1 : dst0, q0 ? *q0 : none, q1 ? *q1 : none);
     return hipGetLastError();
 }
 
@@ -125,14 +176,20 @@ hipError_t launch_census(const View& src, const View& dst, int cw, int ch, int r
 }
 
 hipError_t launch_census_pair(const View& src0, const View& dst0, const View* src1, const View* dst1, int cw, int ch,
-                              hipStream_t s)
+                              hipStream_t s, const View* q0, const View* q1)
 {
     if (src1 && (src1->w != src0.w || src1->h != src0.h)) return hipErrorInvalidValue;
-    if (cw == 9 && ch == 7) return launch_wide<9, 7>(src0, dst0, src1, dst1, s);
-    if (cw == 7 && ch == 9) return launch_wide<7, 9>(src0, dst0, src1, dst1, s);
-    if (cw == 5 && ch == 5) return launch_wide<5, 5>(src0, dst0, src1, dst1, s);
-    if (cw == 7 && ch == 7) return launch_wide<7, 7>(src0, dst0, src1, dst1, s);
-    if (cw == 3 && ch == 3) return launch_wide<3, 3>(src0, dst0, src1, dst1, s);
+    if (q0 || q1) {
+        // quarter-resolution outputs: sizes a multiple of 4, one per image
+        if ((src0.w & 3) || (src0.h & 3) || (q1 && !src1)) return hipErrorInvalidValue;
+        if (q0 && (q0->w != src0.w / 4 || q0->h != src0.h / 4)) return hipErrorInvalidValue;
+        if (q1 && (q1->w != src0.w / 4 || q1->h != src0.h / 4)) return hipErrorInvalidValue;
+    }
+    if (cw == 9 && ch == 7) return launch_wide<9, 7>(src0, dst0, src1, dst1, q0, q1, s);
+    if (cw == 7 && ch == 9) return launch_wide<7, 9>(src0, dst0, src1, dst1, q0, q1, s);
+    if (cw == 5 && ch == 5) return launch_wide<5, 5>(src0, dst0, src1, dst1, q0, q1, s);
+    if (cw == 7 && ch == 7) return launch_wide<7, 7>(src0, dst0, src1, dst1, q0, q1, s);
+    if (cw == 3 && ch == 3) return launch_wide<3, 3>(src0, dst0, src1, dst1, q0, q1, s);
     return hipErrorInvalidValue;
 }
 

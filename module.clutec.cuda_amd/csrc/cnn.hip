@@ -14,13 +14,17 @@
 //
 // Convolutions: 3x3, zero padding, dilation r, NHWC bf16 activations with 16 or 32
 // channels, weights [Cout][9][Cin] bf16 (tap t = 3*ky + kx at offset
-// ((ky-1) r, (kx-1) r)).  One wave computes 16 consecutive pixels of a row for all
-// output channels with v_mfma_f32_16x16x16_bf16:
+// ((ky-1) r, (kx-1) r)), r <= CNN_MAX_DIL.  A workgroup computes a tile of
+// CNN_TILE_ROWS rows x 64 pixels: it stages the input tile with its halo of r pixels
+// in LDS once (each input pixel is then read by 9 taps from LDS, not from L2), and
+// wave w computes row w of the tile, four 16-pixel blocks sharing every weight
+// operand, with v_mfma_f32_16x16x16_bf16:
 //   D[co][px] += A[co][ci] * B[ci][px]
 //   A (weights): lane (col, grp) holds co = col, ci = 4 grp .. 4 grp + 3 -> one 8-byte
 //                load per tap and channel block, all kept in VGPRs
-//   B (pixels):  lane (col, grp) holds px = col, ci = 4 grp .. +3 -> one 8-byte load
-//                of the NHWC pixel
+//   B (pixels):  lane (col, grp) holds px = col, ci = 4 grp .. +3 -> one ds_read_b64
+//                of the staged NHWC pixel (pixel stride 2 CIN + 8 bytes: the 16
+//                columns of a read fall on distinct bank pairs)
 //   D:           lane (col, grp) holds px = col, co = 4 grp .. +3 -> one 8-byte store
 // The data gradient is the same kernel with the transposed weights [Cin][9][Cout]
 // and the tap offsets negated.  The weight gradient uses the pixels as the K
@@ -47,10 +51,59 @@ template <int CIN, int COUT, int EPI>
 __global__ __launch_bounds__(256) void conv3x3_kernel(CnnConvArgs a)
 {
     constexpr int KC = CIN / 16, MB = COUT / 16;
+    constexpr int PS = 2 * CIN + 8;  // staged pixel stride in bytes
+    extern __shared__ __align__(16) uint8_t tile[];
     const int lane = threadIdx.x & 63;
     const int col = lane & 15, grp = lane >> 4;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int nwaves = gridDim.x * 4;
+    const int wv = threadIdx.x >> 6;
+    const int ntx = (a.W + 63) >> 6, nty = (a.H + CNN_TILE_ROWS - 1) / CNN_TILE_ROWS;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = b % ntx, r = b / ntx;
+    const int ty = r % nty, n = r / nty;
+    const int x0 = tx * 64, y0 = ty * CNN_TILE_ROWS;
+    const int d = a.dil;
+    const int TWp = 64 + 2 * d, THp = CNN_TILE_ROWS + 2 * d;
+
+    // stage the input tile (zero outside the image).  A staged row holds TWp * QP
+    // 16-byte chunks (at most 288); thread i owns chunks i and i + 256 of every row,
+    // so the column arithmetic is done once and each row only adds its offset.  The
+    // rows go in groups of four, all loads of a group in flight together; the
+    // chunks are written to LDS as 8-byte halves.
+    constexpr int QP = CIN / 8;  // 16-byte chunks per pixel
+    const int cpr = TWp * QP;
+    const int c0 = threadIdx.x, c1 = threadIdx.x + 256;
+    const int rx0 = c0 / QP, q0 = c0 % QP, rx1 = c1 / QP, q1 = c1 % QP;
+    const int gx0 = x0 - d + rx0, gx1 = x0 - d + rx1;
+    const bool h0 = c0 < cpr, h1 = c1 < cpr;
+    const bool ok0 = h0 && gx0 >= 0 && gx0 < a.W, ok1 = h1 && gx1 >= 0 && gx1 < a.W;
+    const size_t col0 = (size_t)gx0 * CIN + q0 * 8, col1 = (size_t)gx1 * CIN + q1 * 8;
+    const uint16_t* img = a.x + (size_t)n * a.H * a.W * CIN;
+    for (int r0 = 0; r0 < THp; r0 += 4) {
+        uint4 v0[4], v1[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gy = y0 - d + r0 + j;
+            const bool rok = r0 + j < THp && gy >= 0 && gy < a.H;
+            const uint16_t* row = img + (rok ? (size_t)gy * a.W * CIN : 0);
+            v0[j] = (rok && ok0) ? *reinterpret_cast<const uint4*>(row + col0) : make_uint4(0u, 0u, 0u, 0u);
+            v1[j] = (rok && ok1) ? *reinterpret_cast<const uint4*>(row + col1) : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (r0 + j >= THp) break;
+            uint8_t* trow = tile + (size_t)(r0 + j) * TWp * PS;
+            if (h0) {
+                uint2* dst = reinterpret_cast<uint2*>(trow + rx0 * PS + q0 * 16);
+                dst[0] = make_uint2(v0[j].x, v0[j].y);
+                dst[1] = make_uint2(v0[j].z, v0[j].w);
+            }
+            if (h1) {
+                uint2* dst = reinterpret_cast<uint2*>(trow + rx1 * PS + q1 * 16);
+                dst[0] = make_uint2(v1[j].x, v1[j].y);
+                dst[1] = make_uint2(v1[j].z, v1[j].w);
+            }
+        }
+    }
     v4s wr[MB][9][KC];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
@@ -59,80 +112,63 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(CnnConvArgs a)
 #pragma unroll
             for (int kc = 0; kc < KC; ++kc)
                 wr[mb][t][kc] = *reinterpret_cast<const v4s*>(a.w + ((size_t)(mb * 16 + col) * 9 + t) * CIN + kc * 16 + 4 * grp);
-    const int nbx = (a.W + 15) >> 4;
-    const int total = a.N * a.H * nbx;
-    const int step = a.dil * a.sign;
-    // the input vectors of block b (9 taps x KC channel blocks), zero outside the image
-    auto load = [&](int b, v4s (&in)[9][KC]) {
-        const int px = (b % nbx) * 16 + col;
-        const int r = b / nbx;
-        const int yy = r % a.H, n = r / a.H;
+    __syncthreads();
+
+    const int yy = y0 + wv;
+    if (yy >= a.H) return;
+    const int step = d * a.sign;
+    v4f acc[4][MB];
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int sy = yy + (t / 3 - 1) * step;
-            const int sx = px + (t % 3 - 1) * step;
-            const bool ok = sy >= 0 && sy < a.H && sx >= 0 && sx < a.W;
-            const uint16_t* src = ok ? a.x + (((size_t)n * a.H + sy) * a.W + sx) * CIN + 4 * grp : a.x;
+    for (int k = 0; k < 4; ++k)
 #pragma unroll
-            for (int kc = 0; kc < KC; ++kc) {
-                v4s bv = *reinterpret_cast<const v4s*>(src + kc * 16);
-                in[t][kc] = ok ? bv : v4s{0, 0, 0, 0};
-            }
-        }
-    };
-    // software pipeline: the loads of the wave's next block are in flight while
-    // the MFMAs of the current one run
-    v4s cur[9][KC], nxt[9][KC];
-    if (wave < total) load(wave, cur);
-    for (int b = wave; b < total; b += nwaves) {
-        if (b + nwaves < total) load(b + nwaves, nxt);
-        const int bxi = b % nbx;
-        const int r = b / nbx;
-        const int yy = r % a.H;
-        const int n = r / a.H;
-        const int px = bxi * 16 + col;
-        v4f acc[MB];
+        for (int mb = 0; mb < MB; ++mb) acc[k][mb] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int mb = 0; mb < MB; ++mb) acc[mb] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 9; ++t) {
+        const int ry = wv + d + (t / 3 - 1) * step;
+        const int rx = col + d + (t % 3 - 1) * step;
+        const uint8_t* src = tile + (ry * TWp + rx) * PS + 8 * grp;
 #pragma unroll
-        for (int t = 0; t < 9; ++t)
+        for (int kc = 0; kc < KC; ++kc) {
+            v4s bv[4];
 #pragma unroll
-            for (int kc = 0; kc < KC; ++kc)
+            for (int k = 0; k < 4; ++k) bv[k] = *reinterpret_cast<const v4s*>(src + k * 16 * PS + kc * 32);
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
 #pragma unroll
                 for (int mb = 0; mb < MB; ++mb)
-                    acc[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[mb][t][kc], cur[t][kc], acc[mb], 0, 0, 0);
+                    acc[k][mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[mb][t][kc], bv[k], acc[k][mb], 0, 0, 0);
+        }
+    }
 #pragma unroll
-        for (int t = 0; t < 9; ++t)
+    for (int k = 0; k < 4; ++k) {
+        const int px = x0 + k * 16 + col;
+        if (px >= a.W) continue;
+        const size_t o = (((size_t)n * a.H + yy) * a.W + px) * COUT;
 #pragma unroll
-            for (int kc = 0; kc < KC; ++kc) cur[t][kc] = nxt[t][kc];
-        if (px < a.W) {
-            const size_t o = (((size_t)n * a.H + yy) * a.W + px) * COUT;
+        for (int mb = 0; mb < MB; ++mb) {
+            const int c0 = mb * 16 + 4 * grp;
+            float v[4] = {acc[k][mb][0], acc[k][mb][1], acc[k][mb][2], acc[k][mb][3]};
+            if (a.bias) {
 #pragma unroll
-            for (int mb = 0; mb < MB; ++mb) {
-                const int c0 = mb * 16 + 4 * grp;
-                float v[4] = {acc[mb][0], acc[mb][1], acc[mb][2], acc[mb][3]};
-                if (a.bias) {
+                for (int i = 0; i < 4; ++i) v[i] += a.bias[c0 + i];
+            }
+            if (EPI == CNN_EPI_F32) {
+                *reinterpret_cast<float4*>(static_cast<float*>(a.y) + o + c0) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
+                if (EPI == CNN_EPI_RELU_BF16) {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) v[i] += a.bias[c0 + i];
+                    for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.f ? v[i] : 0.f;
+                } else {  // CNN_EPI_MASK_BF16: gradient through the ReLU of the input activation
+                    const uint2 m = *reinterpret_cast<const uint2*>(a.mask + o + c0);
+                    if (!bf_positive(m.x)) v[0] = 0.f;
+                    if (!bf_positive(m.x >> 16)) v[1] = 0.f;
+                    if (!bf_positive(m.y)) v[2] = 0.f;
+                    if (!bf_positive(m.y >> 16)) v[3] = 0.f;
                 }
-                if (EPI == CNN_EPI_F32) {
-                    *reinterpret_cast<float4*>(static_cast<float*>(a.y) + o + c0) = make_float4(v[0], v[1], v[2], v[3]);
-                } else {
-                    if (EPI == CNN_EPI_RELU_BF16) {
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.f ? v[i] : 0.f;
-                    } else {  // CNN_EPI_MASK_BF16: gradient through the ReLU of the input activation
-                        const uint2 m = *reinterpret_cast<const uint2*>(a.mask + o + c0);
-                        if (!bf_positive(m.x)) v[0] = 0.f;
-                        if (!bf_positive(m.x >> 16)) v[1] = 0.f;
-                        if (!bf_positive(m.y)) v[2] = 0.f;
-                        if (!bf_positive(m.y >> 16)) v[3] = 0.f;
-                    }
-                    uint2 pk;
-                    pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-                    pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
-                    *reinterpret_cast<uint2*>(static_cast<uint16_t*>(a.y) + o + c0) = pk;
-                }
+                uint2 pk;
+                pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                *reinterpret_cast<uint2*>(static_cast<uint16_t*>(a.y) + o + c0) = pk;
             }
         }
     }
@@ -290,65 +326,43 @@ __global__ __launch_bounds__(256) void adam_kernel(float* param, const float* gr
 
 // ---- data preparation -----------------------------------------------------------
 
-// 4x4 box average with rounding: out(x, y) = (sum of in(4x.., 4y..) + 8) >> 4.
-__global__ __launch_bounds__(256) void downsample4_kernel(View in, View out)
-{
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= out.w || y >= out.h) return;
-    uint32_t s = 0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const uint32_t v = *reinterpret_cast<const uint32_t*>(row_ptr<const uint8_t>(in, 4 * y + j) + 4 * x);
-        s += __builtin_amdgcn_sad_u8(v, 0u, 0u);
-    }
-    row_ptr<uint8_t>(out, y)[x] = (uint8_t)((s + 8) >> 4);
-}
-
 // Network input of frame n: ch0 = L/255, ch1 = R(x - dfill)/255 (bf16), ch2..15 = 0,
 // where dfill is the coarse disparity with holes filled by the smaller of the
 // nearest valid values to the left and right within 64 pixels (0 when none).
-// dfill is written as float for the range kernel.  One workgroup per row; the
-// row of coarse disparities is searched in LDS.
+// dfill is written as float for the range kernel.  One workgroup per row: each wave
+// takes 64-pixel chunks and records which pixels are valid as a ballot in LDS; the
+// nearest valid pixel within 64 then lies in the pixel's own chunk or in the
+// neighbouring one, found with a leading / trailing bit count.
 constexpr int CNN_MAX_W4 = 4096;
 
 __global__ __launch_bounds__(256) void cnn_input_kernel(View L4, View R4, View dc, int n, uint16_t* x, float* dfill)
 {
-    // nearest valid index to the left (max-scan) and to the right (min-scan) of
-    // every column, one row per workgroup
-    __shared__ int16_t drow[CNN_MAX_W4];
-    __shared__ int16_t lft[CNN_MAX_W4], rgt[CNN_MAX_W4];
+    __shared__ uint64_t valid[CNN_MAX_W4 / 64];
     const int Y = blockIdx.x;
     const int W = L4.w, H = L4.h;
+    const int nch = (W + 63) >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int16_t* src = row_ptr<const int16_t>(dc, Y);
-    for (int X = threadIdx.x; X < W; X += 256) {
-        const int16_t v = src[X];
-        drow[X] = v;
-        lft[X] = v >= 0 ? (int16_t)X : (int16_t)-1;
-        rgt[X] = v >= 0 ? (int16_t)X : (int16_t)CNN_MAX_W4;
+    for (int c = wv; c < nch; c += 4) {
+        const int X = c * 64 + lane;
+        const uint64_t m = __ballot(X < W && src[X] >= 0);
+        if (lane == 0) valid[c] = m;
     }
     __syncthreads();
-    for (int off = 1; off < W; off <<= 1) {
-        int16_t a[CNN_MAX_W4 / 256], b[CNN_MAX_W4 / 256];
-        int k = 0;
-        for (int X = threadIdx.x; X < W; X += 256, ++k) {
-            a[k] = X >= off ? max(lft[X], lft[X - off]) : lft[X];
-            b[k] = X + off < W ? min(rgt[X], rgt[X + off]) : rgt[X];
-        }
-        __syncthreads();
-        k = 0;
-        for (int X = threadIdx.x; X < W; X += 256, ++k) {
-            lft[X] = a[k];
-            rgt[X] = b[k];
-        }
-        __syncthreads();
-    }
-    for (int X = threadIdx.x; X < W; X += 256) {
-        int d = drow[X];
+    const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);  // lanes 0 .. lane
+    for (int c = wv; c < nch; c += 4) {
+        const int X = c * 64 + lane;
+        if (X >= W) continue;
+        int d = src[X];
         if (d < 0) {
-            const int l = lft[X], r = rgt[X];
-            const int a = (l >= 0 && X - l <= 64) ? drow[l] : -1;
-            const int b = (r < W && r - X <= 64) ? drow[r] : -1;
+            const uint64_t ml = valid[c] & le, mr = valid[c] & ~le;
+            int l = -1, r = W;
+            if (ml) l = c * 64 + 63 - __clzll((long long)ml);
+            else if (c > 0 && valid[c - 1]) l = (c - 1) * 64 + 63 - __clzll((long long)valid[c - 1]);
+            if (mr) r = c * 64 + __ffsll((unsigned long long)mr) - 1;
+            else if (c + 1 < nch && valid[c + 1]) r = (c + 1) * 64 + __ffsll((unsigned long long)valid[c + 1]) - 1;
+            const int a = (l >= 0 && X - l <= 64) ? src[l] : -1;
+            const int b = (r < W && r - X <= 64) ? src[r] : -1;
             d = a < 0 ? (b < 0 ? 0 : b) : (b < 0 ? a : min(a, b));
         }
         const size_t p = ((size_t)n * H + Y) * W + X;
@@ -390,61 +404,65 @@ __global__ __launch_bounds__(256) void labels_kernel(View disp, int n, int W4, i
 // Per-tile disparity sets, one wave per tile.  Tile (tx, ty) covers quarter pixels
 // [16 tx, 16 tx + 15] x [14 ty, 14 ty + 13] (64 x 56 full-resolution pixels); with a
 // margin of one quarter pixel, every quarter pixel proposes the band
-// [4 c - band/2 - 4, 4 c + band/2 + 4] around its centre c = dfill + out0.  The
-// tile keeps the union as a mask of 8-disparity bins (bit b: [8b, 8b + 8)), or all
-// bins below D when its smallest confidence logit is below the threshold.
+// [4 c - band/2 - 4, 4 c + band/2 + 4] around its centre c = dfill (+ the network's
+// residual when centre_residual is set).  The tile keeps the union as a mask of
+// 8-disparity bins (bit b: [8b, 8b + 8)), or all bins below D when its smallest
+// confidence logit is below the threshold.
 __global__ __launch_bounds__(256) void band_mask_kernel(CnnBandArgs a)
 {
     const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int ntiles = a.ntx * a.nty;
-    if (t >= ntiles) return;
-    const int tx = t % a.ntx, ty = t / a.ntx;
-    const int qx0 = max(0, 16 * tx - 1), qx1 = min(a.W4 - 1, 16 * tx + 16);
-    const int qy0 = max(0, 14 * ty - 1), qy1 = min(a.H4 - 1, 14 * ty + 14);
-    const int nx = qx1 - qx0 + 1, ny = qy1 - qy0 + 1;
-    const int nbins = a.D >> 3;
-    uint64_t mask = 0;
-    float lmin = 1e30f;
-    for (int k = lane; k < nx * ny && nx > 0 && ny > 0; k += 64) {
-        const int qy = qy0 + k / nx, qx = qx0 + k % nx;
-        const size_t p = (size_t)qy * a.W4 + qx;
-        const float c = 4.f * (a.dfill[p] + a.out[p * 16]);
-        lmin = fminf(lmin, a.out[p * 16 + 1]);
-        const int lo = max(0, (int)floorf(c) - a.band / 2 - 4);
-        const int hi = min(a.D - 1, (int)ceilf(c) + a.band / 2 + 4);
-        if (lo <= hi) {
-            const int b0 = lo >> 3, b1 = hi >> 3;
-            mask |= ((2ull << b1) - 1ull) & ~((1ull << b0) - 1ull);
+    if (t < ntiles) {
+        const int tx = t % a.ntx, ty = t / a.ntx;
+        const int qx0 = max(0, 16 * tx - 1), qx1 = min(a.W4 - 1, 16 * tx + 16);
+        const int qy0 = max(0, 14 * ty - 1), qy1 = min(a.H4 - 1, 14 * ty + 14);
+        const int nx = qx1 - qx0 + 1, ny = qy1 - qy0 + 1;
+        const int nbins = a.D >> 3;
+        uint64_t mask = 0;
+        float lmin = 1e30f;
+        for (int k = lane; k < nx * ny && nx > 0 && ny > 0; k += 64) {
+            const int qy = qy0 + k / nx, qx = qx0 + k % nx;
+            const size_t p = (size_t)qy * a.W4 + qx;
+            const float c = 4.f * (a.centre_residual ? a.dfill[p] + a.out[p * 16] : a.dfill[p]);
+            lmin = fminf(lmin, a.out[p * 16 + 1]);
+            const int lo = max(0, (int)floorf(c) - a.band / 2 - 4);
+            const int hi = min(a.D - 1, (int)ceilf(c) + a.band / 2 + 4);
+            if (lo <= hi) {
+                const int b0 = lo >> 3, b1 = hi >> 3;
+                mask |= ((2ull << b1) - 1ull) & ~((1ull << b0) - 1ull);
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            mask |= (uint64_t)__shfl_xor((unsigned long long)mask, o);
+            lmin = fminf(lmin, __shfl_xor(lmin, o));
+        }
+        const uint64_t all = (nbins >= 64) ? ~0ull : ((1ull << nbins) - 1ull);
+        const bool fallback = nx <= 0 || ny <= 0 || !(lmin >= a.logit_thresh) || mask == 0;
+        if (lane == 0) {
+            a.tile_mask[t] = (uint32_t)(fallback ? all : (mask & all));
+            a.tile_evals[t] = fallback ? (1 << 16) : 0;  // the count is added with the items
         }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        mask |= (uint64_t)__shfl_xor((unsigned long long)mask, o);
-        lmin = fminf(lmin, __shfl_xor(lmin, o));
-    }
-    const uint64_t all = (nbins >= 64) ? ~0ull : ((1ull << nbins) - 1ull);
-    const bool fallback = nx <= 0 || ny <= 0 || !(lmin >= a.logit_thresh) || mask == 0;
-    if (lane == 0) {
-        a.tile_mask[t] = (uint32_t)(fallback ? all : (mask & all));
-        a.tile_evals[t] = fallback ? (1 << 16) : 0;  // the count is filled in by the item kernel
-    }
+
 }
 
-// Work items of the bin masks (one workgroup of 1024 lanes): every run of set bins
-// becomes items of at most `chunk` disparities; a block-wide scan places them.
+// The work list of the tiles' masks (one workgroup): every run of set bins is cut
+// into items of at most `chunk` disparities; a block-wide scan places each tile's
+// items and the tile table records them.
 __global__ __launch_bounds__(1024) void band_items_kernel(CnnBandArgs a)
 {
+    const int ntiles = a.ntx * a.nty;
     __shared__ int scan[1024];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    const int ntiles = a.ntx * a.nty;
     for (int base = 0; base < ntiles; base += 1024) {
-        const int t = base + threadIdx.x;
+        const int tt = base + threadIdx.x;
         uint32_t m = 0;
         int cnt = 0;
-        if (t < ntiles) {
-            m = a.tile_mask[t];
+        if (tt < ntiles) {
+            m = a.tile_mask[tt];
             for (uint32_t r = m; r;) {
                 const int r0 = __builtin_ctz(r);
                 const uint32_t sh = r >> r0;
@@ -462,7 +480,7 @@ __global__ __launch_bounds__(1024) void band_items_kernel(CnnBandArgs a)
             __syncthreads();
         }
         const int first = carry + scan[threadIdx.x] - cnt;
-        if (t < ntiles) {
+        if (tt < ntiles) {
             int k = first, evals = 0, lo = 1 << 30, hi = 0;
             for (uint32_t r = m; r;) {
                 const int r0 = __builtin_ctz(r);
@@ -472,8 +490,8 @@ __global__ __launch_bounds__(1024) void band_items_kernel(CnnBandArgs a)
                 for (int ds = ds0; ds < de0; ds += a.chunk, ++k) {
                     BmItem it;
                     it.frame = 0;
-                    it.tx = t % a.ntx;
-                    it.ty = t / a.ntx;
+                    it.tx = tt % a.ntx;
+                    it.ty = tt / a.ntx;
                     it.ds = ds;
                     it.de = min(de0, ds + a.chunk);
                     a.items[k] = it;
@@ -488,32 +506,32 @@ __global__ __launch_bounds__(1024) void band_items_kernel(CnnBandArgs a)
             bt.hi = cnt ? hi : 0;
             bt.first_item = first;
             bt.nitems = cnt;
-            a.tiles[t] = bt;
-            a.tile_evals[t] = evals | (a.tile_evals[t] & (1 << 16));
+            a.tiles[tt] = bt;
+            a.tile_evals[tt] = evals | (a.tile_evals[tt] & (1 << 16));
         }
         __syncthreads();
         if (threadIdx.x == 1023) carry += scan[1023];
         __syncthreads();
     }
-    if (threadIdx.x == 0) a.nitems[0] = carry;
+    if (threadIdx.x == 0) {
+        a.nitems[0] = carry;
+        a.list_counter[0] = 0;
+    }
 }
 
 }  // namespace cnn
 
 hipError_t launch_cnn_conv(int cin, int cout, int epi, const CnnConvArgs& a, hipStream_t s)
 {
-    // each wave loads the layer's weights into VGPRs once and then walks about four
-    // 16-pixel blocks, so that the weight loads are amortised
-    const int nbx = (a.W + 15) / 16;
-    const long total = (long)a.N * a.H * nbx;
-    long g = (total + 15) / 16;
-    if (g > 1024) g = 1024;
-    if (g < 1) g = 1;
-    const int grid = (int)g;
-    if (total < 1) return hipErrorInvalidValue;
+    // one workgroup per tile of CNN_TILE_ROWS x 64 output pixels
+    if (a.N < 1 || a.H < 1 || a.W < 1 || a.dil < 1 || a.dil > CNN_MAX_DIL) return hipErrorInvalidValue;
+    const long ntiles = (long)a.N * ((a.H + CNN_TILE_ROWS - 1) / CNN_TILE_ROWS) * ((a.W + 63) / 64);
+    if (ntiles > 0x7fffffffL) return hipErrorInvalidValue;
+    const int grid = (int)ntiles;
+    const size_t lds = (size_t)(64 + 2 * a.dil) * (CNN_TILE_ROWS + 2 * a.dil) * (2 * cin + 8);
 #define CNN_CONV(CI, CO, E)                                                                             \
     if (cin == CI && cout == CO && epi == E) {                                                          \
-        hipLaunchKernelGGL((cnn::conv3x3_kernel<CI, CO, E>), dim3(grid), dim3(256), 0, s, a);         \
+        hipLaunchKernelGGL((cnn::conv3x3_kernel<CI, CO, E>), dim3(grid), dim3(256), lds, s, a);       \
         return hipGetLastError();                                                                       \
     }
     CNN_CONV(16, 32, CNN_EPI_RELU_BF16)
@@ -574,13 +592,6 @@ hipError_t launch_cnn_adam(float* param, const float* grad, float* m, float* v, 
     return hipGetLastError();
 }
 
-hipError_t launch_downsample4(const View& in, const View& out, hipStream_t s)
-{
-    dim3 grid((out.w + 63) / 64, (out.h + 3) / 4);
-    hipLaunchKernelGGL(cnn::downsample4_kernel, grid, dim3(256), 0, s, in, out);
-    return hipGetLastError();
-}
-
 hipError_t launch_cnn_input(const View& L4, const View& R4, const View& dc, int n, uint16_t* x, float* dfill,
                             hipStream_t s)
 {
@@ -593,6 +604,7 @@ hipError_t launch_band_items(const CnnBandArgs& a, hipStream_t s)
 {
     if (a.D > 256 || a.D % 8 || a.chunk < 8 || a.chunk % 8) return hipErrorInvalidValue;
     const int ntiles = a.ntx * a.nty;
+    if (ntiles < 1 || !a.list_counter) return hipErrorInvalidValue;
     hipLaunchKernelGGL(cnn::band_mask_kernel, dim3((ntiles + 3) / 4), dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -10,9 +10,10 @@ namespace stereo {
 // ---- census ----------------------------------------------------------------
 bool census_supported(int cw, int ch, int ref3x3);
 hipError_t launch_census(const View& src, const View& dst, int cw, int ch, int ref3x3, hipStream_t s);
-// both images of a pair in one launch (src1/dst1 may be null for a single image)
+// both images of a pair in one launch (src1/dst1 may be null for a single image);
+// q0/q1, when given, receive the 4 x 4 box means (sum + 8) >> 4 of the images
 hipError_t launch_census_pair(const View& src0, const View& dst0, const View* src1, const View* dst1, int cw, int ch,
-                              hipStream_t s);
+                              hipStream_t s, const View* q0 = nullptr, const View* q1 = nullptr);
 
 // ---- fused SAD / census matcher -------------------------------------------------
 enum { BM_COST_SAD = 0, BM_COST_CENSUS = 1 };
@@ -87,6 +88,8 @@ struct BmListParams {
     const BmItem* items;
     const int* nitems;  // device-side item count
     const int* sched;   // order in which the persistent grid takes the items (nullptr: 0, 1, ...)
+    int* counter;       // next item to hand out: 0 before the matcher launch; the finish launch
+                        // resets it, so a matcher launch is always followed by its finish
     uint32_t* lkey;
     uint32_t* rpart;
     int rp_stride;      // bm_list_rp_stride(longest item)
@@ -96,6 +99,8 @@ struct BmListParams {
 
 hipError_t launch_bm_list(int kind, int ww, const BmListParams& p, int grid, hipStream_t s);
 hipError_t launch_bm_list_finish(const BmListParams& p, int max_width, hipStream_t s);
+// widest frame the work-list merge takes (its LDS row and item table stay within 64 KiB)
+constexpr int BM_LIST_MAX_WIDTH = 11264;
 size_t bm_list_lkey_elems(int max_items);
 int bm_list_rp_stride(int chunk_max);
 size_t bm_list_rpart_elems(int max_items, int chunk_max);
@@ -146,6 +151,11 @@ struct AwParams {
 
 // ---- learned proposal network -------------------------------------------------------
 enum { CNN_EPI_RELU_BF16 = 0, CNN_EPI_F32 = 1, CNN_EPI_MASK_BF16 = 2 };
+// 3x3 convolution tiles: CNN_TILE_ROWS rows x 64 pixels per workgroup; the staged
+// input tile (CNN_TILE_ROWS + 2 dil) x (64 + 2 dil) x (2 cin + 8) bytes stays below
+// 64 KiB of LDS for dil <= CNN_MAX_DIL
+constexpr int CNN_TILE_ROWS = 4;
+constexpr int CNN_MAX_DIL = 4;
 
 struct CnnConvArgs {
     const uint16_t* x;     // [N][H][W][cin] bf16
@@ -169,10 +179,12 @@ struct CnnBandArgs {
     BmTile* tiles;         // [nty][ntx]
     BmItem* items;
     int* nitems;
+    int* list_counter;     // item counter of the list matcher, set to 0 with the item count
     int* tile_evals;       // [nty][ntx] disparities searched per tile | fallback << 16
     uint32_t* tile_mask;   // [nty][ntx] searched 8-disparity bins
     int W4, H4, ntx, nty, D;
     int band, chunk;
+    int centre_residual;   // 1: centre = dfill + network residual, 0: dfill
     float logit_thresh;
 };
 
@@ -184,7 +196,6 @@ hipError_t launch_cnn_loss(const float* out, const float* label, const float* df
 hipError_t launch_cnn_labels(const View& disp, int n, int W4, int H4, float* labels, hipStream_t s);
 hipError_t launch_cnn_adam(float* param, const float* grad, float* m, float* v, int n, int cin, int cout, float lr,
                            float b1, float b2, float eps, float c1, float c2, uint16_t* wf, uint16_t* wb, hipStream_t s);
-hipError_t launch_downsample4(const View& in, const View& out, hipStream_t s);
 hipError_t launch_cnn_input(const View& L4, const View& R4, const View& dc, int n, uint16_t* x, float* dfill,
                             hipStream_t s);
 hipError_t launch_band_items(const CnnBandArgs& a, hipStream_t s);

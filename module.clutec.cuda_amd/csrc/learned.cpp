@@ -403,6 +403,10 @@ stereo_status stereo_conv3x3_bf16(stereo_context* ctx, const uint16_t* x, int32_
     if (!ctx || !x || !w || !y || N < 1 || H < 1 || W < 1 || dil < 1) return STEREO_ERROR_INVALID_ARGUMENT;
     if ((cin != 16 && cin != 32) || (cout != 16 && cout != 32))
         return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "channels %d -> %d not supported (16 or 32)", cin, cout);
+    if (dil > stereo::CNN_MAX_DIL)
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "dilation %d above %d", dil, stereo::CNN_MAX_DIL);
+    if (((uintptr_t)x | (uintptr_t)w | (uintptr_t)y) & 15)
+        return fail(ctx, STEREO_ERROR_INVALID_ARGUMENT, "x, w and y must be 16-byte aligned device pointers");
     stereo::CnnConvArgs a;
     a.x = x;
     a.w = w;
@@ -430,6 +434,10 @@ stereo_status stereo_conv3x3_bwd_bf16(stereo_context* ctx, const uint16_t* x, in
         return STEREO_ERROR_INVALID_ARGUMENT;
     if ((cin != 16 && cin != 32) || (cout != 16 && cout != 32) || (cin == 16 && cout == 16))
         return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "channels %d -> %d not supported", cin, cout);
+    if (dil > stereo::CNN_MAX_DIL)
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "dilation %d above %d", dil, stereo::CNN_MAX_DIL);
+    if (((uintptr_t)x | (uintptr_t)wt | (uintptr_t)dy | (uintptr_t)dx | (uintptr_t)dwdb) & 15)
+        return fail(ctx, STEREO_ERROR_INVALID_ARGUMENT, "device pointers must be 16-byte aligned");
     const hipStream_t s = (hipStream_t)stream;
     if (dx) {
         stereo::CnnConvArgs a;
@@ -489,6 +497,7 @@ struct LearnedExt : DriverExt {
     int* nitems = nullptr;
     int* tile_evals = nullptr;
     uint32_t* tile_mask = nullptr;
+    int* counter = nullptr;  // item counter of the list matcher (reset by its finish)
     uint32_t* lkey = nullptr;
     uint32_t* rpart = nullptr;
     int ntx = 0, nty = 0, max_items = 0;
@@ -518,7 +527,7 @@ struct LearnedExt : DriverExt {
         for (stereo_image* im : {&L4, &R4, &dc})
             if (im->data) (void)hipFree(im->data);
         for (void* p : {(void*)padL, (void*)padR, (void*)frame_dev, (void*)tiles, (void*)items, (void*)nitems,
-                        (void*)tile_evals, (void*)tile_mask, (void*)lkey, (void*)rpart})
+                        (void*)tile_evals, (void*)tile_mask, (void*)counter, (void*)lkey, (void*)rpart})
             if (p) (void)hipFree(p);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -537,6 +546,8 @@ stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** out,
     if (!format_valid(fmt)) return fail(ctx, STEREO_ERROR_INVALID_FORMAT, "Invalid image format");
     if (fmt->pixel_type != STEREO_PIXEL_LUM_U8)
         return fail(ctx, STEREO_ERROR_UNSUPPORTED_PIXEL_TYPE, "learned matcher: LUM_U8 images only");
+    if (fmt->width > stereo::BM_LIST_MAX_WIDTH)
+        return fail(ctx, STEREO_ERROR_INVALID_FORMAT, "learned matcher: width above %d", stereo::BM_LIST_MAX_WIDTH);
     if (fmt->width % 4 || fmt->height % 4 || fmt->width < 128 || fmt->height < 96)
         return fail(ctx, STEREO_ERROR_INVALID_FORMAT, "learned matcher: size %dx%d must be multiples of 4, at least 128x96",
                     fmt->width, fmt->height);
@@ -551,6 +562,10 @@ stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** out,
     if (Dc % 8 || Dc > 256 || p->band < 8 || p->band % 8 || p->chunk < 8 || p->chunk % 8)
         return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "learned matcher: coarse D %d, band %d, chunk %d must be multiples of 8",
                     Dc, p->band, p->chunk);
+    if ((p->centre_residual != 0 && p->centre_residual != 1) || (p->direct_launch != 0 && p->direct_launch != 1) ||
+        p->reserved[0] || p->reserved[1])
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "learned matcher: centre_residual and direct_launch are 0 or 1, "
+                    "reserved fields 0");
     if ((st = set_device(ctx))) return st;
 
     stereo_driver* d = new stereo_driver(stereo_driver::LEARNED);
@@ -598,11 +613,13 @@ stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** out,
         (e = hipMalloc((void**)&x->items, x->max_items * sizeof(stereo::BmItem))) ||
         (e = hipMalloc((void**)&x->nitems, sizeof(int))) || (e = hipMalloc((void**)&x->tile_evals, nt * sizeof(int))) ||
         (e = hipMalloc((void**)&x->tile_mask, nt * sizeof(uint32_t))) ||
+        (e = hipMalloc((void**)&x->counter, sizeof(int))) ||
         (e = hipStreamCreateWithFlags(&x->cap, hipStreamNonBlocking)) ||
         (e = hipMalloc((void**)&x->lkey, stereo::bm_list_lkey_elems(x->max_items) * 4)) ||
         (e = hipMalloc((void**)&x->rpart, stereo::bm_list_rpart_elems(x->max_items, p->chunk) * 4)))
         return cleanup(hip_fail(ctx, e, "hipMalloc"));
-    if ((e = hipMemset(x->padL, 0, x->pad_pitch * H)) || (e = hipMemset(x->padR, 0, x->pad_pitch * H)))
+    if ((e = hipMemset(x->padL, 0, x->pad_pitch * H)) || (e = hipMemset(x->padR, 0, x->pad_pitch * H)) ||
+        (e = hipMemset(x->counter, 0, sizeof(int))) || (e = hipDeviceSynchronize()))
         return cleanup(hip_fail(ctx, e, "hipMemset"));
     stereo::BmFrame& f = x->frame;
     std::memset(&f, 0, sizeof(f));
@@ -636,9 +653,9 @@ stereo_status learned_enqueue(stereo_driver* d, LearnedExt* x, const stereo_imag
     stereo_status st;
     if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[0], s), "hipEventRecord");
     const View vl = view_of(left), vr = view_of(right);
-    HIP_TRY(ctx, stereo::launch_census_pair(vl, f.L, &vr, &f.R, d->mp.census_w, d->mp.census_h, s), "census launch");
-    HIP_TRY(ctx, stereo::launch_downsample4(vl, view_of(&x->L4), s), "downsample launch");
-    HIP_TRY(ctx, stereo::launch_downsample4(vr, view_of(&x->R4), s), "downsample launch");
+    const View q0 = view_of(&x->L4), q1 = view_of(&x->R4);
+    HIP_TRY(ctx, stereo::launch_census_pair(vl, f.L, &vr, &f.R, d->mp.census_w, d->mp.census_h, s, &q0, &q1),
+            "census launch");
     if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[1], s), "hipEventRecord");
     if ((st = stereo_matcher_process(x->coarse, &x->dc, nullptr, nullptr, &x->L4, &x->R4, s))) return st;
     if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[2], s), "hipEventRecord");
@@ -651,8 +668,10 @@ stereo_status learned_enqueue(stereo_driver* d, LearnedExt* x, const stereo_imag
     ba.tiles = x->tiles;
     ba.items = x->items;
     ba.nitems = x->nitems;
+    ba.list_counter = x->counter;
     ba.tile_evals = x->tile_evals;
     ba.tile_mask = x->tile_mask;
+    ba.centre_residual = x->lp.centre_residual;
     ba.W4 = W / 4;
     ba.H4 = H / 4;
     ba.ntx = x->ntx;
@@ -668,6 +687,7 @@ stereo_status learned_enqueue(stereo_driver* d, LearnedExt* x, const stereo_imag
     lp.items = x->items;
     lp.nitems = x->nitems;
     lp.sched = nullptr;
+    lp.counter = x->counter;
     lp.lkey = x->lkey;
     lp.rpart = x->rpart;
     lp.rp_stride = stereo::bm_list_rp_stride(x->lp.chunk);
@@ -716,6 +736,7 @@ stereo_status stereo_learned_process(stereo_driver* d, stereo_image* disp, stere
     // sequence is captured once per pair of input images and replayed as a graph
     x->timed = ctx->timing != 0;
     if (x->timed) return learned_enqueue(d, x, left, right, s, true);
+    if (x->lp.direct_launch) return learned_enqueue(d, x, left, right, s, false);
     if (!x->exec || x->graph_key[0] != left->data || x->graph_key[1] != right->data) {
         x->drop_graph();
         HIP_TRY(ctx, hipStreamBeginCapture(x->cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");

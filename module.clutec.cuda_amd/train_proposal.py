@@ -93,11 +93,11 @@ def train(ctx, pool, steps, batch, lr, seed, log=print, init=None):
     return net.params(), history
 
 
-def calibrate(ctx, params, W, H, D, seeds, thresholds, min_agree=0.996, log=print):
+def calibrate(ctx, params, W, H, D, seeds, thresholds, min_agree=0.995, band=8, log=print):
     """Agreement with the full-range matcher and searched fraction of the cost volume
-    per confidence threshold, on held-out frames at the deployment size; returns
-    the smallest threshold whose mean agreement reaches min_agree (most tiles on
-    the band), and the table."""
+    per confidence threshold, on held-out frames at the deployment size (bands around
+    the quarter-resolution census map).  Returns the smallest threshold (most tiles on
+    their band) at which every frame reaches min_agree, and the table."""
     full = cs.Matcher(ctx, W, H, algorithm=cs.ALGO_CENSUS, window=(9, 9), census=(9, 7), num_disp=D, tol=1)
     frames = []
     for seed in seeds:
@@ -109,7 +109,7 @@ def calibrate(ctx, params, W, H, D, seeds, thresholds, min_agree=0.996, log=prin
         frames.append((dl, dr, ref.download()))
     table = []
     for t in thresholds:
-        lm = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, band=32, chunk=32, logit_thresh=t)
+        lm = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, band=band, chunk=64, logit_thresh=t)
         out = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_I16)
         agree, frac = [], []
         for dl, dr, ref in frames:
@@ -121,7 +121,7 @@ def calibrate(ctx, params, W, H, D, seeds, thresholds, min_agree=0.996, log=prin
         log(f"  threshold {t:+.2f}: agreement {row['agree_mean']:.5f} (min {row['agree_min']:.5f}) "
             f"searched {row['evals_frac']:.3f}")
         table.append(row)
-    ok = [r for r in table if r["agree_mean"] >= min_agree]
+    ok = [r for r in table if r["agree_min"] >= min_agree]
     best = min(ok, key=lambda r: r["thresh"]) if ok else max(table, key=lambda r: r["thresh"])
     return best["thresh"], table
 
@@ -138,10 +138,27 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--seed0", type=int, default=10000, help="first synthetic pair seed")
     ap.add_argument("--out", required=True)
-    ap.add_argument("--calib-frames", type=int, default=4, help="held-out 1920x1080 D=256 frames for the threshold")
+    ap.add_argument("--calib-frames", type=int, default=8, help="held-out 1920x1080 D=256 frames for the threshold")
+    ap.add_argument("--band", type=int, default=8, help="band of the searched disparities around each centre")
+    ap.add_argument("--calibrate-only", action="store_true",
+                    help="keep the parameters in --out and only recalibrate the threshold")
     a = ap.parse_args()
     ctx = cs.Context(0)
     log = lambda s: print(s, flush=True)
+    thresholds = [-8.0, -7.0, -6.0, -5.5, -5.0, -4.5, -4.0, -3.5, -3.0, -2.0, -1.0, 0.0, 1.0, 2.0]
+    if a.calibrate_only:
+        params = np.fromfile(a.out, "<f4")
+        with open(a.out + ".json") as f:
+            meta = json.load(f)
+        log(f"calibrating the confidence threshold at 1920x1080, D=256, band {a.band}")
+        thr, table = calibrate(ctx, params, 1920, 1080, 256, range(20000, 20000 + a.calib_frames), thresholds,
+                               band=a.band, log=log)
+        meta.update({"logit_thresh": thr, "calibration": table, "band": a.band, "centre_residual": 0,
+                     "calibration_rule": "smallest threshold with every held-out frame at agreement >= 0.995"})
+        with open(a.out + ".json", "w") as f:
+            json.dump(meta, f, indent=1)
+        log(json.dumps({"logit_thresh": thr}))
+        return
     log(f"preparing {a.frames} frames {a.width}x{a.height} D={a.disparities}")
     t0 = time.time()
     pool = FramePool(ctx, a.width, a.height, a.disparities, a.frames, a.seed0, log=log)
@@ -151,8 +168,8 @@ def main():
     params, hist = train(ctx, pool, a.steps, a.batch, a.lr, a.seed, log=log)
     t_train = time.time() - t1
     log("calibrating the confidence threshold at 1920x1080, D=256")
-    thr, table = calibrate(ctx, params, 1920, 1080, 256, range(20000, 20000 + a.calib_frames),
-                           [-6.0, -4.0, -3.0, -2.0, -1.5, -1.0, -0.5, 0.0, 0.5, 1.0, 2.0, 3.0], log=log)
+    thr, table = calibrate(ctx, params, 1920, 1080, 256, range(20000, 20000 + a.calib_frames), thresholds,
+                           band=a.band, log=log)
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     params.astype("<f4").tofile(a.out)
     tail = np.array(hist[-100:])
@@ -161,7 +178,8 @@ def main():
         "steps": a.steps, "batch": a.batch, "lr": a.lr, "seed": a.seed, "seed0": a.seed0,
         "first_loss": hist[0], "final_loss_mean100": tail.mean(axis=0).tolist(),
         "data_seconds": round(t_data, 1), "train_seconds": round(t_train, 1),
-        "logit_thresh": thr, "calibration": table,
+        "logit_thresh": thr, "calibration": table, "band": a.band, "centre_residual": 0,
+        "calibration_rule": "smallest threshold with every held-out frame at agreement >= 0.995",
         "loss_fields": ["huber", "bce", "labelled_pixels", "within_tau"],
     }
     with open(a.out + ".json", "w") as f:

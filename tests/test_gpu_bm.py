@@ -179,3 +179,12 @@ def test_crosscheck_format_mismatch(ctx):
     with pytest.raises(cs.StereoError) as e:
         cc.process(a, a, b)
     assert e.value.status == cs.ERROR_FORMAT_MISMATCH
+
+
+@pytest.mark.parametrize("op", [0, 1])
+def test_valu_peak_probe(ctx, op):
+    """The measured INT32 issue rate is positive and within a factor of the
+    specified 39.3 T lane-ops/s (256 CUs x 64 lanes x 2.4 GHz)."""
+    v = ctx.valu_peak(op)
+    print(f"valu peak op {op}: {v:.2f} T lane-ops/s", flush=True)
+    assert 5.0 < v < 100.0

@@ -123,12 +123,14 @@ def test_learned_full_fallback_equals_full_range(ctx):
         assert np.array_equal(a, c)
 
 
-def test_learned_band_matches_oracle(ctx):
+@pytest.mark.parametrize("centre_residual", [False, True])
+def test_learned_band_matches_oracle(ctx, centre_residual):
     """Band search (no fallback): every tile searches only its proposed disparity
     bins; the maps equal the oracle restricted to the same per-tile bins."""
     W, H, D = 384, 224, 128
     L, R = cs.synth_pair(78, W, H, 2, 8, D - 1, 8)
-    m = cs.LearnedMatcher(ctx, W, H, cs.cnn_init_params(0), num_disp=D, logit_thresh=-1e30, chunk=32)
+    m = cs.LearnedMatcher(ctx, W, H, cs.cnn_init_params(0), num_disp=D, logit_thresh=-1e30, chunk=32,
+                          centre_residual=centre_residual)
     got = m.run(L, R, maps=True)
     masks = m.tile_masks()
     st = m.stats()
@@ -141,6 +143,55 @@ def test_learned_band_matches_oracle(ctx):
                       census_win=(9, 7), tol=1, nthreads=8)
     for a, c in zip(got, ref):
         assert np.array_equal(a, c)
+
+
+def _coarse_fill(dc):
+    """Holes of a quarter-resolution map filled with the smaller of the nearest valid
+    values within 64 pixels to the left and right along the row (0 when none)."""
+    out = dc.astype(np.int64).copy()
+    H4, W4 = dc.shape
+    for y in range(H4):
+        for x in range(W4):
+            if dc[y, x] >= 0:
+                continue
+            left = [dc[y, i] for i in range(x - 1, max(-1, x - 65), -1) if dc[y, i] >= 0][:1]
+            right = [dc[y, i] for i in range(x + 1, min(W4, x + 65)) if dc[y, i] >= 0][:1]
+            cand = left + right
+            out[y, x] = min(cand) if cand else 0
+    return out
+
+
+def test_learned_masks_from_coarse_map(ctx):
+    """The learned matcher's tile bins equal the bins built on the CPU: 4 x 4 box
+    means (sum + 8) >> 4 of both images, the oracle's census matcher at quarter
+    resolution (D/4), the row-wise hole fill, and every quarter pixel of the tile and
+    its one-pixel margin proposing [4 c - 8, 4 c + 8] (band 8)."""
+    W, H, D = 384, 224, 128
+    L, R = cs.synth_pair(79, W, H, 1, 8, D - 1, 8)
+    m = cs.LearnedMatcher(ctx, W, H, cs.cnn_init_params(0), num_disp=D, logit_thresh=-1e30, band=8)
+    m.run(L, R)
+    got = m.tile_masks()
+
+    def ds4(img):
+        return ((img.reshape(H // 4, 4, W // 4, 4).astype(np.int32).sum(axis=(1, 3)) + 8) >> 4).astype(np.uint8)
+
+    L4, R4 = ds4(L), ds4(R)
+    dc = orc.bm(orc.COST_CENSUS, orc.census(L4, 9, 7), orc.census(R4, 9, 7), D // 4, window=(9, 9),
+                census_win=(9, 7), tol=1)[0]
+    c = 4 * _coarse_fill(dc)
+    W4, H4 = W // 4, H // 4
+    ntx, nty = (W + 63) // 64, (H + 55) // 56
+    want = np.zeros((nty, ntx), np.uint64)
+    for ty in range(nty):
+        for tx in range(ntx):
+            blk = c[max(0, 14 * ty - 1):min(H4, 14 * ty + 15), max(0, 16 * tx - 1):min(W4, 16 * tx + 17)]
+            mask = 0
+            for v in np.unique(blk):
+                lo, hi = max(0, int(v) - 8), min(D - 1, int(v) + 8)
+                for b in range(lo >> 3, (hi >> 3) + 1):
+                    mask |= 1 << b
+            want[ty, tx] = mask
+    assert np.array_equal(got.astype(np.uint64), want)
 
 
 def test_learned_graph_replays(ctx):
@@ -163,3 +214,5 @@ def test_learned_graph_replays(ctx):
         want = fresh.run(L, R)
         ctx.set_timing(False)
         assert np.array_equal(got, want), f"replay {rep}"
+        direct = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, logit_thresh=-2.0, direct_launch=True)
+        assert np.array_equal(direct.run(L, R), want), f"direct launch {rep}"
