@@ -89,6 +89,8 @@ def learned_leg(cs, np, ctx, L, R, dl, dr, full_disp, full_ms, steps):
     if not os.path.exists(wpath):
         return {"status": "no trained parameters in module.clutec.cuda_amd/weights"}
     params = np.fromfile(wpath, "<f4")
+    if params.size != cs.CNN_PARAMS:
+        return {"status": f"parameter file holds {params.size} values, the network has {cs.CNN_PARAMS}"}
     meta = {}
     if os.path.exists(wpath + ".json"):
         with open(wpath + ".json") as f:
@@ -96,7 +98,7 @@ def learned_leg(cs, np, ctx, L, R, dl, dr, full_disp, full_ms, steps):
     thr = float(meta.get("logit_thresh", 0.0))
     band = int(meta.get("band", 8))
     centre_residual = bool(meta.get("centre_residual", 0))
-    chunk = 64
+    chunk = int(meta.get("chunk", 32))
 
     def run(t, direct=False, stages=True):
         lm = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, band=band, chunk=chunk, logit_thresh=t,

@@ -251,11 +251,11 @@ stereo_status stereo_batch_get_tuning(const stereo_driver* drv, stereo_batch_tun
  * which searches a band around a given disparity.  A 5-layer 3x3 convolutional
  * network (BF16 MFMA, FP32 accumulation) sees the pair at 1/4 resolution -- the left
  * image and the right image warped by a census match at that resolution -- and
- * proposes a centre and a confidence per quarter pixel.  Parameters: 28930 FP32
- * values, layer by layer: weights [cout][3][3][cin] then bias [cout], for
- * (cin, cout, dilation) = (2,32,1) (32,32,2) (32,32,4) (32,32,2) (32,2,1). */
+ * proposes a centre residual and a confidence per quarter pixel.  Parameters: 7554
+ * FP32 values, layer by layer: weights [cout][3][3][cin] then bias [cout], for
+ * (cin, cout, dilation) = (2,16,1) (16,16,2) (16,16,4) (16,16,2) (16,2,1). */
 
-#define STEREO_CNN_PARAMS 28930
+#define STEREO_CNN_PARAMS 7554
 
 typedef struct stereo_cnn stereo_cnn;
 
@@ -280,6 +280,11 @@ stereo_status stereo_cnn_set_input(stereo_cnn* cnn, int32_t index, const stereo_
                                    const stereo_image* coarse, stereo_stream stream);
 /* Labels of slot `index` from a full-resolution checked map (LUM_I16, 4 W4 x 4 H4). */
 stereo_status stereo_cnn_set_labels(stereo_cnn* cnn, int32_t index, const stereo_image* disp, stereo_stream stream);
+/* Confidence targets of slot `index` (LUM_U8, W4 x H4): 1 where the quarter pixel's
+ * band search agrees with the full range, 0 where it does not, other values leave
+ * the pixel without a target.  Without them the target is |residual error| <= 2. */
+stereo_status stereo_cnn_set_conf_labels(stereo_cnn* cnn, int32_t index, const stereo_image* conf,
+                                         stereo_stream stream);
 /* Forward over slots [0, n): out (device, may be NULL) receives n x H4 x W4 x 2 floats
  * (centre residual in quarter pixels, confidence logit). */
 stereo_status stereo_cnn_forward(stereo_cnn* cnn, int32_t n, float* out, stereo_stream stream);

@@ -90,9 +90,9 @@ class LearnedParams(C.Structure):
                 ("direct_launch", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
-CNN_PARAMS = 28930
+CNN_PARAMS = 7554
 # (cin, cout, dilation) of the proposal network's layers
-CNN_LAYERS = [(2, 32, 1), (32, 32, 2), (32, 32, 4), (32, 32, 2), (32, 2, 1)]
+CNN_LAYERS = [(2, 16, 1), (16, 16, 2), (16, 16, 4), (16, 16, 2), (16, 2, 1)]
 
 
 # every exported entry point: (name, restype, argtypes)
@@ -143,6 +143,7 @@ _PROTOS = [
     ("stereo_cnn_load_state", C.c_int, [_P, _P, C.c_size_t]),
     ("stereo_cnn_set_input", C.c_int, [_P, C.c_int32, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_cnn_set_labels", C.c_int, [_P, C.c_int32, C.POINTER(Image), _P]),
+    ("stereo_cnn_set_conf_labels", C.c_int, [_P, C.c_int32, C.POINTER(Image), _P]),
     ("stereo_cnn_forward", C.c_int, [_P, C.c_int32, _P, _P]),
     ("stereo_cnn_train_step", C.c_int, [_P, C.c_int32, C.c_float, _P, _P]),
     ("stereo_conv3x3_bf16", C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P, C.c_int32,
@@ -543,6 +544,11 @@ class ProposalNet:
 
     def set_labels(self, index, disp, stream=None):
         _check(lib.stereo_cnn_set_labels(self.handle, index, disp.ref, stream), self.ctx)
+
+    def set_conf_labels(self, index, conf, stream=None):
+        """Confidence targets of slot `index`: LUM_U8 quarter-resolution image, 1 where
+        the band search agrees with the full range, 0 where not, other values unset."""
+        _check(lib.stereo_cnn_set_conf_labels(self.handle, index, conf.ref, stream), self.ctx)
 
     def forward_ptr(self, n, out_ptr, stream=None):
         _check(lib.stereo_cnn_forward(self.handle, n, out_ptr, stream), self.ctx)

@@ -9,8 +9,8 @@
 // the scaled initial disparities, widened by 4).  Here the initial map is the
 // census matcher at 1/4 resolution; the network sees the quarter-resolution left
 // image and the right image warped by that map, and predicts a residual for the
-// centre and a confidence logit.  Tiles whose confidence is low search the full
-// range.
+// centre and a confidence logit.  The bands are centred on the coarse map (the
+// residual is an option); tiles whose confidence is low search the full range.
 //
 // Convolutions: 3x3, zero padding, dilation r, NHWC bf16 activations with 16 or 32
 // channels, weights [Cout][9][Cin] bf16 (tap t = 3*ky + kx at offset
@@ -251,12 +251,15 @@ __global__ __launch_bounds__(256) void reduce_parts_kernel(const float* part, in
 
 // Loss over the network output (FP32, 16 channels; channel 0 = centre residual in
 // quarter-resolution pixels, channel 1 = confidence logit) against the label
-// (quarter-resolution disparity, NaN = none) relative to the filled coarse disparity:
-//   Huber(e), e = out0 - (label - dfill);   BCE(out1, c) with c = (|e| <= tau)
-// averaged over labelled pixels (scale = 1 / count).  Writes the bf16 gradient of
-// the output and per-workgroup sums (huber, bce, count, correct).
-__global__ __launch_bounds__(256) void loss_kernel(const float* out, const float* label, const float* dfill, uint16_t* g,
-                                                   float* part, int P, float scale, float lambda, float tau)
+// (quarter-resolution disparity, NaN = none) relative to the filled coarse disparity
+// and the confidence label (0 or 1, NaN = none):
+//   Huber(e), e = out0 - (label - dfill)       where the disparity label is set
+//   BCE(out1, c), c = the confidence label     where it is set, else c = (|e| <= tau)
+// with scale = 1 / P.  Writes the bf16 gradient of the output and per-workgroup
+// sums (huber, bce, labelled pixels, confident targets).
+__global__ __launch_bounds__(256) void loss_kernel(const float* out, const float* label, const float* clabel,
+                                                   const float* dfill, uint16_t* g, float* part, int P, float scale,
+                                                   float lambda, float tau)
 {
     __shared__ float red[4][256];
     const int p = blockIdx.x * 256 + threadIdx.x;
@@ -264,18 +267,24 @@ __global__ __launch_bounds__(256) void loss_kernel(const float* out, const float
     if (p < P) {
         const float o0 = out[(size_t)p * 16], o1 = out[(size_t)p * 16 + 1];
         const float t = label[p] - dfill[p];
+        const float cl = clabel[p];
         float g0 = 0.f, g1 = 0.f;
+        float c = -1.f;  // confidence target: the given label, else |e| <= tau, else none
         if (t == t) {
             const float e = o0 - t;
             const float ae = fabsf(e);
             hub = ae <= 1.f ? 0.5f * e * e : ae - 0.5f;
-            const float c = ae <= tau ? 1.f : 0.f;
+            g0 = fminf(fmaxf(e, -1.f), 1.f) * scale;
+            c = ae <= tau ? 1.f : 0.f;
+            cnt = 1.f;
+        }
+        if (cl == cl) c = cl;
+        if (c >= 0.f) {
             const float sg = 1.f / (1.f + expf(-o1));
             bce = fmaxf(o1, 0.f) - o1 * c + log1pf(expf(-fabsf(o1)));
-            g0 = fminf(fmaxf(e, -1.f), 1.f) * scale;
             g1 = lambda * (sg - c) * scale;
-            cnt = 1.f;
             good = c;
+            if (!(t == t)) cnt = 1.f;
         }
         uint4 z = make_uint4(0, 0, 0, 0);
         z.x = (uint32_t)f2bf(g0) | ((uint32_t)f2bf(g1) << 16);
@@ -399,6 +408,17 @@ __global__ __launch_bounds__(256) void labels_kernel(View disp, int n, int W4, i
     }
     const float v = (cnt >= 8 && mx - mn <= 4) ? (float)sum / (float)(4 * cnt) : __int_as_float(0x7fc00000);
     labels[((size_t)n * H4 + Y) * W4 + X] = v;
+}
+
+// Confidence targets of frame n from a quarter-resolution LUM_U8 map: 0 -> 0, 1 -> 1,
+// anything else -> NaN (no target).
+__global__ __launch_bounds__(256) void conf_labels_kernel(View conf, int n, float* out)
+{
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (X >= conf.w || Y >= conf.h) return;
+    const uint8_t v = row_ptr<const uint8_t>(conf, Y)[X];
+    out[((size_t)n * conf.h + Y) * conf.w + X] = v <= 1 ? (float)v : __int_as_float(0x7fc00000);
 }
 
 // Per-tile disparity sets, one wave per tile.  Tile (tx, ty) covers quarter pixels
@@ -534,6 +554,9 @@ hipError_t launch_cnn_conv(int cin, int cout, int epi, const CnnConvArgs& a, hip
         hipLaunchKernelGGL((cnn::conv3x3_kernel<CI, CO, E>), dim3(grid), dim3(256), lds, s, a);       \
         return hipGetLastError();                                                                       \
     }
+    CNN_CONV(16, 16, CNN_EPI_RELU_BF16)
+    CNN_CONV(16, 16, CNN_EPI_F32)
+    CNN_CONV(16, 16, CNN_EPI_MASK_BF16)
     CNN_CONV(16, 32, CNN_EPI_RELU_BF16)
     CNN_CONV(32, 32, CNN_EPI_RELU_BF16)
     CNN_CONV(32, 16, CNN_EPI_F32)
@@ -558,7 +581,8 @@ int cnn_dw_parts(const CnnDwArgs& a)
 hipError_t launch_cnn_dw(int cin, int cout, const CnnDwArgs& a, float* out, hipStream_t s)
 {
     const int grid = cnn_dw_parts(a);
-    if (cin == 16 && cout == 32) hipLaunchKernelGGL((cnn::conv3x3_dw_kernel<16, 32>), dim3(grid), dim3(256), 0, s, a);
+    if (cin == 16 && cout == 16) hipLaunchKernelGGL((cnn::conv3x3_dw_kernel<16, 16>), dim3(grid), dim3(256), 0, s, a);
+    else if (cin == 16 && cout == 32) hipLaunchKernelGGL((cnn::conv3x3_dw_kernel<16, 32>), dim3(grid), dim3(256), 0, s, a);
     else if (cin == 32 && cout == 32) hipLaunchKernelGGL((cnn::conv3x3_dw_kernel<32, 32>), dim3(grid), dim3(256), 0, s, a);
     else if (cin == 32 && cout == 16) hipLaunchKernelGGL((cnn::conv3x3_dw_kernel<32, 16>), dim3(grid), dim3(256), 0, s, a);
     else return hipErrorInvalidValue;
@@ -569,11 +593,18 @@ hipError_t launch_cnn_dw(int cin, int cout, const CnnDwArgs& a, float* out, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_cnn_loss(const float* out, const float* label, const float* dfill, uint16_t* g, float* part, int P,
-                           float scale, float lambda, float tau, hipStream_t s)
+hipError_t launch_cnn_loss(const float* out, const float* label, const float* clabel, const float* dfill, uint16_t* g,
+                           float* part, int P, float scale, float lambda, float tau, hipStream_t s)
 {
-    hipLaunchKernelGGL(cnn::loss_kernel, dim3((P + 255) / 256), dim3(256), 0, s, out, label, dfill, g, part, P, scale,
-                       lambda, tau);
+    hipLaunchKernelGGL(cnn::loss_kernel, dim3((P + 255) / 256), dim3(256), 0, s, out, label, clabel, dfill, g, part, P,
+                       scale, lambda, tau);
+    return hipGetLastError();
+}
+
+hipError_t launch_cnn_conf_labels(const View& conf, int n, float* out, hipStream_t s)
+{
+    dim3 grid((conf.w + 63) / 64, (conf.h + 3) / 4);
+    hipLaunchKernelGGL(cnn::conf_labels_kernel, grid, dim3(256), 0, s, conf, n, out);
     return hipGetLastError();
 }
 

@@ -191,9 +191,11 @@ struct CnnBandArgs {
 hipError_t launch_cnn_conv(int cin, int cout, int epi, const CnnConvArgs& a, hipStream_t s);
 int cnn_dw_parts(const CnnDwArgs& a);
 hipError_t launch_cnn_dw(int cin, int cout, const CnnDwArgs& a, float* out, hipStream_t s);
-hipError_t launch_cnn_loss(const float* out, const float* label, const float* dfill, uint16_t* g, float* part, int P,
-                           float scale, float lambda, float tau, hipStream_t s);
+// clabel: per-pixel confidence target (0 or 1, NaN: from |e| <= tau where label is set)
+hipError_t launch_cnn_loss(const float* out, const float* label, const float* clabel, const float* dfill, uint16_t* g,
+                           float* part, int P, float scale, float lambda, float tau, hipStream_t s);
 hipError_t launch_cnn_labels(const View& disp, int n, int W4, int H4, float* labels, hipStream_t s);
+hipError_t launch_cnn_conf_labels(const View& conf, int n, float* out, hipStream_t s);
 hipError_t launch_cnn_adam(float* param, const float* grad, float* m, float* v, int n, int cin, int cout, float lr,
                            float b1, float b2, float eps, float c1, float c2, uint16_t* wf, uint16_t* wb, hipStream_t s);
 hipError_t launch_cnn_input(const View& L4, const View& R4, const View& dc, int n, uint16_t* x, float* dfill,

@@ -15,10 +15,10 @@ struct LayerDef {
     int cin_real, cout_real; // channels of the public parameter layout
 };
 
-// (2,32,1) (32,32,2) (32,32,4) (32,32,2) (32,2,1); the 2-channel ends are padded
+// (2,16,1) (16,16,2) (16,16,4) (16,16,2) (16,2,1); the 2-channel ends are padded
 // to 16 channels with zero weights, so every convolution runs on whole MFMA blocks.
-constexpr LayerDef kLayers[5] = {{16, 32, 1, 2, 32}, {32, 32, 2, 32, 32}, {32, 32, 4, 32, 32}, {32, 32, 2, 32, 32},
-                                 {32, 16, 1, 32, 2}};
+constexpr LayerDef kLayers[5] = {{16, 16, 1, 2, 16}, {16, 16, 2, 16, 16}, {16, 16, 4, 16, 16}, {16, 16, 2, 16, 16},
+                                 {16, 16, 1, 16, 2}};
 constexpr int kNumLayers = 5;
 
 size_t layer_w(int l) { return (size_t)kLayers[l].cout * 9 * kLayers[l].cin; }
@@ -51,6 +51,7 @@ struct stereo_cnn {
     uint16_t* g[2] = {};             // gradient ping-pong [P][32] bf16
     float* dfill = nullptr;
     float* labels = nullptr;
+    float* clabels = nullptr;  // confidence targets (0, 1 or NaN)
     float* dwpart = nullptr;
     int dwparts = 0;
     float* lossp = nullptr;
@@ -64,7 +65,7 @@ namespace {
 void cnn_release(stereo_cnn* c)
 {
     for (void* p : {(void*)c->param, (void*)c->grad, (void*)c->m, (void*)c->v, (void*)c->wf, (void*)c->wb, (void*)c->out,
-                    (void*)c->dfill, (void*)c->labels, (void*)c->dwpart, (void*)c->lossp, (void*)c->g[0], (void*)c->g[1]})
+                    (void*)c->dfill, (void*)c->labels, (void*)c->clabels, (void*)c->dwpart, (void*)c->lossp, (void*)c->g[0], (void*)c->g[1]})
         if (p) (void)hipFree(p);
     for (uint16_t* a : c->act)
         if (a) (void)hipFree(a);
@@ -179,7 +180,8 @@ stereo_status stereo_cnn_create(stereo_context* ctx, stereo_cnn** out, int32_t W
         (e = hipMalloc((void**)&c->m, np * 4)) || (e = hipMalloc((void**)&c->v, np * 4)) ||
         (e = hipMalloc((void**)&c->wf, np * 2)) || (e = hipMalloc((void**)&c->wb, np * 2)) ||
         (e = hipMalloc((void**)&c->out, P * 16 * 4)) || (e = hipMalloc((void**)&c->dfill, P * 4)) ||
-        (e = hipMalloc((void**)&c->labels, P * 4)) || (e = hipMalloc((void**)&c->g[0], P * 32 * 2)) ||
+        (e = hipMalloc((void**)&c->labels, P * 4)) || (e = hipMalloc((void**)&c->clabels, P * 4)) ||
+        (e = hipMalloc((void**)&c->g[0], P * 32 * 2)) ||
         (e = hipMalloc((void**)&c->g[1], P * 32 * 2)) || (e = hipMalloc((void**)&c->lossp, ((P + 255) / 256) * 16)))
         return bail(e);
     if ((e = hipMalloc((void**)&c->act[0], P * 16 * 2))) return bail(e);
@@ -192,7 +194,8 @@ stereo_status stereo_cnn_create(stereo_context* ctx, stereo_cnn** out, int32_t W
     c->dwparts = stereo::cnn_dw_parts(da);
     if ((e = hipMalloc((void**)&c->dwpart, (size_t)c->dwparts * (32 * 9 * 32 + 32) * 4))) return bail(e);
     if ((e = hipMemset(c->param, 0, np * 4)) || (e = hipMemset(c->grad, 0, np * 4)) || (e = hipMemset(c->m, 0, np * 4)) ||
-        (e = hipMemset(c->v, 0, np * 4)) || (e = hipMemset(c->labels, 0xFF, P * 4)) || (e = hipMemset(c->dfill, 0, P * 4)) ||
+        (e = hipMemset(c->v, 0, np * 4)) || (e = hipMemset(c->labels, 0xFF, P * 4)) ||
+        (e = hipMemset(c->clabels, 0xFF, P * 4)) || (e = hipMemset(c->dfill, 0, P * 4)) ||
         (e = hipMemset(c->act[0], 0, P * 16 * 2)))
         return bail(e);
     st = refresh_weights(c, 0.f, nullptr);
@@ -320,6 +323,17 @@ stereo_status stereo_cnn_set_labels(stereo_cnn* c, int32_t index, const stereo_i
     return STEREO_OK;
 }
 
+stereo_status stereo_cnn_set_conf_labels(stereo_cnn* c, int32_t index, const stereo_image* conf, stereo_stream stream)
+{
+    if (!c || !conf || index < 0 || index >= c->maxn) return STEREO_ERROR_INVALID_ARGUMENT;
+    stereo_status st;
+    if ((st = check_image(c->ctx, conf, c->W4, c->H4, STEREO_PIXEL_LUM_U8, "confidence labels"))) return st;
+    if ((st = set_device(c->ctx))) return st;
+    HIP_TRY(c->ctx, stereo::launch_cnn_conf_labels(view_of(conf), index, c->clabels, (hipStream_t)stream),
+            "confidence labels launch");
+    return STEREO_OK;
+}
+
 stereo_status stereo_cnn_forward(stereo_cnn* c, int32_t n, float* out, stereo_stream stream)
 {
     if (!c) return STEREO_ERROR_INVALID_ARGUMENT;
@@ -347,7 +361,7 @@ stereo_status stereo_cnn_train_step(stereo_cnn* c, int32_t n, float lr, float* l
     const int P = n * c->W4 * c->H4;
     const int nblk = (P + 255) / 256;
     HIP_TRY(c->ctx,
-            stereo::launch_cnn_loss(c->out, c->labels, c->dfill, c->g[0], c->lossp, P, 1.0f / (float)P, kLambda, kTau, s),
+            stereo::launch_cnn_loss(c->out, c->labels, c->clabels, c->dfill, c->g[0], c->lossp, P, 1.0f / (float)P, kLambda, kTau, s),
             "loss launch");
     // backward, last layer first; g[cur] holds the gradient of the current layer's output
     int cur = 0;
@@ -432,7 +446,7 @@ stereo_status stereo_conv3x3_bwd_bf16(stereo_context* ctx, const uint16_t* x, in
 {
     if (!ctx || !x || !dy || !dwdb || N < 1 || H < 1 || W < 1 || dil < 1 || (dx && !wt))
         return STEREO_ERROR_INVALID_ARGUMENT;
-    if ((cin != 16 && cin != 32) || (cout != 16 && cout != 32) || (cin == 16 && cout == 16))
+    if ((cin != 16 && cin != 32) || (cout != 16 && cout != 32))
         return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "channels %d -> %d not supported", cin, cout);
     if (dil > stereo::CNN_MAX_DIL)
         return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "dilation %d above %d", dil, stereo::CNN_MAX_DIL);

@@ -9,8 +9,9 @@ learned.cpp):
   z_l = conv3x3(a_l, bf16(w_l), dil_l) + b_l     FP64 accumulation
   a_{l+1} = bf16(relu(z_l))                     l = 0..3
   out = z_4                                     (centre residual, confidence logit)
-  loss = (1/P) sum over labelled pixels of Huber(e) + lambda * BCE(out1, |e| <= tau),
-         e = out0 - (label - dfill)
+  loss = (1/P) [sum over pixels with a disparity label of Huber(e)
+                + lambda * sum over pixels with a confidence target of BCE(out1, c)],
+         e = out0 - (label - dfill), c = the confidence label where given, else |e| <= tau
   backward: g4 = bf16(dloss/dout); dW_l, db_l from (a_l, g_l);
             g_{l-1} = bf16(conv3x3_bwd_data(g_l, bf16(w_l)) * (a_l > 0))
 The convolution and its gradients are the C oracle's (cnn_oracle.c).
@@ -19,7 +20,7 @@ import numpy as np
 
 import orc
 
-LAYERS = [(2, 32, 1), (32, 32, 2), (32, 32, 4), (32, 32, 2), (32, 2, 1)]
+LAYERS = [(2, 16, 1), (16, 16, 2), (16, 16, 4), (16, 16, 2), (16, 2, 1)]
 NPARAMS = sum(co * 9 * ci + co for ci, co, _ in LAYERS)
 LAMBDA = 0.5
 TAU = 2.0
@@ -60,8 +61,9 @@ def forward(params, x0, rounding=True):
     return out, acts
 
 
-def loss_grad(out, label, dfill):
-    """Loss sums and dloss/dout (FP32) with the GPU's normalisation 1/P."""
+def loss_grad(out, label, dfill, clabel=None):
+    """Loss sums and dloss/dout (FP32) with the GPU's normalisation 1/P.  clabel:
+    optional confidence targets (0 or 1, NaN = none) that replace |e| <= tau."""
     P = out.shape[0] * out.shape[1] * out.shape[2]
     o0 = out[..., 0].astype(np.float64)
     o1 = out[..., 1].astype(np.float64)
@@ -69,15 +71,19 @@ def loss_grad(out, label, dfill):
     lab = np.isfinite(t)
     e = np.where(lab, o0 - np.where(lab, t, 0.0), 0.0)
     ae = np.abs(e)
-    c = (ae <= TAU).astype(np.float64)
+    c = np.where(lab, (ae <= TAU).astype(np.float64), -1.0)
+    if clabel is not None:
+        cl = np.asarray(clabel, np.float64)
+        c = np.where(np.isfinite(cl), cl, c)
+    has_c = c >= 0
     sg = 1.0 / (1.0 + np.exp(-o1))
     g = np.zeros(out.shape, np.float64)
     g[..., 0] = np.where(lab, np.clip(e, -1, 1) / P, 0.0)
-    g[..., 1] = np.where(lab, LAMBDA * (sg - c) / P, 0.0)
+    g[..., 1] = np.where(has_c, LAMBDA * (sg - c) / P, 0.0)
     hub = np.where(ae <= 1, 0.5 * e * e, ae - 0.5)
     bce = np.maximum(o1, 0) - o1 * c + np.log1p(np.exp(-np.abs(o1)))
-    n = lab.sum()
-    return g.astype(np.float32), {"huber": float(hub[lab].sum() / max(n, 1)), "bce": float(bce[lab].sum() / max(n, 1)),
+    n = (lab | has_c).sum()
+    return g.astype(np.float32), {"huber": float(hub[lab].sum() / max(n, 1)), "bce": float(bce[has_c].sum() / max(n, 1)),
                                   "count": int(n)}
 
 

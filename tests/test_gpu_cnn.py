@@ -151,29 +151,40 @@ def _frame(ctx, seed, W, H, D):
     return imgs, x0.astype(np.float32), dfill.astype(np.float32), _labels(disp, W // 4, H // 4)
 
 
-def test_network_gradients_match_oracle(ctx):
+@pytest.mark.parametrize("conf_labels", [False, True])
+def test_network_gradients_match_oracle(ctx, conf_labels):
+    """Gradients of every layer against the oracle's network, with the confidence
+    target from |e| <= tau or from given confidence labels (0, 1 and unset)."""
     W, H, D = 256, 128, 64
     net = cs.ProposalNet(ctx, W // 4, H // 4, max_frames=2)
     p = cs.cnn_init_params(1)
     p[-2:] = [0.3, -0.2]  # non-zero output biases
     net.set_params(p)
-    xs, dfs, labs = [], [], []
+    xs, dfs, labs, cls = [], [], [], []
+    rng = np.random.default_rng(5)
     for i, seed in enumerate((41, 42)):
         (dl, dr, dc, disp), x0, dfill, lab = _frame(ctx, seed, W, H, D)
         net.set_input(i, dl, dr, dc)
         net.set_labels(i, disp)
+        cl = np.full(lab.shape, np.nan, np.float32)
+        if conf_labels:
+            conf = rng.integers(0, 3, lab.shape).astype(np.uint8)  # 2 = no target
+            net.set_conf_labels(i, cs.DeviceImage.from_array(ctx, conf, cs.PIXEL_LUM_U8))
+            cl = np.where(conf <= 1, conf.astype(np.float32), np.float32(np.nan))
         xs.append(x0)
         dfs.append(dfill)
         labs.append(lab)
+        cls.append(cl)
     loss = net.train_step(2, 0.0)
     g = net.grads()
     x0, dfill, lab = np.stack(xs), np.stack(dfs), np.stack(labs)
     out, acts = cnn_ref.forward(p, x0)
-    gout, ref_loss = cnn_ref.loss_grad(out, lab, dfill)
+    gout, ref_loss = cnn_ref.loss_grad(out, lab, dfill, np.stack(cls))
     ref = cnn_ref.backward(p, acts, gout)
     print(f"loss gpu {loss} oracle {ref_loss}", flush=True)
     assert int(loss[2]) == ref_loss["count"]
     assert abs(loss[0] - ref_loss["huber"]) <= 1e-3 * max(1.0, ref_loss["huber"])
+    assert abs(loss[1] - ref_loss["bce"]) <= 1e-3 * max(1.0, ref_loss["bce"])
     k = 0
     for l, (ci, co, _) in enumerate(cs.CNN_LAYERS):
         n = co * 9 * ci + co
