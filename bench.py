@@ -291,6 +291,16 @@ def main():
     peak = min([PEAK_INT_TOPS] + [v for v in measured.values() if v > 0])
 
     disp = out.download()
+    # the same frame from host buffers: upload both images, process, download the map
+    # (PCIe included; reported beside the value, never as it)
+    host_ms = []
+    for _ in range(max(3, min(args.steps, 10))):
+        t0 = time.perf_counter()
+        dl.upload(L)
+        dr.upload(R)
+        m.process(out, None, None, dl, dr)
+        out.download()
+        host_ms.append((time.perf_counter() - t0) * 1e3)
     result = {
         "metric": "M disparity-evals/s at 1080p D=256; learned-proposal vs full-range speedup",
         "value": round(value, 1),
@@ -324,6 +334,7 @@ def main():
                       "event_total": round(event_total, 4)},
         "pipeline_frac_of_valu_peak": round(value * 1e6 * OPS_PER_EVAL / (peak * 1e12), 4),
         "valid_fraction": round(float((disp >= 0).mean()), 4),
+        "host_buffers_ms_per_frame": round(statistics.median(host_ms), 4),
     }
 
     if not args.no_learned:
