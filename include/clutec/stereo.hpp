@@ -272,6 +272,15 @@ public:
         EnsureOK(stereo_matcher_process(m_drv, disp.Raw(), lr.Raw(), rl.Raw(), L.Raw(), R.Raw(), nullptr), m_ctx,
                  "Matcher::Process");
     }
+    // With an initial disparity map (LUM_I16 or RGBA_UInt16 ids, any size up to the
+    // images'): the band search of MatcherUseDisparity (BlockMatcherNCC.cu:894-1255).
+    // The driver must be configured with iUseDispInput.
+    void Process(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R, const CDeviceImage& dispInit)
+    {
+        EnsureOK(stereo_matcher_process_init(m_drv, disp.Raw(), nullptr, nullptr, L.Raw(), R.Raw(), dispInit.Raw(),
+                                             nullptr),
+                 m_ctx, "Matcher::Process");
+    }
 
 protected:
     void Configure(const CDevice& dev, const SImageFormat& fmt, const stereo_matcher_params& p)
@@ -282,9 +291,10 @@ protected:
         Reset(dev.Context(), d);
     }
     static stereo_matcher_params Params(int32_t algo, int32_t ww, int32_t wh, int32_t D, int32_t tol, int32_t cw = 1,
-                                        int32_t ch = 1)
+                                        int32_t ch = 1, int32_t use_disp_input = 0)
     {
         stereo_matcher_params p{};
+        p.use_disp_input = use_disp_input;
         p.algorithm = algo;
         p.window_w = ww;
         p.window_h = wh;
@@ -303,11 +313,13 @@ struct SParameter
     _SDisparityConfig xDispConfig;
     int32_t iPatchSize = 9;
     int32_t iLrTolerance = 1;
-    void Set(const _SDisparityConfig& cfg, int32_t patch = 9, int32_t tol = 1)
+    int32_t iUseDispInput = 0;  // 1: Process may take an initial disparity map
+    void Set(const _SDisparityConfig& cfg, int32_t patch = 9, int32_t tol = 1, int32_t useDispInput = 0)
     {
         xDispConfig = cfg;
         iPatchSize = patch;
         iLrTolerance = tol;
+        iUseDispInput = useDispInput;
     }
 };
 class CDriver : public CMatcherDriver
@@ -316,7 +328,8 @@ public:
     CDriver() : CMatcherDriver("Block Matcher SAD 1") {}
     void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
     {
-        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_SAD, p.iPatchSize, p.iPatchSize, p.xDispConfig.Range(), p.iLrTolerance));
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_SAD, p.iPatchSize, p.iPatchSize, p.xDispConfig.Range(),
+                                                   p.iLrTolerance, 1, 1, p.iUseDispInput));
     }
 };
 }  // namespace BlockMatcherSAD1
@@ -327,13 +340,16 @@ struct SParameter
 {
     _SDisparityConfig xDispConfig;
     int32_t iCensusW = 9, iCensusH = 7, iPatchSize = 9, iLrTolerance = 1;
-    void Set(const _SDisparityConfig& cfg, int32_t cw = 9, int32_t ch = 7, int32_t patch = 9, int32_t tol = 1)
+    int32_t iUseDispInput = 0;  // 1: Process may take an initial disparity map
+    void Set(const _SDisparityConfig& cfg, int32_t cw = 9, int32_t ch = 7, int32_t patch = 9, int32_t tol = 1,
+             int32_t useDispInput = 0)
     {
         xDispConfig = cfg;
         iCensusW = cw;
         iCensusH = ch;
         iPatchSize = patch;
         iLrTolerance = tol;
+        iUseDispInput = useDispInput;
     }
 };
 class CDriver : public CMatcherDriver
@@ -344,7 +360,7 @@ public:
     {
         CMatcherDriver::Configure(dev, fmt,
                                   Params(STEREO_ALGO_CENSUS, p.iPatchSize, p.iPatchSize, p.xDispConfig.Range(), p.iLrTolerance,
-                                         p.iCensusW, p.iCensusH));
+                                         p.iCensusW, p.iCensusH, p.iUseDispInput));
     }
 };
 }  // namespace BlockMatcherCensus

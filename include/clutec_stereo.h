@@ -147,7 +147,9 @@ typedef struct stereo_matcher_params {
     int32_t census_h;
     int32_t num_disparities; /* D: candidates 0..D-1, multiple of 8, <= 256 */
     int32_t lr_tolerance;    /* |dR(x-dL) - dL| <= tol keeps the pixel */
-    int32_t reserved[5];     /* must be 0 */
+    int32_t use_disp_input;  /* 1: also configure stereo_matcher_process_init (SParameter::iUseDispInput,
+                                BlockMatcherNCC.cu:1330); SAD and CENSUS only */
+    int32_t reserved[4];     /* must be 0 */
 } stereo_matcher_params;
 
 /* Checks a configuration without touching a device.  Returns STEREO_OK or
@@ -174,6 +176,17 @@ stereo_status stereo_matcher_configure(stereo_context* ctx, stereo_driver** drv,
 stereo_status stereo_matcher_process(stereo_driver* drv, stereo_image* disp, stereo_image* disp_lr,
                                      stereo_image* disp_rl, const stereo_image* left, const stereo_image* right,
                                      stereo_stream stream);
+
+/* MatcherUseDisparity (BlockMatcherNCC.cu:894-1255), the Process overload with an initial
+ * disparity map (BlockMatcherNCC.h:137-143): disp_init is LUM_I16 (d, -1 invalid) or
+ * RGBA_U16 (EDisparityId in channel 0) of any size up to the images'; scale = width /
+ * its width.  Each 64x56 tile searches the 8-disparity bins covering
+ * [min - 4, max + 4] of scale x d over its pixels, d read at (x / scale, y / scale);
+ * a tile with no valid value searches the full range.  Outputs as
+ * stereo_matcher_process.  Requires use_disp_input at configure. */
+stereo_status stereo_matcher_process_init(stereo_driver* drv, stereo_image* disp, stereo_image* disp_lr,
+                                          stereo_image* disp_rl, const stereo_image* left, const stereo_image* right,
+                                          const stereo_image* disp_init, stereo_stream stream);
 
 /* Winning NCC score of every left pixel for the last process call on the same
  * images (FP32, NaN where the left map has no match).  Scores carry the exact
@@ -340,6 +353,19 @@ stereo_status stereo_learned_tile_ranges(stereo_driver* drv, int32_t* lohi, int3
 /* The searched disparities of each tile in the last process call as a mask of
  * 8-disparity bins (bit b: d in [8b, 8b + 8)); n >= tiles.  Waits for the call. */
 stereo_status stereo_learned_tile_masks(stereo_driver* drv, uint32_t* masks, int32_t n);
+
+/* ---- disparity mip maps (CreateDispMipMaps, CreateDispMipMaps.h:40-70) ------- */
+
+/* CreateDispMipMaps::CDriver::Run (CreateDispMipMaps.cu:250-330): levels[0..nlevels) are
+ * LUM_F32 images, levels[0] the size of disp, level i+1 ((w-1)/2) x ((h-1)/2) of level i.
+ * Level 0 is the map as float: for RGBA_U16 the channel-0 id when First <= id <= Last,
+ * else Unknown (0); for LUM_I16 d >= 0, else -1.  Each next level is a [1 2 1] filter
+ * along x and then along y that skips the unknown marker and divides by the weights it
+ * kept, the vertical pass halving the value (CreateDispMipMaps.cu:97-218).  Levels that
+ * do not fit are STEREO_ERROR_INVALID_CONFIG, as the reference's "Mip-map level count
+ * larger than the maximal possible level" (:262-265). */
+stereo_status stereo_disp_mipmaps(stereo_context* ctx, const stereo_image* disp, stereo_image* levels, int32_t nlevels,
+                                  stereo_stream stream);
 
 /* ---- device calibration ------------------------------------------------------ */
 

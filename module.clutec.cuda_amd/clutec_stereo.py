@@ -69,7 +69,7 @@ class CensusParams(C.Structure):
 class MatcherParams(C.Structure):
     _fields_ = [("algorithm", C.c_int32), ("window_w", C.c_int32), ("window_h", C.c_int32),
                 ("census_w", C.c_int32), ("census_h", C.c_int32), ("num_disparities", C.c_int32),
-                ("lr_tolerance", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("lr_tolerance", C.c_int32), ("use_disp_input", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class CrossCheckParams(C.Structure):
@@ -122,6 +122,9 @@ _PROTOS = [
     ("stereo_matcher_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(MatcherParams)]),
     ("stereo_matcher_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image),
                                           C.POINTER(Image), C.POINTER(Image), _P]),
+    ("stereo_matcher_process_init", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image),
+                                               C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
+    ("stereo_disp_mipmaps", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.c_int32, _P]),
     ("stereo_matcher_scores", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_stage_times", C.c_int, [_P, C.POINTER(C.c_double), C.c_int32]),
     ("stereo_crosscheck_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(CrossCheckParams)]),
@@ -212,13 +215,14 @@ def validate_matcher(W, H, params):
     return st, buf.value.decode()
 
 
-def matcher_params(algorithm, window=(9, 9), census=(9, 7), num_disp=64, tol=1):
+def matcher_params(algorithm, window=(9, 9), census=(9, 7), num_disp=64, tol=1, use_disp_input=False):
     p = MatcherParams()
     p.algorithm = algorithm
     p.window_w, p.window_h = window
     p.census_w, p.census_h = census
     p.num_disparities = num_disp
     p.lr_tolerance = tol
+    p.use_disp_input = 1 if use_disp_input else 0
     return p
 
 
@@ -376,11 +380,12 @@ class Census(_Driver):
 
 
 class Matcher(_Driver):
-    def __init__(self, ctx, W, H, algorithm=ALGO_CENSUS, window=(9, 9), census=(9, 7), num_disp=64, tol=1):
+    def __init__(self, ctx, W, H, algorithm=ALGO_CENSUS, window=(9, 9), census=(9, 7), num_disp=64, tol=1,
+                 use_disp_input=False):
         super().__init__(ctx)
         self.W, self.H = W, H
         fmt = ImageFormat(W, H, PIXEL_LUM_U8)
-        self.params = matcher_params(algorithm, window, census, num_disp, tol)
+        self.params = matcher_params(algorithm, window, census, num_disp, tol, use_disp_input)
         _check(lib.stereo_matcher_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(self.params)), ctx)
 
     def process(self, disp, disp_lr, disp_rl, left, right, stream=None):
@@ -391,6 +396,27 @@ class Matcher(_Driver):
         v = (C.c_double * 3)()
         _check(lib.stereo_matcher_stage_times(self.handle, v, 3), self.ctx)
         return list(v)
+
+    def process_init(self, disp, disp_lr, disp_rl, left, right, init, stream=None):
+        """Band search around the initial disparity map `init` (stereo_matcher_process_init)."""
+        _check(lib.stereo_matcher_process_init(self.handle, disp.ref if disp else None,
+                                               disp_lr.ref if disp_lr else None, disp_rl.ref if disp_rl else None,
+                                               left.ref, right.ref, init.ref, stream), self.ctx)
+
+    def run_init(self, L, R, init, maps=False):
+        """numpy in, numpy out: the band search around `init` (int16 map, -1 invalid, or
+        (h, w, 4) uint16 EDisparityId pixels)."""
+        dl = DeviceImage.from_array(self.ctx, L, PIXEL_LUM_U8)
+        dr = DeviceImage.from_array(self.ctx, R, PIXEL_LUM_U8)
+        init = np.asarray(init)
+        di = DeviceImage.from_array(self.ctx, init, PIXEL_RGBA_U16 if init.ndim == 3 else PIXEL_LUM_I16)
+        out = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16)
+        lr = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16) if maps else None
+        rl = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16) if maps else None
+        self.process_init(out, lr, rl, dl, dr, di)
+        if maps:
+            return out.download(), lr.download(), rl.download()
+        return out.download()
 
     def run(self, L, R, maps=False, dispid=False, scores=False):
         """numpy in, numpy out: the checked map (and the two one-sided maps, and for
@@ -409,6 +435,21 @@ class Matcher(_Driver):
             _check(lib.stereo_matcher_scores(self.handle, sc.ref, dl.ref, dr.ref, None), self.ctx)
             res.append(sc.download())
         return res[0] if len(res) == 1 else tuple(res)
+
+
+def disp_mipmaps(ctx, disp, nlevels, stream=None):
+    """CreateDispMipMaps: LUM_F32 levels of a LUM_I16 or RGBA_U16 disparity image
+    (stereo_disp_mipmaps); level i+1 is ((w-1)/2) x ((h-1)/2) of level i."""
+    w, h = disp.img.format.width, disp.img.format.height
+    levels = []
+    for i in range(nlevels):
+        if i:
+            w, h = (w - 1) // 2, (h - 1) // 2
+        # a level that does not fit is left to the library to reject
+        levels.append(DeviceImage(ctx, max(1, w), max(1, h), PIXEL_LUM_F32))
+    arr = (Image * nlevels)(*[lv.img for lv in levels])
+    _check(lib.stereo_disp_mipmaps(ctx.handle, disp.ref, arr, nlevels, stream), ctx)
+    return levels
 
 
 class CrossCheck(_Driver):

@@ -236,6 +236,77 @@ stereo_status stereo_census_process(stereo_driver* d, stereo_image* trg, const s
 
 // ---- matchers -----------------------------------------------------------------------
 
+namespace {
+
+// Work-list buffers of the band search around an initial map: one run of bins per
+// tile, cut into items of kBandChunk disparities.
+constexpr int kBandChunk = 32;
+
+struct BandExt : DriverExt {
+    stereo::BmFrame* frame_dev = nullptr;
+    stereo::BmFrame frame{};  // last frame table written to the device
+    stereo::BmTile* tiles = nullptr;
+    stereo::BmItem* items = nullptr;
+    int* nitems = nullptr;
+    int* counter = nullptr;
+    int* tile_evals = nullptr;
+    uint32_t* tile_mask = nullptr;
+    uint32_t* lkey = nullptr;
+    uint32_t* rpart = nullptr;
+    int ntx = 0, nty = 0, max_items = 0;
+    ~BandExt() override
+    {
+        for (void* p : {(void*)frame_dev, (void*)tiles, (void*)items, (void*)nitems, (void*)counter, (void*)tile_evals,
+                        (void*)tile_mask, (void*)lkey, (void*)rpart})
+            if (p) (void)hipFree(p);
+    }
+};
+
+stereo_status band_alloc(stereo_driver* d)
+{
+    BandExt* x = new BandExt();
+    d->ext = x;
+    const int W = d->fmt.width, H = d->fmt.height, D = d->mp.num_disparities;
+    x->ntx = (W + stereo::BM_TILE_W - 1) / stereo::BM_TILE_W;
+    x->nty = (H + stereo::BM_TILE_H - 1) / stereo::BM_TILE_H;
+    const int nt = x->ntx * x->nty;
+    x->max_items = nt * ((D + kBandChunk - 1) / kBandChunk);
+    hipError_t e;
+    if ((e = hipMalloc((void**)&x->frame_dev, sizeof(stereo::BmFrame))) ||
+        (e = hipMalloc((void**)&x->tiles, nt * sizeof(stereo::BmTile))) ||
+        (e = hipMalloc((void**)&x->items, (size_t)x->max_items * sizeof(stereo::BmItem))) ||
+        (e = hipMalloc((void**)&x->nitems, sizeof(int))) || (e = hipMalloc((void**)&x->counter, sizeof(int))) ||
+        (e = hipMalloc((void**)&x->tile_evals, nt * sizeof(int))) ||
+        (e = hipMalloc((void**)&x->tile_mask, nt * sizeof(uint32_t))) ||
+        (e = hipMalloc((void**)&x->lkey, stereo::bm_list_lkey_elems(x->max_items) * 4)) ||
+        (e = hipMalloc((void**)&x->rpart, stereo::bm_list_rpart_elems(x->max_items, kBandChunk) * 4)) ||
+        (e = hipMemset(x->counter, 0, sizeof(int))) || (e = hipDeviceSynchronize()))
+        return hip_fail(d->ctx, e, "hipMalloc");
+    stereo::BmFrame& f = x->frame;
+    std::memset(&f, 0, sizeof(f));
+    f.L.pitch = f.R.pitch = d->pad_pitch;
+    f.L.w = f.R.w = W;
+    f.L.h = f.R.h = H;
+    f.L.ptr = d->padL + (size_t)stereo::BM_PAD_LEFT * d->pad_elem;
+    f.R.ptr = d->padR + (size_t)stereo::BM_PAD_LEFT * d->pad_elem;
+    f.W = W;
+    f.H = H;
+    f.D = D;
+    f.bx = d->bx;
+    f.by = d->by;
+    f.tol = d->mp.lr_tolerance;
+    f.value_base = 0;
+    f.ntx = x->ntx;
+    f.nty = x->nty;
+    f.tile_base = 0;
+    f.row_base = 0;
+    HIP_TRY(d->ctx, hipMemcpy(x->frame_dev, &f, sizeof(f), hipMemcpyHostToDevice), "hipMemcpy");
+    return STEREO_OK;
+}
+
+}  // namespace
+
+
 stereo_status stereo_matcher_validate(const stereo_image_format* fmt, const stereo_matcher_params* p, char* msg,
                                       size_t msg_len)
 {
@@ -247,8 +318,15 @@ stereo_status stereo_matcher_validate(const stereo_image_format* fmt, const ster
     if (!format_valid(fmt)) return say(STEREO_ERROR_INVALID_FORMAT, "Invalid image format");
     if (fmt->pixel_type != STEREO_PIXEL_LUM_U8)
         return say(STEREO_ERROR_UNSUPPORTED_PIXEL_TYPE, "matchers take LUM_U8 images");
-    for (int i = 0; i < 5; ++i)
+    for (int i = 0; i < 4; ++i)
         if (p->reserved[i]) return say(STEREO_ERROR_INVALID_CONFIG, "reserved parameters must be 0");
+    if (p->use_disp_input != 0 && p->use_disp_input != 1)
+        return say(STEREO_ERROR_INVALID_CONFIG, "use_disp_input must be 0 or 1");
+    if (p->use_disp_input && p->algorithm != STEREO_ALGO_SAD && p->algorithm != STEREO_ALGO_CENSUS)
+        return say(STEREO_ERROR_INVALID_CONFIG, "the band search around an initial disparity map is built for the SAD "
+                                                "and census matchers");
+    if (p->use_disp_input && fmt->width > stereo::BM_LIST_MAX_WIDTH)
+        return say(STEREO_ERROR_INVALID_FORMAT, "the band search takes widths up to 11264");
     if (p->lr_tolerance < 0) return say(STEREO_ERROR_INVALID_CONFIG, "lr_tolerance must be >= 0");
     if (p->num_disparities < 8 || p->num_disparities > 256 || p->num_disparities % 8)
         return say(STEREO_ERROR_INVALID_CONFIG, "num_disparities must be a multiple of 8 in [8, 256]");
@@ -332,6 +410,7 @@ stereo_status stereo_matcher_configure(stereo_context* ctx, stereo_driver** out,
     // uninitialised memory is read
     if ((e = hipMemset(d->padL, 0, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMemset"));
     if ((e = hipMemset(d->padR, 0, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMemset"));
+    if (p->use_disp_input && (st = band_alloc(d))) return cleanup(st);
     *out = d;
     return STEREO_OK;
 }
@@ -452,6 +531,99 @@ stereo_status stereo_matcher_process(stereo_driver* d, stereo_image* disp, stere
     f.tol = d->mp.lr_tolerance;
     f.value_base = d->mp.algorithm == STEREO_ALGO_AW1 ? 255 * 3 * d->mp.window_w * d->mp.window_w : 0;
     HIP_TRY(ctx, stereo::launch_bm_finish(f, s), "finish launch");
+    return end(d, s);
+}
+
+stereo_status stereo_matcher_process_init(stereo_driver* d, stereo_image* disp, stereo_image* disp_lr,
+                                          stereo_image* disp_rl, const stereo_image* left, const stereo_image* right,
+                                          const stereo_image* disp_init, stereo_stream stream)
+{
+    if (!d || d->kind != stereo_driver::MATCHER || !left || !right || !disp_init) return STEREO_ERROR_INVALID_ARGUMENT;
+    stereo_context* ctx = d->ctx;
+    BandExt* x = dynamic_cast<BandExt*>(d->ext);
+    if (!x)
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "%s: configured without use_disp_input", d->name);
+    const int W = d->fmt.width, H = d->fmt.height;
+    stereo_status st;
+    if ((st = check_image(ctx, left, W, H, STEREO_PIXEL_LUM_U8, "left image"))) return st;
+    if ((st = check_image(ctx, right, W, H, STEREO_PIXEL_LUM_U8, "right image"))) return st;
+    bool dispid = false;
+    if (disp) {
+        if (disp->format.pixel_type == STEREO_PIXEL_RGBA_U16) dispid = true;
+        if ((st = check_image(ctx, disp, W, H, dispid ? STEREO_PIXEL_RGBA_U16 : STEREO_PIXEL_LUM_I16, "disparity output")))
+            return st;
+    }
+    if (disp_lr && (st = check_image(ctx, disp_lr, W, H, STEREO_PIXEL_LUM_I16, "left-to-right output"))) return st;
+    if (disp_rl && (st = check_image(ctx, disp_rl, W, H, STEREO_PIXEL_LUM_I16, "right-to-left output"))) return st;
+    const int iw = disp_init->format.width, ih = disp_init->format.height;
+    const int ipt = disp_init->format.pixel_type;
+    if (ipt != STEREO_PIXEL_LUM_I16 && ipt != STEREO_PIXEL_RGBA_U16)
+        return fail(ctx, STEREO_ERROR_UNSUPPORTED_PIXEL_TYPE, "initial disparity: LUM_I16 or RGBA_U16");
+    if (iw < 1 || ih < 1 || iw > W || ih > H)
+        return fail(ctx, STEREO_ERROR_FORMAT_MISMATCH, "initial disparity %dx%d: at most the images' %dx%d", iw, ih, W, H);
+    if ((st = check_image(ctx, disp_init, iw, ih, ipt, "initial disparity"))) return st;
+    if ((st = set_device(ctx))) return st;
+    GUARD(ctx, "stereo_matcher_process_init");
+    const hipStream_t s = (hipStream_t)stream;
+
+    // outputs live in the frame table; rewrite it (synchronously) when they change
+    stereo::BmFrame f = x->frame;
+    f.disp = (disp && !dispid) ? view_of(disp) : null_view();
+    f.dispid = (disp && dispid) ? view_of(disp) : null_view();
+    f.disp_l = disp_lr ? view_of(disp_lr) : null_view();
+    f.disp_r = disp_rl ? view_of(disp_rl) : null_view();
+    if (std::memcmp(&f, &x->frame, sizeof(f)) != 0) {
+        HIP_TRY(ctx, hipStreamSynchronize(s), "hipStreamSynchronize");
+        HIP_TRY(ctx, hipMemcpy(x->frame_dev, &f, sizeof(f), hipMemcpyHostToDevice), "hipMemcpy");
+        x->frame = f;
+    }
+    if ((st = begin(d, s))) return st;
+    const bool census = d->mp.algorithm == STEREO_ALGO_CENSUS;
+    if (census) {
+        const View vl = view_of(left), vr = view_of(right);
+        HIP_TRY(ctx, stereo::launch_census_pair(vl, f.L, &vr, &f.R, d->mp.census_w, d->mp.census_h, s), "census launch");
+    } else {
+        HIP_TRY(ctx, hipMemcpy2DAsync(f.L.ptr, f.L.pitch, left->data, left->pitch, W, H, hipMemcpyDeviceToDevice, s),
+                "hipMemcpy2DAsync");
+        HIP_TRY(ctx, hipMemcpy2DAsync(f.R.ptr, f.R.pitch, right->data, right->pitch, W, H, hipMemcpyDeviceToDevice, s),
+                "hipMemcpy2DAsync");
+    }
+    if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[0], s), "hipEventRecord");
+    stereo::InitBandArgs ib;
+    ib.init = view_of(disp_init);
+    ib.ref_format = ipt == STEREO_PIXEL_RGBA_U16 ? 1 : 0;
+    ib.scale = W / iw;
+    ib.W = W;
+    ib.H = H;
+    stereo::CnnBandArgs ba{};
+    ba.tiles = x->tiles;
+    ba.items = x->items;
+    ba.nitems = x->nitems;
+    ba.list_counter = x->counter;
+    ba.tile_evals = x->tile_evals;
+    ba.tile_mask = x->tile_mask;
+    ba.ntx = x->ntx;
+    ba.nty = x->nty;
+    ba.D = d->mp.num_disparities;
+    ba.chunk = kBandChunk;
+    HIP_TRY(ctx, stereo::launch_init_band(ib, ba, s), "band launch");
+    stereo::BmListParams lp;
+    lp.frames = x->frame_dev;
+    lp.tiles = x->tiles;
+    lp.items = x->items;
+    lp.nitems = x->nitems;
+    lp.sched = nullptr;
+    lp.counter = x->counter;
+    lp.lkey = x->lkey;
+    lp.rpart = x->rpart;
+    lp.rp_stride = stereo::bm_list_rp_stride(kBandChunk);
+    lp.nframes = 1;
+    lp.total_rows = H;
+    const int grid = x->max_items < 768 ? x->max_items : 768;
+    HIP_TRY(ctx, stereo::launch_bm_list(census ? stereo::BM_COST_CENSUS : stereo::BM_COST_SAD, d->mp.window_w, lp, grid, s),
+            "list matcher launch");
+    if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[1], s), "hipEventRecord");
+    HIP_TRY(ctx, stereo::launch_bm_list_finish(lp, W, s), "list finish launch");
     return end(d, s);
 }
 

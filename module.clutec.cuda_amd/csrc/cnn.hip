@@ -467,6 +467,59 @@ __global__ __launch_bounds__(256) void band_mask_kernel(CnnBandArgs a)
 
 }
 
+// Tile bins from a caller's initial disparity map, one wave per tile (the role of
+// MatcherUseDisparity, BlockMatcherNCC.cu:894-1255: scale = W / init width, the values
+// scaled, min and max over the block, searched from min - 4 to max + 4).  Every pixel
+// of the tile reads the initial map at (x / scale, y / scale), clamped; a valid value d
+// (>= 0, or an id in [First, Last] as id - First) proposes scale x d.  The tile keeps
+// the bins of [max(0, min - 4), min(D - 1, max + 4)], or all bins when none is valid.
+__global__ __launch_bounds__(256) void init_band_mask_kernel(InitBandArgs ib, CnnBandArgs a)
+{
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int ntiles = a.ntx * a.nty;
+    if (t >= ntiles) return;
+    const int tx = t % a.ntx, ty = t / a.ntx;
+    const int x0 = tx * BM_TILE_W, y0 = ty * BM_TILE_H;
+    const int nx = min(BM_TILE_W, ib.W - x0), ny = min(BM_TILE_H, ib.H - y0);
+    int lo = 1 << 30, hi = -1;
+    for (int k = lane; k < nx * ny; k += 64) {
+        const int y = y0 + k / nx, x = x0 + k % nx;
+        const int X = min(x / ib.scale, ib.init.w - 1), Y = min(y / ib.scale, ib.init.h - 1);
+        int d;
+        if (ib.ref_format) {
+            const int id = row_ptr<const ushort4>(ib.init, Y)[X].x;
+            d = (id >= 1 && id <= 0xFFEF) ? id - 1 : -1;
+        } else {
+            d = row_ptr<const int16_t>(ib.init, Y)[X];
+        }
+        if (d >= 0) {
+            lo = min(lo, ib.scale * d);
+            hi = max(hi, ib.scale * d);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    const int nbins = a.D >> 3;
+    const uint32_t all = nbins >= 32 ? 0xFFFFFFFFu : ((1u << nbins) - 1u);
+    uint32_t mask = all;
+    bool fallback = true;
+    if (hi >= 0) {
+        const int blo = max(0, lo - 4), bhi = min(a.D - 1, hi + 4);
+        if (blo <= bhi) {
+            const int b0 = blo >> 3, b1 = bhi >> 3;
+            mask = (uint32_t)((((2ull << b1) - 1ull) & ~((1ull << b0) - 1ull)) & all);
+            fallback = false;
+        }
+    }
+    if (lane == 0) {
+        a.tile_mask[t] = mask;
+        a.tile_evals[t] = fallback ? (1 << 16) : 0;
+    }
+}
+
 // The work list of the tiles' masks (one workgroup): every run of set bins is cut
 // into items of at most `chunk` disparities; a block-wide scan places each tile's
 // items and the tile table records them.
@@ -628,6 +681,18 @@ hipError_t launch_cnn_input(const View& L4, const View& R4, const View& dc, int 
 {
     if (L4.w > cnn::CNN_MAX_W4) return hipErrorInvalidValue;
     hipLaunchKernelGGL(cnn::cnn_input_kernel, dim3(L4.h), dim3(256), 0, s, L4, R4, dc, n, x, dfill);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_band(const InitBandArgs& ib, const CnnBandArgs& a, hipStream_t s)
+{
+    if (a.D > 256 || a.D % 8 || a.chunk < 8 || a.chunk % 8 || ib.scale < 1 || !a.list_counter) return hipErrorInvalidValue;
+    const int ntiles = a.ntx * a.nty;
+    if (ntiles < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cnn::init_band_mask_kernel, dim3((ntiles + 3) / 4), dim3(256), 0, s, ib, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(cnn::band_items_kernel, dim3(1), dim3(1024), 0, s, a);
     return hipGetLastError();
 }
 

@@ -201,6 +201,17 @@ hipError_t launch_cnn_adam(float* param, const float* grad, float* m, float* v, 
 hipError_t launch_cnn_input(const View& L4, const View& R4, const View& dc, int n, uint16_t* x, float* dfill,
                             hipStream_t s);
 hipError_t launch_band_items(const CnnBandArgs& a, hipStream_t s);
+// Band search around a caller's initial disparity map (the role of MatcherUseDisparity):
+// per 64 x BM_TILE_H tile, the 8-disparity bins that cover [min - 4, max + 4] of
+// scale x d over the tile's pixels, d read at (x / scale, y / scale); all bins when the
+// tile has no valid value.  Then the items as launch_band_items.
+struct InitBandArgs {
+    View init;             // LUM_I16 (d, -1 invalid) or RGBA_U16 (EDisparityId in channel 0)
+    int ref_format;        // 1: RGBA_U16
+    int scale;             // W / init width
+    int W, H;
+};
+hipError_t launch_init_band(const InitBandArgs& ib, const CnnBandArgs& a, hipStream_t s);
 
 bool aw_supported(int sub, int D);
 void aw_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len);

@@ -604,6 +604,7 @@ stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** out,
     stereo_image_format f4{W4, H4, STEREO_PIXEL_LUM_U8};
     stereo_matcher_params cp = p->matcher;
     cp.num_disparities = Dc;
+    cp.use_disp_input = 0;
     if ((st = stereo_matcher_configure(ctx, &x->coarse, &f4, &cp))) return cleanup(st);
     if ((st = stereo_cnn_create(ctx, &x->cnn, W4, H4, 1))) return cleanup(st);
     if ((st = stereo_cnn_set_params(x->cnn, cnn_params, n))) return cleanup(st);

@@ -52,8 +52,12 @@ def _load():
     lib.orc_aw.argtypes = [i, P, P, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_crosscheck_plain.argtypes = [P, P, i, i, i, P]
     lib.orc_crosscheck_ref.argtypes = [P, P, i, i, i, i, i, P, P, P]
+    lib.orc_mip_level0_ref.argtypes = [P, i, i, P]
+    lib.orc_mip_level0_plain.argtypes = [P, i, i, P]
+    lib.orc_mip_next.argtypes = [P, i, i, C.c_float, P]
     for name in ["orc_census", "orc_census3x3_ref", "orc_bm", "orc_bm_brute", "orc_ncc", "orc_aw",
-                 "orc_crosscheck_plain", "orc_crosscheck_ref"]:
+                 "orc_crosscheck_plain", "orc_crosscheck_ref", "orc_mip_level0_ref", "orc_mip_level0_plain",
+                 "orc_mip_next"]:
         getattr(lib, name).restype = i
     return lib
 
@@ -242,3 +246,24 @@ def conv3x3_bwd(x, w, dy, dil=1, want_dx=True):
     db = np.empty(Co, np.float64)
     _chk(lib.orc_conv3x3_bwd(_p(x), N, H, W, Ci, _p(w), _p(dy), Co, dil, _p(dx), _p(dw), _p(db)), "orc_conv3x3_bwd")
     return dx, dw, db
+
+
+def disp_mipmaps(disp, nlevels):
+    """CreateDispMipMaps: float levels of a disparity map.  disp: (H, W, 4) uint16
+    EDisparityId pixels (Unknown marker 0) or (H, W) int16 plain map (marker -1)."""
+    disp = np.ascontiguousarray(disp)
+    H, W = disp.shape[:2]
+    lvl = np.empty((H, W), np.float32)
+    if disp.ndim == 3:
+        _chk(lib.orc_mip_level0_ref(_p(np.ascontiguousarray(disp, np.uint16)), W, H, _p(lvl)), "orc_mip_level0_ref")
+        unknown = 0.0
+    else:
+        _chk(lib.orc_mip_level0_plain(_p(np.ascontiguousarray(disp, np.int16)), W, H, _p(lvl)), "orc_mip_level0_plain")
+        unknown = -1.0
+    out = [lvl]
+    for _ in range(nlevels - 1):
+        h, w = out[-1].shape
+        nxt = np.empty(((h - 1) // 2, (w - 1) // 2), np.float32)
+        _chk(lib.orc_mip_next(_p(out[-1]), w, h, unknown, _p(nxt)), "orc_mip_next")
+        out.append(nxt)
+    return out

@@ -487,3 +487,60 @@ int orc_crosscheck_ref(const uint16_t* lr, const uint16_t* rl, int W, int H, int
     if (dmax) *dmax = (uint16_t)mx;
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* Disparity mip maps (CreateDispMipMaps.cu).                                             */
+
+int orc_mip_level0_ref(const uint16_t* rgba, int W, int H, float* out)
+{
+    /* CopyDispChannel (CreateDispMipMaps.cu:66-94): ids in [First, Last] as float, the
+     * rest Unknown */
+    if (!rgba || !out || W < 1 || H < 1) return -1;
+    for (size_t i = 0; i < (size_t)W * H; ++i) {
+        const uint16_t id = rgba[4 * i];
+        out[i] = (id >= ORC_DISP_FIRST && id <= ORC_DISP_LAST) ? (float)id : (float)ORC_DISP_UNKNOWN;
+    }
+    return 0;
+}
+
+int orc_mip_level0_plain(const int16_t* d, int W, int H, float* out)
+{
+    if (!d || !out || W < 1 || H < 1) return -1;
+    for (size_t i = 0; i < (size_t)W * H; ++i) out[i] = d[i] >= 0 ? (float)d[i] : -1.0f;
+    return 0;
+}
+
+/* one [1 2 1] tap of three values in the reference's order: the first value, twice the
+ * second, the third, each only when it is not `unknown`; sum / weight, or unknown */
+static float mip_tap(float v0, float v1, float v2, float unknown)
+{
+    float cnt = 0.0f, sum = 0.0f;
+    if (v0 != unknown) { cnt += 1.0f; sum += v0; }
+    if (v1 != unknown) { cnt += 2.0f; sum += v1 * 2.0f; }
+    if (v2 != unknown) { cnt += 1.0f; sum += v2; }
+    return cnt > 0.0f ? sum / cnt : unknown;
+}
+
+int orc_mip_next(const float* src, int W, int H, float unknown, float* out)
+{
+    /* SmoothHorizontalLum (:97-150) to a ((W-1)/2) x H buffer, then SmoothVerticalLum
+     * (:153-218) with the factor 0.5 on the normalised sum; NextLevelWidth/Height are
+     * (n - 1) / 2 (DeviceSurfMipMap.h:78-86), so every 3-tap window lies inside */
+    if (!src || !out || W < 3 || H < 3) return -1;
+    const int Wn = (W - 1) / 2, Hn = (H - 1) / 2;
+    float* tmp = (float*)malloc(sizeof(float) * (size_t)Wn * H);
+    if (!tmp) return -2;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < Wn; ++x) {
+            const float* r = src + (size_t)y * W + 2 * x;
+            tmp[(size_t)y * Wn + x] = mip_tap(r[0], r[1], r[2], unknown);
+        }
+    for (int y = 0; y < Hn; ++y)
+        for (int x = 0; x < Wn; ++x) {
+            const float* c = tmp + (size_t)(2 * y) * Wn + x;
+            const float v = mip_tap(c[0], c[Wn], c[2 * Wn], unknown);
+            out[(size_t)y * Wn + x] = v != unknown ? 0.5f * v : unknown;
+        }
+    free(tmp);
+    return 0;
+}

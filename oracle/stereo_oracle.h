@@ -78,6 +78,16 @@ int orc_crosscheck_plain(const int16_t* dL, const int16_t* dR, int W, int H, int
 int orc_crosscheck_ref(const uint16_t* lr, const uint16_t* rl, int W, int H, int thresh, int ltor,
                        int offset, uint16_t* out, uint16_t* dmin, uint16_t* dmax);
 
+/* CreateDispMipMaps (CreateDispMipMaps.cu).  Level 0 as float: the reference map's
+ * channel 0 id when First <= id <= Last, else Unknown (0) (CopyDispChannel :66-94); the
+ * plain map's d >= 0, else -1. */
+int orc_mip_level0_ref(const uint16_t* rgba, int W, int H, float* out);
+int orc_mip_level0_plain(const int16_t* d, int W, int H, float* out);
+/* The next level, ((W-1)/2) x ((H-1)/2): [1 2 1] horizontally then vertically, values
+ * equal to `unknown` skipped and the weights of the rest normalised, the vertical pass
+ * halving (SmoothHorizontalLum :97-150, SmoothVerticalLum :153-218). */
+int orc_mip_next(const float* src, int W, int H, float unknown, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,128 @@
+"""Band search around a caller's initial disparity map (MatcherUseDisparity) and the
+disparity mip maps (CreateDispMipMaps) against the CPU oracle."""
+import numpy as np
+import pytest
+
+import clutec_stereo as cs
+import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _tile_bins(init, W, H, D, ref_format=False):
+    """Per-pixel bins of the band search: every 64x56 tile keeps the 8-disparity bins of
+    [min - 4, max + 4] over scale * d at (x / scale, y / scale), all bins when none is valid."""
+    ih, iw = init.shape[:2]
+    s = W // iw
+    if ref_format:
+        ids = init[..., 0].astype(np.int64)
+        d = np.where((ids >= 1) & (ids <= 0xFFEF), ids - 1, -1)
+    else:
+        d = init.astype(np.int64)
+    ys = np.minimum(np.arange(H) // s, ih - 1)
+    xs = np.minimum(np.arange(W) // s, iw - 1)
+    full = d[ys][:, xs]
+    allm = (1 << (D // 8)) - 1
+    bins = np.zeros((H, W), np.uint32)
+    for y0 in range(0, H, 56):
+        for x0 in range(0, W, 64):
+            blk = full[y0:y0 + 56, x0:x0 + 64]
+            v = blk[blk >= 0] * s
+            if v.size == 0:
+                m = allm
+            else:
+                lo, hi = max(0, int(v.min()) - 4), min(D - 1, int(v.max()) + 4)
+                m = 0
+                for b in range(lo >> 3, (hi >> 3) + 1):
+                    m |= 1 << b
+                m &= allm
+            bins[y0:y0 + 56, x0:x0 + 64] = m
+    return bins
+
+
+def _oracle(algo, L, R, D, bins, window, census):
+    if algo == cs.ALGO_CENSUS:
+        return orc.bm_bins(orc.COST_CENSUS, orc.census(L, *census), orc.census(R, *census), D, bins, window=window,
+                           census_win=census, tol=1, nthreads=8)
+    return orc.bm_bins(orc.COST_SAD, L, R, D, bins, window=window, tol=1, nthreads=8)
+
+
+@pytest.mark.parametrize("algo,window,scale,fmt", [(cs.ALGO_CENSUS, (9, 9), 2, "i16"), (cs.ALGO_CENSUS, (9, 9), 1, "ref"),
+                                                   (cs.ALGO_SAD, (7, 7), 3, "i16"), (cs.ALGO_SAD, (9, 9), 2, "ref")])
+def test_band_search_matches_oracle(ctx, algo, window, scale, fmt):
+    W, H, D = 400, 250, 128
+    print(f"band search algo {algo} window {window} scale {scale} init {fmt}", flush=True)
+    L, R = cs.synth_pair(91, W, H, 2, 8, D - 1, 8)
+    census = (9, 7)
+    full = cs.Matcher(ctx, W, H, algorithm=algo, window=window, census=census, num_disp=D, tol=1)
+    ref_full = full.run(L, R)
+    # the initial map: the full-range result at 1/scale (every scale-th pixel, d / scale),
+    # with some values made invalid
+    rng = np.random.default_rng(scale)
+    small = ref_full[::scale, ::scale][: H // scale, : W // scale] // scale
+    small = np.where(rng.random(small.shape) < 0.1, -1, small).astype(np.int16)
+    if fmt == "ref":
+        init = np.zeros(small.shape + (4,), np.uint16)
+        init[..., 0] = np.where(small >= 0, small + 1, 0xFFFA)
+    else:
+        init = small
+    m = cs.Matcher(ctx, W, H, algorithm=algo, window=window, census=census, num_disp=D, tol=1, use_disp_input=True)
+    got = m.run_init(L, R, init, maps=True)
+    bins = _tile_bins(init, W, H, D, ref_format=(fmt == "ref"))
+    want = _oracle(algo, L, R, D, bins, window, census)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+    # most of the frame keeps the full-range result
+    print(f"  agreement with the full range {(got[0] == ref_full).mean():.4f}", flush=True)
+
+
+def test_band_search_without_valid_init_is_full_range(ctx):
+    W, H, D = 320, 224, 64
+    L, R = cs.synth_pair(92, W, H, 0, 6, D - 1, 8)
+    init = np.full((H // 2, W // 2), -1, np.int16)
+    m = cs.Matcher(ctx, W, H, algorithm=cs.ALGO_CENSUS, num_disp=D, use_disp_input=True)
+    got = m.run_init(L, R, init, maps=True)
+    want = m.run(L, R, maps=True)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+
+
+def test_band_search_configuration_errors(ctx):
+    W, H, D = 256, 128, 64
+    with pytest.raises(cs.StereoError) as e:
+        cs.Matcher(ctx, W, H, algorithm=cs.ALGO_NCC, window=(11, 11), num_disp=D, use_disp_input=True)
+    assert e.value.status == cs.ERROR_INVALID_CONFIG
+    m = cs.Matcher(ctx, W, H, algorithm=cs.ALGO_SAD, window=(9, 9), num_disp=D)
+    L, R = cs.synth_pair(93, W, H, 0, 4, D - 1, 8)
+    with pytest.raises(cs.StereoError) as e:
+        m.run_init(L, R, np.zeros((H // 2, W // 2), np.int16))
+    assert e.value.status == cs.ERROR_INVALID_CONFIG
+
+
+@pytest.mark.parametrize("fmt,size", [("i16", (331, 197)), ("ref", (256, 160)), ("i16", (1920, 1080))])
+def test_disp_mipmaps_match_oracle(ctx, fmt, size):
+    W, H = size
+    print(f"mip maps {fmt} {W}x{H}", flush=True)
+    rng = np.random.default_rng(W)
+    d = rng.integers(0, 250, (H, W)).astype(np.int16)
+    d[rng.random(d.shape) < 0.25] = -1
+    if fmt == "ref":
+        arr = np.zeros((H, W, 4), np.uint16)
+        arr[..., 0] = np.where(d >= 0, d + 1, np.where(rng.random(d.shape) < 0.5, 0, 0xFFFD))
+        img = cs.DeviceImage.from_array(ctx, arr, cs.PIXEL_RGBA_U16)
+    else:
+        arr = d
+        img = cs.DeviceImage.from_array(ctx, arr, cs.PIXEL_LUM_I16)
+    n = 5
+    got = [lv.download() for lv in cs.disp_mipmaps(ctx, img, n)]
+    want = orc.disp_mipmaps(arr, n)
+    for k, (a, b) in enumerate(zip(got, want)):
+        assert a.shape == b.shape, k
+        assert np.array_equal(a, b), k
+
+
+def test_disp_mipmaps_too_many_levels(ctx):
+    img = cs.DeviceImage.from_array(ctx, np.zeros((10, 10), np.int16), cs.PIXEL_LUM_I16)
+    with pytest.raises(cs.StereoError) as e:
+        cs.disp_mipmaps(ctx, img, 4)  # 10 -> 4 -> 1 -> below one pixel
+    assert e.value.status == cs.ERROR_INVALID_CONFIG
