@@ -91,10 +91,12 @@ size_t stereo_pixel_size(int32_t pixel_type);
 stereo_status stereo_image_alloc(stereo_context* ctx, stereo_image* img, int32_t width, int32_t height,
                                  int32_t pixel_type);
 stereo_status stereo_image_free(stereo_context* ctx, stereo_image* img);
-/* CDeviceImage::CopyFrom: host rows (host_pitch bytes apart) -> device. */
+/* CDeviceImage::CopyFrom: host rows (host_pitch bytes apart) -> device.  Synchronous;
+ * the rows pass through a pinned staging buffer of the context (8 MiB, allocated on the
+ * first transfer), so transfers on one context are serialised. */
 stereo_status stereo_image_upload(stereo_context* ctx, stereo_image* img, const void* host, size_t host_pitch,
                                   stereo_stream stream);
-/* CDeviceImage::CopyInto: device -> host rows. */
+/* CDeviceImage::CopyInto: device -> host rows, as stereo_image_upload. */
 stereo_status stereo_image_download(stereo_context* ctx, const stereo_image* img, void* host, size_t host_pitch,
                                     stereo_stream stream);
 

@@ -45,6 +45,10 @@ struct stereo_context {
     int device = 0;
     int timing = 0;
     std::string last_error;
+    // pinned staging buffer of the host image transfers (allocated on the first one)
+    std::mutex stage_mu;
+    void* stage = nullptr;
+    size_t stage_bytes = 0;
 };
 
 namespace capi {
