@@ -17,7 +17,7 @@
 // their horizontal sums over 2*AX+1 columns (halo from the neighbouring runs by DPP)
 // into LDS (one buffer, two barriers per disparity).  Column phase: lane = column,
 // wave = 13 rows: vertical running sums give SIJ, the right window's (SJ, rJ) come
-// from an LDS tile of the right statistics that is refreshed every 16 disparities,
+// from an LDS tile of the right statistics that is refreshed every 8 disparities,
 // the left window's are registers.  Left winner in
 // a register, right winner carried along the wave by a DPP shift (see bm.hip).
 #include "common.hpp"
@@ -33,7 +33,8 @@ constexpr int HROWS = 64;
 constexpr int RUN = 19;
 constexpr int COLS = 4 * RUN;  // 76 cost columns: tile columns [-6, 70)
 constexpr int HSTRIDE = 76;
-constexpr int DCH = 16;        // disparities per right-statistics tile
+constexpr int DCH = 8;         // disparities per right-statistics tile
+constexpr int EMIT = 32;       // right-view values parked per wave between flushes
 constexpr int SCOLS = TW + DCH - 1;
 
 // Per-pixel window statistics: .x = window sum S (int), .y = float bits of the
@@ -98,7 +99,8 @@ __global__ __launch_bounds__(256) void ncc_stats_kernel(View img, int2* out, int
 }
 
 template <int AX, int AY, bool EDGE>
-__device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int2* stile, uint32_t (*ebuf)[RG][64], int tx, int ty, int seg)
+__device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16_t* ssum, float* srinv,
+                                         uint32_t (*ebuf)[RG][EMIT], int tx, int ty, int seg)
 {
     static_assert(TH + 2 * AY <= HROWS, "tile too tall for the window");
     static_assert(AX <= 6 && 2 * AX + 64 <= COLS, "window wider than the run layout");
@@ -126,7 +128,7 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int2*
     const int x = x0 + lane;
     const int rows_here = H - (y0 + gw * RG);
     uint32_t* const rp = p.rpart + (((size_t)seg * p.ntx + tx) * H + y0 + gw * RG) * BM_NPART;
-    uint32_t* const eb = &ebuf[gw][0][0];
+    uint32_t* const eb = &ebuf[gw][0][0];  // [RG][EMIT]
     const bool lane_ok = x >= p.bx && x < W - p.bx;
     const int dlim = x - p.bx;
     int nsi[RG];     // -SI of the left window
@@ -149,7 +151,9 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int2*
             const int r = k / SCOLS, c = k - r * SCOLS;
             const int y = min(y0 + r, H - 1);
             const int xc = clampi(x0 - dc - (DCH - 1) + c, 0, W - 1);
-            stile[k] = p.rstats[(size_t)y * p.stats_pitch + xc];
+            const int2 st = p.rstats[(size_t)y * p.stats_pitch + xc];
+            ssum[k] = (int16_t)st.x;
+            srinv[k] = __int_as_float(st.y);
         }
         const int dce = min(de, dc + DCH);
         for (int d = dc; d < dce; ++d) {
@@ -182,7 +186,9 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int2*
             uint32_t sij = 0;
 #pragma unroll
             for (int i = 0; i <= 2 * AY; ++i) sij += hv[i];
-            const int2* srow = stile + (gw * RG) * SCOLS + lane + (DCH - 1) - (d - dc);
+            const int soff = (gw * RG) * SCOLS + lane + (DCH - 1) - (d - dc);
+            const int16_t* srow_s = ssum + soff;
+            const float* srow_r = srinv + soff;
             const uint32_t dd = 255u - (uint32_t)d;
             bool ok = true;
             if (EDGE) ok = lane_ok && d <= dlim;
@@ -190,10 +196,13 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int2*
 #pragma unroll
             for (int i = 0; i < RG; ++i) {
                 if (i > 0) sij = sij + hv[i + 2 * AY] - hv[i - 1];
-                const int2 rs = srow[i * SCOLS];
-                const int cov = __mul24(nsi[i], rs.x) + (int)__umul24((uint32_t)N, sij);
-                const float sc = (float)cov * rfi[i] * __int_as_float(rs.y);
-                const uint32_t q = sc > 0.0f ? (uint32_t)sc : 0u;
+                const int sj = srow_s[i * SCOLS];
+                const int cov = __mul24(nsi[i], sj) + (int)__umul24((uint32_t)N, sij);
+                const float sc = (float)cov * rfi[i] * srow_r[i * SCOLS];
+                // v_cvt_u32_f32 truncates and clamps negative values to 0: the key's
+                // floor(max(score, 0) * 2^23) in one instruction
+                uint32_t q;
+                asm("v_cvt_u32_f32 %0, %1" : "=v"(q) : "v"(sc));
                 uint32_t k = (q << 8) | dd;
                 if (EDGE) k = ok ? k : 0u;
                 key[i] = k;
@@ -203,15 +212,15 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int2*
             max_shift_in13(racc, key);
             if (lane == 63) {
 #pragma unroll
-                for (int i = 0; i < RG; ++i) eb[i * 64 + (d & 63)] = racc[i];
+                for (int i = 0; i < RG; ++i) eb[i * EMIT + (d & (EMIT - 1))] = racc[i];
             }
-            if ((d & 63) == 63 || d == de - 1) {
-                const int base = d & ~63;
-                if (lane <= (d & 63) && base + lane >= ds) {
+            if ((d & (EMIT - 1)) == EMIT - 1 || d == de - 1) {
+                const int base = d & ~(EMIT - 1);
+                if (lane <= (d & (EMIT - 1)) && base + lane >= ds) {
                     const int kk = 318 - (base + lane);
 #pragma unroll
                     for (int i = 0; i < RG; ++i)
-                        if (i < rows_here) rp[i * BM_NPART + kk] = eb[i * 64 + lane];
+                        if (i < rows_here) rp[i * BM_NPART + kk] = eb[i * EMIT + lane];
                 }
             }
         }
@@ -234,8 +243,10 @@ template <int AX, int AY>
 __global__ __launch_bounds__(256, 2) void ncc_fused_kernel(NccParams p)
 {
     __shared__ uint32_t hbuf[HROWS * HSTRIDE];
-    __shared__ int2 stile[TH * SCOLS];
-    __shared__ uint32_t ebuf[4][RG][64];
+    // right statistics of the tile: window sums (at most 121 x 255, an int16) and 1/sigma
+    __shared__ int16_t ssum[TH * SCOLS];
+    __shared__ float srinv[TH * SCOLS];
+    __shared__ uint32_t ebuf[4][RG][EMIT];
     // one-dimensional grid of ntx * nty * nseg workgroups, renumbered per XCD
     const int nty = (p.H + TH - 1) / TH;
     const int w = xcd_remap(blockIdx.x, gridDim.x);
@@ -245,9 +256,9 @@ __global__ __launch_bounds__(256, 2) void ncc_fused_kernel(NccParams p)
     const int de = min(p.D, (seg + 1) * p.seg_len);
     const bool edge = (x0 < p.bx + de - 1) || (x0 + TW > p.W - p.bx);
     if (edge)
-        ncc_tile<AX, AY, true>(p, hbuf, stile, ebuf, tx, ty, seg);
+        ncc_tile<AX, AY, true>(p, hbuf, ssum, srinv, ebuf, tx, ty, seg);
     else
-        ncc_tile<AX, AY, false>(p, hbuf, stile, ebuf, tx, ty, seg);
+        ncc_tile<AX, AY, false>(p, hbuf, ssum, srinv, ebuf, tx, ty, seg);
 }
 
 // Winning score of every left pixel: the same FP32 arithmetic as the matcher, on a
