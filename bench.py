@@ -257,6 +257,46 @@ def other_matchers_leg(cs, np, ctx, steps):
     return res
 
 
+def post_filters_leg(cs, np, ctx, disp, steps):
+    """The filters on the 1080p matcher map, resident in HBM: de-noise (5x5 window, plain
+    int16 map and the 4 x uint16 id map) and five disparity mip-map levels.  Algorithmic
+    bytes: every input pixel read once and every output pixel written once.  ms_per_frame
+    is wall-clock per back-to-back call from Python, which for kernels this short is
+    bound by the host call; the kernels' own durations come from a kernel trace."""
+    res = {}
+    ids = np.zeros(disp.shape + (4,), np.uint16)
+    ids[..., 0] = np.where(disp >= 0, disp + 1, 0xFFFA)
+    cases = []
+    for name, arr, pt, opt, inb, outb in (("denoise5x5_i16", disp, cs.PIXEL_LUM_I16, cs.PIXEL_LUM_I16, 2, 2),
+                                          ("denoise5x5_ids", ids, cs.PIXEL_RGBA_U16, cs.PIXEL_LUM_U16, 8, 2)):
+        dn = cs.Denoise(ctx, W, H, patch=5, ref_format=(pt == cs.PIXEL_RGBA_U16))
+        src = cs.DeviceImage.from_array(ctx, arr, pt)
+        dst = cs.DeviceImage(ctx, W, H, opt)
+        cases.append((name, (lambda dn=dn, src=src, dst=dst: dn.process(dst, src)), W * H * (inb + outb)))
+    src = cs.DeviceImage.from_array(ctx, disp, cs.PIXEL_LUM_I16)
+    levels, w, h, nbytes = [], W, H, W * H * (2 + 4)
+    for i in range(5):
+        if i:
+            w, h = (w - 1) // 2, (h - 1) // 2
+            nbytes += 3 * 4 * w * (2 * h + 1) + 4 * w * h  # three source rows per target row, target written
+        levels.append(cs.DeviceImage(ctx, w, h, cs.PIXEL_LUM_F32))
+    arr_t = (cs.Image * 5)(*[lv.img for lv in levels])
+    cases.append(("mipmaps5_i16", lambda: cs._check(cs.lib.stereo_disp_mipmaps(ctx.handle, src.ref, arr_t, 5, None), ctx),
+                  nbytes))
+    for name, fn, nb in cases:
+        for _ in range(3):
+            fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        res[name] = {"ms_per_frame": round(ms, 4), "algorithmic_bytes": nb,
+                     "GB_per_s_wall": round(nb / (ms * 1e-3) / 1e9, 1)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -375,6 +415,7 @@ def main():
         result["batch"] = batch_leg(cs, np, ctx, max(3, min(args.steps, 10)))
     if not args.no_others:
         result["other_matchers"] = other_matchers_leg(cs, np, ctx, max(5, min(args.steps, 20)))
+        result["post_filters"] = post_filters_leg(cs, np, ctx, disp, max(20, args.steps))
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

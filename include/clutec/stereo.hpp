@@ -498,5 +498,63 @@ public:
 };
 }  // namespace CrossCheckDisparityMaps
 
+// Statistics::DispDenoise (Statistics.DispDenoise.h:45-190): the input is an RGBA_U16
+// EDisparityId map (uint16 ids out, LUM_U16) or a LUM_I16 map (LUM_I16 out).
+namespace Statistics
+{
+namespace DispDenoise
+{
+enum class EConfig
+{
+    Patch_16x16 = 0,
+    Patch_11x11,
+    Patch_9x9,
+    Patch_7x7,
+    Patch_5x5,
+    Patch_3x3,
+};
+
+inline int32_t PatchSize(EConfig e)
+{
+    static const int32_t sizes[] = {16, 11, 9, 7, 5, 3};
+    const int i = static_cast<int>(e);
+    return i >= 0 && i < 6 ? sizes[i] : 0;
+}
+
+struct SParameter
+{
+    float fSupportRatio = 0.5f;
+    float fMeanCutOff = 1.0f;
+    int32_t iDoFill = 1;
+    EConfig eConfig = EConfig::Patch_5x5;
+    void Set(float ratio, float cutoff, int32_t fill, EConfig e)
+    {
+        fSupportRatio = ratio;
+        fMeanCutOff = cutoff;
+        iDoFill = fill;
+        eConfig = e;
+    }
+};
+
+class CDriver : public CKernelDriverBase
+{
+public:
+    CDriver() : CKernelDriverBase("Statistics disparity de-noise Filter") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    {
+        const stereo_image_format f = fmt.Raw();
+        const stereo_denoise_params dp{PatchSize(p.eConfig), p.fSupportRatio, p.fMeanCutOff, p.iDoFill, {0, 0, 0, 0}};
+        stereo_driver* d = nullptr;
+        EnsureOK(stereo_denoise_configure(dev.Context(), &d, &f, &dp), dev.Context(), "DispDenoise::Configure");
+        Reset(dev.Context(), d);
+    }
+    void Process(CDeviceImage& target, const CDeviceImage& source)
+    {
+        EnsureOK(stereo_denoise_process(m_drv, target.Raw(), source.Raw(), nullptr), m_ctx, "DispDenoise::Process");
+    }
+};
+}  // namespace DispDenoise
+}  // namespace Statistics
+
 }  // namespace Cuda
 }  // namespace Clu

@@ -369,6 +369,30 @@ stereo_status stereo_learned_tile_masks(stereo_driver* drv, uint32_t* masks, int
 stereo_status stereo_disp_mipmaps(stereo_context* ctx, const stereo_image* disp, stereo_image* levels, int32_t nlevels,
                                   stereo_stream stream);
 
+/* ---- disparity de-noise (Statistics::DispDenoise, Statistics.DispDenoise.h:40-190) ---- */
+
+typedef struct stereo_denoise_params {
+    int32_t patch;         /* window P: 3, 5, 7, 9, 11 or 16 (EConfig::Patch_PxP) */
+    float support_ratio;   /* fSupportRatio: at least P*P*ratio valid values (0.5) */
+    float mean_cutoff;     /* fMeanCutOff (1.0) */
+    int32_t fill;          /* iDoFill: fill invalid pixels in flat windows (1) */
+    int32_t reserved[4];   /* must be 0 */
+} stereo_denoise_params;
+
+/* CDriver::Configure (Statistics.DispDenoise.cu:360-380): fmt is the input map, RGBA_U16
+ * (EDisparityId in channel 0) or LUM_I16 (d, -1 invalid), at least P x P. */
+stereo_status stereo_denoise_configure(stereo_context* ctx, stereo_driver** drv, const stereo_image_format* fmt,
+                                       const stereo_denoise_params* params);
+/* CDriver::Process (FilterNoBorder, Statistics.DispDenoise.cu:110-330) on the whole image:
+ * per pixel, a P x P window (offsets -P/2 .. P-1-P/2, positions outside the image moved by
+ * P into it) gives the count, sum and sum of squares of the valid values.  Fewer than
+ * P*P*ratio valid: Unknown (0, or -1).  A valid pixel v: m = (sum - v) / (count - 1) in
+ * FP32; floor(m + 0.5) when |v - m| < cut-off, else Unknown.  An invalid pixel (with fill):
+ * m = sum / count, m2 = sumsq / count; floor(m + 0.5) when sqrt(|m2 - m*m|) < cut-off,
+ * else unchanged.  out: LUM_U16 ids for RGBA_U16 input, LUM_I16 for LUM_I16. */
+stereo_status stereo_denoise_process(stereo_driver* drv, stereo_image* out, const stereo_image* in,
+                                     stereo_stream stream);
+
 /* ---- device calibration ------------------------------------------------------ */
 
 /* INT32 VALU issue rate of the device in tera lane-operations per second, measured

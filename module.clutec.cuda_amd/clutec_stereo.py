@@ -77,6 +77,11 @@ class CrossCheckParams(C.Structure):
                 ("disp_offset", C.c_int32), ("reserved", C.c_int32)]
 
 
+class DenoiseParams(C.Structure):
+    _fields_ = [("patch", C.c_int32), ("support_ratio", C.c_float), ("mean_cutoff", C.c_float), ("fill", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
+
+
 class BatchTuning(C.Structure):
     _fields_ = [("chunk", C.c_int32), ("grid", C.c_int32), ("order", C.c_int32), ("use_graph", C.c_int32)]
 
@@ -125,6 +130,8 @@ _PROTOS = [
     ("stereo_matcher_process_init", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image),
                                                C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_disp_mipmaps", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.c_int32, _P]),
+    ("stereo_denoise_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(DenoiseParams)]),
+    ("stereo_denoise_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_scores", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_stage_times", C.c_int, [_P, C.POINTER(C.c_double), C.c_int32]),
     ("stereo_crosscheck_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(CrossCheckParams)]),
@@ -450,6 +457,27 @@ def disp_mipmaps(ctx, disp, nlevels, stream=None):
     arr = (Image * nlevels)(*[lv.img for lv in levels])
     _check(lib.stereo_disp_mipmaps(ctx.handle, disp.ref, arr, nlevels, stream), ctx)
     return levels
+
+
+class Denoise(_Driver):
+    """Statistics::DispDenoise (stereo_denoise_*): ref_format maps are (h, w, 4) uint16
+    EDisparityId pixels in and uint16 ids out; plain maps int16 (-1 invalid) in and out."""
+
+    def __init__(self, ctx, W, H, patch=5, support_ratio=0.5, mean_cutoff=1.0, fill=True, ref_format=False):
+        super().__init__(ctx)
+        self.W, self.H, self.ref_format = W, H, ref_format
+        fmt = ImageFormat(W, H, PIXEL_RGBA_U16 if ref_format else PIXEL_LUM_I16)
+        p = DenoiseParams(patch, support_ratio, mean_cutoff, 1 if fill else 0)
+        _check(lib.stereo_denoise_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(p)), ctx)
+
+    def process(self, out, disp, stream=None):
+        _check(lib.stereo_denoise_process(self.handle, out.ref, disp.ref, stream), self.ctx)
+
+    def run(self, disp):
+        src = DeviceImage.from_array(self.ctx, disp, PIXEL_RGBA_U16 if self.ref_format else PIXEL_LUM_I16)
+        out = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_U16 if self.ref_format else PIXEL_LUM_I16)
+        self.process(out, src)
+        return out.download()
 
 
 class CrossCheck(_Driver):

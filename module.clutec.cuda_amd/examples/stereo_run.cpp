@@ -3,10 +3,12 @@
 //   1. generate a synthetic pair, 2. upload into device images, 3. Configure the
 //   drivers, 4. ProcessKernel: matcher (census or SAD; L->R, R->L and check fused)
 //   and, from the two one-sided maps in disparity-id form, the reference-style
-//   cross-check, 5. download, 6. write the maps and report milliseconds.
+//   cross-check, then Statistics::DispDenoise over the checked id map, 5. download,
+//   6. write the maps and report milliseconds.
 //
 // usage: stereo_run <algo:sad|census|ncc|aw1|aw2> <W> <H> <D> <seed> <out_prefix> [runs]
-// Writes <out_prefix>_checked.i16, _lr.i16, _rl.i16 (row-major int16, -1 invalid)
+// Writes <out_prefix>_checked.i16, _lr.i16, _rl.i16 (row-major int16, -1 invalid),
+// _checkid.u16x4 (the cross-check's id map) and _denoised.u16 (its de-noised ids)
 // and prints one line "ms <median> valid <fraction> range <min> <max>".
 #include <algorithm>
 #include <cstdio>
@@ -120,11 +122,23 @@ int main(int argc, char** argv)
         const double cms = ProcessKernel(cc, dchk, dl, dr);
         const int dmin = cc.DispMin(), dmax = cc.DispMax();
 
+        // Statistics::DispDenoise over the cross-checked id map (5x5, defaults)
+        Statistics::DispDenoise::SParameter np;
+        Statistics::DispDenoise::CDriver dn;
+        dn.Configure(dev, fmtId, np);
+        CDeviceImage dden(dev, SImageFormat(W, H, EPixelType::Lum_UInt16));
+        ProcessKernel(dn, dden, dchk);
+        std::vector<uint16_t> hid((size_t)W * H * 4), hden((size_t)W * H);
+        dchk.CopyInto(hid.data(), (size_t)W * 8);
+        dden.CopyInto(hden.data(), (size_t)W * 2);
+
         size_t valid = 0;
         for (int16_t v : hchk) valid += v >= 0;
         std::sort(ms.begin(), ms.end());
         if (!write_file(out + "_checked.i16", hchk.data(), hchk.size() * 2) ||
-            !write_file(out + "_lr.i16", hlr.data(), hlr.size() * 2) || !write_file(out + "_rl.i16", hrl.data(), hrl.size() * 2)) {
+            !write_file(out + "_lr.i16", hlr.data(), hlr.size() * 2) || !write_file(out + "_rl.i16", hrl.data(), hrl.size() * 2) ||
+            !write_file(out + "_checkid.u16x4", hid.data(), hid.size() * 2) ||
+            !write_file(out + "_denoised.u16", hden.data(), hden.size() * 2)) {
             std::fprintf(stderr, "cannot write %s_*.i16\n", out.c_str());
             return 1;
         }

@@ -55,9 +55,10 @@ def _load():
     lib.orc_mip_level0_ref.argtypes = [P, i, i, P]
     lib.orc_mip_level0_plain.argtypes = [P, i, i, P]
     lib.orc_mip_next.argtypes = [P, i, i, C.c_float, P]
+    lib.orc_disp_denoise.argtypes = [i, P, i, i, i, C.c_float, C.c_float, i, P]
     for name in ["orc_census", "orc_census3x3_ref", "orc_bm", "orc_bm_brute", "orc_ncc", "orc_aw",
                  "orc_crosscheck_plain", "orc_crosscheck_ref", "orc_mip_level0_ref", "orc_mip_level0_plain",
-                 "orc_mip_next"]:
+                 "orc_mip_next", "orc_disp_denoise"]:
         getattr(lib, name).restype = i
     return lib
 
@@ -266,4 +267,16 @@ def disp_mipmaps(disp, nlevels):
         nxt = np.empty(((h - 1) // 2, (w - 1) // 2), np.float32)
         _chk(lib.orc_mip_next(_p(out[-1]), w, h, unknown, _p(nxt)), "orc_mip_next")
         out.append(nxt)
+    return out
+
+
+def disp_denoise(disp, P=5, support_ratio=0.5, mean_cutoff=1.0, fill=True):
+    """Statistics::DispDenoise: (H, W, 4) uint16 EDisparityId pixels -> (H, W) uint16 ids,
+    or (H, W) int16 d (-1 invalid) -> int16."""
+    disp = np.ascontiguousarray(disp)
+    H, W = disp.shape[:2]
+    ref = disp.ndim == 3
+    out = np.empty((H, W), np.uint16 if ref else np.int16)
+    _chk(lib.orc_disp_denoise(1 if ref else 0, _p(np.ascontiguousarray(disp, np.uint16 if ref else np.int16)), W, H, P,
+                              support_ratio, mean_cutoff, 1 if fill else 0, _p(out)), "orc_disp_denoise")
     return out

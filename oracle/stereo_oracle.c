@@ -544,3 +544,68 @@ int orc_mip_next(const float* src, int W, int H, float unknown, float* out)
     free(tmp);
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* Disparity de-noise (Statistics.DispDenoise.cu).                                        */
+
+int orc_disp_denoise(int ref_format, const void* in, int W, int H, int P, float support_ratio, float mean_cutoff,
+                     int fill, void* out)
+{
+    if (!in || !out || P < 1 || W < P || H < P) return -1;
+    const uint16_t* rin = (const uint16_t*)in;
+    const int16_t* pin = (const int16_t*)in;
+    /* value of pixel (x, y): the id (reference) or d (plain) when valid, else -1 */
+    #define ORC_DN_VAL(x, y) (ref_format ? ((rin[4 * ((size_t)(y) * W + (x))] >= ORC_DISP_FIRST && \
+                                              rin[4 * ((size_t)(y) * W + (x))] <= ORC_DISP_LAST) ? \
+                                              (int)rin[4 * ((size_t)(y) * W + (x))] : -1) \
+                                           : (pin[(size_t)(y) * W + (x)] >= 0 ? (int)pin[(size_t)(y) * W + (x)] : -1))
+    const int unknown = ref_format ? ORC_DISP_UNKNOWN : -1;
+    const float need = (float)(P * P) * support_ratio;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            int cnt = 0, sum = 0;
+            int64_t sum2 = 0;
+            for (int j = 0; j < P; ++j) {
+                int yy = y - P / 2 + j;
+                if (yy < 0) yy += P;
+                if (yy >= H) yy -= P;
+                for (int i = 0; i < P; ++i) {
+                    int xx = x - P / 2 + i;
+                    if (xx < 0) xx += P;
+                    if (xx >= W) xx -= P;
+                    const int v = ORC_DN_VAL(xx, yy);
+                    if (v >= 0) {
+                        ++cnt;
+                        sum += v;
+                        sum2 += (int64_t)v * v;
+                    }
+                }
+            }
+            const int raw = ref_format ? (int)rin[4 * ((size_t)y * W + x)] : (int)pin[(size_t)y * W + x];
+            const int v = ORC_DN_VAL(x, y);
+            int res;
+            if (!((float)cnt >= need)) {
+                res = unknown;                       /* too little support (:303-304) */
+            } else if (v >= 0) {
+                /* the mean of the other valid values; keep it when the pixel is near it (:308-316) */
+                const float fd = (float)v;
+                const float mean = ((float)sum - fd) / (float)(cnt - 1);
+                res = fabsf(fd - mean) < mean_cutoff ? (int)floorf(mean + 0.5f) : unknown;
+            } else if (fill) {
+                /* fill from the window when it is flat (:318-325) */
+                const float mean = (float)sum / (float)cnt;
+                const float mean2 = (float)sum2 / (float)cnt;
+                const float sq = mean * mean;
+                const float sd = sqrtf(fabsf(mean2 - sq));
+                res = sd < mean_cutoff ? (int)floorf(mean + 0.5f) : raw;
+            } else {
+                res = raw;
+            }
+            if (ref_format) ((uint16_t*)out)[(size_t)y * W + x] = (uint16_t)res;
+            else ((int16_t*)out)[(size_t)y * W + x] = (int16_t)res;
+        }
+    }
+    #undef ORC_DN_VAL
+    return 0;
+}

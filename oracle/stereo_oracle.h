@@ -88,6 +88,15 @@ int orc_mip_level0_plain(const int16_t* d, int W, int H, float* out);
  * halving (SmoothHorizontalLum :97-150, SmoothVerticalLum :153-218). */
 int orc_mip_next(const float* src, int W, int H, float unknown, float* out);
 
+/* Statistics::DispDenoise::FilterNoBorder (Statistics.DispDenoise.cu:110-330) on the whole
+ * image: a P x P window (offsets -P/2 .. P-1-P/2) whose positions outside the image are
+ * moved by P into it (:160-165); count, sum and sum of squares of the valid values; see
+ * the .c file for the decision.  ref_format 1: 4 x uint16 EDisparityId pixels in, uint16
+ * ids out (valid: First..Last, Unknown 0); 0: int16 d in and out (valid: d >= 0,
+ * unknown -1).  W, H >= P. */
+int orc_disp_denoise(int ref_format, const void* in, int W, int H, int P, float support_ratio, float mean_cutoff,
+                     int fill, void* out);
+
 #ifdef __cplusplus
 }
 #endif

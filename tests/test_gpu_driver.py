@@ -37,3 +37,7 @@ def test_stereo_run_matches_oracle(tmp_path, algo, W, H, D):
     # the reference-style cross-check over the one-sided maps reports the kept range
     kept = maps[0][maps[0] >= 0]
     assert int(line[5]) == kept.min() + 1 and int(line[6]) == kept.max() + 1
+    # DispDenoise (5x5 defaults) over the cross-check's id map
+    ids = np.fromfile(prefix + "_checkid.u16x4", np.uint16).reshape(H, W, 4)
+    den = np.fromfile(prefix + "_denoised.u16", np.uint16).reshape(H, W)
+    assert np.array_equal(den, orc.disp_denoise(ids, P=5))
