@@ -257,6 +257,40 @@ def other_matchers_leg(cs, np, ctx, steps):
     return res
 
 
+def bm1_leg(cs, np, ctx, steps):
+    """BlockMatcher1 (9x9 window, AbsoluteDifference, left to right) with the reference's
+    block-tiled results, on the 640x480 pair (D=64, seed 0) and the 1080p pair (D=256,
+    seed 1).  Evaluations are counted as W*H*D per frame."""
+    res = {}
+    for name, w, h, d, seed, nreg in (("bm1_9x9_640x480_d64", 640, 480, 64, 0, 8),
+                                      ("bm1_9x9_1920x1080_d256", 1920, 1080, 256, 1, 24)):
+        L, R = cs.synth_pair(seed, w, h, 0, nreg, d - 1, 8)
+        m = cs.BlockMatcher1(ctx, w, h, patch=9, disp_min=0, disp_max=d, min_delta_thresh=1.0, sim_thresh=60.0,
+                             grad_thresh=2.0)
+        dl = cs.DeviceImage.from_array(ctx, L, cs.PIXEL_LUM_U8)
+        dr = cs.DeviceImage.from_array(ctx, R, cs.PIXEL_LUM_U8)
+        out = cs.DeviceImage(ctx, w, h, cs.PIXEL_RGBA_U16)
+        for _ in range(2):
+            m.process(out, dl, dr)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.process(out, dl, dr)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        ctx.set_timing(True)
+        kms = []
+        for _ in range(3):
+            m.process(out, dl, dr)
+            kms.append(m.last_time_ms())
+        ctx.set_timing(False)
+        c = out.download()[..., 0]
+        res[name] = {"ms_per_frame": round(ms, 4), "kernel_ms": round(statistics.median(kms), 4),
+                     "M_evals_per_s": round(w * h * d / (ms * 1e-3) / 1e6, 1),
+                     "disparity_fraction": round(float(((c >= 1) & (c <= 0xFFEF)).mean()), 4)}
+    return res
+
+
 def post_filters_leg(cs, np, ctx, disp, steps):
     """The filters on the 1080p matcher map, resident in HBM: de-noise (5x5 window, plain
     int16 map and the 4 x uint16 id map) and five disparity mip-map levels.  Algorithmic
@@ -416,6 +450,7 @@ def main():
     if not args.no_others:
         result["other_matchers"] = other_matchers_leg(cs, np, ctx, max(5, min(args.steps, 20)))
         result["post_filters"] = post_filters_leg(cs, np, ctx, disp, max(20, args.steps))
+        result["block_matcher1"] = bm1_leg(cs, np, ctx, max(5, min(args.steps, 10)))
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

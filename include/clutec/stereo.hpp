@@ -452,6 +452,73 @@ public:
 };
 }  // namespace BlockMatcherAW2
 
+// BlockMatcher1 (BlockMatcher1.h:40-180) with the reference's block-tiled results.
+namespace BlockMatcher1
+{
+enum class EConfig
+{
+    Patch_5x5 = 0,
+    Patch_9x9,
+    Patch_11x11,
+    Patch_15x15,
+};
+enum class ESimType
+{
+    AbsoluteDifference = 0,
+    SignedProduct,
+};
+inline int32_t PatchSize(EConfig e)
+{
+    static const int32_t sizes[] = {5, 9, 11, 15};
+    const int i = static_cast<int>(e);
+    return i >= 0 && i < 4 ? sizes[i] : 0;
+}
+struct SParameter
+{
+    _SDisparityConfig xDispConfig;
+    float fMinDeltaThresh = 0.0f;
+    float fSimThresh = 0.0f;
+    float fGradThresh = 0.0f;
+    int32_t iUseDispInput = 0;
+    bool bLeftToRight = true;
+    EConfig eConfig = EConfig::Patch_9x9;
+    ESimType eSimType = ESimType::AbsoluteDifference;
+    void Set(const _SDisparityConfig& cfg, float minDelta, float simThresh, float gradThresh, int32_t useDispInput,
+             bool leftToRight, EConfig config, ESimType simType)
+    {
+        xDispConfig = cfg;
+        fMinDeltaThresh = minDelta;
+        fSimThresh = simThresh;
+        fGradThresh = gradThresh;
+        iUseDispInput = useDispInput;
+        bLeftToRight = leftToRight;
+        eConfig = config;
+        eSimType = simType;
+    }
+};
+class CDriver : public CKernelDriverBase
+{
+public:
+    CDriver() : CKernelDriverBase("Block Matcher Adaptive Window 1") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    {
+        const stereo_image_format f = fmt.Raw();
+        const stereo_bm1_params bp{PatchSize(p.eConfig), static_cast<int32_t>(p.eSimType), p.xDispConfig.iMin,
+                                   p.xDispConfig.iMax, p.xDispConfig.iOffset, p.fMinDeltaThresh, p.fSimThresh,
+                                   p.fGradThresh, p.iUseDispInput, p.bLeftToRight ? 1 : 0, {0, 0, 0, 0}};
+        stereo_driver* d = nullptr;
+        EnsureOK(stereo_bm1_configure(dev.Context(), &d, &f, &bp), dev.Context(), "BlockMatcher1::Configure");
+        Reset(dev.Context(), d);
+    }
+    // Process(disp, L, R, dispInit): dispInit is used when iUseDispInput is set
+    void Process(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R, const CDeviceImage* dispInit = nullptr)
+    {
+        EnsureOK(stereo_bm1_process(m_drv, disp.Raw(), L.Raw(), R.Raw(), dispInit ? dispInit->Raw() : nullptr, nullptr),
+                 m_ctx, "BlockMatcher1::Process");
+    }
+};
+}  // namespace BlockMatcher1
+
 namespace CrossCheckDisparityMaps
 {
 struct SParameter

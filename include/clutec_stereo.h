@@ -369,6 +369,36 @@ stereo_status stereo_learned_tile_masks(stereo_driver* drv, uint32_t* masks, int
 stereo_status stereo_disp_mipmaps(stereo_context* ctx, const stereo_image* disp, stereo_image* levels, int32_t nlevels,
                                   stereo_stream stream);
 
+/* ---- BlockMatcher1 (BlockMatcher1.h:40-180) ---------------------------------------- */
+
+typedef struct stereo_bm1_params {
+    int32_t patch;            /* EConfig: 5, 9, 11 or 15 */
+    int32_t sim_type;         /* ESimType: 0 AbsoluteDifference, 1 SignedProduct */
+    int32_t disp_min;         /* _SDisparityConfig: disparities disp_min <= d < disp_max */
+    int32_t disp_max;         /*   (<= 4096) */
+    int32_t disp_offset;      /*   and the pixel offset of MapPixelPos */
+    float min_delta_thresh;   /* fMinDeltaThresh */
+    float sim_thresh;         /* fSimThresh */
+    float grad_thresh;        /* fGradThresh */
+    int32_t use_disp_input;   /* iUseDispInput: MatcherUseDisparity around an initial map */
+    int32_t left_to_right;    /* bLeftToRight */
+    int32_t reserved[4];      /* must be 0 */
+} stereo_bm1_params;
+
+/* CDriver::Configure (BlockMatcher1.cu:1218-1268): fmt is the LUM_U8 pair format, at least
+ * 48 wide and patch + 3 tall. */
+stereo_status stereo_bm1_configure(stereo_context* ctx, stereo_driver** drv, const stereo_image_format* fmt,
+                                   const stereo_bm1_params* params);
+/* CDriver::Process (BlockMatcher1.cu:1360-1400) with the reference's block-tiled results:
+ * 32-thread blocks of PCX x 4 result origins (PCX = 8 * ((33 - patch) / 8)), each with the
+ * disparity groups, test patches and edge rules of the reference (Matcher :590-847,
+ * MatcherUseDisparity :857-1203).  disp: RGBA_U16 (EDisparityId, similarity maximum and the
+ * two neighbour deltas as 16-bit fractions); pixels the reference does not write are left
+ * as they are.  disp_init: an RGBA_U16 id map of at most the image size (scale = width /
+ * its width; positions outside it read as Unknown), required iff use_disp_input. */
+stereo_status stereo_bm1_process(stereo_driver* drv, stereo_image* disp, const stereo_image* left,
+                                 const stereo_image* right, const stereo_image* disp_init, stereo_stream stream);
+
 /* ---- disparity de-noise (Statistics::DispDenoise, Statistics.DispDenoise.h:40-190) ---- */
 
 typedef struct stereo_denoise_params {

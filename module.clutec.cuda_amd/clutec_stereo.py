@@ -77,6 +77,13 @@ class CrossCheckParams(C.Structure):
                 ("disp_offset", C.c_int32), ("reserved", C.c_int32)]
 
 
+class Bm1Params(C.Structure):
+    _fields_ = [("patch", C.c_int32), ("sim_type", C.c_int32), ("disp_min", C.c_int32), ("disp_max", C.c_int32),
+                ("disp_offset", C.c_int32), ("min_delta_thresh", C.c_float), ("sim_thresh", C.c_float),
+                ("grad_thresh", C.c_float), ("use_disp_input", C.c_int32), ("left_to_right", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
+
+
 class DenoiseParams(C.Structure):
     _fields_ = [("patch", C.c_int32), ("support_ratio", C.c_float), ("mean_cutoff", C.c_float), ("fill", C.c_int32),
                 ("reserved", C.c_int32 * 4)]
@@ -131,6 +138,8 @@ _PROTOS = [
                                                C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_disp_mipmaps", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.c_int32, _P]),
     ("stereo_denoise_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(DenoiseParams)]),
+    ("stereo_bm1_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(Bm1Params)]),
+    ("stereo_bm1_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_denoise_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_scores", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_stage_times", C.c_int, [_P, C.POINTER(C.c_double), C.c_int32]),
@@ -457,6 +466,33 @@ def disp_mipmaps(ctx, disp, nlevels, stream=None):
     arr = (Image * nlevels)(*[lv.img for lv in levels])
     _check(lib.stereo_disp_mipmaps(ctx.handle, disp.ref, arr, nlevels, stream), ctx)
     return levels
+
+
+class BlockMatcher1(_Driver):
+    """BlockMatcher1 with the reference's block-tiled results (stereo_bm1_*): an
+    (h, w, 4) uint16 map of EDisparityId and three similarity channels."""
+
+    def __init__(self, ctx, W, H, patch=9, sim_type=0, disp_min=0, disp_max=64, disp_offset=0,
+                 min_delta_thresh=0.0, sim_thresh=0.0, grad_thresh=0.0, use_disp_input=False, left_to_right=True):
+        super().__init__(ctx)
+        self.W, self.H = W, H
+        p = Bm1Params(patch, sim_type, disp_min, disp_max, disp_offset, min_delta_thresh, sim_thresh, grad_thresh,
+                      1 if use_disp_input else 0, 1 if left_to_right else 0)
+        fmt = ImageFormat(W, H, PIXEL_LUM_U8)
+        _check(lib.stereo_bm1_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(p)), ctx)
+
+    def process(self, disp, left, right, init=None, stream=None):
+        _check(lib.stereo_bm1_process(self.handle, disp.ref, left.ref, right.ref, init.ref if init else None, stream),
+               self.ctx)
+
+    def run(self, L, R, init=None, fill=0):
+        """Host images in, the (H, W, 4) map out; pixels the matcher leaves are `fill`."""
+        dl = DeviceImage.from_array(self.ctx, L, PIXEL_LUM_U8)
+        dr = DeviceImage.from_array(self.ctx, R, PIXEL_LUM_U8)
+        out = DeviceImage.from_array(self.ctx, np.full((self.H, self.W, 4), fill, np.uint16), PIXEL_RGBA_U16)
+        di = DeviceImage.from_array(self.ctx, init, PIXEL_RGBA_U16) if init is not None else None
+        self.process(out, dl, dr, di)
+        return out.download()
 
 
 class Denoise(_Driver):

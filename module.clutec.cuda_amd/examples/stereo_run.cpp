@@ -6,7 +6,7 @@
 //   cross-check, then Statistics::DispDenoise over the checked id map, 5. download,
 //   6. write the maps and report milliseconds.
 //
-// usage: stereo_run <algo:sad|census|ncc|aw1|aw2> <W> <H> <D> <seed> <out_prefix> [runs]
+// usage: stereo_run <algo:sad|census|ncc|aw1|aw2|bm1> <W> <H> <D> <seed> <out_prefix> [runs]
 // Writes <out_prefix>_checked.i16, _lr.i16, _rl.i16 (row-major int16, -1 invalid),
 // _checkid.u16x4 (the cross-check's id map) and _denoised.u16 (its de-noised ids)
 // and prints one line "ms <median> valid <fraction> range <min> <max>".
@@ -33,7 +33,7 @@ static bool write_file(const std::string& path, const void* p, size_t n)
 int main(int argc, char** argv)
 {
     if (argc < 7) {
-        std::fprintf(stderr, "usage: %s <sad|census|ncc|aw1|aw2> W H D seed out_prefix [runs]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <sad|census|ncc|aw1|aw2|bm1> W H D seed out_prefix [runs]\n", argv[0]);
         return 2;
     }
     const std::string algo = argv[1];
@@ -96,6 +96,23 @@ int main(int argc, char** argv)
                 drv.Configure(dev, fmtIn, p);
                 run(drv);
             }
+        } else if (algo == "bm1") {
+            // BlockMatcher1 writes the reference's id map with similarity channels; the
+            // program writes it to <out_prefix>_bm1.u16x4 and stops there
+            BlockMatcher1::SParameter p;
+            p.Set(dcfg, 1.0f, 60.0f, 2.0f, 0, true, BlockMatcher1::EConfig::Patch_9x9,
+                  BlockMatcher1::ESimType::AbsoluteDifference);
+            BlockMatcher1::CDriver drv;
+            drv.Configure(dev, fmtIn, p);
+            CDeviceImage ids(dev, fmtId);
+            std::vector<uint16_t> hids((size_t)W * H * 4, 0x5A5A);
+            ids.CopyFrom(hids.data(), (size_t)W * 8);
+            for (int i = 0; i < runs; ++i) ms.push_back(ProcessKernel(drv, ids, imgL, imgR));
+            ids.CopyInto(hids.data(), (size_t)W * 8);
+            if (!write_file(out + "_bm1.u16x4", hids.data(), hids.size() * 2)) return 1;
+            std::sort(ms.begin(), ms.end());
+            std::printf("ms %.4f\n", ms[ms.size() / 2]);
+            return 0;
         } else {
             std::fprintf(stderr, "unknown algorithm '%s'\n", algo.c_str());
             return 2;

@@ -52,13 +52,14 @@ def _load():
     lib.orc_aw.argtypes = [i, P, P, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_crosscheck_plain.argtypes = [P, P, i, i, i, P]
     lib.orc_crosscheck_ref.argtypes = [P, P, i, i, i, i, i, P, P, P]
+    lib.orc_bm1.argtypes = [P, P, i, i, i, i, i, i, i, i, C.c_float, C.c_float, C.c_float, P, i, i, P]
     lib.orc_mip_level0_ref.argtypes = [P, i, i, P]
     lib.orc_mip_level0_plain.argtypes = [P, i, i, P]
     lib.orc_mip_next.argtypes = [P, i, i, C.c_float, P]
     lib.orc_disp_denoise.argtypes = [i, P, i, i, i, C.c_float, C.c_float, i, P]
     for name in ["orc_census", "orc_census3x3_ref", "orc_bm", "orc_bm_brute", "orc_ncc", "orc_aw",
                  "orc_crosscheck_plain", "orc_crosscheck_ref", "orc_mip_level0_ref", "orc_mip_level0_plain",
-                 "orc_mip_next", "orc_disp_denoise"]:
+                 "orc_mip_next", "orc_disp_denoise", "orc_bm1"]:
         getattr(lib, name).restype = i
     return lib
 
@@ -279,4 +280,25 @@ def disp_denoise(disp, P=5, support_ratio=0.5, mean_cutoff=1.0, fill=True):
     out = np.empty((H, W), np.uint16 if ref else np.int16)
     _chk(lib.orc_disp_denoise(1 if ref else 0, _p(np.ascontiguousarray(disp, np.uint16 if ref else np.int16)), W, H, P,
                               support_ratio, mean_cutoff, 1 if fill else 0, _p(out)), "orc_disp_denoise")
+    return out
+
+
+def bm1(L, R, P=9, sim_type=0, dmin=0, dmax=64, offset=0, ltor=True, min_delta=0.0, sim_thresh=0.0,
+        grad_thresh=0.0, init=None, fill=0):
+    """BlockMatcher1 emulated block by block: (H, W, 4) uint16 pixels (EDisparityId,
+    similarity max, prev delta, next delta).  Pixels the reference never writes keep
+    `fill` (every channel)."""
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    out = np.full((H, W, 4), fill, np.uint16)
+    if init is not None:
+        init = np.ascontiguousarray(init, np.uint16)
+        iH, iW = init.shape[:2]
+        ip = _p(init)
+    else:
+        iH = iW = 0
+        ip = None
+    _chk(lib.orc_bm1(_p(L), _p(R), W, H, P, sim_type, dmin, dmax, offset, 1 if ltor else 0, min_delta, sim_thresh,
+                     grad_thresh, ip, iW, iH, _p(out)), "orc_bm1")
     return out

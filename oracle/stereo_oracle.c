@@ -609,3 +609,234 @@ int orc_disp_denoise(int ref_format, const void* in, int W, int H, int P, float 
     #undef ORC_DN_VAL
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* BlockMatcher1, emulated block by block (CluTec.Cuda.MultiView/BlockMatcher1.cu).      */
+/*                                                                                        */
+/* The reference launches one 32-thread block per PCX x 4 result origins                 */
+/* (AlgoArraySum<32, 2, P, P>, Kernel.Algo.ArraySum.h:38-60: PCX = ((32 - P + 1) / 8) * 8), */
+/* blocks counted by EvalThreadConfig_ReadDepBlockSize (KernelDriverBase.cpp:166-197)     */
+/* from a 33 x (P + 3) read area.  Which disparities a result sees, and where its test    */
+/* patch lies, is decided per block; this function follows those decisions.              */
+
+/* Similarity of two pixels (EvalSim, BlockMatcher1.cu:184-221). */
+static inline int bm1_sim(int sim_type, int a, int b)
+{
+    if (sim_type == 0) return (255 - abs(a - b)) >> 1; /* AbsoluteDifference */
+    float f1 = 2.0f * ((float)a / 255.0f - 0.5f);      /* SignedProduct */
+    const float f2 = 2.0f * ((float)b / 255.0f - 0.5f);
+    f1 *= f2;
+    f1 = f1 * 127.0f;
+    return (int)f1;
+}
+
+/* SimValueToFloat (:155-159) and SimFloatToInteger (:172-176) */
+static inline float bm1_sim_float(int v, int P) { return ((float)v / (float)(P * P)) * (100.0f / 127.0f); }
+static inline uint16_t bm1_sim_u16(float f)
+{
+    float t = (f / 100.0f + 1.0f) / 2.0f;
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    return (uint16_t)(t * 65535.0f);
+}
+
+typedef struct {
+    int16_t prev2, prev, max, next, next2, last;
+} bm1_store;
+
+/* One disparity of one result (the update lambda of EvalSim, :262-300). */
+static inline void bm1_update(bm1_store* s, uint16_t* disp, int v, int d)
+{
+    const int above = v > s->max;
+    const int behind1 = !above && (int)*disp + 1 == d + ORC_DISP_FIRST;
+    const int behind2 = !above && (int)*disp + 2 == d + ORC_DISP_FIRST;
+    if (above) {
+        s->prev2 = s->prev; /* the value before the previous maximum, not the one 2 before d */
+        s->prev = s->last;
+        s->max = (int16_t)v;
+        s->next = (int16_t)v;
+        s->next2 = (int16_t)v;
+        *disp = (uint16_t)(d + ORC_DISP_FIRST);
+    }
+    if (behind1) s->next = (int16_t)v;
+    if (behind2) s->next2 = (int16_t)v;
+    s->last = (int16_t)v;
+}
+
+int orc_bm1(const uint8_t* L, const uint8_t* R, int W, int H, int P, int sim_type, int dmin, int dmax, int offset,
+            int ltor, float min_delta, float sim_thresh, float grad_thresh, const uint16_t* init, int iW, int iH,
+            uint16_t* out)
+{
+    if (!L || !R || !out || (P != 5 && P != 9 && P != 11 && P != 15) || W < 48 || H < P + 3 || dmin < 0 ||
+        dmax <= dmin || (init && (iW < 1 || iH < 1 || W / iW < 1)))
+        return -1;
+    const int CG = (32 - P + 1) / 8, PCX = CG * 8, PCY = 4, TBX = 33, TBY = P + 3, SDC = 16;
+    const unsigned effW = (unsigned)W - (unsigned)(TBX - PCX), effH = (unsigned)H - (unsigned)(TBY - PCY);
+    const int nbx = (int)(effW / PCX + (effW % PCX > 0)), nby = (int)(effH / PCY + (effH % PCY > 0));
+    const int NR = PCX * PCY;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int by = 0; by < nby; ++by) {
+        bm1_store st[24 * 4];
+        uint16_t disp[24 * 4];
+        for (int bx = 0; bx < nbx; ++bx) {
+            const int X0 = bx * PCX, Y0 = by * PCY;
+            if (!(X0 + P <= W && Y0 + P <= H)) continue; /* :626-630 */
+            int any_eval = 0;
+            for (int k = 0; k < NR; ++k) {
+                st[k] = (bm1_store){0, 0, -32767, 0, 0, 0};
+                disp[k] = ORC_DISP_UNKNOWN;
+            }
+            int dlo = ORC_DISP_LAST, dhi = ORC_DISP_FIRST;
+            if (init) {
+                /* MatcherUseDisparity (:912-963).  Thread t holds the results
+                   ((t / 4) * CG + r, t % 4), r < CG.  A written result takes the initial id at
+                   its centre over the scale (positions outside the initial map read as
+                   Unknown): an id up to Last scaled (16-bit), a code above Last as it is.  The
+                   thread's plain min / max of the scaled values (from Last / First) are then
+                   reduced over the warp with MinDisp / MaxDisp, which drop invalid
+                   operands: the result is the min / max over the threads whose own value is
+                   valid. */
+                const int scale = W / iW;
+                int red_lo = -1, red_hi = -1;
+                for (int t = 0; t < 32; ++t) {
+                    int tlo = ORC_DISP_LAST, thi = ORC_DISP_FIRST;
+                    for (int r = 0; r < CG; ++r) {
+                        const int ix = (t / 4) * CG + r, iy = t % 4, k = iy * PCX + ix;
+                        if (!(X0 + ix + P < W && Y0 + iy + P < H)) continue;
+                        const int px = (X0 + ix + P / 2) / scale, py = (Y0 + iy + P / 2) / scale;
+                        const uint16_t id = (px < iW && py < iH) ? init[4 * ((size_t)py * iW + px)] : ORC_DISP_UNKNOWN;
+                        if (id <= ORC_DISP_LAST) {
+                            const uint16_t v = (uint16_t)(scale * id);
+                            if (v < tlo) tlo = v;
+                            if (v > thi) thi = v;
+                            disp[k] = v;
+                        } else {
+                            disp[k] = id;
+                        }
+                    }
+                    const int vlo = tlo >= ORC_DISP_FIRST && tlo <= ORC_DISP_LAST;
+                    const int vhi = thi >= ORC_DISP_FIRST && thi <= ORC_DISP_LAST;
+                    if (vlo) red_lo = red_lo < 0 ? tlo : (tlo < red_lo ? tlo : red_lo);
+                    if (vhi) red_hi = red_hi < 0 ? thi : (thi > red_hi ? thi : red_hi);
+                }
+                dlo = red_lo; /* -1: no thread holds a valid value */
+                dhi = red_hi;
+            }
+            for (int k = 0; k < NR && !init; ++k) {
+                const int ix = k % PCX, iy = k / PCX;
+                /* EvalGrad (:325-361): the standard deviation of the left patch */
+                int64_t s = 0, s2 = 0;
+                for (int j = 0; j < P; ++j)
+                    for (int i = 0; i < P; ++i) {
+                        const int x = X0 + ix + i, y = Y0 + iy + j;
+                        const int v = (x < W && y < H) ? L[(size_t)y * W + x] : 0;
+                        s += v;
+                        s2 += v * v;
+                    }
+                const float mean = (float)s / (float)(P * P);
+                const float mean2 = (float)s2 / (float)(P * P);
+                const float sq = mean * mean;
+                const float sd = sqrtf(fabsf(mean2 - sq)) / 255.0f * 100.0f;
+                if (sd < grad_thresh) disp[k] = ORC_DISP_CANNOT_EVALUATE;
+                else any_eval = 1;
+            }
+            int d_start, d_end_excl, lastgroup_cap;
+            if (init) {
+                /* the block's range of valid scaled values (:955-963, the warp's shuffle
+                   reduction), else every result keeps its initial code (:967-999) */
+                if (dlo < 0 || dhi < 0 || dlo > dhi) {
+                    for (int k = 0; k < NR; ++k) {
+                        const int ix = k % PCX, iy = k / PCX;
+                        if (X0 + ix + P < W && Y0 + iy + P < H) {
+                            uint16_t* o = out + 4 * ((size_t)(Y0 + iy + P / 2) * W + X0 + ix + P / 2);
+                            o[0] = disp[k];
+                            o[1] = o[2] = o[3] = 0;
+                        }
+                    }
+                    continue;
+                }
+                d_start = dlo < 4 ? 0 : dlo - 4;
+                d_end_excl = dhi + 4 + 1; /* iDispIdx <= iDispEnd */
+                lastgroup_cap = d_end_excl - d_start < SDC ? d_end_excl - d_start : SDC; /* :1088 */
+                /* the loop variables are 16-bit: an end beyond 32767 evaluates nothing */
+                any_eval = dhi + 4 <= 32767;
+            } else {
+                const int maxd = ltor ? X0 - offset + 1 : W - X0 - offset; /* MaxDisparity */
+                d_start = dmin;
+                d_end_excl = dmax < maxd ? dmax : maxd;
+                lastgroup_cap = SDC;
+            }
+            if (any_eval) {
+                for (int D0 = d_start; D0 < d_end_excl; D0 += SDC) {
+                    const int tx = (ltor ? X0 - D0 - offset : X0 + D0 + offset) - ltor * (SDC - 1);
+                    int n, base;
+                    if (tx >= 0 && Y0 >= 0 && tx + TBX + SDC - 1 <= W && Y0 + TBY <= H) {
+                        n = lastgroup_cap;
+                        base = tx + ltor * (SDC - 1);
+                    } else {
+                        if (!(Y0 >= 0 && Y0 + TBY <= H)) continue;
+                        int dx, sx;
+                        if (tx >= 0) {
+                            dx = W - tx - TBX;
+                            sx = tx;
+                        } else {
+                            dx = tx + (SDC - 1);
+                            sx = 0;
+                        }
+                        if (dx < 0) continue;
+                        n = dx + 1;
+                        base = sx + ltor * (n - 1);
+                    }
+                    for (int r = 0; r < n; ++r) {
+                        const int d = D0 + r, t0 = ltor ? base - r : base + r;
+                        for (int k = 0; k < NR; ++k) {
+                            if (disp[k] > ORC_DISP_LAST) continue;
+                            const int ix = k % PCX, iy = k / PCX;
+                            int v = 0;
+                            for (int j = 0; j < P; ++j) {
+                                const int y = Y0 + iy + j;
+                                for (int i = 0; i < P; ++i) {
+                                    const int x = X0 + ix + i;
+                                    const int a = x < W ? L[(size_t)y * W + x] : 0;
+                                    v += bm1_sim(sim_type, a, R[(size_t)y * W + t0 + ix + i]);
+                                }
+                            }
+                            bm1_update(&st[k], &disp[k], (int16_t)v, d);
+                        }
+                    }
+                }
+            }
+            /* WriteDisparity (:447-575) */
+            for (int k = 0; k < NR; ++k) {
+                const int ix = k % PCX, iy = k / PCX;
+                if (!(X0 + ix + P < W && Y0 + iy + P < H)) continue;
+                uint16_t* o = out + 4 * ((size_t)(Y0 + iy + P / 2) * W + X0 + ix + P / 2);
+                const int dv = disp[k];
+                if (dv > ORC_DISP_LAST || dv == ORC_DISP_UNKNOWN) {
+                    o[0] = (uint16_t)dv;
+                    o[1] = o[2] = o[3] = 0;
+                    continue;
+                }
+                const bm1_store* s = &st[k];
+                const float fmax = bm1_sim_float(s->max, P);
+                const float fp2 = bm1_sim_float(s->prev2, P), fp = bm1_sim_float(s->prev, P);
+                const float fn = bm1_sim_float(s->next, P), fn2 = bm1_sim_float(s->next2, P);
+                const float dprev = fmaxf(fabsf(fp - fmax), fabsf(fp2 - fmax));
+                const float dnext = fmaxf(fabsf(fn - fmax), fabsf(fn2 - fmax));
+                const int specific = dprev >= min_delta && dnext >= min_delta;
+                const int above = fmax >= sim_thresh;
+                uint16_t code;
+                if (above) {
+                    const int sat = dv <= ORC_DISP_FIRST + dmin + 1 || dv >= ORC_DISP_FIRST + dmax - 1;
+                    code = sat ? ORC_DISP_SATURATED : (specific ? (uint16_t)dv : ORC_DISP_NOT_SPECIFIC);
+                } else {
+                    code = ORC_DISP_NOT_FOUND;
+                }
+                o[0] = code;
+                o[1] = bm1_sim_u16(fmax);
+                o[2] = bm1_sim_u16(dprev);
+                o[3] = bm1_sim_u16(dnext);
+            }
+        }
+    }
+    return 0;
+}

@@ -97,6 +97,17 @@ int orc_mip_next(const float* src, int W, int H, float unknown, float* out);
 int orc_disp_denoise(int ref_format, const void* in, int W, int H, int P, float support_ratio, float mean_cutoff,
                      int fill, void* out);
 
+/* BlockMatcher1 (BlockMatcher1.cu:590-847, and MatcherUseDisparity :857-1203 when init is
+ * given), emulating the reference's block tiling: 32-thread blocks of PCX x 4 result
+ * origins, P in {5, 9, 11, 15}, sim_type 0 (AbsoluteDifference) or 1 (SignedProduct),
+ * disparities dmin <= d < dmax with offset, ltor 1 for left to right.  out is W*H*4
+ * uint16 (EDisparityId, similarity max, prev delta, next delta); pixels the reference
+ * does not write are left as they are.  init: an iW x iH map of 4 x uint16 ids with
+ * W / iW >= 1, or NULL.  Needs W >= 48 and H >= P + 3. */
+int orc_bm1(const uint8_t* L, const uint8_t* R, int W, int H, int P, int sim_type, int dmin, int dmax, int offset,
+            int ltor, float min_delta, float sim_thresh, float grad_thresh, const uint16_t* init, int iW, int iH,
+            uint16_t* out);
+
 #ifdef __cplusplus
 }
 #endif

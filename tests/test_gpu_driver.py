@@ -41,3 +41,18 @@ def test_stereo_run_matches_oracle(tmp_path, algo, W, H, D):
     ids = np.fromfile(prefix + "_checkid.u16x4", np.uint16).reshape(H, W, 4)
     den = np.fromfile(prefix + "_denoised.u16", np.uint16).reshape(H, W)
     assert np.array_equal(den, orc.disp_denoise(ids, P=5))
+
+
+def test_stereo_run_bm1_matches_emulation(tmp_path):
+    """BlockMatcher1::CDriver through the C++ layer (9x9, AbsoluteDifference, left to right)."""
+    W, H, D, seed = 320, 160, 64, 7
+    prefix = str(tmp_path / "bm1")
+    print(f"stereo_run bm1 {W}x{H} D={D}", flush=True)
+    r = subprocess.run([EXE, "bm1", str(W), str(H), str(D), str(seed), prefix, "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(prefix + "_bm1.u16x4", np.uint16).reshape(H, W, 4)
+    L, R = cs.synth_pair(seed, W, H, 0, 8, D - 1, min(8, D - 1))
+    want = orc.bm1(L, R, P=9, sim_type=0, dmin=0, dmax=D, offset=0, ltor=True, min_delta=1.0, sim_thresh=60.0,
+                   grad_thresh=2.0, fill=0x5A5A)
+    assert np.array_equal(got, want)
