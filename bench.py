@@ -292,8 +292,9 @@ def bm1_leg(cs, np, ctx, steps):
 
 
 def post_filters_leg(cs, np, ctx, disp, steps):
-    """The filters on the 1080p matcher map, resident in HBM: de-noise (5x5 window, plain
-    int16 map and the 4 x uint16 id map) and five disparity mip-map levels.  Algorithmic
+    """The steps around the matcher at 1080p, inputs resident in HBM: de-noise of the
+    census map (5x5 window, plain int16 map and the 4 x uint16 id map), five disparity
+    mip-map levels, rectification of the left image and colouring of the id map.  Algorithmic
     bytes: every input pixel read once and every output pixel written once.  ms_per_frame
     is wall-clock per back-to-back call from Python, which for kernels this short is
     bound by the host call; the kernels' own durations come from a kernel trace."""
@@ -317,6 +318,22 @@ def post_filters_leg(cs, np, ctx, disp, steps):
     arr_t = (cs.Image * 5)(*[lv.img for lv in levels])
     cases.append(("mipmaps5_i16", lambda: cs._check(cs.lib.stereo_disp_mipmaps(ctx.handle, src.ref, arr_t, 5, None), ctx),
                   nbytes))
+    # rectification of the left image: a 1.15 degree rotation about the optical axis and
+    # radial distortion (k1 = -0.05), source and target 1920x1080
+    L = cs.synth_pair(1, W, H, 0, 24, 255, 8)[0]
+    c, sn = 0.9998, 0.0199987
+    params = [0.8 * W, 0.8 * W, 0.5 * (W - 1), 0.5 * (H - 1), -0.05, 0, 0, 0, 0,
+              0.8 * W, 0.8 * W, 0.5 * (W - 1), 0.5 * (H - 1), c, -sn, 0, sn, c, 0, 0, 0, 1]
+    rect = cs.Rectify(ctx, W, H, params)
+    rsrc = cs.DeviceImage.from_array(ctx, L, cs.PIXEL_LUM_U8)
+    rdst = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_U8)
+    cases.append(("rectify_1080p", lambda: rect.process(rdst, rsrc), W * H * 2))
+    # colouring of the id map: Analysis (palette) and Color (ramp) into RGBA8
+    col = cs.ColorDisparity(ctx, W, H, cs.PIXEL_RGBA_U16, 1, D + 1)
+    csrc = cs.DeviceImage.from_array(ctx, ids, cs.PIXEL_RGBA_U16)
+    cdst = cs.DeviceImage(ctx, W, H, cs.PIXEL_RGBA_U8)
+    cases.append(("color_analysis_ids", lambda: col.process(cdst, csrc, cs.COLOR_ANALYSIS), W * H * (8 + 4)))
+    cases.append(("color_ramp_ids", lambda: col.process(cdst, csrc, cs.COLOR_COLOR), W * H * (8 + 4)))
     for name, fn, nb in cases:
         for _ in range(3):
             fn()

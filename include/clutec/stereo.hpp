@@ -50,6 +50,7 @@ enum class EPixelType : int32_t
     Lum_UInt64 = STEREO_PIXEL_LUM_U64,
     Lum_Single = STEREO_PIXEL_LUM_F32,
     RGBA_UInt16 = STEREO_PIXEL_RGBA_U16,
+    RGBA_UInt8 = STEREO_PIXEL_RGBA_U8,
 };
 
 // Clu::SImageFormat subset: width, height, pixel type.
@@ -564,6 +565,100 @@ public:
     }
 };
 }  // namespace CrossCheckDisparityMaps
+
+// RectifyImage (RectifyImage.h:40-90).  The reference's parameters are CluTec.Core camera
+// objects (_CPinhole, _CDistortion<OpenCV>, _CFrame3D), which are not part of the
+// reference; _SParameter holds the same model as numbers: the source camera with its
+// OpenCV distortion, the rectified camera, and the rotation from the rectified to the
+// source camera (row-major).
+namespace RectifyImage
+{
+struct _SParameter
+{
+    float fSrcFx = 1.0f, fSrcFy = 1.0f, fSrcCx = 0.0f, fSrcCy = 0.0f;
+    float fK1 = 0.0f, fK2 = 0.0f, fP1 = 0.0f, fP2 = 0.0f, fK3 = 0.0f;
+    float fRectFx = 1.0f, fRectFy = 1.0f, fRectCx = 0.0f, fRectCy = 0.0f;
+    float mRot[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+};
+class CDriver : public CKernelDriverBase
+{
+public:
+    CDriver() : CKernelDriverBase("Rectify Image") {}
+    // fmt: the rectified (target) image
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const _SParameter& p)
+    {
+        const stereo_image_format f = fmt.Raw();
+        stereo_rectify_params rp{p.fSrcFx, p.fSrcFy, p.fSrcCx, p.fSrcCy, p.fK1, p.fK2, p.fP1, p.fP2, p.fK3,
+                                 p.fRectFx, p.fRectFy, p.fRectCx, p.fRectCy, {}, {0, 0, 0, 0}};
+        for (int i = 0; i < 9; ++i) rp.R[i] = p.mRot[i];
+        stereo_driver* d = nullptr;
+        EnsureOK(stereo_rectify_configure(dev.Context(), &d, &f, &rp), dev.Context(), "RectifyImage::Configure");
+        Reset(dev.Context(), d);
+    }
+    void Process(CDeviceImage& target, const CDeviceImage& source)
+    {
+        EnsureOK(stereo_rectify_process(m_drv, target.Raw(), source.Raw(), nullptr), m_ctx, "RectifyImage::Process");
+    }
+};
+}  // namespace RectifyImage
+
+// ColorDisparityImage (ColorDisparityImage.h:40-125).
+namespace ColorDisparityImage
+{
+enum class EStyle
+{
+    Analysis = STEREO_COLOR_ANALYSIS,
+    Gray = STEREO_COLOR_GRAY,
+    Color = STEREO_COLOR_COLOR,
+    DispAbs = STEREO_COLOR_DISP_ABS,
+    DispRel = STEREO_COLOR_DISP_REL,
+};
+struct SParameter
+{
+    uint16_t uDispMin = (uint16_t)EDisparityId::First, uDispMax = (uint16_t)EDisparityId::Last;
+    uint16_t uDispInfOffset = 0;
+    uint8_t pixUnknown[4] = {10, 100, 180, 255};
+    uint8_t pixCannotEvaluate[4] = {0, 150, 50, 255};
+    uint8_t pixSaturated[4] = {0, 200, 200, 255};
+    uint8_t pixNotSpecific[4] = {200, 0, 200, 255};
+    uint8_t pixNotFound[4] = {200, 50, 50, 255};
+    uint8_t pixNotUnique[4] = {240, 50, 150, 255};
+    uint8_t pixInconsistent[4] = {100, 10, 10, 255};
+    void Set(uint16_t dispMin, uint16_t dispMax, uint16_t dispInfOffset)
+    {
+        uDispMin = dispMin;
+        uDispMax = dispMax;
+        uDispInfOffset = dispInfOffset;
+    }
+};
+class CDriver : public CKernelDriverBase
+{
+public:
+    CDriver() : CKernelDriverBase("Color Disparity Image") {}
+    // fmt: the disparity image (RGBA_UInt16, Lum_UInt16 or Lum_Single ids)
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    {
+        const stereo_image_format f = fmt.Raw();
+        stereo_color_params cp{};
+        cp.disp_min = p.uDispMin;
+        cp.disp_max = p.uDispMax;
+        cp.disp_inf_offset = p.uDispInfOffset;
+        const uint8_t* pal[7] = {p.pixUnknown, p.pixCannotEvaluate, p.pixSaturated, p.pixNotSpecific,
+                                 p.pixNotFound, p.pixNotUnique, p.pixInconsistent};
+        for (int k = 0; k < 7; ++k)
+            for (int c = 0; c < 4; ++c) cp.colors[k][c] = pal[k][c];
+        stereo_driver* d = nullptr;
+        EnsureOK(stereo_color_configure(dev.Context(), &d, &f, &cp), dev.Context(), "ColorDisparityImage::Configure");
+        Reset(dev.Context(), d);
+    }
+    // color: RGBA_UInt8 for Analysis and Color, Lum_UInt16 for the other styles
+    void Process(CDeviceImage& color, const CDeviceImage& disp, EStyle style)
+    {
+        EnsureOK(stereo_color_process(m_drv, color.Raw(), disp.Raw(), (int32_t)style, nullptr), m_ctx,
+                 "ColorDisparityImage::Process");
+    }
+};
+}  // namespace ColorDisparityImage
 
 // Statistics::DispDenoise (Statistics.DispDenoise.h:45-190): the input is an RGBA_U16
 // EDisparityId map (uint16 ids out, LUM_U16) or a LUM_I16 map (LUM_I16 out).

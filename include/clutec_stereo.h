@@ -47,7 +47,8 @@ typedef enum stereo_pixel_type {
     STEREO_PIXEL_LUM_U32 = 4,
     STEREO_PIXEL_LUM_U64 = 5,   /* wide census descriptors */
     STEREO_PIXEL_LUM_F32 = 6,   /* NCC score maps */
-    STEREO_PIXEL_RGBA_U16 = 7   /* _SDisparityConfig::TPixelEx: 4 x uint16, EDisparityId in channel 0 */
+    STEREO_PIXEL_RGBA_U16 = 7,  /* _SDisparityConfig::TPixelEx: 4 x uint16, EDisparityId in channel 0 */
+    STEREO_PIXEL_RGBA_U8 = 8    /* TPixel_RGBA_UInt8: colour images (r, g, b, a bytes) */
 } stereo_pixel_type;
 
 /* Image format (Clu::SImageFormat: width, height, pixel type). */
@@ -398,6 +399,59 @@ stereo_status stereo_bm1_configure(stereo_context* ctx, stereo_driver** drv, con
  * its width; positions outside it read as Unknown), required iff use_disp_input. */
 stereo_status stereo_bm1_process(stereo_driver* drv, stereo_image* disp, const stereo_image* left,
                                  const stereo_image* right, const stereo_image* disp_init, stereo_stream stream);
+
+/* ---- rectification (RectifyImage.h:40-90) ------------------------------------------- */
+
+/* The camera model of the rectification: the rectified camera's pixel becomes a ray,
+ * rotated into the source camera (R, row-major, rectified to source), projected, distorted
+ * with the OpenCV model and mapped through the source camera.  It stands in for the
+ * reference's _CPinhole / _CDistortion<OpenCV> / _CFrame3D parameters, whose library is
+ * not part of the reference. */
+typedef struct stereo_rectify_params {
+    float src_fx, src_fy, src_cx, src_cy;      /* source camera, pixels */
+    float k1, k2, p1, p2, k3;                  /* its OpenCV distortion */
+    float rect_fx, rect_fy, rect_cx, rect_cy;  /* rectified camera, pixels */
+    float R[9];
+    int32_t reserved[4];                       /* must be 0 */
+} stereo_rectify_params;
+
+/* CDriver::Configure (RectifyImage.cu:166-203): fmt is the rectified (target) LUM_U8 image. */
+stereo_status stereo_rectify_configure(stereo_context* ctx, stereo_driver** drv, const stereo_image_format* fmt,
+                                       const stereo_rectify_params* params);
+/* CDriver::Process (RectifyImage.cu:205-232, RectifyUndistort :140-159): per target pixel,
+ * bilinear over the 2x2 source block at the floor of its source position when the block is
+ * inside (values / 255, result rounded), else the nearest source pixel when inside, else 0.
+ * The source is LUM_U8 of any size. */
+stereo_status stereo_rectify_process(stereo_driver* drv, stereo_image* target, const stereo_image* source,
+                                     stereo_stream stream);
+
+/* ---- disparity colouring (ColorDisparityImage.h:40-125) ------------------------------ */
+
+enum {
+    STEREO_COLOR_ANALYSIS = 0, /* RGBA_U8: grey ids, palette for the codes */
+    STEREO_COLOR_GRAY = 1,     /* LUM_U16 */
+    STEREO_COLOR_COLOR = 2,    /* RGBA_U8: colour ramp */
+    STEREO_COLOR_DISP_ABS = 3, /* LUM_U16: the id */
+    STEREO_COLOR_DISP_REL = 4  /* LUM_U16: (0x7FFF + inf offset) - (id - First) */
+};
+
+typedef struct stereo_color_params {
+    uint16_t disp_min, disp_max;  /* uDispMin, uDispMax: the ids mapped to 0 and 1 */
+    uint16_t disp_inf_offset;     /* uDispInfOffset (DispRel) */
+    uint16_t pad;
+    /* RGBA for Unknown, CannotEvaluate, Saturated, NotSpecific, NotFound, NotUnique,
+     * Inconsistent (SParameter defaults, ColorDisparityImage.h:76-84) */
+    uint8_t colors[7][4];
+    int32_t reserved[4];          /* must be 0 */
+} stereo_color_params;
+
+/* CDriver::Configure: fmt is the disparity image, RGBA_U16 (EDisparityId), LUM_U16 or LUM_F32. */
+stereo_status stereo_color_configure(stereo_context* ctx, stereo_driver** drv, const stereo_image_format* fmt,
+                                     const stereo_color_params* params);
+/* CDriver::Process (ColorDisparityImage.cu:330-410) for one STEREO_COLOR_* style; out is
+ * RGBA_U8 (Analysis, Color) or LUM_U16 (Gray, DispAbs, DispRel) of the same size. */
+stereo_status stereo_color_process(stereo_driver* drv, stereo_image* out, const stereo_image* disp, int32_t style,
+                                   stereo_stream stream);
 
 /* ---- disparity de-noise (Statistics::DispDenoise, Statistics.DispDenoise.h:40-190) ---- */
 

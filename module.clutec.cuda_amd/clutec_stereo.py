@@ -36,6 +36,7 @@ PIXEL_LUM_U32 = 4
 PIXEL_LUM_U64 = 5
 PIXEL_LUM_F32 = 6
 PIXEL_RGBA_U16 = 7
+PIXEL_RGBA_U8 = 8
 
 ALGO_SAD = 0
 ALGO_CENSUS = 1
@@ -51,6 +52,7 @@ _NP_OF = {
     PIXEL_LUM_U64: (np.uint64, 1),
     PIXEL_LUM_F32: (np.float32, 1),
     PIXEL_RGBA_U16: (np.uint16, 4),
+    PIXEL_RGBA_U8: (np.uint8, 4),
 }
 
 
@@ -82,6 +84,24 @@ class Bm1Params(C.Structure):
                 ("disp_offset", C.c_int32), ("min_delta_thresh", C.c_float), ("sim_thresh", C.c_float),
                 ("grad_thresh", C.c_float), ("use_disp_input", C.c_int32), ("left_to_right", C.c_int32),
                 ("reserved", C.c_int32 * 4)]
+
+
+class RectifyParams(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("src_fx", "src_fy", "src_cx", "src_cy", "k1", "k2", "p1", "p2", "k3",
+                                         "rect_fx", "rect_fy", "rect_cx", "rect_cy")] + \
+               [("R", C.c_float * 9), ("reserved", C.c_int32 * 4)]
+
+
+class ColorParams(C.Structure):
+    _fields_ = [("disp_min", C.c_uint16), ("disp_max", C.c_uint16), ("disp_inf_offset", C.c_uint16),
+                ("pad", C.c_uint16), ("colors", (C.c_uint8 * 4) * 7), ("reserved", C.c_int32 * 4)]
+
+
+COLOR_ANALYSIS, COLOR_GRAY, COLOR_COLOR, COLOR_DISP_ABS, COLOR_DISP_REL = range(5)
+# ColorDisparityImage's default palette: Unknown, CannotEvaluate, Saturated, NotSpecific,
+# NotFound, NotUnique, Inconsistent
+DEFAULT_COLORS = ((10, 100, 180, 255), (0, 150, 50, 255), (0, 200, 200, 255), (200, 0, 200, 255),
+                  (200, 50, 50, 255), (240, 50, 150, 255), (100, 10, 10, 255))
 
 
 class DenoiseParams(C.Structure):
@@ -139,6 +159,10 @@ _PROTOS = [
     ("stereo_disp_mipmaps", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.c_int32, _P]),
     ("stereo_denoise_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(DenoiseParams)]),
     ("stereo_bm1_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(Bm1Params)]),
+    ("stereo_rectify_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(RectifyParams)]),
+    ("stereo_rectify_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), _P]),
+    ("stereo_color_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(ColorParams)]),
+    ("stereo_color_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.c_int32, _P]),
     ("stereo_bm1_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_denoise_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_scores", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
@@ -492,6 +516,54 @@ class BlockMatcher1(_Driver):
         out = DeviceImage.from_array(self.ctx, np.full((self.H, self.W, 4), fill, np.uint16), PIXEL_RGBA_U16)
         di = DeviceImage.from_array(self.ctx, init, PIXEL_RGBA_U16) if init is not None else None
         self.process(out, dl, dr, di)
+        return out.download()
+
+
+class Rectify(_Driver):
+    """RectifyImage (stereo_rectify_*): params is the 22-value camera model (source fx,
+    fy, cx, cy; k1, k2, p1, p2, k3; rectified fx, fy, cx, cy; R row-major)."""
+
+    def __init__(self, ctx, W, H, params):
+        super().__init__(ctx)
+        self.W, self.H = W, H
+        p = RectifyParams(*[float(v) for v in params[:13]])
+        for i in range(9):
+            p.R[i] = float(params[13 + i])
+        fmt = ImageFormat(W, H, PIXEL_LUM_U8)
+        _check(lib.stereo_rectify_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(p)), ctx)
+
+    def process(self, target, source, stream=None):
+        _check(lib.stereo_rectify_process(self.handle, target.ref, source.ref, stream), self.ctx)
+
+    def run(self, src):
+        s = DeviceImage.from_array(self.ctx, src, PIXEL_LUM_U8)
+        out = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_U8)
+        self.process(out, s)
+        return out.download()
+
+
+class ColorDisparity(_Driver):
+    """ColorDisparityImage (stereo_color_*): disparity images of RGBA_U16, LUM_U16 or
+    LUM_F32 ids to RGBA_U8 (Analysis, Color) or LUM_U16 (Gray, DispAbs, DispRel)."""
+
+    def __init__(self, ctx, W, H, pixel_type=PIXEL_RGBA_U16, disp_min=1, disp_max=0xFFEF, inf_offset=0,
+                 colors=DEFAULT_COLORS):
+        super().__init__(ctx)
+        self.W, self.H, self.pixel_type = W, H, pixel_type
+        p = ColorParams(disp_min, disp_max, inf_offset, 0)
+        for k in range(7):
+            for c in range(4):
+                p.colors[k][c] = int(colors[k][c])
+        fmt = ImageFormat(W, H, pixel_type)
+        _check(lib.stereo_color_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(p)), ctx)
+
+    def process(self, out, disp, style, stream=None):
+        _check(lib.stereo_color_process(self.handle, out.ref, disp.ref, style, stream), self.ctx)
+
+    def run(self, disp, style):
+        d = DeviceImage.from_array(self.ctx, disp, self.pixel_type)
+        out = DeviceImage(self.ctx, self.W, self.H, PIXEL_RGBA_U8 if style in (0, 2) else PIXEL_LUM_U16)
+        self.process(out, d, style)
         return out.download()
 
 

@@ -91,6 +91,7 @@ size_t stereo_pixel_size(int32_t t)
     case STEREO_PIXEL_LUM_U64: return 8;
     case STEREO_PIXEL_LUM_F32: return 4;
     case STEREO_PIXEL_RGBA_U16: return 8;
+    case STEREO_PIXEL_RGBA_U8: return 4;
     default: return 0;
     }
 }

@@ -3,12 +3,14 @@
 //   1. generate a synthetic pair, 2. upload into device images, 3. Configure the
 //   drivers, 4. ProcessKernel: matcher (census or SAD; L->R, R->L and check fused)
 //   and, from the two one-sided maps in disparity-id form, the reference-style
-//   cross-check, then Statistics::DispDenoise over the checked id map, 5. download,
-//   6. write the maps and report milliseconds.
+//   cross-check, then Statistics::DispDenoise and ColorDisparityImage over the checked id
+//   map and RectifyImage of the left image, 5. download, 6. write the maps and report
+//   milliseconds.
 //
 // usage: stereo_run <algo:sad|census|ncc|aw1|aw2|bm1> <W> <H> <D> <seed> <out_prefix> [runs]
 // Writes <out_prefix>_checked.i16, _lr.i16, _rl.i16 (row-major int16, -1 invalid),
-// _checkid.u16x4 (the cross-check's id map) and _denoised.u16 (its de-noised ids)
+// _checkid.u16x4 (the cross-check's id map), _denoised.u16 (its de-noised ids),
+// _color.rgba (its Analysis colouring) and _rectified.u8 (the rectified left image)
 // and prints one line "ms <median> valid <fraction> range <min> <max>".
 #include <algorithm>
 #include <cstdio>
@@ -149,13 +151,39 @@ int main(int argc, char** argv)
         dchk.CopyInto(hid.data(), (size_t)W * 8);
         dden.CopyInto(hden.data(), (size_t)W * 2);
 
+        // ColorDisparityImage (Analysis style) of the checked id map, and RectifyImage of the
+        // left image by a small rotation about the optical axis
+        ColorDisparityImage::SParameter colp;
+        colp.Set(1, (uint16_t)(D + 1), 0);
+        ColorDisparityImage::CDriver col;
+        col.Configure(dev, fmtId, colp);
+        CDeviceImage dcol(dev, SImageFormat(W, H, EPixelType::RGBA_UInt8));
+        ProcessKernel(col, dcol, dchk, ColorDisparityImage::EStyle::Analysis);
+        std::vector<uint8_t> hcol((size_t)W * H * 4), hrect((size_t)W * H);
+        dcol.CopyInto(hcol.data(), (size_t)W * 4);
+        RectifyImage::_SParameter rp;
+        rp.fSrcFx = rp.fSrcFy = rp.fRectFx = rp.fRectFy = 0.8f * (float)W;
+        rp.fSrcCx = rp.fRectCx = 0.5f * (float)(W - 1);
+        rp.fSrcCy = rp.fRectCy = 0.5f * (float)(H - 1);
+        rp.fK1 = -0.05f;
+        const float c = 0.9998f, sn = 0.0199987f;  // about 1.15 degrees
+        const float rot[9] = {c, -sn, 0.0f, sn, c, 0.0f, 0.0f, 0.0f, 1.0f};
+        for (int i = 0; i < 9; ++i) rp.mRot[i] = rot[i];
+        RectifyImage::CDriver rect;
+        rect.Configure(dev, fmtIn, rp);
+        CDeviceImage drect(dev, fmtIn);
+        ProcessKernel(rect, drect, imgL);
+        drect.CopyInto(hrect.data(), (size_t)W);
+
         size_t valid = 0;
         for (int16_t v : hchk) valid += v >= 0;
         std::sort(ms.begin(), ms.end());
         if (!write_file(out + "_checked.i16", hchk.data(), hchk.size() * 2) ||
             !write_file(out + "_lr.i16", hlr.data(), hlr.size() * 2) || !write_file(out + "_rl.i16", hrl.data(), hrl.size() * 2) ||
             !write_file(out + "_checkid.u16x4", hid.data(), hid.size() * 2) ||
-            !write_file(out + "_denoised.u16", hden.data(), hden.size() * 2)) {
+            !write_file(out + "_denoised.u16", hden.data(), hden.size() * 2) ||
+            !write_file(out + "_color.rgba", hcol.data(), hcol.size()) ||
+            !write_file(out + "_rectified.u8", hrect.data(), hrect.size())) {
             std::fprintf(stderr, "cannot write %s_*.i16\n", out.c_str());
             return 1;
         }

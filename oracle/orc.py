@@ -53,13 +53,16 @@ def _load():
     lib.orc_crosscheck_plain.argtypes = [P, P, i, i, i, P]
     lib.orc_crosscheck_ref.argtypes = [P, P, i, i, i, i, i, P, P, P]
     lib.orc_bm1.argtypes = [P, P, i, i, i, i, i, i, i, i, C.c_float, C.c_float, C.c_float, P, i, i, P]
+    lib.orc_rectify.argtypes = [P, i, i, P, i, i, P]
+    lib.orc_color_disparity.argtypes = [P, i, i, i, i, i, i, i, P, P]
     lib.orc_mip_level0_ref.argtypes = [P, i, i, P]
     lib.orc_mip_level0_plain.argtypes = [P, i, i, P]
     lib.orc_mip_next.argtypes = [P, i, i, C.c_float, P]
     lib.orc_disp_denoise.argtypes = [i, P, i, i, i, C.c_float, C.c_float, i, P]
     for name in ["orc_census", "orc_census3x3_ref", "orc_bm", "orc_bm_brute", "orc_ncc", "orc_aw",
                  "orc_crosscheck_plain", "orc_crosscheck_ref", "orc_mip_level0_ref", "orc_mip_level0_plain",
-                 "orc_mip_next", "orc_disp_denoise", "orc_bm1"]:
+                 "orc_mip_next", "orc_disp_denoise", "orc_bm1",
+                 "orc_rectify", "orc_color_disparity"]:
         getattr(lib, name).restype = i
     return lib
 
@@ -301,4 +304,46 @@ def bm1(L, R, P=9, sim_type=0, dmin=0, dmax=64, offset=0, ltor=True, min_delta=0
         ip = None
     _chk(lib.orc_bm1(_p(L), _p(R), W, H, P, sim_type, dmin, dmax, offset, 1 if ltor else 0, min_delta, sim_thresh,
                      grad_thresh, ip, iW, iH, _p(out)), "orc_bm1")
+    return out
+
+
+RECTIFY_PARAMS = ("src_fx", "src_fy", "src_cx", "src_cy", "k1", "k2", "p1", "p2", "k3", "rect_fx", "rect_fy",
+                  "rect_cx", "rect_cy")
+
+
+def rectify_params(src=(1.0, 1.0, 0.0, 0.0), dist=(0.0, 0.0, 0.0, 0.0, 0.0), rect=None, R=None):
+    """The 22 floats of the rectification: source fx, fy, cx, cy; k1, k2, p1, p2, k3;
+    rectified fx, fy, cx, cy (default: the source's); R (rectified to source)."""
+    rect = src if rect is None else rect
+    R = np.eye(3) if R is None else np.asarray(R)
+    return np.array(list(src) + list(dist) + list(rect) + list(np.asarray(R, np.float64).ravel()), np.float32)
+
+
+def rectify(src, params, W=None, H=None):
+    """RectifyImage: uint8 source (sH, sW) -> uint8 (H, W)."""
+    src = np.ascontiguousarray(src, np.uint8)
+    sH, sW = src.shape
+    W = sW if W is None else W
+    H = sH if H is None else H
+    params = np.ascontiguousarray(params, np.float32)
+    assert params.size == 22
+    out = np.empty((H, W), np.uint8)
+    _chk(lib.orc_rectify(_p(src), sW, sH, _p(params), W, H, _p(out)), "orc_rectify")
+    return out
+
+
+DEFAULT_COLORS = np.array([[10, 100, 180, 255], [0, 150, 50, 255], [0, 200, 200, 255], [200, 0, 200, 255],
+                           [200, 50, 50, 255], [240, 50, 150, 255], [100, 10, 10, 255]], np.uint8)
+
+
+def color_disparity(disp, style, dmin=1, dmax=0xFFEF, inf_offset=0, colors=None):
+    """ColorDisparityImage: disp (H, W, 4) uint16, (H, W) uint16 or (H, W) float32 ids;
+    styles 0 Analysis / 2 Color -> (H, W, 4) uint8, 1 Gray / 3 DispAbs / 4 DispRel -> (H, W) uint16."""
+    disp = np.ascontiguousarray(disp)
+    H, W = disp.shape[:2]
+    fmt = 0 if disp.ndim == 3 else (2 if disp.dtype == np.float32 else 1)
+    colors = np.ascontiguousarray(DEFAULT_COLORS if colors is None else colors, np.uint8)
+    out = np.empty((H, W, 4), np.uint8) if style in (0, 2) else np.empty((H, W), np.uint16)
+    _chk(lib.orc_color_disparity(_p(disp), fmt, W, H, style, dmin, dmax, inf_offset, _p(colors), _p(out)),
+         "orc_color_disparity")
     return out

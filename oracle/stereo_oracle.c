@@ -840,3 +840,159 @@ int orc_bm1(const uint8_t* L, const uint8_t* R, int W, int H, int P, int sim_typ
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* Image rectification (CluTec.Cuda.ImgProc/RectifyImage.cu) and disparity colouring     */
+/* (CluTec.Cuda.MultiView/ColorDisparityImage.cu).                                        */
+
+/* NormFloatTo<T>: clamp to [0, 1] (NaN to 0), scale by the type's maximum, round half up. */
+static inline uint32_t orc_norm_to(float f, float maxv)
+{
+    f = fminf(fmaxf(f, 0.0f), 1.0f);
+    return (uint32_t)(f * maxv + 0.5f);
+}
+
+/* The source position of target pixel (u, v): rectified pixel -> normalised ray ->
+ * rotation into the source camera -> OpenCV distortion -> source pixel (MapPixelTrgToSrc
+ * and DistortPX, RectifyImage.cu:111-121,150-158). */
+static void rect_map(const float* p, int u, int v, float* sx, float* sy)
+{
+    const float x = ((float)u - p[11]) / p[9], y = ((float)v - p[12]) / p[10];
+    const float* R = p + 13;
+    const float X = R[0] * x + R[1] * y + R[2];
+    const float Y = R[3] * x + R[4] * y + R[5];
+    const float Z = R[6] * x + R[7] * y + R[8];
+    const float xn = X / Z, yn = Y / Z;
+    const float r2 = xn * xn + yn * yn;
+    const float radial = 1.0f + r2 * (p[4] + r2 * (p[5] + r2 * p[8]));
+    const float xy = xn * yn;
+    const float xd = xn * radial + 2.0f * p[6] * xy + p[7] * (r2 + 2.0f * xn * xn);
+    const float yd = yn * radial + p[6] * (r2 + 2.0f * yn * yn) + 2.0f * p[7] * xy;
+    *sx = p[0] * xd + p[2];
+    *sy = p[1] * yd + p[3];
+}
+
+/* params: src fx, fy, cx, cy, k1, k2, p1, p2, k3, rect fx, fy, cx, cy, R (9, rectified to
+ * source, row-major): 22 floats.  CopyPixel (:64-107): bilinear when the 2x2 block at the
+ * floor lies inside the source, else the nearest pixel when inside, else 0. */
+int orc_rectify(const uint8_t* src, int sW, int sH, const float* params, int W, int H, uint8_t* out)
+{
+    if (!src || !params || !out || sW < 1 || sH < 1 || W < 1 || H < 1) return -1;
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < H; ++v)
+        for (int u = 0; u < W; ++u) {
+            float sx, sy;
+            rect_map(params, u, v, &sx, &sy);
+            const float fx0 = floorf(sx), fy0 = floorf(sy);
+            const float nx = floorf(sx + 0.5f), ny = floorf(sy + 0.5f);
+            uint8_t o = 0;
+            /* positions beyond +-2^30 (or NaN) are outside */
+            const int ok = fabsf(fx0) < 1073741824.0f && fabsf(fy0) < 1073741824.0f;
+            const int ix = ok ? (int)fx0 : -2, iy = ok ? (int)fy0 : -2;
+            const int okn = fabsf(nx) < 1073741824.0f && fabsf(ny) < 1073741824.0f;
+            const int jx = okn ? (int)nx : -1, jy = okn ? (int)ny : -1;
+            if (ix >= 0 && iy >= 0 && ix + 2 <= sW && iy + 2 <= sH) {
+                const float ax = sx - fx0, ay = sy - fy0;
+                const uint8_t* s0 = src + (size_t)iy * sW + ix;
+                const uint8_t* s1 = s0 + sW;
+                float f = (1.0f - ax) * (1.0f - ay) * ((float)s0[0] / 255.0f);
+                f += ax * (1.0f - ay) * ((float)s0[1] / 255.0f);
+                f += (1.0f - ax) * ay * ((float)s1[0] / 255.0f);
+                f += ax * ay * ((float)s1[1] / 255.0f);
+                o = (uint8_t)orc_norm_to(f, 255.0f);
+            } else if (jx >= 0 && jy >= 0 && jx < sW && jy < sH) {
+                o = src[(size_t)jy * sW + jx];
+            }
+            out[(size_t)v * W + u] = o;
+        }
+    return 0;
+}
+
+/* ColorDisparityImage::Kernel::Process (ColorDisparityImage.cu:45-270).  disp: fmt 0
+ * (4 x uint16 ids), 1 (uint16 ids), 2 (float ids); style 0 Analysis (RGBA8), 1 Gray
+ * (uint16), 2 Color (RGBA8), 3 DispAbs (uint16), 4 DispRel (uint16).  colors: RGBA8 for
+ * Unknown, CannotEvaluate, Saturated, NotSpecific, NotFound, NotUnique, Inconsistent. */
+int orc_color_disparity(const void* disp, int fmt, int W, int H, int style, int dmin, int dmax, int inf_offset,
+                        const uint8_t* colors, void* out)
+{
+    if (!disp || !out || !colors || W < 1 || H < 1 || fmt < 0 || fmt > 2 || style < 0 || style > 4 || dmax <= dmin)
+        return -1;
+    const float range = (float)(dmax - dmin);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t i = (size_t)y * W + x;
+            float v;    /* the id as the pixel type holds it */
+            int valid;
+            float fd;   /* (id - min) / range */
+            if (fmt == 2) {
+                v = ((const float*)disp)[i];
+                valid = v >= (float)ORC_DISP_FIRST && v <= (float)ORC_DISP_LAST;
+                fd = (v - (float)dmin) / range;
+            } else {
+                const int id = fmt == 0 ? ((const uint16_t*)disp)[4 * i] : ((const uint16_t*)disp)[i];
+                v = (float)id;
+                valid = id >= ORC_DISP_FIRST && id <= ORC_DISP_LAST;
+                fd = (float)(id - dmin) / range;
+            }
+            uint8_t* rgba = (uint8_t*)out + 4 * i;
+            uint16_t* g = (uint16_t*)out + i;
+            switch (style) {
+            case 0: {
+                static const int codes[7] = {ORC_DISP_UNKNOWN, ORC_DISP_CANNOT_EVALUATE, ORC_DISP_SATURATED,
+                                             ORC_DISP_NOT_SPECIFIC, ORC_DISP_NOT_FOUND, ORC_DISP_NOT_UNIQUE,
+                                             ORC_DISP_INCONSISTENT};
+                if (valid) {
+                    const uint8_t c = (uint8_t)orc_norm_to(fd, 255.0f);
+                    rgba[0] = rgba[1] = rgba[2] = c;
+                    rgba[3] = 255;
+                } else {
+                    int k = 0;
+                    while (k < 7 && v != (float)codes[k]) ++k;
+                    if (k < 7) {
+                        memcpy(rgba, colors + 4 * k, 4);
+                    } else {
+                        rgba[0] = 255;
+                        rgba[1] = 0;
+                        rgba[2] = 0;
+                        rgba[3] = 255;
+                    }
+                }
+                break;
+            }
+            case 1:
+                *g = valid ? (uint16_t)orc_norm_to(fd, 65535.0f) : 0;
+                break;
+            case 2:
+                if (valid) {
+                    /* NormFloatToColor (Conversion.h:56-71) */
+                    const float fB = 1.0f - fminf(fd / 0.7f, 1.0f);
+                    const float fG = 1.0f - fminf(fabsf(fd - 0.5f) / 0.59f, 1.0f);
+                    const float fR = 1.0f - fminf(fabsf(fd - 0.8f) / 0.7f, 1.0f);
+                    const float fM = fmaxf(fmaxf(fR, fG), fB);
+                    rgba[0] = (uint8_t)orc_norm_to(fR / fM, 255.0f);
+                    rgba[1] = (uint8_t)orc_norm_to(fG / fM, 255.0f);
+                    rgba[2] = (uint8_t)orc_norm_to(fB / fM, 255.0f);
+                } else {
+                    rgba[0] = rgba[1] = rgba[2] = 0;
+                }
+                rgba[3] = 255;
+                break;
+            case 3:
+                *g = valid ? (uint16_t)v : 0;
+                break;
+            default:
+                if (!valid) {
+                    *g = 0;
+                } else if (fmt == 2) {
+                    /* float arithmetic, converted to 16 bits with truncation, saturating */
+                    const float r = ((float)0x7FFF + (float)inf_offset) - (v - (float)ORC_DISP_FIRST);
+                    *g = !(r > 0.0f) ? 0 : (r >= 65535.0f ? 65535 : (uint16_t)r);
+                } else {
+                    *g = (uint16_t)((0x7FFF + inf_offset) - ((int)v - ORC_DISP_FIRST));
+                }
+                break;
+            }
+        }
+    return 0;
+}

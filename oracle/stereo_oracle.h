@@ -108,6 +108,18 @@ int orc_bm1(const uint8_t* L, const uint8_t* R, int W, int H, int P, int sim_typ
             int ltor, float min_delta, float sim_thresh, float grad_thresh, const uint16_t* init, int iW, int iH,
             uint16_t* out);
 
+/* RectifyImage (RectifyImage.cu:64-158): target pixel -> source position through the
+ * rectified camera (fx, fy, cx, cy), a rotation into the source camera and the source
+ * camera with OpenCV distortion; bilinear / nearest / 0 sampling.  params: src fx, fy, cx,
+ * cy, k1, k2, p1, p2, k3, rect fx, fy, cx, cy, R[9] (row-major, rectified to source). */
+int orc_rectify(const uint8_t* src, int sW, int sH, const float* params, int W, int H, uint8_t* out);
+/* ColorDisparityImage (ColorDisparityImage.cu:45-270).  fmt 0: 4 x uint16 ids, 1: uint16
+ * ids, 2: float ids; style 0 Analysis / 2 Color: RGBA8 out, 1 Gray / 3 DispAbs / 4 DispRel:
+ * uint16 out.  colors: 7 RGBA8 entries (Unknown, CannotEvaluate, Saturated, NotSpecific,
+ * NotFound, NotUnique, Inconsistent). */
+int orc_color_disparity(const void* disp, int fmt, int W, int H, int style, int dmin, int dmax, int inf_offset,
+                        const uint8_t* colors, void* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,14 @@ def test_stereo_run_matches_oracle(tmp_path, algo, W, H, D):
     ids = np.fromfile(prefix + "_checkid.u16x4", np.uint16).reshape(H, W, 4)
     den = np.fromfile(prefix + "_denoised.u16", np.uint16).reshape(H, W)
     assert np.array_equal(den, orc.disp_denoise(ids, P=5))
+    # ColorDisparityImage (Analysis, ids 1 .. D + 1) and RectifyImage of the left image
+    col = np.fromfile(prefix + "_color.rgba", np.uint8).reshape(H, W, 4)
+    assert np.array_equal(col, orc.color_disparity(ids, 0, 1, D + 1))
+    c, sn = np.float32(0.9998), np.float32(0.0199987)
+    f, cx, cy = np.float32(0.8) * np.float32(W), np.float32(0.5) * np.float32(W - 1), np.float32(0.5) * np.float32(H - 1)
+    p = orc.rectify_params((f, f, cx, cy), (-0.05, 0, 0, 0, 0), (f, f, cx, cy), [[c, -sn, 0], [sn, c, 0], [0, 0, 1]])
+    rect = np.fromfile(prefix + "_rectified.u8", np.uint8).reshape(H, W)
+    assert np.array_equal(rect, orc.rectify(L, p))
 
 
 def test_stereo_run_bm1_matches_emulation(tmp_path):
