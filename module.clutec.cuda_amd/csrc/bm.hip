@@ -302,78 +302,36 @@ __device__ __forceinline__ uint32_t rkey_at(const BmFinishParams& f, int xr, int
 {
     // right pixel xr meets left pixels xr+d; segment s covers d in [ds, de), and the
     // tile holding left pixel x keeps the partial under index xr - x0 + 255.  A
-    // segment spans at most 5 tiles (seg_len <= 256); the loads are independent.
+    // segment spans at most 5 tiles (seg_len <= 256).  Two segments' loads are issued
+    // together and unconditionally: past the segment's last tile a load repeats that
+    // tile (the max is unchanged), and a segment with no tile reads a fixed address
+    // whose value is discarded.
     uint32_t best = 0;
-    for (int s = 0; s < f.nseg; ++s) {
-        const int ds = s * f.seg_len;
-        const int de = min(f.D, ds + f.seg_len);
-        const int tlo = (xr + ds) / 64;
-        int thi = (xr + de - 1) / 64;
-        if (thi > f.ntx - 1) thi = f.ntx - 1;
-        const uint32_t* base = f.rpart + ((size_t)s * f.ntx * f.H + y) * BM_NPART + (xr + 255);
+    for (int s0 = 0; s0 < f.nseg; s0 += 2) {
+        uint32_t v[2][5];
+        bool any[2];
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            const int t = tlo + j;
-            if (t <= thi) {
-                const uint32_t v = base[(size_t)t * f.H * BM_NPART - 64 * t];
-                best = v > best ? v : best;
+        for (int q = 0; q < 2; ++q) {
+            const int s = min(s0 + q, f.nseg - 1);
+            const int ds = s * f.seg_len;
+            const int de = min(f.D, ds + f.seg_len);
+            const int tlo = (xr + ds) / 64;
+            const int thi = min((xr + de - 1) / 64, f.ntx - 1);
+            any[q] = tlo <= thi;
+            const uint32_t* base = f.rpart + ((size_t)s * f.ntx * f.H + y) * BM_NPART + (xr + 255);
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                const int t = min(tlo + j, thi);
+                const uint32_t* a = any[q] ? base + ((size_t)t * f.H * BM_NPART - 64 * t) : f.rpart;
+                v[q][j] = *a;
             }
         }
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) best = (any[q] && v[q][j] > best) ? v[q][j] : best;
     }
     return best;
-}
-
-// One workgroup per image row: first every right pixel of the row merges its partial
-// keys (contiguous reads, one pass) into LDS, then every left pixel looks up the right
-// winner at x - dL for the check.
-__global__ __launch_bounds__(256) void bm_finish_kernel(BmFinishParams f)
-{
-    extern __shared__ uint32_t rrow[];
-    const int y = blockIdx.x;
-    const int W = f.W;
-    const bool row_ok = y >= f.by && y < f.H - f.by;
-#pragma unroll 4
-    for (int x = threadIdx.x; x < W; x += 256) {
-        const bool col_ok = x >= f.bx && x < W - f.bx;
-        rrow[x] = (row_ok && col_ok) ? rkey_at(f, x, y) : 0u;
-    }
-    __syncthreads();
-    for (int x = threadIdx.x; x < W; x += 256) {
-        const bool col_ok = x >= f.bx && x < W - f.bx;
-        int dl = -1, dr = -1, outd = -1;
-        uint32_t lk = 0;
-        if (row_ok && col_ok) {
-            for (int s = 0; s < f.nseg; ++s) {
-                const uint32_t v = f.lkey[(size_t)s * W * f.H + (size_t)y * W + x];
-                lk = v > lk ? v : lk;
-            }
-            dl = key_disp(lk);
-            if (dl >= 0) {
-                const int dxr = key_disp(rrow[x - dl]);
-                outd = (dxr >= 0 && abs(dxr - dl) <= f.tol) ? dl : -1;
-            }
-            dr = key_disp(rrow[x]);
-        }
-        if (f.disp.ptr) row_ptr<int16_t>(f.disp, y)[x] = (int16_t)outd;
-        if (f.disp_l.ptr) row_ptr<int16_t>(f.disp_l, y)[x] = (int16_t)dl;
-        if (f.disp_r.ptr) row_ptr<int16_t>(f.disp_r, y)[x] = (int16_t)dr;
-        if (f.dispid.ptr) {
-            // reference pixel layout (EDisparityId in channel 0, DisparityId.h:33-45)
-            uint16_t id;
-            if (!(row_ok && col_ok)) id = 0;                       // Unknown
-            else if (outd >= 0) id = (uint16_t)(outd + 1);        // First + d
-            else if (dl >= 0) id = 0xFFFFu - 5u;                   // Inconsistent
-            else id = 0xFFFFu - 2u;                                // NotFound
-            int cost = lk ? key_cost(lk) : 0;
-            if (lk && f.value_base) cost = f.value_base - cost;
-            ushort4 v;
-            v.x = id;
-            v.y = (uint16_t)(cost > 65535 ? 65535 : cost);
-            v.z = 0;
-            v.w = 0;
-            row_ptr<ushort4>(f.dispid, y)[x] = v;
-        }
-    }
 }
 
 // Output of one left pixel: the checked disparity and the maps, in every requested
@@ -408,6 +366,74 @@ __device__ __forceinline__ void write_pixel(const View& disp, const View& disp_l
         v.z = 0;
         v.w = 0;
         row_ptr<ushort4>(dispid, y)[x] = v;
+    }
+}
+
+// One workgroup per image row: first every right pixel of the row merges its partial
+// keys (contiguous reads, one pass) into LDS, then every left pixel looks up the right
+// winner at x - dL for the check.  The left keys of a thread's first FIN_PF pixels are
+// loaded before the barrier, so their latency overlaps the right-key merge.
+constexpr int FIN_PF = 8;
+
+__device__ __forceinline__ uint32_t lkey_at(const BmFinishParams& f, int x, int y)
+{
+    uint32_t lk = 0;
+    for (int s = 0; s < f.nseg; ++s) {
+        const uint32_t v = f.lkey[(size_t)s * f.W * f.H + (size_t)y * f.W + x];
+        lk = v > lk ? v : lk;
+    }
+    return lk;
+}
+
+__global__ __launch_bounds__(256) void bm_finish_kernel(BmFinishParams f)
+{
+    extern __shared__ uint32_t rrow[];
+    const int y = blockIdx.x;
+    const int W = f.W;
+    const bool row_ok = y >= f.by && y < f.H - f.by;
+    // the first two segments' left keys of FIN_PF pixels, loaded unconditionally (a
+    // position past the row reads the row's last pixel; the value is discarded)
+    uint32_t lpre[FIN_PF];
+    {
+        const size_t plane = (size_t)W * f.H;
+        const size_t s1 = (size_t)min(1, f.nseg - 1) * plane;
+        uint32_t la[FIN_PF], lb[FIN_PF];
+#pragma unroll
+        for (int k = 0; k < FIN_PF; ++k) {
+            const size_t i = (size_t)y * W + min((int)threadIdx.x + 256 * k, W - 1);
+            la[k] = f.lkey[i];
+            lb[k] = f.lkey[s1 + i];
+        }
+#pragma unroll
+        for (int k = 0; k < FIN_PF; ++k) {
+            const int x = threadIdx.x + 256 * k;
+            uint32_t lk = la[k] > lb[k] ? la[k] : lb[k];
+            for (int sg = 2; sg < f.nseg; ++sg) {
+                const uint32_t v = f.lkey[(size_t)sg * plane + (size_t)y * W + min(x, W - 1)];
+                lk = v > lk ? v : lk;
+            }
+            const bool inside = row_ok && x >= f.bx && x < W - f.bx;
+            lpre[k] = inside ? lk : 0u;
+        }
+    }
+#pragma unroll 2
+    for (int x = threadIdx.x; x < W; x += 256) {
+        const bool col_ok = x >= f.bx && x < W - f.bx;
+        const uint32_t r = rkey_at(f, x, y);
+        rrow[x] = (row_ok && col_ok) ? r : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < FIN_PF; ++k) {
+        const int x = threadIdx.x + 256 * k;
+        if (x < W)
+            write_pixel(f.disp, f.disp_l, f.disp_r, f.dispid, x, y, row_ok && x >= f.bx && x < W - f.bx, lpre[k], rrow,
+                        f.tol, f.value_base);
+    }
+    for (int x = threadIdx.x + 256 * FIN_PF; x < W; x += 256) {
+        const bool inside = row_ok && x >= f.bx && x < W - f.bx;
+        write_pixel(f.disp, f.disp_l, f.disp_r, f.dispid, x, y, inside, inside ? lkey_at(f, x, y) : 0u, rrow, f.tol,
+                    f.value_base);
     }
 }
 
