@@ -391,29 +391,36 @@ __global__ __launch_bounds__(256) void bm_finish_kernel(BmFinishParams f)
     const int y = blockIdx.x;
     const int W = f.W;
     const bool row_ok = y >= f.by && y < f.H - f.by;
-    // the first two segments' left keys of FIN_PF pixels, loaded unconditionally (a
-    // position past the row reads the row's last pixel; the value is discarded)
+    // the left keys of the thread's first FIN_PF pixels, two segments per batch of loads
+    // (a position past the row reads the row's last pixel; the value is discarded;
+    // slots wholly past the row are skipped)
     uint32_t lpre[FIN_PF];
     {
         const size_t plane = (size_t)W * f.H;
-        const size_t s1 = (size_t)min(1, f.nseg - 1) * plane;
-        uint32_t la[FIN_PF], lb[FIN_PF];
 #pragma unroll
-        for (int k = 0; k < FIN_PF; ++k) {
-            const size_t i = (size_t)y * W + min((int)threadIdx.x + 256 * k, W - 1);
-            la[k] = f.lkey[i];
-            lb[k] = f.lkey[s1 + i];
+        for (int k = 0; k < FIN_PF; ++k) lpre[k] = 0u;
+        for (int s0 = 0; s0 < f.nseg; s0 += 2) {
+            const size_t sa = (size_t)s0 * plane, sb = (size_t)min(s0 + 1, f.nseg - 1) * plane;
+            uint32_t la[FIN_PF], lb[FIN_PF];
+#pragma unroll
+            for (int k = 0; k < FIN_PF; ++k) {
+                la[k] = lb[k] = 0u;
+                if (256 * k < W) {
+                    const size_t i = (size_t)y * W + min((int)threadIdx.x + 256 * k, W - 1);
+                    la[k] = f.lkey[sa + i];
+                    lb[k] = f.lkey[sb + i];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < FIN_PF; ++k) {
+                const uint32_t m = la[k] > lb[k] ? la[k] : lb[k];
+                lpre[k] = m > lpre[k] ? m : lpre[k];
+            }
         }
 #pragma unroll
         for (int k = 0; k < FIN_PF; ++k) {
             const int x = threadIdx.x + 256 * k;
-            uint32_t lk = la[k] > lb[k] ? la[k] : lb[k];
-            for (int sg = 2; sg < f.nseg; ++sg) {
-                const uint32_t v = f.lkey[(size_t)sg * plane + (size_t)y * W + min(x, W - 1)];
-                lk = v > lk ? v : lk;
-            }
-            const bool inside = row_ok && x >= f.bx && x < W - f.bx;
-            lpre[k] = inside ? lk : 0u;
+            if (!(row_ok && x >= f.bx && x < W - f.bx)) lpre[k] = 0u;
         }
     }
 #pragma unroll 2
@@ -497,22 +504,21 @@ __global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p, int
             // element (k, c): column c of staged item k, right pixel x0 - (de - 1) + c
             const int tot = toff[ntx] * p.rp_stride;
             for (int f0 = threadIdx.x; f0 < tot; f0 += 8 * 256) {
+                // all eight loads are issued before any is used: an element past the end
+                // or outside the item's columns reads a fixed address, and is discarded
                 uint32_t v[8];
                 int xr[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const int e = f0 + u * 256;
-                    v[u] = 0u;
-                    xr[u] = -1;
-                    if (e < tot) {
-                        const int k = e / p.rp_stride, c = e - k * p.rp_stride;
-                        const int2 rg = irng[k];
-                        const int x = itx[k] * TW - (rg.y - 1) + c;
-                        if (c < TW - 1 + rg.y - rg.x && x >= f.bx && x < W - f.bx) {
-                            v[u] = p.rpart[((size_t)islot[k] * TH + ry) * p.rp_stride + c];
-                            xr[u] = x;
-                        }
-                    }
+                    const int ec = min(e, tot - 1);
+                    const int k = ec / p.rp_stride, c = ec - k * p.rp_stride;
+                    const int2 rg = irng[k];
+                    const int x = itx[k] * TW - (rg.y - 1) + c;
+                    const bool ok = e < tot && c < TW - 1 + rg.y - rg.x && x >= f.bx && x < W - f.bx;
+                    const uint32_t* a = ok ? p.rpart + ((size_t)islot[k] * TH + ry) * p.rp_stride + c : p.rpart;
+                    v[u] = *a;
+                    xr[u] = ok ? x : -1;
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
@@ -551,12 +557,14 @@ __global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p, int
             uint32_t lv[BM_LIST_LEFT_UNROLL];
 #pragma unroll
             for (int q = 0; q < BM_LIST_LEFT_UNROLL; ++q) {
+                // unconditional: past the tile's items a load reads a fixed address
                 const int k = kb + q;
                 const int j = k < ke ? (staged ? islot[k] : k) : 0;
-                lv[q] = k < ke ? p.lkey[((size_t)j * TH + ry) * TW + (x & (TW - 1))] : 0u;
+                const uint32_t* a = k < ke ? p.lkey + ((size_t)j * TH + ry) * TW + (x & (TW - 1)) : p.lkey;
+                lv[q] = *a;
             }
 #pragma unroll
-            for (int q = 0; q < BM_LIST_LEFT_UNROLL; ++q) lk = lv[q] > lk ? lv[q] : lk;
+            for (int q = 0; q < BM_LIST_LEFT_UNROLL; ++q) lk = (kb + q < ke && lv[q] > lk) ? lv[q] : lk;
             for (int k = kb + BM_LIST_LEFT_UNROLL; k < ke; ++k) {
                 const int j = staged ? islot[k] : k;
                 const uint32_t v = p.lkey[((size_t)j * TH + ry) * TW + (x & (TW - 1))];
