@@ -18,7 +18,9 @@
 // take none).  The strip's left pixels, and per group of 16 disparities the right pixels,
 // sit in LDS as bytes.  Per disparity, each thread forms running P-row column sums of the
 // similarity over a segment of rows into LDS (double-buffered), then a running P-column
-// row sum over its results and the peak update.  The band search around an initial map
+// row sum over its results and the peak update.  With 16 blocks per strip, each wave's
+// column sums are its own 16 rows, so the waves synchronise only per group of 16
+// disparities, not per disparity.  The band search around an initial map
 // (MatcherUseDisparity :857-1203) gives every block its own range, and runs one block per
 // workgroup.
 #include <hip/hip_runtime.h>
@@ -135,6 +137,7 @@ __global__ __launch_bounds__(256) void bm1_kernel(Bm1Args a)
     const int Y0 = Y0s + 4 * bro;           // its origin row
     const bool block_ok = has_row && by0 + bro < a.nby && X0 + P <= W && Y0 + P <= H;
     const bool rows_fit = Y0 + S::TBY <= H;  // the block's test rows lie in the image
+    const bool upd = block_ok && rows_fit;   // the block evaluates disparities
     int disp[RPT], mx[RPT], prev[RPT], prev2[RPT], nxt[RPT], nxt2[RPT], last[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
@@ -243,50 +246,64 @@ __global__ __launch_bounds__(256) void bm1_kernel(Bm1Args a)
             // column sums of P rows over this thread's segment
             if (t < VT) {
                 const int c = t % 32, r0 = (t / 32) * SEG;
-                int sim[SEG + P - 1];
+                // AbsoluteDifference: (255 - x) >> 1 == 127 - (x >> 1) for 0 <= x <= 255, and
+                // v_sad_u8(l, q, 0) gives |l - q|, so a column sum is 127 P minus a sum of
+                // t = |l - q| >> 1.  SignedProduct: the similarity, negated.
+                int t[SEG + P - 1];
 #pragma unroll
                 for (int j = 0; j < SEG + P - 1; ++j) {
-                    const int l = sl[r0 + j][c], q = sr[r0 + j][c + sh];
+                    const uint32_t l = sl[r0 + j][c], q = sr[r0 + j][c + sh];
                     if (SIM == 0) {
-                        sim[j] = (255 - abs(l - q)) >> 1;
+                        t[j] = (int)(__builtin_amdgcn_sad_u8(l, q, 0u) >> 1);
                     } else {
                         const float f = stab[l] * stab[q];
-                        sim[j] = (int)(f * 127.0f);
+                        t[j] = -(int)(f * 127.0f);
                     }
                 }
+                constexpr int base = SIM == 0 ? 127 * P : 0;
                 int s = 0;
 #pragma unroll
-                for (int j = 0; j < P; ++j) s += sim[j];
-                sv[buf][r0][c] = s;
+                for (int j = 0; j < P; ++j) s += t[j];
+                sv[buf][r0][c] = base - s;
 #pragma unroll
                 for (int j = 1; j < SEG; ++j) {
-                    s += sim[j + P - 1] - sim[j - 1];
-                    sv[buf][r0 + j][c] = s;
+                    s += t[j + P - 1] - t[j - 1];
+                    sv[buf][r0 + j][c] = base - s;
                 }
             }
-            __syncthreads();
-            // row sums of P columns and the peak update (:262-300)
-            if (block_ok && rows_fit) {
+            if (NB == 16) {
+                // wave w writes and reads only the column-sum rows 16 w .. 16 w + 15, and a
+                // wave's LDS operations execute in order: ordering the compiler suffices
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            } else {
+                __syncthreads();
+            }
+            // row sums of P columns and the peak update (:262-300); every thread reads
+            // its row (in LDS for all threads), results outside an evaluating block
+            // leave their state unchanged
+            {
+                const int (&vrow)[32] = sv[buf][has_row ? rr : 0];
                 int s = 0;
 #pragma unroll
-                for (int i = 0; i < P; ++i) s += sv[buf][rr][ix0 + i];
+                for (int i = 0; i < P; ++i) s += vrow[ix0 + i];
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
-                    if (k > 0) s += sv[buf][rr][ix0 + k + P - 1] - sv[buf][rr][ix0 + k - 1];
+                    if (k > 0) s += vrow[ix0 + k + P - 1] - vrow[ix0 + k - 1];
+                    // branch-free: every state value is a select
                     const int v = (int)(int16_t)s;
-                    if (disp[k] > B1_LAST) continue;
-                    const bool above = v > mx[k];
-                    const bool b1 = !above && disp[k] + 1 == d + B1_FIRST;
-                    const bool b2 = !above && disp[k] + 2 == d + B1_FIRST;
-                    if (above) {
-                        prev2[k] = prev[k];
-                        prev[k] = last[k];
-                        mx[k] = v;
-                        disp[k] = d + B1_FIRST;
-                    }
-                    if (above || b1) nxt[k] = v;
-                    if (above || b2) nxt2[k] = v;
-                    last[k] = v;
+                    const bool act = upd && disp[k] <= B1_LAST;
+                    const bool above = act && v > mx[k];
+                    const bool b1 = act && !above && disp[k] + 1 == d + B1_FIRST;
+                    const bool b2 = act && !above && disp[k] + 2 == d + B1_FIRST;
+                    prev2[k] = above ? prev[k] : prev2[k];
+                    prev[k] = above ? last[k] : prev[k];
+                    mx[k] = above ? v : mx[k];
+                    disp[k] = above ? d + B1_FIRST : disp[k];
+                    nxt[k] = (above || b1) ? v : nxt[k];
+                    nxt2[k] = (above || b2) ? v : nxt2[k];
+                    last[k] = act ? v : last[k];
                 }
             }
             buf ^= 1;

@@ -52,10 +52,27 @@ __global__ __launch_bounds__(256) void ncc_stats_kernel(View img, int2* out, int
     __shared__ int hss[ST_H + 2 * ST_MAXA][ST_W];
     const int x0 = blockIdx.x * ST_W, y0 = blockIdx.y * ST_H;
     const int rows = ST_H + 2 * ay, cols = ST_W + 2 * ax;
-    for (int k = threadIdx.x; k < rows * cols; k += 256) {
-        const int r = k / cols, c = k - r * cols;
-        const int y = clampi(y0 - ay + r, 0, img.h - 1), x = clampi(x0 - ax + c, 0, img.w - 1);
-        pix[r][c] = row_ptr<const uint8_t>(img, y)[x];
+    {
+        // lane = column (and column + 64 for the halo), wave = every fourth row; all
+        // loads (clamped to the image) are issued before the first store
+        constexpr int NR = (ST_H + 2 * ST_MAXA + 3) / 4;
+        const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+        const int xa = clampi(x0 - ax + c, 0, img.w - 1), xb = clampi(x0 - ax + 64 + c, 0, img.w - 1);
+        uint8_t va[NR], vb[NR];
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const uint8_t* row = row_ptr<const uint8_t>(img, clampi(y0 - ay + r0 + 4 * i, 0, img.h - 1));
+            va[i] = row[xa];
+            vb[i] = row[xb];
+        }
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const int r = r0 + 4 * i;
+            if (r < rows) {
+                pix[r][c] = va[i];
+                if (64 + c < cols) pix[r][64 + c] = vb[i];
+            }
+        }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < rows * ST_W; k += 256) {

@@ -276,3 +276,12 @@ def test_signed_product_values():
     assert v[0] == 127 and v[1] == 127 and v[2] == -127
     # the sign of the product of the centred values
     assert np.array_equal(np.sign(v[3:]), np.sign((a[3:] - 127.5) * (b[3:] - 127.5)).astype(int) * (v[3:] != 0))
+
+
+def test_absolute_difference_identity_used_by_the_kernel():
+    """(255 - |a - b|) >> 1 == 127 - (|a - b| >> 1) for all byte pairs: the GPU kernel sums
+    |a - b| >> 1 (one v_sad_u8 and a shift) and subtracts from 127 P."""
+    a, b = np.meshgrid(np.arange(256), np.arange(256))
+    x = np.abs(a - b)
+    assert np.array_equal((255 - x) >> 1, 127 - (x >> 1))
+    assert not np.array_equal((255 - x) >> 1, 127 - ((x + 1) >> 1))
