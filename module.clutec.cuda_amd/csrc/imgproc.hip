@@ -41,6 +41,10 @@ struct RectArgs {
 __global__ __launch_bounds__(256) void rectify_kernel(RectArgs a)
 {
 #pragma clang fp contract(off)
+    // the 256 values v / 255 of the bilinear tap, each the correctly rounded quotient
+    __shared__ float norm[256];
+    norm[threadIdx.x] = (float)threadIdx.x / 255.0f;
+    __syncthreads();
     const int u = blockIdx.x * 64 + (threadIdx.x & 63);
     const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (u >= a.dst.w || v >= a.dst.h) return;
@@ -71,10 +75,10 @@ __global__ __launch_bounds__(256) void rectify_kernel(RectArgs a)
         const float ax = sx - fx0, ay = sy - fy0;
         const uint8_t* s0 = row_ptr<const uint8_t>(a.src, iy) + ix;
         const uint8_t* s1 = row_ptr<const uint8_t>(a.src, iy + 1) + ix;
-        float f = (1.0f - ax) * (1.0f - ay) * ((float)s0[0] / 255.0f);
-        f += ax * (1.0f - ay) * ((float)s0[1] / 255.0f);
-        f += (1.0f - ax) * ay * ((float)s1[0] / 255.0f);
-        f += ax * ay * ((float)s1[1] / 255.0f);
+        float f = (1.0f - ax) * (1.0f - ay) * norm[s0[0]];
+        f += ax * (1.0f - ay) * norm[s0[1]];
+        f += (1.0f - ax) * ay * norm[s1[0]];
+        f += ax * ay * norm[s1[1]];
         o = norm_to(f, 255.0f);
     } else if (jx >= 0 && jy >= 0 && jx < sW && jy < sH) {
         o = row_ptr<const uint8_t>(a.src, jy)[jx];
