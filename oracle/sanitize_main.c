@@ -94,8 +94,66 @@ static void run_cnn(void)
     free(x); free(w); free(b); free(y); free(dx); free(dw); free(db);
 }
 
+/* The operators around the matchers: mip maps, de-noise, BlockMatcher1 (plain and band
+ * search), rectification and colouring, at edge and ragged sizes. */
+static void run_post(int W, int H)
+{
+    const size_t N = (size_t)W * H;
+    uint8_t* L = malloc(N);
+    uint8_t* R = malloc(N);
+    int16_t* d = malloc(N * 2);
+    uint16_t* ids = malloc(N * 8);
+    uint16_t* o16 = malloc(N * 2);
+    uint16_t* o4 = malloc(N * 8);
+    uint8_t* o8 = malloc(N * 4);
+    float* ff = malloc(N * 4);
+    float* lv = malloc(N * 4);
+    for (size_t i = 0; i < N; ++i) {
+        L[i] = (uint8_t)next();
+        R[i] = (uint8_t)next();
+        d[i] = (int16_t)((next() % 5 == 0) ? -1 : (int)(next() % 200));
+        ids[4 * i] = (uint16_t)((next() % 7 == 0) ? 0xFFFA : 1 + next() % 300);
+        ids[4 * i + 1] = ids[4 * i + 2] = ids[4 * i + 3] = 0;
+        ff[i] = (float)(next() % 400) * 0.5f;
+    }
+    EXPECT(orc_mip_level0_ref(ids, W, H, lv) == 0);
+    EXPECT(orc_mip_level0_plain(d, W, H, lv) == 0);
+    if (W >= 3 && H >= 3) EXPECT(orc_mip_next(lv, W, H, -1.0f, ff) == 0);
+    const int P = W < H ? (W < 5 ? W : 5) : (H < 5 ? H : 5);
+    EXPECT(orc_disp_denoise(0, d, W, H, P, 0.5f, 1.0f, 1, o16) == 0);
+    EXPECT(orc_disp_denoise(1, ids, W, H, P, 0.3f, 2.0f, 0, o16) == 0);
+    if (W >= 48) {
+        static const int patches[4] = {5, 9, 11, 15};
+        for (int k = 0; k < 4; ++k) {
+            const int Pb = patches[k];
+            if (H < Pb + 3) continue;
+            EXPECT(orc_bm1(L, R, W, H, Pb, k & 1, 0, 64, k - 1, k != 3, 1.0f, 10.0f, 2.0f, NULL, 0, 0, o4) == 0);
+            EXPECT(orc_bm1(L, R, W, H, Pb, 0, 0, 64, 0, 1, 0.0f, 0.0f, 0.0f, ids, W / 2 ? W / 2 : 1, H / 2 ? H / 2 : 1,
+                           o4) == 0);
+        }
+    } else {
+        EXPECT(orc_bm1(L, R, W, H, 5, 0, 0, 16, 0, 1, 0.0f, 0.0f, 0.0f, NULL, 0, 0, o4) != 0);
+    }
+    float rp[22] = {1.1f * W, 1.1f * W, 0.5f * W, 0.5f * H, -0.2f, 0.05f, 0.001f, -0.001f, 0.01f,
+                    1.0f * W, 1.0f * W, 0.5f * W, 0.5f * H, 0.999f, -0.04f, 0.0f, 0.04f, 0.999f, 0.0f, 0.0f, 0.0f, 1.0f};
+    EXPECT(orc_rectify(L, W, H, rp, W + 3, H + 1, o8) == 0);
+    rp[13 + 8] = 0.0f; /* Z = 0 for the optical axis: infinite positions are outside */
+    EXPECT(orc_rectify(L, W, H, rp, W, H, o8) == 0);
+    static const uint8_t colors[28] = {0};
+    for (int style = 0; style < 5; ++style) {
+        EXPECT(orc_color_disparity(ids, 0, W, H, style, 1, 300, 7, colors, style == 0 || style == 2 ? (void*)o8 : (void*)o16) == 0);
+        EXPECT(orc_color_disparity(ff, 2, W, H, style, 20, 100, 40000, colors, style == 0 || style == 2 ? (void*)o8 : (void*)o16) == 0);
+    }
+    EXPECT(orc_color_disparity(ids, 0, W, H, 0, 5, 5, 0, colors, o8) != 0);
+    free(L); free(R); free(d); free(ids); free(o16); free(o4); free(o8); free(ff); free(lv);
+}
+
 int main(void)
 {
+    run_post(48, 8);    /* the smallest BlockMatcher1 image */
+    run_post(97, 61);
+    run_post(5, 5);
+    run_post(131, 19);
     run_matchers(40, 30, 16);
     run_matchers(23, 17, 32);   /* D larger than the matchable width */
     run_matchers(64, 12, 8);    /* fewer rows than the window */
