@@ -254,6 +254,25 @@ def other_matchers_leg(cs, np, ctx, steps):
             r["kernel_valu_frac"] = round(w * h * d * ops / (kms * 1e-3) / (PEAK_INT_TOPS * 1e12), 4)
             r["ops_per_eval"] = ops
         res[name] = r
+        if algo == cs.ALGO_NCC:
+            # the band search around an initial map (MatcherUseDisparity): the full-range
+            # map at 1/4 resolution (made before the timed region)
+            full = out.download()
+            init = np.ascontiguousarray(full[::4, ::4] // 4).astype(np.int16)
+            di = cs.DeviceImage.from_array(ctx, init, cs.PIXEL_LUM_I16)
+            mb = cs.Matcher(ctx, w, h, algorithm=algo, window=win, num_disp=d, tol=1, use_disp_input=True)
+            ob = cs.DeviceImage(ctx, w, h, cs.PIXEL_LUM_I16)
+            for _ in range(3):
+                mb.process_init(ob, None, None, dl, dr, di)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                mb.process_init(ob, None, None, dl, dr, di)
+            ctx.synchronize()
+            bms = (time.perf_counter() - t0) * 1e3 / steps
+            res[name + "_band"] = {"ms_per_frame": round(bms, 4), "speedup_vs_full": round(ms / bms, 3),
+                                   "agreement_with_full": round(float((ob.download() == full).mean()), 5),
+                                   "note": "initial map: the full-range map at 1/4 resolution, d / 4"}
     return res
 
 

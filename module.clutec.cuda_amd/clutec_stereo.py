@@ -443,9 +443,9 @@ class Matcher(_Driver):
                                                disp_lr.ref if disp_lr else None, disp_rl.ref if disp_rl else None,
                                                left.ref, right.ref, init.ref, stream), self.ctx)
 
-    def run_init(self, L, R, init, maps=False):
+    def run_init(self, L, R, init, maps=False, scores=False):
         """numpy in, numpy out: the band search around `init` (int16 map, -1 invalid, or
-        (h, w, 4) uint16 EDisparityId pixels)."""
+        (h, w, 4) uint16 EDisparityId pixels); for NCC also the winning scores."""
         dl = DeviceImage.from_array(self.ctx, L, PIXEL_LUM_U8)
         dr = DeviceImage.from_array(self.ctx, R, PIXEL_LUM_U8)
         init = np.asarray(init)
@@ -454,9 +454,14 @@ class Matcher(_Driver):
         lr = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16) if maps else None
         rl = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_I16) if maps else None
         self.process_init(out, lr, rl, dl, dr, di)
+        res = [out.download()]
         if maps:
-            return out.download(), lr.download(), rl.download()
-        return out.download()
+            res += [lr.download(), rl.download()]
+        if scores:
+            sc = DeviceImage(self.ctx, self.W, self.H, PIXEL_LUM_F32)
+            _check(lib.stereo_matcher_scores(self.handle, sc.ref, dl.ref, dr.ref, None), self.ctx)
+            res.append(sc.download())
+        return res[0] if len(res) == 1 else tuple(res)
 
     def run(self, L, R, maps=False, dispid=False, scores=False):
         """numpy in, numpy out: the checked map (and the two one-sided maps, and for

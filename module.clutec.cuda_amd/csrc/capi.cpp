@@ -314,6 +314,14 @@ stereo_status band_alloc(stereo_driver* d)
     x->ntx = (W + stereo::BM_TILE_W - 1) / stereo::BM_TILE_W;
     x->nty = (H + stereo::BM_TILE_H - 1) / stereo::BM_TILE_H;
     const int nt = x->ntx * x->nty;
+    if (d->mp.algorithm == STEREO_ALGO_NCC) {
+        // the NCC matcher masks its own tiles: only the per-tile bins
+        hipError_t e;
+        if ((e = hipMalloc((void**)&x->tile_evals, nt * sizeof(int))) ||
+            (e = hipMalloc((void**)&x->tile_mask, nt * sizeof(uint32_t))))
+            return hip_fail(d->ctx, e, "hipMalloc");
+        return STEREO_OK;
+    }
     x->max_items = nt * ((D + kBandChunk - 1) / kBandChunk);
     hipError_t e;
     if ((e = hipMalloc((void**)&x->frame_dev, sizeof(stereo::BmFrame))) ||
@@ -366,9 +374,10 @@ stereo_status stereo_matcher_validate(const stereo_image_format* fmt, const ster
         if (p->reserved[i]) return say(STEREO_ERROR_INVALID_CONFIG, "reserved parameters must be 0");
     if (p->use_disp_input != 0 && p->use_disp_input != 1)
         return say(STEREO_ERROR_INVALID_CONFIG, "use_disp_input must be 0 or 1");
-    if (p->use_disp_input && p->algorithm != STEREO_ALGO_SAD && p->algorithm != STEREO_ALGO_CENSUS)
-        return say(STEREO_ERROR_INVALID_CONFIG, "the band search around an initial disparity map is built for the SAD "
-                                                "and census matchers");
+    if (p->use_disp_input && p->algorithm != STEREO_ALGO_SAD && p->algorithm != STEREO_ALGO_CENSUS &&
+        p->algorithm != STEREO_ALGO_NCC)
+        return say(STEREO_ERROR_INVALID_CONFIG, "the band search around an initial disparity map is built for the SAD, "
+                                                "census and NCC matchers");
     if (p->use_disp_input && fmt->width > stereo::BM_LIST_MAX_WIDTH)
         return say(STEREO_ERROR_INVALID_FORMAT, "the band search takes widths up to 11264");
     if (p->lr_tolerance < 0) return say(STEREO_ERROR_INVALID_CONFIG, "lr_tolerance must be >= 0");
@@ -459,28 +468,15 @@ stereo_status stereo_matcher_configure(stereo_context* ctx, stereo_driver** out,
     return STEREO_OK;
 }
 
-stereo_status stereo_matcher_process(stereo_driver* d, stereo_image* disp, stereo_image* disp_lr, stereo_image* disp_rl,
-                                     const stereo_image* left, const stereo_image* right, stereo_stream stream)
+// The launch sequence of one frame: inputs into the padded buffers (census or copies), the
+// NCC statistics, the matcher and the merge / check.  bins: the NCC band search's per-tile
+// masks, or nullptr for the full range.  begin() has been called; this calls end().
+static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool dispid, stereo_image* disp_lr,
+                                 stereo_image* disp_rl, const stereo_image* left, const stereo_image* right,
+                                 const uint32_t* bins, int bins_nty, hipStream_t s)
 {
-    if (!d || d->kind != stereo_driver::MATCHER || !left || !right) return STEREO_ERROR_INVALID_ARGUMENT;
     stereo_context* ctx = d->ctx;
     const int W = d->fmt.width, H = d->fmt.height;
-    stereo_status st;
-    if ((st = check_image(ctx, left, W, H, STEREO_PIXEL_LUM_U8, "left image"))) return st;
-    if ((st = check_image(ctx, right, W, H, STEREO_PIXEL_LUM_U8, "right image"))) return st;
-    bool dispid = false;
-    if (disp) {
-        if (disp->format.pixel_type == STEREO_PIXEL_RGBA_U16) dispid = true;
-        if ((st = check_image(ctx, disp, W, H, dispid ? STEREO_PIXEL_RGBA_U16 : STEREO_PIXEL_LUM_I16, "disparity output")))
-            return st;
-    }
-    if (disp_lr && (st = check_image(ctx, disp_lr, W, H, STEREO_PIXEL_LUM_I16, "left-to-right output"))) return st;
-    if (disp_rl && (st = check_image(ctx, disp_rl, W, H, STEREO_PIXEL_LUM_I16, "right-to-left output"))) return st;
-    if ((st = set_device(ctx))) return st;
-    GUARD(ctx, "stereo_matcher_process");
-    hipStream_t s = (hipStream_t)stream;
-    if ((st = begin(d, s))) return st;
-
     View pl, pr;
     pl.pitch = pr.pitch = d->pad_pitch;
     pl.w = pr.w = W;
@@ -507,7 +503,7 @@ stereo_status stereo_matcher_process(stereo_driver* d, stereo_image* disp, stere
     }
     if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[0], s), "hipEventRecord");
     if (ncc) {
-        stereo::NccParams np;
+        stereo::NccParams np{};
         np.L = pl;
         np.R = pr;
         np.lstats = d->lstats;
@@ -523,6 +519,8 @@ stereo_status stereo_matcher_process(stereo_driver* d, stereo_image* disp, stere
         np.ntx = d->ntx;
         np.nseg = d->nseg;
         np.seg_len = d->seg_len;
+        np.bins = bins;
+        np.bins_nty = bins_nty;
         HIP_TRY(ctx, stereo::launch_ncc(d->mp.window_w, np, s), "ncc launch");
     } else if (aw) {
         stereo::AwParams ap;
@@ -578,6 +576,31 @@ stereo_status stereo_matcher_process(stereo_driver* d, stereo_image* disp, stere
     return end(d, s);
 }
 
+stereo_status stereo_matcher_process(stereo_driver* d, stereo_image* disp, stereo_image* disp_lr, stereo_image* disp_rl,
+                                     const stereo_image* left, const stereo_image* right, stereo_stream stream)
+{
+    if (!d || d->kind != stereo_driver::MATCHER || !left || !right) return STEREO_ERROR_INVALID_ARGUMENT;
+    stereo_context* ctx = d->ctx;
+    const int W = d->fmt.width, H = d->fmt.height;
+    stereo_status st;
+    if ((st = check_image(ctx, left, W, H, STEREO_PIXEL_LUM_U8, "left image"))) return st;
+    if ((st = check_image(ctx, right, W, H, STEREO_PIXEL_LUM_U8, "right image"))) return st;
+    bool dispid = false;
+    if (disp) {
+        if (disp->format.pixel_type == STEREO_PIXEL_RGBA_U16) dispid = true;
+        if ((st = check_image(ctx, disp, W, H, dispid ? STEREO_PIXEL_RGBA_U16 : STEREO_PIXEL_LUM_I16, "disparity output")))
+            return st;
+    }
+    if (disp_lr && (st = check_image(ctx, disp_lr, W, H, STEREO_PIXEL_LUM_I16, "left-to-right output"))) return st;
+    if (disp_rl && (st = check_image(ctx, disp_rl, W, H, STEREO_PIXEL_LUM_I16, "right-to-left output"))) return st;
+    if ((st = set_device(ctx))) return st;
+    GUARD(ctx, "stereo_matcher_process");
+    hipStream_t s = (hipStream_t)stream;
+    if ((st = begin(d, s))) return st;
+
+    return matcher_run(d, disp, dispid, disp_lr, disp_rl, left, right, nullptr, 0, s);
+}
+
 stereo_status stereo_matcher_process_init(stereo_driver* d, stereo_image* disp, stereo_image* disp_lr,
                                           stereo_image* disp_rl, const stereo_image* left, const stereo_image* right,
                                           const stereo_image* disp_init, stereo_stream stream)
@@ -609,6 +632,25 @@ stereo_status stereo_matcher_process_init(stereo_driver* d, stereo_image* disp, 
     if ((st = set_device(ctx))) return st;
     GUARD(ctx, "stereo_matcher_process_init");
     const hipStream_t s = (hipStream_t)stream;
+    stereo::InitBandArgs ib;
+    ib.init = view_of(disp_init);
+    ib.ref_format = ipt == STEREO_PIXEL_RGBA_U16 ? 1 : 0;
+    ib.scale = W / iw;
+    ib.W = W;
+    ib.H = H;
+    if (d->mp.algorithm == STEREO_ALGO_NCC) {
+        // NCC: the per-tile bins, then the NCC frame with every candidate outside its
+        // pixel's bins masked and groups of disparities no pixel searches skipped
+        if ((st = begin(d, s))) return st;
+        stereo::CnnBandArgs ba{};
+        ba.tile_evals = x->tile_evals;
+        ba.tile_mask = x->tile_mask;
+        ba.ntx = x->ntx;
+        ba.nty = x->nty;
+        ba.D = d->mp.num_disparities;
+        HIP_TRY(ctx, stereo::launch_init_band_mask(ib, ba, s), "band launch");
+        return matcher_run(d, disp, dispid, disp_lr, disp_rl, left, right, x->tile_mask, x->nty, s);
+    }
 
     // outputs live in the frame table; rewrite it (synchronously) when they change
     stereo::BmFrame f = x->frame;
@@ -633,12 +675,6 @@ stereo_status stereo_matcher_process_init(stereo_driver* d, stereo_image* disp, 
                 "hipMemcpy2DAsync");
     }
     if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[0], s), "hipEventRecord");
-    stereo::InitBandArgs ib;
-    ib.init = view_of(disp_init);
-    ib.ref_format = ipt == STEREO_PIXEL_RGBA_U16 ? 1 : 0;
-    ib.scale = W / iw;
-    ib.W = W;
-    ib.H = H;
     stereo::CnnBandArgs ba{};
     ba.tiles = x->tiles;
     ba.items = x->items;

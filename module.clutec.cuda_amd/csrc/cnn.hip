@@ -696,6 +696,15 @@ hipError_t launch_init_band(const InitBandArgs& ib, const CnnBandArgs& a, hipStr
     return hipGetLastError();
 }
 
+hipError_t launch_init_band_mask(const InitBandArgs& ib, const CnnBandArgs& a, hipStream_t s)
+{
+    if (a.D > 256 || a.D % 8 || ib.scale < 1) return hipErrorInvalidValue;
+    const int ntiles = a.ntx * a.nty;
+    if (ntiles < 1 || !a.tile_mask || !a.tile_evals) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cnn::init_band_mask_kernel, dim3((ntiles + 3) / 4), dim3(256), 0, s, ib, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_band_items(const CnnBandArgs& a, hipStream_t s)
 {
     if (a.D > 256 || a.D % 8 || a.chunk < 8 || a.chunk % 8) return hipErrorInvalidValue;

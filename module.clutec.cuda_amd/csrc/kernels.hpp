@@ -121,6 +121,10 @@ struct NccParams {
     int W, H, D;
     int bx, by;
     int ntx, nseg, seg_len;
+    // band search: searched 8-disparity bins per 64 x BM_TILE_H tile ([bins_nty][ntx]);
+    // nullptr for the full range
+    const uint32_t* bins;
+    int bins_nty;
 };
 
 struct NccScoreParams {
@@ -212,6 +216,8 @@ struct InitBandArgs {
     int W, H;
 };
 hipError_t launch_init_band(const InitBandArgs& ib, const CnnBandArgs& a, hipStream_t s);
+// only the per-tile bin masks of the band search (a.tile_mask, a.tile_evals)
+hipError_t launch_init_band_mask(const InitBandArgs& ib, const CnnBandArgs& a, hipStream_t s);
 
 bool aw_supported(int sub, int D);
 void aw_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len);

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void ncc_stats_kernel(View img, int2* out, int
     }
 }
 
-template <int AX, int AY, bool EDGE>
+template <int AX, int AY, bool EDGE, bool BAND>
 __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16_t* ssum, float* srinv,
                                          uint32_t (*ebuf)[RG][EMIT], int tx, int ty, int seg)
 {
@@ -160,8 +160,48 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
         lacc[i] = 0;
         racc[i] = 0;
     }
+    // band search: the tile's rows lie in at most two 64 x BM_TILE_H bin tiles; rows before
+    // ysplit take m0, the others m1 (uniform per wave and row); a group of disparities
+    // whose bins neither mask holds is skipped
+    uint32_t m0 = ~0u, m1 = ~0u;
+    int ysplit = 1 << 30;
+    if (BAND) {
+        const int tb0 = y0 / BM_TILE_H, tb1 = min((y0 + TH - 1) / BM_TILE_H, p.bins_nty - 1);
+        m0 = p.bins[tb0 * p.ntx + tx];
+        m1 = p.bins[tb1 * p.ntx + tx];
+        ysplit = (tb0 + 1) * BM_TILE_H;
+    }
 
     for (int dc = ds; dc < de; dc += DCH) {
+        const int dce = min(de, dc + DCH);
+        if (BAND) {
+            const int b0 = dc >> 3, b1 = (dce - 1) >> 3;
+            const uint32_t range = (uint32_t)((2ull << b1) - (1ull << b0));
+            if (((m0 | m1) & range) == 0u) {
+                // no pixel of the tile searches these disparities: the right-view chain
+                // still advances one lane per disparity, with no new candidates
+                for (int d = dc; d < dce; ++d) {
+                    uint32_t zk[RG];
+#pragma unroll
+                    for (int i = 0; i < RG; ++i) zk[i] = 0u;
+                    max_shift_in13(racc, zk);
+                    if (lane == 63) {
+#pragma unroll
+                        for (int i = 0; i < RG; ++i) eb[i * EMIT + (d & (EMIT - 1))] = racc[i];
+                    }
+                    if ((d & (EMIT - 1)) == EMIT - 1 || d == de - 1) {
+                        const int base = d & ~(EMIT - 1);
+                        if (lane <= (d & (EMIT - 1)) && base + lane >= ds) {
+                            const int kk = 318 - (base + lane);
+#pragma unroll
+                            for (int i = 0; i < RG; ++i)
+                                if (i < rows_here) rp[i * BM_NPART + kk] = eb[i * EMIT + lane];
+                        }
+                    }
+                }
+                continue;
+            }
+        }
         // right statistics for right columns x0-dc-(DCH-1) .. x0+63-dc, rows of the tile
         __syncthreads();
         for (int k = tid; k < TH * SCOLS; k += 256) {
@@ -172,7 +212,6 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
             ssum[k] = (int16_t)st.x;
             srinv[k] = __int_as_float(st.y);
         }
-        const int dce = min(de, dc + DCH);
         for (int d = dc; d < dce; ++d) {
             // ---- row phase: products and horizontal sums for disparity d
             uint32_t e[RUN + 2 * AX];
@@ -222,6 +261,10 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
                 asm("v_cvt_u32_f32 %0, %1" : "=v"(q) : "v"(sc));
                 uint32_t k = (q << 8) | dd;
                 if (EDGE) k = ok ? k : 0u;
+                if (BAND) {
+                    const uint32_t mm = (y0 + gw * RG + i < ysplit) ? m0 : m1;
+                    k = ((mm >> (d >> 3)) & 1u) ? k : 0u;
+                }
                 key[i] = k;
                 lacc[i] = lacc[i] > k ? lacc[i] : k;
             }
@@ -256,7 +299,9 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
     }
 }
 
-template <int AX, int AY>
+// BAND: the band search's instantiation, a kernel of its own so that the full-range
+// kernel keeps its register count (and three workgroups per CU)
+template <int AX, int AY, bool BAND>
 __global__ __launch_bounds__(256, 2) void ncc_fused_kernel(NccParams p)
 {
     __shared__ uint32_t hbuf[HROWS * HSTRIDE];
@@ -273,9 +318,9 @@ __global__ __launch_bounds__(256, 2) void ncc_fused_kernel(NccParams p)
     const int de = min(p.D, (seg + 1) * p.seg_len);
     const bool edge = (x0 < p.bx + de - 1) || (x0 + TW > p.W - p.bx);
     if (edge)
-        ncc_tile<AX, AY, true>(p, hbuf, ssum, srinv, ebuf, tx, ty, seg);
+        ncc_tile<AX, AY, true, BAND>(p, hbuf, ssum, srinv, ebuf, tx, ty, seg);
     else
-        ncc_tile<AX, AY, false>(p, hbuf, ssum, srinv, ebuf, tx, ty, seg);
+        ncc_tile<AX, AY, false, BAND>(p, hbuf, ssum, srinv, ebuf, tx, ty, seg);
 }
 
 // Winning score of every left pixel: the same FP32 arithmetic as the matcher, on a
@@ -344,12 +389,19 @@ hipError_t launch_ncc_stats(const View& img, int2* out, int pitch_px, int ww, in
 hipError_t launch_ncc(int ww, const NccParams& p, hipStream_t s)
 {
     dim3 grid(p.ntx * ((p.H + ncc::TH - 1) / ncc::TH) * p.nseg);
+#define NCC_LAUNCH(A)                                                                          \
+    if (p.bins)                                                                                   \
+        hipLaunchKernelGGL((ncc::ncc_fused_kernel<A, A, true>), grid, dim3(256), 0, s, p);         \
+    else                                                                                          \
+        hipLaunchKernelGGL((ncc::ncc_fused_kernel<A, A, false>), grid, dim3(256), 0, s, p);        \
+    break
     switch (ww) {
-    case 11: hipLaunchKernelGGL((ncc::ncc_fused_kernel<5, 5>), grid, dim3(256), 0, s, p); break;
-    case 9: hipLaunchKernelGGL((ncc::ncc_fused_kernel<4, 4>), grid, dim3(256), 0, s, p); break;
-    case 7: hipLaunchKernelGGL((ncc::ncc_fused_kernel<3, 3>), grid, dim3(256), 0, s, p); break;
-    default: hipLaunchKernelGGL((ncc::ncc_fused_kernel<2, 2>), grid, dim3(256), 0, s, p); break;
+    case 11: NCC_LAUNCH(5);
+    case 9: NCC_LAUNCH(4);
+    case 7: NCC_LAUNCH(3);
+    default: NCC_LAUNCH(2);
     }
+#undef NCC_LAUNCH
     return hipGetLastError();
 }
 

@@ -49,6 +49,7 @@ def _load():
     lib.orc_bm_bins.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P, i]
     lib.orc_bm_bins.restype = i
     lib.orc_ncc.argtypes = [P, P, i, i, i, i, i, i, P, P, P, P, P, i]
+    lib.orc_ncc_bins.argtypes = [P, P, i, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_aw.argtypes = [i, P, P, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_crosscheck_plain.argtypes = [P, P, i, i, i, P]
     lib.orc_crosscheck_ref.argtypes = [P, P, i, i, i, i, i, P, P, P]
@@ -62,7 +63,7 @@ def _load():
     for name in ["orc_census", "orc_census3x3_ref", "orc_bm", "orc_bm_brute", "orc_ncc", "orc_aw",
                  "orc_crosscheck_plain", "orc_crosscheck_ref", "orc_mip_level0_ref", "orc_mip_level0_plain",
                  "orc_mip_next", "orc_disp_denoise", "orc_bm1",
-                 "orc_rectify", "orc_color_disparity"]:
+                 "orc_rectify", "orc_color_disparity", "orc_ncc_bins"]:
         getattr(lib, name).restype = i
     return lib
 
@@ -166,6 +167,22 @@ def ncc(L, R, D, window=(11, 11), tol=1, nthreads=1, with_volume=False):
     _chk(lib.orc_ncc(_p(L), _p(R), W, H, D, window[0], window[1], tol, _p(dL), _p(dR), _p(out), _p(sc), _p(vol),
                      nthreads), "orc_ncc")
     return (out, dL, dR, sc, vol) if with_volume else (out, dL, dR, sc)
+
+
+def ncc_bins(L, R, D, bins, window=(11, 11), tol=1, nthreads=1):
+    """orc_ncc restricted to the 8-disparity bins set in bins[y, x] (uint32) of each left pixel."""
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    bins = np.ascontiguousarray(bins, np.uint32)
+    assert bins.shape == (H, W)
+    dL = np.empty((H, W), np.int16)
+    dR = np.empty((H, W), np.int16)
+    out = np.empty((H, W), np.int16)
+    sc = np.empty((H, W), np.float64)
+    _chk(lib.orc_ncc_bins(_p(L), _p(R), W, H, D, window[0], window[1], tol, _p(bins), _p(dL), _p(dR), _p(out), _p(sc),
+                          nthreads), "orc_ncc_bins")
+    return out, dL, dR, sc
 
 
 def aw(mode, L, R, D, sub=5, tol=1, nthreads=1, with_volume=False):

@@ -293,10 +293,15 @@ static void box_stats(const uint8_t* img, int W, int H, int ax, int ay, int64_t*
         }
 }
 
-int orc_ncc(const uint8_t* L, const uint8_t* R, int W, int H, int D, int ww, int wh, int tol,
-            int16_t* dL, int16_t* dR, int16_t* out, double* score_out, double* volume, int nthreads)
+/* NCC over the candidates of every left pixel p, or only those whose 8-disparity bin
+ * (d >> 3) is set in bins[p] (the band search, MatcherUseDisparity BlockMatcherNCC.cu:894-1255,
+ * with the engine's per-tile bins). */
+static int ncc_impl(const uint8_t* L, const uint8_t* R, int W, int H, int D, int ww, int wh, int tol,
+                    const uint32_t* bins, int16_t* dL, int16_t* dR, int16_t* out, double* score_out, double* volume,
+                    int nthreads)
 {
     if (!L || !R || !dL || !dR || W <= 0 || H <= 0 || D <= 0 || ww < 1 || wh < 1 || !(ww & 1) || !(wh & 1)) return -1;
+    if (bins && D > 256) return -1;
     const int ax = ww / 2, ay = wh / 2, bx = ax, by = ay;
     const int64_t Nw = (int64_t)ww * wh;
     const size_t N = (size_t)W * H;
@@ -345,6 +350,7 @@ int orc_ncc(const uint8_t* L, const uint8_t* R, int W, int H, int D, int ww, int
                     if (vI >= Nw * Nw && vJ >= Nw * Nw)
                         s = (double)(Nw * sij - SI[pl] * SJ[pr]) / sqrt((double)vI * (double)vJ);
                     if (volume) volume[pl * D + d] = s;
+                    if (bins && !((bins[pl] >> (d >> 3)) & 1u)) continue;
                     const int64_t q = (int64_t)floor((s > 0.0 ? s : 0.0) * 8388608.0);
                     if (q > bestL[pl]) { bestL[pl] = q; dL[pl] = (int16_t)d; sc[pl] = s; }
                     if (q > bestR[pr]) { bestR[pr] = q; dR[pr] = (int16_t)d; }
@@ -357,6 +363,19 @@ int orc_ncc(const uint8_t* L, const uint8_t* R, int W, int H, int D, int ww, int
     if (score_out) memcpy(score_out, sc, N * sizeof(double));
     free(SI); free(SII); free(SJ); free(SJJ); free(bestL); free(bestR); free(sc);
     return 0;
+}
+
+int orc_ncc(const uint8_t* L, const uint8_t* R, int W, int H, int D, int ww, int wh, int tol,
+            int16_t* dL, int16_t* dR, int16_t* out, double* score_out, double* volume, int nthreads)
+{
+    return ncc_impl(L, R, W, H, D, ww, wh, tol, NULL, dL, dR, out, score_out, volume, nthreads);
+}
+
+int orc_ncc_bins(const uint8_t* L, const uint8_t* R, int W, int H, int D, int ww, int wh, int tol,
+                 const uint32_t* bins, int16_t* dL, int16_t* dR, int16_t* out, double* score_out, int nthreads)
+{
+    if (!bins) return -1;
+    return ncc_impl(L, R, W, H, D, ww, wh, tol, bins, dL, dR, out, score_out, NULL, nthreads);
 }
 
 /* ---------------------------------------------------------------------------------

@@ -63,6 +63,10 @@ int orc_bm_brute(int cost_kind, const void* L, const void* R, int W, int H, int 
  * left map (NaN where invalid); volume (optional, (y*W+x)*D+d) = every score. */
 int orc_ncc(const uint8_t* L, const uint8_t* R, int W, int H, int D, int ww, int wh, int tol,
             int16_t* dL, int16_t* dR, int16_t* out, double* score_out, double* volume, int nthreads);
+/* orc_ncc restricted to candidates whose 8-disparity bin d >> 3 is set in bins[p] of the
+ * left pixel p (D <= 256): the band search. */
+int orc_ncc_bins(const uint8_t* L, const uint8_t* R, int W, int H, int D, int ww, int wh, int tol,
+                 const uint32_t* bins, int16_t* dL, int16_t* dR, int16_t* out, double* score_out, int nthreads);
 
 /* Adaptive-window matchers.  mode 1 = AW1 similarity (255-|a-b|, two best of four
  * neighbouring sub-patches, maximise); mode 2 = AW2 SAD (two smallest, minimise).

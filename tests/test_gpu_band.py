@@ -76,6 +76,50 @@ def test_band_search_matches_oracle(ctx, algo, window, scale, fmt):
     print(f"  agreement with the full range {(got[0] == ref_full).mean():.4f}", flush=True)
 
 
+@pytest.mark.parametrize("scale,win,fmt", [(2, (11, 11), "i16"), (1, (7, 7), "ref"), (4, (9, 9), "i16")])
+def test_ncc_band_search_matches_oracle(ctx, scale, win, fmt):
+    """NCC around an initial map (MatcherUseDisparity of BlockMatcherNCC): the oracle's NCC
+    restricted to the same per-pixel bins, with the NCC matcher's parity rules: the
+    GPU's winning scores within 1e-5 relative of FP64, maps equal except at FP32
+    near-ties, and every winner inside its pixel's bins."""
+    W, H, D = 400, 250, 128
+    print(f"ncc band search scale {scale} window {win} init {fmt}", flush=True)
+    L, R = cs.synth_pair(95, W, H, 2, 8, D - 1, 8)
+    full = cs.Matcher(ctx, W, H, algorithm=cs.ALGO_NCC, window=win, num_disp=D, tol=1)
+    ref_full = full.run(L, R)
+    rng = np.random.default_rng(10 + scale)
+    small = ref_full[::scale, ::scale][: H // scale, : W // scale] // scale
+    small = np.where(rng.random(small.shape) < 0.1, -1, small).astype(np.int16)
+    small[: 60 // scale, : 130 // scale] = -1  # tiles with no valid value search the full range
+    if fmt == "ref":
+        init = np.zeros(small.shape + (4,), np.uint16)
+        init[..., 0] = np.where(small >= 0, small + 1, 0xFFFD)
+    else:
+        init = small
+    m = cs.Matcher(ctx, W, H, algorithm=cs.ALGO_NCC, window=win, num_disp=D, tol=1, use_disp_input=True)
+    out, lr, rl, score = m.run_init(L, R, init, maps=True, scores=True)
+    bins = _tile_bins(init, W, H, D, ref_format=(fmt == "ref"))
+    ref_out, ref_l, ref_r, _ = orc.ncc_bins(L, R, D, bins, window=win, tol=1, nthreads=8)
+    vol = orc.ncc(L, R, D, window=win, tol=1, nthreads=8, with_volume=True)[4]
+    valid = ref_l >= 0
+    assert np.array_equal(valid, lr >= 0)
+    yy, xx = np.nonzero(valid)
+    assert ((bins[yy, xx] >> (lr[yy, xx] >> 3).astype(np.uint32)) & 1).all()
+    s64 = vol[yy, xx, lr[yy, xx]]
+    s32 = score[yy, xx].astype(np.float64)
+    zero = s64 == 0.0
+    assert (s32[zero] == 0.0).all()
+    assert (np.abs(s32[~zero] - s64[~zero]) / np.abs(s64[~zero])).max() <= 1e-5
+    for a, b in ((lr, ref_l), (rl, ref_r), (out, ref_out)):
+        assert (a == b).mean() >= 0.999
+    dif = np.nonzero((lr != ref_l) & valid)
+    if dif[0].size:
+        assert np.all(np.abs(vol[dif[0], dif[1], lr[dif]] - vol[dif[0], dif[1], ref_l[dif]]) <= 1e-6)
+    searched = np.mean([bin(int(b)).count("1") for b in np.unique(bins)]) * 8 / D
+    print(f"  agreement with the full range {(out == ref_full).mean():.4f}, mean searched fraction {searched:.3f}",
+          flush=True)
+
+
 def test_band_search_without_valid_init_is_full_range(ctx):
     W, H, D = 320, 224, 64
     L, R = cs.synth_pair(92, W, H, 0, 6, D - 1, 8)
@@ -90,7 +134,7 @@ def test_band_search_without_valid_init_is_full_range(ctx):
 def test_band_search_configuration_errors(ctx):
     W, H, D = 256, 128, 64
     with pytest.raises(cs.StereoError) as e:
-        cs.Matcher(ctx, W, H, algorithm=cs.ALGO_NCC, window=(11, 11), num_disp=D, use_disp_input=True)
+        cs.Matcher(ctx, W, H, algorithm=cs.ALGO_AW1, window=(5, 5), num_disp=D, use_disp_input=True)
     assert e.value.status == cs.ERROR_INVALID_CONFIG
     m = cs.Matcher(ctx, W, H, algorithm=cs.ALGO_SAD, window=(9, 9), num_disp=D)
     L, R = cs.synth_pair(93, W, H, 0, 4, D - 1, 8)

@@ -331,3 +331,39 @@ def test_aw1_aw2_equivalent_winners():
     v = a1[1] >= 0
     # similarity of the three chosen sub-patches = 255 * 3 * 25 - their SAD
     assert np.array_equal(a1[3][v], 255 * 75 - a2[3][v])
+
+
+def test_ncc_bins_against_the_score_volume():
+    """The bins-restricted NCC: with every bin set it is orc_ncc; with a random mask per left
+    pixel, both views equal the argmax over the allowed candidates of the FP64 score volume
+    (key floor(max(s, 0) * 2^23), lowest d on ties)."""
+    import numpy as np
+    rng = np.random.default_rng(4)
+    H, W, D = 28, 90, 32
+    L = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    R = np.roll(L, -5, axis=1)
+    R = np.clip(R.astype(int) + rng.integers(-6, 7, R.shape), 0, 255).astype(np.uint8)
+    full = orc.ncc(L, R, D, window=(7, 7), tol=1)
+    every = orc.ncc_bins(L, R, D, np.full((H, W), 0xF, np.uint32), window=(7, 7), tol=1)
+    for a, b in zip(full, every):
+        assert np.array_equal(np.nan_to_num(a, nan=-7), np.nan_to_num(b, nan=-7))
+    bins = rng.integers(0, 16, (H, W)).astype(np.uint32)
+    out, dl, dr, _ = orc.ncc_bins(L, R, D, bins, window=(7, 7), tol=1)
+    vol = orc.ncc(L, R, D, window=(7, 7), tol=1, with_volume=True)[4]
+    ok = ~np.isnan(vol)
+    d = np.arange(D)
+    allowed = ok & (((bins[:, :, None] >> (d[None, None, :] >> 3)) & 1) == 1)
+    q = np.where(allowed, np.floor(np.maximum(np.nan_to_num(vol), 0.0) * 8388608.0), -1)
+    want_l = np.where(allowed.any(-1), q.argmax(-1), -1)
+    assert np.array_equal(dl, want_l)
+    # right view: right pixel xr takes d over the left pixels xr + d that allow d
+    want_r = np.full((H, W), -1)
+    best = np.full((H, W), -1.0)
+    for dd in range(D):
+        xs = np.arange(W - dd)
+        cand = q[:, xs + dd, dd]
+        better = cand > best[:, xs]
+        best[:, xs] = np.where(better, cand, best[:, xs])
+        want_r[:, xs] = np.where(better, dd, want_r[:, xs])
+    assert np.array_equal(dr, want_r)
+    assert (dl >= 0).mean() > 0.5
