@@ -170,53 +170,127 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
         m0 = p.bins[tb0 * p.ntx + tx];
         m1 = p.bins[tb1 * p.ntx + tx];
         ysplit = (tb0 + 1) * BM_TILE_H;
+        const int b0 = ds >> 3, b1 = (de - 1) >> 3;
+        if (((m0 | m1) & (uint32_t)((2ull << b1) - (1ull << b0))) == 0u) {
+            // nothing to search in this segment: every right-view partial the loop below
+            // would write (slots 256 - de .. 318 - ds) and every left key is the empty key
+            const int lane = tid & 63, gw = tid >> 6;
+            const int rows_here = min(RG, H - (y0 + gw * RG));
+            uint32_t* const rp = p.rpart + (((size_t)seg * p.ntx + tx) * H + y0 + gw * RG) * BM_NPART;
+            for (int kk = 256 - de + lane; kk <= 318 - ds; kk += 64)
+                for (int i = 0; i < rows_here; ++i) rp[i * BM_NPART + kk] = 0u;
+            const int x = x0 + lane;
+            if (x < W) {
+                uint32_t* lk = p.lkey + (size_t)seg * W * H + (size_t)(y0 + gw * RG) * W + x;
+                for (int i = 0; i < rows_here; ++i) lk[(size_t)i * W] = 0u;
+            }
+            return;
+        }
     }
 
+    // right pixels of the run for the current disparity d, byte i of rw = R(xs + i - d)
+    // (little-endian over RWW words), and the one that enters next, R(xs - d - 1): one load
+    // per disparity instead of RUN (every offset down to xs - de - 1 lies in the matcher's
+    // left padding)
+    constexpr int RWW = (RUN + 3) / 4;
+    // after disparity d: at the end of a block of EMIT disparities (or of the segment) the
+    // right-view values parked in the wave's LDS rows go to the partials
+    auto flush = [&](int d) {
+        if ((d & (EMIT - 1)) == EMIT - 1 || d == de - 1) {
+            const int base = d & ~(EMIT - 1);
+            if (lane <= (d & (EMIT - 1)) && base + lane >= ds) {
+                const int kk = 318 - (base + lane);
+#pragma unroll
+                for (int i = 0; i < RG; ++i)
+                    if (i < rows_here) rp[i * BM_NPART + kk] = eb[i * EMIT + lane];
+            }
+        }
+    };
+    uint32_t rw[RWW], rnext = 0;
+    int wd = -1;  // the disparity rw holds
     for (int dc = ds; dc < de; dc += DCH) {
         const int dce = min(de, dc + DCH);
+        // the band search: rows of this wave that search the group (bit r: row r), wave-uniform
+        uint32_t keep = (1u << RG) - 1u;
         if (BAND) {
-            const int b0 = dc >> 3, b1 = (dce - 1) >> 3;
-            const uint32_t range = (uint32_t)((2ull << b1) - (1ull << b0));
-            if (((m0 | m1) & range) == 0u) {
-                // no pixel of the tile searches these disparities: the right-view chain
-                // still advances one lane per disparity, with no new candidates
-                for (int d = dc; d < dce; ++d) {
-                    uint32_t zk[RG];
+            const uint32_t bit = 1u << (dc >> 3);  // the group's bin (dc is a multiple of 8)
+            const uint32_t lo_rows = (1u << clampi(ysplit - (y0 + gw * RG), 0, RG)) - 1u;
+            keep = ((m0 & bit) ? lo_rows : 0u) | ((m1 & bit) ? (((1u << RG) - 1u) & ~lo_rows) : 0u);
+            if (((m0 | m1) & bit) == 0u) {
+                // no pixel of the tile searches these 8 disparities: the right-view chain
+                // moves on by 8 lanes with no new candidates.  The values that leave the
+                // wave at lane 63 during those 8 steps sit in lanes 62 .. 55 now.
+                if (lane >= 63 - DCH && lane < 63) {
+                    const int slot = (dc + 62 - lane) & (EMIT - 1);
 #pragma unroll
-                    for (int i = 0; i < RG; ++i) zk[i] = 0u;
-                    max_shift_in13(racc, zk);
-                    if (lane == 63) {
-#pragma unroll
-                        for (int i = 0; i < RG; ++i) eb[i * EMIT + (d & (EMIT - 1))] = racc[i];
-                    }
-                    if ((d & (EMIT - 1)) == EMIT - 1 || d == de - 1) {
-                        const int base = d & ~(EMIT - 1);
-                        if (lane <= (d & (EMIT - 1)) && base + lane >= ds) {
-                            const int kk = 318 - (base + lane);
-#pragma unroll
-                            for (int i = 0; i < RG; ++i)
-                                if (i < rows_here) rp[i * BM_NPART + kk] = eb[i * EMIT + lane];
-                        }
-                    }
+                    for (int i = 0; i < RG; ++i) eb[i * EMIT + slot] = racc[i];
                 }
+#pragma unroll
+                for (int i = 0; i < RG; ++i) {
+                    const uint32_t v = __shfl_up(racc[i], DCH);
+                    racc[i] = lane >= DCH ? v : 0u;
+                }
+                flush(dce - 1);
                 continue;
             }
         }
-        // right statistics for right columns x0-dc-(DCH-1) .. x0+63-dc, rows of the tile
-        __syncthreads();
-        for (int k = tid; k < TH * SCOLS; k += 256) {
-            const int r = k / SCOLS, c = k - r * SCOLS;
-            const int y = min(y0 + r, H - 1);
-            const int xc = clampi(x0 - dc - (DCH - 1) + c, 0, W - 1);
-            const int2 st = p.rstats[(size_t)y * p.stats_pitch + xc];
-            ssum[k] = (int16_t)st.x;
-            srinv[k] = __int_as_float(st.y);
+        // right statistics for right columns x0-dc-(DCH-1) .. x0+63-dc, rows of the tile:
+        // the thread's loads (the last index repeated past the end) are all issued before
+        // the barrier, the LDS stores after it
+        {
+            constexpr int SN = TH * SCOLS, SIT = (SN + 255) / 256, SB = SIT;
+            static_assert(SIT % SB == 0, "whole batches");
+            // an opaque copy of the thread index: the addresses are recomputed here rather
+            // than hoisted out of the disparity loop into live registers
+            int t0 = tid;
+            asm volatile("" : "+v"(t0));
+            auto sload = [&](int j) {
+                const int k = min(t0 + 256 * j, SN - 1);
+                const int r = k / SCOLS, c = k - r * SCOLS;
+                const int y = min(y0 + r, H - 1);
+                const int xc = clampi(x0 - dc - (DCH - 1) + c, 0, W - 1);
+                return p.rstats[(size_t)y * p.stats_pitch + xc];
+            };
+#pragma unroll
+            for (int j0 = 0; j0 < SIT; j0 += SB) {
+                int2 v[SB];
+#pragma unroll
+                for (int j = 0; j < SB; ++j) v[j] = sload(j0 + j);
+                if (j0 == 0) __syncthreads();
+#pragma unroll
+                for (int j = 0; j < SB; ++j) {
+                    const int k = tid + 256 * (j0 + j);
+                    if (k < SN) {
+                        ssum[k] = (int16_t)v[j].x;
+                        srinv[k] = __int_as_float(v[j].y);
+                    }
+                }
+            }
+        }
+        if (wd != dc) {
+            uint32_t b[RUN];
+#pragma unroll
+            for (int i = 0; i < RUN; ++i) b[i] = rowR[i - dc];
+            rnext = rowR[-dc - 1];
+#pragma unroll
+            for (int k = 0; k < RWW; ++k) {
+                uint32_t w = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (4 * k + j < RUN) w |= b[4 * k + j] << (8 * j);
+                rw[k] = w;
+            }
         }
         for (int d = dc; d < dce; ++d) {
             // ---- row phase: products and horizontal sums for disparity d
             uint32_t e[RUN + 2 * AX];
 #pragma unroll
-            for (int i = 0; i < RUN; ++i) e[AX + i] = il[i] * (uint32_t)rowR[i - d];
+            for (int i = 0; i < RUN; ++i) e[AX + i] = il[i] * ((rw[i >> 2] >> (8 * (i & 3))) & 0xffu);
+#pragma unroll
+            for (int k = RWW - 1; k > 0; --k) rw[k] = __builtin_amdgcn_alignbyte(rw[k], rw[k - 1], 3);
+            rw[0] = (rw[0] << 8) | rnext;
+            rnext = rowR[-d - 2];
+            wd = d + 1;
             __syncthreads();  // the previous column phase has read the H buffer
 #pragma unroll
             for (int i = 0; i < AX; ++i) {
@@ -261,10 +335,7 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
                 asm("v_cvt_u32_f32 %0, %1" : "=v"(q) : "v"(sc));
                 uint32_t k = (q << 8) | dd;
                 if (EDGE) k = ok ? k : 0u;
-                if (BAND) {
-                    const uint32_t mm = (y0 + gw * RG + i < ysplit) ? m0 : m1;
-                    k = ((mm >> (d >> 3)) & 1u) ? k : 0u;
-                }
+                if (BAND) k = ((keep >> i) & 1u) ? k : 0u;
                 key[i] = k;
                 lacc[i] = lacc[i] > k ? lacc[i] : k;
             }
@@ -274,15 +345,7 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
 #pragma unroll
                 for (int i = 0; i < RG; ++i) eb[i * EMIT + (d & (EMIT - 1))] = racc[i];
             }
-            if ((d & (EMIT - 1)) == EMIT - 1 || d == de - 1) {
-                const int base = d & ~(EMIT - 1);
-                if (lane <= (d & (EMIT - 1)) && base + lane >= ds) {
-                    const int kk = 318 - (base + lane);
-#pragma unroll
-                    for (int i = 0; i < RG; ++i)
-                        if (i < rows_here) rp[i * BM_NPART + kk] = eb[i * EMIT + lane];
-                }
-            }
+            flush(d);
         }
     }
     {
@@ -302,7 +365,7 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
 // BAND: the band search's instantiation, a kernel of its own so that the full-range
 // kernel keeps its register count (and three workgroups per CU)
 template <int AX, int AY, bool BAND>
-__global__ __launch_bounds__(256, 2) void ncc_fused_kernel(NccParams p)
+__global__ __launch_bounds__(256, 3) void ncc_fused_kernel(NccParams p)
 {
     __shared__ uint32_t hbuf[HROWS * HSTRIDE];
     // right statistics of the tile: window sums (at most 121 x 255, an int16) and 1/sigma
@@ -311,7 +374,10 @@ __global__ __launch_bounds__(256, 2) void ncc_fused_kernel(NccParams p)
     __shared__ uint32_t ebuf[4][RG][EMIT];
     // one-dimensional grid of ntx * nty * nseg workgroups, renumbered per XCD
     const int nty = (p.H + TH - 1) / TH;
-    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    // the band search keeps the hardware's round-robin over the XCDs: its work sits in a
+    // few segments and image regions, and a contiguous range per XCD would load some
+    // XCDs far more than others
+    const int w = BAND ? (int)blockIdx.x : xcd_remap(blockIdx.x, gridDim.x);
     const int tx = w % p.ntx, rest = w / p.ntx;
     const int ty = rest % nty, seg = rest / nty;
     const int x0 = tx * TW;
