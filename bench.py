@@ -254,7 +254,7 @@ def other_matchers_leg(cs, np, ctx, steps):
             r["kernel_valu_frac"] = round(w * h * d * ops / (kms * 1e-3) / (PEAK_INT_TOPS * 1e12), 4)
             r["ops_per_eval"] = ops
         res[name] = r
-        if algo == cs.ALGO_NCC:
+        if algo in (cs.ALGO_NCC, cs.ALGO_AW1):
             # the band search around an initial map (MatcherUseDisparity): the full-range
             # map at 1/4 resolution (made before the timed region)
             full = out.download()

@@ -151,7 +151,7 @@ typedef struct stereo_matcher_params {
     int32_t num_disparities; /* D: candidates 0..D-1, multiple of 8, <= 256 */
     int32_t lr_tolerance;    /* |dR(x-dL) - dL| <= tol keeps the pixel */
     int32_t use_disp_input;  /* 1: also configure stereo_matcher_process_init (SParameter::iUseDispInput,
-                                BlockMatcherNCC.cu:1330); SAD, CENSUS and NCC */
+                                BlockMatcherNCC.cu:1330); every algorithm */
     int32_t reserved[4];     /* must be 0 */
 } stereo_matcher_params;
 

@@ -45,7 +45,10 @@ __device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b)
                                                                  __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned short, b)));
 }
 
-template <int SP, bool EDGE>
+// BAND: the band search around an initial map (MatcherUseDisparity of BlockMatcherAW1.cu:1336
+// and BlockMatcherAW2.cu:858): only the 8-disparity bins set in the tile's masks p.bins
+// are searched (the same per-tile bins as the SAD, census and NCC matchers).
+template <int SP, bool EDGE, bool BAND>
 __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64], int tx, int ty, int seg)
 {
     constexpr int R = SP / 2;        // sub-patch radius
@@ -58,6 +61,33 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
     const int W = p.W, H = p.H;
     const int ds = seg * p.seg_len;
     const int de = min(p.D, ds + p.seg_len);
+
+    // band search: the tile's rows lie in at most two 64 x BM_TILE_H bin tiles; rows before
+    // ysplit take m0, the others m1
+    uint32_t m0 = ~0u, m1 = ~0u;
+    int ysplit = 1 << 30;
+    if (BAND) {
+        const int tb0 = y0 / BM_TILE_H, tb1 = min((y0 + TH - 1) / BM_TILE_H, p.bins_nty - 1);
+        m0 = p.bins[tb0 * p.ntx + tx];
+        m1 = p.bins[tb1 * p.ntx + tx];
+        ysplit = (tb0 + 1) * BM_TILE_H;
+        const int b0 = ds >> 3, b1 = (de - 1) >> 3;
+        if (((m0 | m1) & (uint32_t)((2ull << b1) - (1ull << b0))) == 0u) {
+            // nothing to search in this segment: every right-view partial the loop below
+            // would write (slots 256 - de .. 318 - ds) and every left key is the empty key
+            const int lane = tid & 63, gw = tid >> 6;
+            const int rows_here = min(RG, H - (y0 + gw * RG));
+            uint32_t* const rp = p.rpart + (((size_t)seg * p.ntx + tx) * H + y0 + gw * RG) * BM_NPART;
+            for (int kk = 256 - de + lane; kk <= 318 - ds; kk += 64)
+                for (int i = 0; i < rows_here; ++i) rp[i * BM_NPART + kk] = 0u;
+            const int x = x0 + lane;
+            if (x < W) {
+                uint32_t* lk = p.lkey + (size_t)seg * W * H + (size_t)(y0 + gw * RG) * W + x;
+                for (int i = 0; i < rows_here; ++i) lk[(size_t)i * W] = 0u;
+            }
+            return;
+        }
+    }
 
     // row phase identity: H row hr is image row y0 - B + hr
     const int run = tid & 3;
@@ -83,8 +113,46 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
 #pragma unroll
     for (int i = 0; i < RG; ++i) { lacc[i] = 0; racc[i] = 0; }
 
+    // after the pair ending at disparity dl: at the end of a block of 64 disparities (or of
+    // the segment) the right-view values parked in the wave's LDS rows go to the partials
+    auto flush = [&](int dl) {
+        if ((dl & 63) == 63 || dl == de - 1) {
+            const int base = dl & ~63;
+            if (lane <= (dl & 63) && base + lane >= ds) {
+                const int k = 318 - (base + lane);
+#pragma unroll
+                for (int i = 0; i < RG; ++i)
+                    if (i < rows_here) rp[i * BM_NPART + k] = eb[i * 64 + lane];
+            }
+        }
+    };
+
     int s = 0;
     for (int d0 = ds; d0 < de; d0 += DC, rowR -= DC) {
+        // the band search: rows of this wave that search the group (bit o: row o), wave-uniform
+        uint32_t keep = (1u << RG) - 1u;
+        if (BAND) {
+            const uint32_t bit = 1u << (d0 >> 3);  // the group's bin (d0 is a multiple of 8)
+            const uint32_t lo_rows = (1u << clampi(ysplit - (y0 + gw * RG), 0, RG)) - 1u;
+            keep = ((m0 & bit) ? lo_rows : 0u) | ((m1 & bit) ? (((1u << RG) - 1u) & ~lo_rows) : 0u);
+            if (((m0 | m1) & bit) == 0u) {
+                // no pixel of the tile searches these 8 disparities: the right-view chain
+                // moves on by 8 lanes with no new candidates.  The values that leave the
+                // wave at lane 63 during those 8 steps sit in lanes 62 .. 55 now.
+                if (lane >= 63 - DC && lane < 63) {
+                    const int slot = (d0 + 62 - lane) & 63;
+#pragma unroll
+                    for (int i = 0; i < RG; ++i) eb[i * 64 + slot] = racc[i];
+                }
+#pragma unroll
+                for (int i = 0; i < RG; ++i) {
+                    const uint32_t v = __shfl_up(racc[i], DC);
+                    racc[i] = lane >= DC ? v : 0u;
+                }
+                flush(d0 + DC - 1);
+                continue;
+            }
+        }
         uint32_t cr[RUN + DC - 1];
 #pragma unroll
         for (int k = 0; k < RUN + DC - 1; ++k) cr[k] = rowR[k];
@@ -165,6 +233,11 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
                 k0[o] = __builtin_amdgcn_perm(ddv, abar, 0x0C010004u);
                 k1[o] = __builtin_amdgcn_perm(ddv, abar, 0x0C030205u);
                 if (EDGE) { k0[o] = ok0 ? k0[o] : 0u; k1[o] = ok1 ? k1[o] : 0u; }
+                if (BAND) {
+                    const bool kp = (keep >> o) & 1u;
+                    k0[o] = kp ? k0[o] : 0u;
+                    k1[o] = kp ? k1[o] : 0u;
+                }
                 const uint32_t kk = k0[o] > k1[o] ? k0[o] : k1[o];
                 lacc[o] = lacc[o] > kk ? lacc[o] : kk;
             }
@@ -179,16 +252,7 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
 #pragma unroll
                 for (int i = 0; i < RG; ++i) eb[i * 64 + ((d + 1) & 63)] = racc[i];
             }
-            const int dl = d + 1;
-            if ((dl & 63) == 63 || dl == de - 1) {
-                const int base = dl & ~63;
-                if (lane <= (dl & 63) && base + lane >= ds) {
-                    const int k = 318 - (base + lane);
-#pragma unroll
-                    for (int i = 0; i < RG; ++i)
-                        if (i < rows_here) rp[i * BM_NPART + k] = eb[i * 64 + lane];
-                }
-            }
+            flush(d + 1);
         }
     }
     {
@@ -205,23 +269,25 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
     }
 }
 
-template <int SP>
+template <int SP, bool BAND>
 __global__ __launch_bounds__(256, 2) void aw_fused_kernel(AwParams p)
 {
     __shared__ uint32_t hbuf[2][HROWS * HSTRIDE];
     __shared__ uint32_t ebuf[4][RG][64];
-    // one-dimensional grid of ntx * nty * nseg workgroups, renumbered per XCD
+    // one-dimensional grid of ntx * nty * nseg workgroups, renumbered per XCD; the band
+    // search keeps the hardware's round-robin over the XCDs, since its work sits in a few
+    // segments and image regions
     const int nty = (p.H + TH - 1) / TH;
-    const int w = xcd_remap(blockIdx.x, gridDim.x);
+    const int w = BAND ? (int)blockIdx.x : xcd_remap(blockIdx.x, gridDim.x);
     const int tx = w % p.ntx, rest = w / p.ntx;
     const int ty = rest % nty, seg = rest / nty;
     const int x0 = tx * TW;
     const int de = min(p.D, (seg + 1) * p.seg_len);
     const bool edge = (x0 < p.bx + de - 1) || (x0 + TW > p.W - p.bx);
     if (edge)
-        aw_tile<SP, true>(p, hbuf, ebuf, tx, ty, seg);
+        aw_tile<SP, true, BAND>(p, hbuf, ebuf, tx, ty, seg);
     else
-        aw_tile<SP, false>(p, hbuf, ebuf, tx, ty, seg);
+        aw_tile<SP, false, BAND>(p, hbuf, ebuf, tx, ty, seg);
 }
 
 }  // namespace aw
@@ -246,10 +312,17 @@ void aw_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len)
 hipError_t launch_aw(int sub, const AwParams& p, hipStream_t s)
 {
     dim3 grid(p.ntx * ((p.H + aw::TH - 1) / aw::TH) * p.nseg);
-    if (sub == 5)
-        hipLaunchKernelGGL((aw::aw_fused_kernel<5>), grid, dim3(256), 0, s, p);
-    else
-        hipLaunchKernelGGL((aw::aw_fused_kernel<3>), grid, dim3(256), 0, s, p);
+    if (sub == 5) {
+        if (p.bins)
+            hipLaunchKernelGGL((aw::aw_fused_kernel<5, true>), grid, dim3(256), 0, s, p);
+        else
+            hipLaunchKernelGGL((aw::aw_fused_kernel<5, false>), grid, dim3(256), 0, s, p);
+    } else {
+        if (p.bins)
+            hipLaunchKernelGGL((aw::aw_fused_kernel<3, true>), grid, dim3(256), 0, s, p);
+        else
+            hipLaunchKernelGGL((aw::aw_fused_kernel<3, false>), grid, dim3(256), 0, s, p);
+    }
     return hipGetLastError();
 }
 

@@ -314,8 +314,9 @@ stereo_status band_alloc(stereo_driver* d)
     x->ntx = (W + stereo::BM_TILE_W - 1) / stereo::BM_TILE_W;
     x->nty = (H + stereo::BM_TILE_H - 1) / stereo::BM_TILE_H;
     const int nt = x->ntx * x->nty;
-    if (d->mp.algorithm == STEREO_ALGO_NCC) {
-        // the NCC matcher masks its own tiles: only the per-tile bins
+    if (d->mp.algorithm == STEREO_ALGO_NCC || d->mp.algorithm == STEREO_ALGO_AW1 ||
+        d->mp.algorithm == STEREO_ALGO_AW2) {
+        // the NCC and adaptive-window matchers mask their own tiles: only the per-tile bins
         hipError_t e;
         if ((e = hipMalloc((void**)&x->tile_evals, nt * sizeof(int))) ||
             (e = hipMalloc((void**)&x->tile_mask, nt * sizeof(uint32_t))))
@@ -375,9 +376,9 @@ stereo_status stereo_matcher_validate(const stereo_image_format* fmt, const ster
     if (p->use_disp_input != 0 && p->use_disp_input != 1)
         return say(STEREO_ERROR_INVALID_CONFIG, "use_disp_input must be 0 or 1");
     if (p->use_disp_input && p->algorithm != STEREO_ALGO_SAD && p->algorithm != STEREO_ALGO_CENSUS &&
-        p->algorithm != STEREO_ALGO_NCC)
+        p->algorithm != STEREO_ALGO_NCC && p->algorithm != STEREO_ALGO_AW1 && p->algorithm != STEREO_ALGO_AW2)
         return say(STEREO_ERROR_INVALID_CONFIG, "the band search around an initial disparity map is built for the SAD, "
-                                                "census and NCC matchers");
+                                                "census, NCC and adaptive-window matchers");
     if (p->use_disp_input && fmt->width > stereo::BM_LIST_MAX_WIDTH)
         return say(STEREO_ERROR_INVALID_FORMAT, "the band search takes widths up to 11264");
     if (p->lr_tolerance < 0) return say(STEREO_ERROR_INVALID_CONFIG, "lr_tolerance must be >= 0");
@@ -469,8 +470,8 @@ stereo_status stereo_matcher_configure(stereo_context* ctx, stereo_driver** out,
 }
 
 // The launch sequence of one frame: inputs into the padded buffers (census or copies), the
-// NCC statistics, the matcher and the merge / check.  bins: the NCC band search's per-tile
-// masks, or nullptr for the full range.  begin() has been called; this calls end().
+// NCC statistics, the matcher and the merge / check.  bins: the NCC or adaptive-window band
+// search's per-tile masks, or nullptr for the full range.  begin() has been called; this calls end().
 static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool dispid, stereo_image* disp_lr,
                                  stereo_image* disp_rl, const stereo_image* left, const stereo_image* right,
                                  const uint32_t* bins, int bins_nty, hipStream_t s)
@@ -536,6 +537,8 @@ static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool disp
         ap.ntx = d->ntx;
         ap.nseg = d->nseg;
         ap.seg_len = d->seg_len;
+        ap.bins = bins;
+        ap.bins_nty = bins_nty;
         HIP_TRY(ctx, stereo::launch_aw(d->mp.window_w, ap, s), "adaptive-window launch");
     } else {
         stereo::BmParams bp;
@@ -638,9 +641,10 @@ stereo_status stereo_matcher_process_init(stereo_driver* d, stereo_image* disp, 
     ib.scale = W / iw;
     ib.W = W;
     ib.H = H;
-    if (d->mp.algorithm == STEREO_ALGO_NCC) {
-        // NCC: the per-tile bins, then the NCC frame with every candidate outside its
-        // pixel's bins masked and groups of disparities no pixel searches skipped
+    if (d->mp.algorithm == STEREO_ALGO_NCC || d->mp.algorithm == STEREO_ALGO_AW1 ||
+        d->mp.algorithm == STEREO_ALGO_AW2) {
+        // NCC, AW1, AW2: the per-tile bins, then the matcher's frame with every candidate
+        // outside its pixel's bins masked and groups of disparities no pixel searches skipped
         if ((st = begin(d, s))) return st;
         stereo::CnnBandArgs ba{};
         ba.tile_evals = x->tile_evals;

@@ -151,6 +151,8 @@ struct AwParams {
     int W, H, D;
     int bx, by;
     int ntx, nseg, seg_len;
+    const uint32_t* bins;  // band search: per 64 x BM_TILE_H tile, the 8-disparity bins to search (or null)
+    int bins_nty;          // tile rows of bins
 };
 
 // ---- learned proposal network -------------------------------------------------------

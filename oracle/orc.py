@@ -51,6 +51,7 @@ def _load():
     lib.orc_ncc.argtypes = [P, P, i, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_ncc_bins.argtypes = [P, P, i, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_aw.argtypes = [i, P, P, i, i, i, i, i, P, P, P, P, P, i]
+    lib.orc_aw_bins.argtypes = [i, P, P, i, i, i, i, i, P, P, P, P, P, i]
     lib.orc_crosscheck_plain.argtypes = [P, P, i, i, i, P]
     lib.orc_crosscheck_ref.argtypes = [P, P, i, i, i, i, i, P, P, P]
     lib.orc_bm1.argtypes = [P, P, i, i, i, i, i, i, i, i, C.c_float, C.c_float, C.c_float, P, i, i, P]
@@ -63,7 +64,7 @@ def _load():
     for name in ["orc_census", "orc_census3x3_ref", "orc_bm", "orc_bm_brute", "orc_ncc", "orc_aw",
                  "orc_crosscheck_plain", "orc_crosscheck_ref", "orc_mip_level0_ref", "orc_mip_level0_plain",
                  "orc_mip_next", "orc_disp_denoise", "orc_bm1",
-                 "orc_rectify", "orc_color_disparity", "orc_ncc_bins"]:
+                 "orc_rectify", "orc_color_disparity", "orc_ncc_bins", "orc_aw_bins"]:
         getattr(lib, name).restype = i
     return lib
 
@@ -197,6 +198,22 @@ def aw(mode, L, R, D, sub=5, tol=1, nthreads=1, with_volume=False):
     _chk(lib.orc_aw(mode, _p(L), _p(R), W, H, D, sub, tol, _p(dL), _p(dR), _p(out), _p(val), _p(vol), nthreads),
          "orc_aw")
     return (out, dL, dR, val, vol) if with_volume else (out, dL, dR, val)
+
+
+def aw_bins(mode, L, R, D, bins, sub=5, tol=1, nthreads=1):
+    """orc_aw restricted to the 8-disparity bins set in bins[y, x] (uint32) of each left pixel."""
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    bins = np.ascontiguousarray(bins, np.uint32)
+    assert bins.shape == (H, W)
+    dL = np.empty((H, W), np.int16)
+    dR = np.empty((H, W), np.int16)
+    out = np.empty((H, W), np.int16)
+    val = np.empty((H, W), np.int32)
+    _chk(lib.orc_aw_bins(mode, _p(L), _p(R), W, H, D, sub, tol, _p(bins), _p(dL), _p(dR), _p(out), _p(val), nthreads),
+         "orc_aw_bins")
+    return out, dL, dR, val
 
 
 def crosscheck_plain(dL, dR, tol=1):

@@ -402,10 +402,16 @@ static inline void ins_min(int* f, int* s, int v)
     *f = *f < v ? *f : v;
 }
 
-int orc_aw(int mode, const uint8_t* L, const uint8_t* R, int W, int H, int D, int s, int tol,
-           int16_t* dL, int16_t* dR, int16_t* out, int32_t* value_out, int32_t* volume, int nthreads)
+/* bins (optional, D <= 256): only candidates whose 8-disparity bin d >> 3 is set in
+ * bins[p] of the left pixel p take part, in both views -- the band search
+ * (MatcherUseDisparity, BlockMatcherAW1.cu:1336, BlockMatcherAW2.cu:858) with the
+ * engine's per-tile bins. */
+static int aw_impl(int mode, const uint8_t* L, const uint8_t* R, int W, int H, int D, int s, int tol,
+                   const uint32_t* bins, int16_t* dL, int16_t* dR, int16_t* out, int32_t* value_out,
+                   int32_t* volume, int nthreads)
 {
     if ((mode != 1 && mode != 2) || !L || !R || !dL || !dR || s < 1 || (s & 1) == 0 || D <= 0) return -1;
+    if (bins && D > 256) return -1;
     const int h = s / 2, b = s + h;
     const size_t N = (size_t)W * H;
     int32_t* bestL = (int32_t*)malloc(N * sizeof(int32_t));
@@ -455,6 +461,7 @@ int orc_aw(int mode, const uint8_t* L, const uint8_t* R, int W, int H, int D, in
                     }
                     const size_t pl = (size_t)y * W + x, pr = pl - d;
                     if (volume) volume[pl * D + d] = v;
+                    if (bins && !((bins[pl] >> (d >> 3)) & 1u)) continue;
                     const int betterL = (mode == 1) ? (v > bestL[pl]) : (v < bestL[pl]);
                     if (betterL) { bestL[pl] = v; dL[pl] = (int16_t)d; }
                     const int betterR = (mode == 1) ? (v > bestR[pr]) : (v < bestR[pr]);
@@ -470,6 +477,19 @@ int orc_aw(int mode, const uint8_t* L, const uint8_t* R, int W, int H, int D, in
     free(bestL);
     free(bestR);
     return 0;
+}
+
+int orc_aw(int mode, const uint8_t* L, const uint8_t* R, int W, int H, int D, int s, int tol,
+           int16_t* dL, int16_t* dR, int16_t* out, int32_t* value_out, int32_t* volume, int nthreads)
+{
+    return aw_impl(mode, L, R, W, H, D, s, tol, NULL, dL, dR, out, value_out, volume, nthreads);
+}
+
+int orc_aw_bins(int mode, const uint8_t* L, const uint8_t* R, int W, int H, int D, int s, int tol,
+                const uint32_t* bins, int16_t* dL, int16_t* dR, int16_t* out, int32_t* value_out, int nthreads)
+{
+    if (!bins) return -1;
+    return aw_impl(mode, L, R, W, H, D, s, tol, bins, dL, dR, out, value_out, NULL, nthreads);
 }
 
 /* ---------------------------------------------------------------------------------

@@ -74,6 +74,10 @@ int orc_ncc_bins(const uint8_t* L, const uint8_t* R, int W, int H, int D, int ww
  * aggregated value of the left map (-1 invalid); volume optional as above. */
 int orc_aw(int mode, const uint8_t* L, const uint8_t* R, int W, int H, int D, int s, int tol,
            int16_t* dL, int16_t* dR, int16_t* out, int32_t* value_out, int32_t* volume, int nthreads);
+/* orc_aw restricted to candidates whose 8-disparity bin d >> 3 is set in bins[p] of the
+ * left pixel p (D <= 256): the band search. */
+int orc_aw_bins(int mode, const uint8_t* L, const uint8_t* R, int W, int H, int D, int s, int tol,
+                const uint32_t* bins, int16_t* dL, int16_t* dR, int16_t* out, int32_t* value_out, int nthreads);
 
 /* Plain left-right check of two int16 maps. */
 int orc_crosscheck_plain(const int16_t* dL, const int16_t* dR, int W, int H, int tol, int16_t* out);

@@ -333,6 +333,45 @@ def test_aw1_aw2_equivalent_winners():
     assert np.array_equal(a1[3][v], 255 * 75 - a2[3][v])
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_aw_bins_against_the_value_volume(mode):
+    """The bins-restricted adaptive-window matcher: with every bin set it is orc_aw; with a
+    random mask per left pixel, both views equal the best allowed value of the full volume
+    (AW1 maximises, AW2 minimises; lowest d on ties)."""
+    import numpy as np
+    rng = np.random.default_rng(5 + mode)
+    H, W, D = 34, 80, 24
+    L = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    R = np.roll(L, -4, axis=1)
+    R = np.clip(R.astype(int) + rng.integers(-8, 9, R.shape), 0, 255).astype(np.uint8)
+    full = orc.aw(mode, L, R, D, sub=3, tol=1)
+    every = orc.aw_bins(mode, L, R, D, np.full((H, W), 0x7, np.uint32), sub=3, tol=1)
+    for a, b in zip(full, every):
+        assert np.array_equal(a, b)
+    bins = rng.integers(0, 8, (H, W)).astype(np.uint32)
+    out, dl, dr, val = orc.aw_bins(mode, L, R, D, bins, sub=3, tol=1)
+    vol = orc.aw(mode, L, R, D, sub=3, tol=1, with_volume=True)[4].astype(np.int64)
+    d = np.arange(D)
+    allowed = (vol >= 0) & (((bins[:, :, None] >> (d[None, None, :] >> 3)) & 1) == 1)
+    # a score to maximise, -1 where not allowed
+    q = np.where(allowed, vol if mode == 1 else 100000 - vol, -1)
+    want_l = np.where(allowed.any(-1), q.argmax(-1), -1)
+    assert np.array_equal(dl, want_l)
+    want_r = np.full((H, W), -1)
+    best = np.full((H, W), -1)
+    for dd in range(D):
+        xs = np.arange(W - dd)
+        cand = q[:, xs + dd, dd]
+        better = cand > best[:, xs]
+        best[:, xs] = np.where(better, cand, best[:, xs])
+        want_r[:, xs] = np.where(better, dd, want_r[:, xs])
+    assert np.array_equal(dr, want_r)
+    assert np.array_equal(out, orc.crosscheck_plain(dl, dr, 1))
+    yy, xx = np.nonzero(dl >= 0)
+    assert np.array_equal(val[yy, xx], vol[yy, xx, dl[yy, xx]])
+    assert (dl >= 0).mean() > 0.4
+
+
 def test_ncc_bins_against_the_score_volume():
     """The bins-restricted NCC: with every bin set it is orc_ncc; with a random mask per left
     pixel, both views equal the argmax over the allowed candidates of the FP64 score volume
