@@ -379,11 +379,13 @@ struct SParameter
     _SDisparityConfig xDispConfig;
     EConfig eConfig = EConfig::Patch_11x11;
     int32_t iLrTolerance = 1;
-    void Set(const _SDisparityConfig& cfg, EConfig config, int32_t tol = 1)
+    int32_t iUseDispInput = 0;  // 1: Process may take an initial disparity map (BlockMatcherNCC.h:88)
+    void Set(const _SDisparityConfig& cfg, EConfig config, int32_t tol = 1, int32_t useDispInput = 0)
     {
         xDispConfig = cfg;
         eConfig = config;
         iLrTolerance = tol;
+        iUseDispInput = useDispInput;
     }
 };
 class CDriver : public CMatcherDriver
@@ -394,7 +396,8 @@ public:
     {
         const int32_t w = p.eConfig == EConfig::Patch_11x11 ? 11 : p.eConfig == EConfig::Patch_9x9 ? 9 : p.eConfig == EConfig::Patch_5x5 ? 5 : -1;
         if (w < 0) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
-        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_NCC, w, w, p.xDispConfig.Range(), p.iLrTolerance));
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_NCC, w, w, p.xDispConfig.Range(), p.iLrTolerance, 1, 1,
+                                                   p.iUseDispInput));
     }
     // winning scores of the last Process (LUM_Single)
     void Scores(CDeviceImage& score, const CDeviceImage& L, const CDeviceImage& R)
@@ -416,11 +419,13 @@ struct SParameter
     _SDisparityConfig xDispConfig;
     EConfig eConfig = EConfig::Patch_15x15;
     int32_t iLrTolerance = 1;
-    void Set(const _SDisparityConfig& cfg, EConfig config, int32_t tol = 1)
+    int32_t iUseDispInput = 0;  // 1: Process may take an initial disparity map (BlockMatcherAW1.h:91)
+    void Set(const _SDisparityConfig& cfg, EConfig config, int32_t tol = 1, int32_t useDispInput = 0)
     {
         xDispConfig = cfg;
         eConfig = config;
         iLrTolerance = tol;
+        iUseDispInput = useDispInput;
     }
 };
 class CDriver : public CMatcherDriver
@@ -431,7 +436,8 @@ public:
     {
         const int32_t s = p.eConfig == EConfig::Patch_15x15 ? 5 : p.eConfig == EConfig::Patch_9x9 ? 3 : -1;
         if (s < 0) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
-        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_AW1, s, s, p.xDispConfig.Range(), p.iLrTolerance));
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_AW1, s, s, p.xDispConfig.Range(), p.iLrTolerance, 1, 1,
+                                                   p.iUseDispInput));
     }
 };
 }  // namespace BlockMatcherAW1
@@ -448,7 +454,8 @@ public:
     {
         const int32_t s = p.eConfig == EConfig::Patch_15x15 ? 5 : p.eConfig == EConfig::Patch_9x9 ? 3 : -1;
         if (s < 0) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
-        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_AW2, s, s, p.xDispConfig.Range(), p.iLrTolerance));
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_AW2, s, s, p.xDispConfig.Range(), p.iLrTolerance, 1, 1,
+                                                   p.iUseDispInput));
     }
 };
 }  // namespace BlockMatcherAW2

@@ -136,7 +136,11 @@ def test_cpp_driver_layer_builds_and_reports_errors(tmp_path):
     import subprocess
     src = tmp_path / "t.cpp"
     src.write_text('#include "clutec/stereo.hpp"\nint main(){ Clu::Cuda::BlockMatcherNCC::SParameter p; '
-                   'p.Set(Clu::Cuda::_SDisparityConfig{}, Clu::Cuda::BlockMatcherNCC::EConfig::Patch_9x9); return 0; }\n')
+                   'p.Set(Clu::Cuda::_SDisparityConfig{}, Clu::Cuda::BlockMatcherNCC::EConfig::Patch_9x9); '
+                   'p.Set(Clu::Cuda::_SDisparityConfig{}, Clu::Cuda::BlockMatcherNCC::EConfig::Patch_9x9, 1, 1); '
+                   'Clu::Cuda::BlockMatcherAW2::SParameter q; '
+                   'q.Set(Clu::Cuda::_SDisparityConfig{}, Clu::Cuda::BlockMatcherAW1::EConfig::Patch_9x9, 1, 1); '
+                   'return p.iUseDispInput + q.iUseDispInput - 2; }\n')
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
