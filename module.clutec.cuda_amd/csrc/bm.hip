@@ -47,6 +47,9 @@ constexpr int RG = TH / 4;
 
 struct CostCensus {
     static constexpr int DC = 4;  // disparities per right-window load
+    // workgroups per CU asked of the work-list kernel: three fit its LDS, and the
+    // register cap (168) costs it only a few spills outside the disparity loop
+    static constexpr int LIST_WG = 3;
     using E = uint64_t;           // element type in memory
     struct T { uint32_t lo, hi; };
     static __device__ __forceinline__ T make(E q) { return T{(uint32_t)q, (uint32_t)(q >> 32)}; }
@@ -60,6 +63,7 @@ struct CostCensus {
 
 struct CostSad {
     static constexpr int DC = 8;
+    static constexpr int LIST_WG = 2;  // three would spill inside the disparity loop
     using E = uint8_t;
     using T = uint32_t;
     static __device__ __forceinline__ T make(E v) { return v; }
@@ -86,13 +90,14 @@ __device__ __forceinline__ bool job_is_edge(const Job& j)
     return (j.x0 < j.bx + j.de - 1) || (j.x0 + TW > j.W - j.bx);
 }
 
+// tid: the thread index (threadIdx.x, or an opaque copy of it, see bm_list_kernel)
 template <class Cost, int AX, int AY, bool EDGE>
-__device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64])
+__device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64],
+                                        int tid)
 {
     static_assert(TH + 2 * AY <= HROWS, "tile too tall for the aggregation window");
     static_assert(AX <= 4, "horizontal radius limited by the run halo");
     constexpr int DC = Cost::DC;
-    const int tid = threadIdx.x;
     const int x0 = p.x0, y0 = p.y0;
     const int W = p.W, H = p.H;
     const int ds = p.ds, de = p.de;
@@ -220,12 +225,13 @@ __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * H
 }
 
 template <class Cost, int AX, int AY>
-__device__ __forceinline__ void run_job(const Job& j, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64])
+__device__ __forceinline__ void run_job(const Job& j, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64],
+                                        int tid)
 {
     if (job_is_edge(j))
-        bm_tile<Cost, AX, AY, true>(j, hbuf, ebuf);
+        bm_tile<Cost, AX, AY, true>(j, hbuf, ebuf, tid);
     else
-        bm_tile<Cost, AX, AY, false>(j, hbuf, ebuf);
+        bm_tile<Cost, AX, AY, false>(j, hbuf, ebuf, tid);
 }
 
 // Grid launch over one frame: one workgroup per (tile x, tile y, disparity segment),
@@ -255,13 +261,13 @@ __global__ __launch_bounds__(256, 2) void bm_fused_kernel(BmParams p)
     j.rp_kofs = 0;
     j.lk = p.lkey + (size_t)seg * p.W * p.H + (size_t)j.y0 * p.W + j.x0;
     j.lk_stride = p.W;
-    run_job<Cost, AX, AY>(j, hbuf, ebuf);
+    run_job<Cost, AX, AY>(j, hbuf, ebuf, threadIdx.x);
 }
 
 // Persistent launch over a work list: each workgroup takes items blockIdx.x,
 // blockIdx.x + gridDim.x, ... up to the device-side count.
 template <class Cost, int AX, int AY>
-__global__ __launch_bounds__(256, 2) void bm_list_kernel(BmListParams p)
+__global__ __launch_bounds__(256, Cost::LIST_WG) void bm_list_kernel(BmListParams p)
 {
     __shared__ uint32_t hbuf[2][HROWS * HSTRIDE];
     __shared__ uint32_t ebuf[4][RG][64];
@@ -294,7 +300,12 @@ __global__ __launch_bounds__(256, 2) void bm_list_kernel(BmListParams p)
         j.rp_kofs = 256 - it.de;
         j.lk = p.lkey + (size_t)i * TH * TW;
         j.lk_stride = TW;
-        run_job<Cost, AX, AY>(j, hbuf, ebuf);
+        // an opaque copy of the thread index per item: the addresses derived from it are
+        // recomputed for each item instead of being hoisted out of this loop into
+        // registers that stay live through the whole item
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        run_job<Cost, AX, AY>(j, hbuf, ebuf, tid);
     }
 }
 
