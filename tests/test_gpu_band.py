@@ -76,14 +76,15 @@ def test_band_search_matches_oracle(ctx, algo, window, scale, fmt):
     print(f"  agreement with the full range {(got[0] == ref_full).mean():.4f}", flush=True)
 
 
-@pytest.mark.parametrize("scale,win,fmt", [(2, (11, 11), "i16"), (1, (7, 7), "ref"), (4, (9, 9), "i16")])
-def test_ncc_band_search_matches_oracle(ctx, scale, win, fmt):
+@pytest.mark.parametrize("scale,win,fmt,size", [(2, (11, 11), "i16", (400, 250, 128)), (1, (7, 7), "ref", (400, 250, 128)),
+                                               (4, (9, 9), "i16", (400, 250, 128)), (3, (5, 5), "i16", (333, 190, 64))])
+def test_ncc_band_search_matches_oracle(ctx, scale, win, fmt, size):
     """NCC around an initial map (MatcherUseDisparity of BlockMatcherNCC): the oracle's NCC
     restricted to the same per-pixel bins, with the NCC matcher's parity rules: the
     GPU's winning scores within 1e-5 relative of FP64, maps equal except at FP32
     near-ties, and every winner inside its pixel's bins."""
-    W, H, D = 400, 250, 128
-    print(f"ncc band search scale {scale} window {win} init {fmt}", flush=True)
+    W, H, D = size
+    print(f"ncc band search scale {scale} window {win} init {fmt} size {size}", flush=True)
     L, R = cs.synth_pair(95, W, H, 2, 8, D - 1, 8)
     full = cs.Matcher(ctx, W, H, algorithm=cs.ALGO_NCC, window=win, num_disp=D, tol=1)
     ref_full = full.run(L, R)
@@ -120,14 +121,19 @@ def test_ncc_band_search_matches_oracle(ctx, scale, win, fmt):
           flush=True)
 
 
-@pytest.mark.parametrize("algo,sub,scale,fmt", [(cs.ALGO_AW1, 5, 2, "i16"), (cs.ALGO_AW2, 5, 4, "ref"),
-                                              (cs.ALGO_AW1, 3, 1, "ref"), (cs.ALGO_AW2, 3, 2, "i16")])
-def test_aw_band_search_matches_oracle(ctx, algo, sub, scale, fmt):
+@pytest.mark.parametrize("algo,sub,scale,fmt,size", [(cs.ALGO_AW1, 5, 2, "i16", (400, 250, 128)),
+                                                   (cs.ALGO_AW2, 5, 4, "ref", (400, 250, 128)),
+                                                   (cs.ALGO_AW1, 3, 1, "ref", (400, 250, 128)),
+                                                   (cs.ALGO_AW2, 3, 2, "i16", (400, 250, 128)),
+                                                   (cs.ALGO_AW1, 5, 3, "i16", (333, 190, 64)),
+                                                   (cs.ALGO_AW2, 3, 3, "ref", (130, 97, 8))])
+def test_aw_band_search_matches_oracle(ctx, algo, sub, scale, fmt, size):
     """AW1 / AW2 around an initial map (MatcherUseDisparity of BlockMatcherAW1 / AW2): the
-    oracle's adaptive-window matcher restricted to the same per-pixel bins, bit-exact."""
-    W, H, D = 400, 250, 128
-    print(f"aw band search algo {algo} sub {sub} scale {scale} init {fmt}", flush=True)
-    L, R = cs.synth_pair(97, W, H, 2, 6, D - 1, 8)
+    oracle's adaptive-window matcher restricted to the same per-pixel bins, bit-exact;
+    ragged sizes and the smallest range (D = 8) included."""
+    W, H, D = size
+    print(f"aw band search algo {algo} sub {sub} scale {scale} init {fmt} size {size}", flush=True)
+    L, R = cs.synth_pair(97, W, H, 2, 6, D - 1, min(8, D - 1))
     full = cs.Matcher(ctx, W, H, algorithm=algo, window=(sub, sub), num_disp=D, tol=1)
     ref_full = full.run(L, R)
     rng = np.random.default_rng(20 + scale)
