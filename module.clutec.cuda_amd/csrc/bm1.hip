@@ -229,6 +229,15 @@ __global__ __launch_bounds__(256) void bm1_kernel(Bm1Args a)
         cap = min(B1_SDC, d_end - d_start + 1);
     }
 
+    // the left pixels of this thread's column segment do not change with the disparity:
+    // they are read from LDS once, into registers
+    uint32_t lreg[SEG + P - 1];
+    {
+        const int c = t % 32, r0 = (t / 32) * SEG;
+#pragma unroll
+        for (int j = 0; j < SEG + P - 1; ++j) lreg[j] = t < VT ? sl[r0 + j][c] : 0u;
+    }
+
     int buf = 0;
     for (int D0 = d_start; any && D0 < d_stop; D0 += B1_SDC) {
         const Bm1Group g = bm1_group(X0, D0, W, a.offset, a.ltor, cap);
@@ -252,7 +261,7 @@ __global__ __launch_bounds__(256) void bm1_kernel(Bm1Args a)
                 int t[SEG + P - 1];
 #pragma unroll
                 for (int j = 0; j < SEG + P - 1; ++j) {
-                    const uint32_t l = sl[r0 + j][c], q = sr[r0 + j][c + sh];
+                    const uint32_t l = lreg[j], q = sr[r0 + j][c + sh];
                     if (SIM == 0) {
                         t[j] = (int)(__builtin_amdgcn_sad_u8(l, q, 0u) >> 1);
                     } else {
