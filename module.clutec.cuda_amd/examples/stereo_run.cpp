@@ -7,8 +7,11 @@
 //   map and RectifyImage of the left image, 5. download, 6. write the maps and report
 //   milliseconds.
 //
-// usage: stereo_run <algo:sad|census|ncc|aw1|aw2|bm1> <W> <H> <D> <seed> <out_prefix> [runs]
-// Writes <out_prefix>_checked.i16, _lr.i16, _rl.i16 (row-major int16, -1 invalid),
+// usage: stereo_run <algo:sad|census|ncc|aw1|aw2|bm1|aw1band|nccband> <W> <H> <D> <seed> <out_prefix> [runs]
+// aw1band / nccband: the full-range map, taken at half resolution (d / 2), as the initial
+// map of Process(disp, L, R, dispInit) with iUseDispInput; writes <out_prefix>_init.i16 and
+// _band.i16 and stops there.
+// Otherwise writes <out_prefix>_checked.i16, _lr.i16, _rl.i16 (row-major int16, -1 invalid),
 // _checkid.u16x4 (the cross-check's id map), _denoised.u16 (its de-noised ids),
 // _color.rgba (its Analysis colouring) and _rectified.u8 (the rectified left image)
 // and prints one line "ms <median> valid <fraction> range <min> <max>".
@@ -35,7 +38,8 @@ static bool write_file(const std::string& path, const void* p, size_t n)
 int main(int argc, char** argv)
 {
     if (argc < 7) {
-        std::fprintf(stderr, "usage: %s <sad|census|ncc|aw1|aw2|bm1> W H D seed out_prefix [runs]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <sad|census|ncc|aw1|aw2|bm1|aw1band|nccband> W H D seed out_prefix [runs]\n",
+                     argv[0]);
         return 2;
     }
     const std::string algo = argv[1];
@@ -98,6 +102,52 @@ int main(int argc, char** argv)
                 drv.Configure(dev, fmtIn, p);
                 run(drv);
             }
+        } else if (algo == "aw1band" || algo == "nccband") {
+            // MatcherUseDisparity through the driver layer: a full-range pass, then the band
+            // search around its map at half resolution
+            const bool aw = algo == "aw1band";
+            BlockMatcherAW1::SParameter pa;
+            BlockMatcherNCC::SParameter pn;
+            BlockMatcherAW1::CDriver fa, ba;
+            BlockMatcherNCC::CDriver fn, bn;
+            if (aw) {
+                pa.Set(dcfg, BlockMatcherAW1::EConfig::Patch_9x9, 1, 0);
+                fa.Configure(dev, fmtIn, pa);
+                pa.Set(dcfg, BlockMatcherAW1::EConfig::Patch_9x9, 1, 1);
+                ba.Configure(dev, fmtIn, pa);
+                fa.Process(disp, imgL, imgR);
+            } else {
+                pn.Set(dcfg, BlockMatcherNCC::EConfig::Patch_9x9, 1, 0);
+                fn.Configure(dev, fmtIn, pn);
+                pn.Set(dcfg, BlockMatcherNCC::EConfig::Patch_9x9, 1, 1);
+                bn.Configure(dev, fmtIn, pn);
+                fn.Process(disp, imgL, imgR);
+            }
+            std::vector<int16_t> full((size_t)W * H);
+            disp.CopyInto(full.data(), (size_t)W * 2);
+            const int iw = W / 2, ih = H / 2;
+            std::vector<int16_t> init((size_t)iw * ih);
+            for (int y = 0; y < ih; ++y)
+                for (int x = 0; x < iw; ++x) {
+                    const int16_t d = full[(size_t)(2 * y) * W + 2 * x];
+                    init[(size_t)y * iw + x] = d >= 0 ? (int16_t)(d / 2) : (int16_t)-1;
+                }
+            CDeviceImage dinit(dev, SImageFormat(iw, ih, EPixelType::Lum_Int16));
+            dinit.CopyFrom(init.data(), (size_t)iw * 2);
+            for (int i = 0; i < runs; ++i) {
+                if (aw)
+                    ms.push_back(ProcessKernel(ba, disp, imgL, imgR, dinit));
+                else
+                    ms.push_back(ProcessKernel(bn, disp, imgL, imgR, dinit));
+            }
+            std::vector<int16_t> band((size_t)W * H);
+            disp.CopyInto(band.data(), (size_t)W * 2);
+            if (!write_file(out + "_init.i16", init.data(), init.size() * 2) ||
+                !write_file(out + "_band.i16", band.data(), band.size() * 2))
+                return 1;
+            std::sort(ms.begin(), ms.end());
+            std::printf("ms %.4f\n", ms[ms.size() / 2]);
+            return 0;
         } else if (algo == "bm1") {
             // BlockMatcher1 writes the reference's id map with similarity channels; the
             // program writes it to <out_prefix>_bm1.u16x4 and stops there

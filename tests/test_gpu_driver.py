@@ -64,3 +64,27 @@ def test_stereo_run_bm1_matches_emulation(tmp_path):
     want = orc.bm1(L, R, P=9, sim_type=0, dmin=0, dmax=D, offset=0, ltor=True, min_delta=1.0, sim_thresh=60.0,
                    grad_thresh=2.0, fill=0x5A5A)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("algo", ["aw1band", "nccband"])
+def test_stereo_run_band_search_matches_oracle(tmp_path, algo):
+    """Process(disp, L, R, dispInit) through the C++ layer with iUseDispInput (AW1 9x9 and
+    NCC 9x9): the full-range map at half resolution as the initial map; AW1 bit-exact
+    against the oracle restricted to the same per-tile bins, NCC equal but for FP32 near-ties."""
+    from test_gpu_band import _tile_bins
+    W, H, D, seed = 288, 150, 64, 7
+    prefix = str(tmp_path / algo)
+    print(f"stereo_run {algo} {W}x{H} D={D}", flush=True)
+    r = subprocess.run([EXE, algo, str(W), str(H), str(D), str(seed), prefix, "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    init = np.fromfile(prefix + "_init.i16", np.int16).reshape(H // 2, W // 2)
+    got = np.fromfile(prefix + "_band.i16", np.int16).reshape(H, W)
+    L, R = cs.synth_pair(seed, W, H, 0, 8, D - 1, min(8, D - 1))
+    bins = _tile_bins(init, W, H, D)
+    if algo == "aw1band":
+        assert np.array_equal(got, orc.aw_bins(1, L, R, D, bins, sub=3, tol=1, nthreads=8)[0])
+    else:
+        want = orc.ncc_bins(L, R, D, bins, window=(9, 9), tol=1, nthreads=8)[0]
+        assert (got == want).mean() >= 0.999
+    assert (got >= 0).mean() > 0.5
