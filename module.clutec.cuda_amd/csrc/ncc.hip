@@ -17,7 +17,7 @@
 // their horizontal sums over 2*AX+1 columns (halo from the neighbouring runs by DPP)
 // into LDS (one buffer, two barriers per disparity).  Column phase: lane = column,
 // wave = 13 rows: vertical running sums give SIJ, the right window's (SJ, rJ) come
-// from an LDS tile of the right statistics that is refreshed every 8 disparities,
+// from an LDS tile of the right statistics that is refreshed every 16 disparities,
 // the left window's are registers.  Left winner in
 // a register, right winner carried along the wave by a DPP shift (see bm.hip).
 #include "common.hpp"
@@ -33,9 +33,10 @@ constexpr int HROWS = 64;
 constexpr int RUN = 19;
 constexpr int COLS = 4 * RUN;  // 76 cost columns: tile columns [-6, 70)
 constexpr int HSTRIDE = 76;
-constexpr int DCH = 8;         // disparities per right-statistics tile
+constexpr int DCH = 8;         // disparities per group (one band-search bin)
+constexpr int SDC = 16;        // disparities per right-statistics tile
 constexpr int EMIT = 32;       // right-view values parked per wave between flushes
-constexpr int SCOLS = TW + DCH - 1;
+constexpr int SCOLS = TW + SDC - 1;
 
 // Per-pixel window statistics: .x = window sum S (int), .y = float bits of the
 // reciprocal standard-deviation factor scale / sqrt(N SS - S^2) (0 when the
@@ -208,6 +209,7 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
     };
     uint32_t rw[RWW], rnext = 0;
     int wd = -1;  // the disparity rw holds
+    int sdc = -SDC;  // the first disparity of the right-statistics tile in LDS (none yet)
     for (int dc = ds; dc < de; dc += DCH) {
         const int dce = min(de, dc + DCH);
         // the band search: rows of this wave that search the group (bit r: row r), wave-uniform
@@ -234,10 +236,12 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
                 continue;
             }
         }
-        // right statistics for right columns x0-dc-(DCH-1) .. x0+63-dc, rows of the tile:
-        // the thread's loads (the last index repeated past the end) are all issued before
-        // the barrier, the LDS stores after it
-        {
+        // right statistics for disparities sdc .. sdc+SDC-1 (right columns x0-sdc-(SDC-1) ..
+        // x0+63-sdc), rows of the tile, loaded when the group leaves the tile in LDS: the
+        // thread's loads (the last index repeated past the end) are all issued before the
+        // barrier, the LDS stores after it
+        if (dc - sdc >= SDC) {
+            sdc = dc;
             constexpr int SN = TH * SCOLS, SIT = (SN + 255) / 256, SB = SIT;
             static_assert(SIT % SB == 0, "whole batches");
             // an opaque copy of the thread index: the addresses are recomputed here rather
@@ -248,7 +252,7 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
                 const int k = min(t0 + 256 * j, SN - 1);
                 const int r = k / SCOLS, c = k - r * SCOLS;
                 const int y = min(y0 + r, H - 1);
-                const int xc = clampi(x0 - dc - (DCH - 1) + c, 0, W - 1);
+                const int xc = clampi(x0 - sdc - (SDC - 1) + c, 0, W - 1);
                 return p.rstats[(size_t)y * p.stats_pitch + xc];
             };
 #pragma unroll
@@ -316,7 +320,7 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
             uint32_t sij = 0;
 #pragma unroll
             for (int i = 0; i <= 2 * AY; ++i) sij += hv[i];
-            const int soff = (gw * RG) * SCOLS + lane + (DCH - 1) - (d - dc);
+            const int soff = (gw * RG) * SCOLS + lane + (SDC - 1) - (d - sdc);
             const int16_t* srow_s = ssum + soff;
             const float* srow_r = srinv + soff;
             const uint32_t dd = 255u - (uint32_t)d;
