@@ -36,6 +36,33 @@ PEAK_INT_TOPS = 256 * 64 * 2.4e9 / 1e12
 PEAK_HBM_GBS = 8000.0
 
 
+def host_cpus():
+    """CPUs this process may use: the affinity set, capped by the cgroup CPU quota (a
+    container granted 16 CPUs' worth of time on a larger host sees every core in its
+    affinity set, but more threads than the quota only queue).  Returns (usable, affinity,
+    quota or None, model)."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = min(affinity, quota) if quota else affinity
+    return usable, affinity, quota, model
+
+
 def cpu_baseline(L, R, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
@@ -67,10 +94,14 @@ def cpu_baseline(L, R, threads):
         orc.bm(orc.COST_CENSUS, cl, cr, D, window=(9, 9), census_win=(9, 7), tol=1, nthreads=1)
         st.append(time.perf_counter() - t0)
     t1 = statistics.median(st)
+    usable, affinity, quota, model = host_cpus()
     return {
         "value": round(evals / t / 1e6, 1),
         "unit": "M disparity-evals/s",
         "cores": threads,
+        "nproc": affinity,
+        "cgroup_cpu_quota": quota,
+        "cpu_model": model,
         "kind": "port",
         "sample": f"full 1920x1080 frame, census 9x7 + 9x9 Hamming + D=256 + LR check, "
                   f"median of {len(times)} runs, OpenMP over row bands",
@@ -489,7 +520,7 @@ def main():
         result["block_matcher1"] = bm1_leg(cs, np, ctx, max(5, min(args.steps, 10)))
 
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cpus()[0]  # every core the process may use
         cpu, ref = cpu_baseline(L, R, threads)
         result["cpu_baseline"] = cpu
         result["parity_vs_oracle"] = "bit-exact" if np.array_equal(disp, ref) else \

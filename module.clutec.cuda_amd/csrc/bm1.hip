@@ -457,8 +457,9 @@ stereo_status stereo_bm1_process(stereo_driver* d, stereo_image* disp, const ste
     // EvalThreadConfig_ReadDepBlockSize: blocks over the area less the read border
     const int PCX = 8 * ((32 - P + 1) / 8);
     const int effW = W - (33 - PCX), effH = H - (P - 1);
-    const int nbx = effW / PCX + (effW % PCX > 0);
-    a.nby = effH / 4 + (effH % 4 > 0);
+    // full blocks only (bOnlyFullBlocks defaults to true, KernelDriverBase.h:117-121)
+    const int nbx = effW / PCX;
+    a.nby = effH / 4;
     hipError_t e;
     switch (P) {
     case 5: e = stereo::bm1_dispatch_sim<5>(a, x->p.sim_type, nbx, s); break;

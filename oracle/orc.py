@@ -44,8 +44,6 @@ def _load():
     lib.orc_census3x3_ref.argtypes = [P, i, i, P]
     lib.orc_bm.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P, i]
     lib.orc_bm_brute.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P]
-    lib.orc_bm_ranges.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P, P, i]
-    lib.orc_bm_ranges.restype = i
     lib.orc_bm_bins.argtypes = [i, P, P, i, i, i, i, i, i, i, i, P, P, P, P, i]
     lib.orc_bm_bins.restype = i
     lib.orc_ncc.argtypes = [P, P, i, i, i, i, i, i, P, P, P, P, P, i]
@@ -55,6 +53,8 @@ def _load():
     lib.orc_crosscheck_plain.argtypes = [P, P, i, i, i, P]
     lib.orc_crosscheck_ref.argtypes = [P, P, i, i, i, i, i, P, P, P]
     lib.orc_bm1.argtypes = [P, P, i, i, i, i, i, i, i, i, C.c_float, C.c_float, C.c_float, P, i, i, P]
+    lib.orc_ncc_ref.argtypes = [P, P, i, i, i, i, i, i, i, i, C.c_float, C.c_float, P, i, i, P]
+    lib.orc_aw1_ref.argtypes = [P, P, i, i, i, i, i, i, i, i, C.c_float, C.c_float, C.c_float, P, i, i, P]
     lib.orc_rectify.argtypes = [P, i, i, P, i, i, P]
     lib.orc_color_disparity.argtypes = [P, i, i, i, i, i, i, i, P, P]
     lib.orc_mip_level0_ref.argtypes = [P, i, i, P]
@@ -63,7 +63,7 @@ def _load():
     lib.orc_disp_denoise.argtypes = [i, P, i, i, i, C.c_float, C.c_float, i, P]
     for name in ["orc_census", "orc_census3x3_ref", "orc_bm", "orc_bm_brute", "orc_ncc", "orc_aw",
                  "orc_crosscheck_plain", "orc_crosscheck_ref", "orc_mip_level0_ref", "orc_mip_level0_plain",
-                 "orc_mip_next", "orc_disp_denoise", "orc_bm1",
+                 "orc_mip_next", "orc_disp_denoise", "orc_bm1", "orc_ncc_ref", "orc_aw1_ref",
                  "orc_rectify", "orc_color_disparity", "orc_ncc_bins", "orc_aw_bins"]:
         getattr(lib, name).restype = i
     return lib
@@ -111,21 +111,6 @@ def bm(kind, L, R, D, window=(9, 9), census_win=(1, 1), tol=1, nthreads=1, with_
     _chk(lib.orc_bm(kind, _p(L), _p(R), W, H, D, window[0], window[1], census_win[0], census_win[1], tol,
                     _p(dL), _p(dR), _p(out), _p(cost), nthreads), "orc_bm")
     return (out, dL, dR, cost) if with_cost else (out, dL, dR)
-
-
-def bm_ranges(kind, L, R, D, lo, hi, window=(9, 9), census_win=(1, 1), tol=1, nthreads=1):
-    """orc_bm restricted to candidates lo <= d < hi per left pixel (int16 arrays)."""
-    L = np.ascontiguousarray(L)
-    R = np.ascontiguousarray(R)
-    H, W = L.shape
-    lo = np.ascontiguousarray(lo, np.int16)
-    hi = np.ascontiguousarray(hi, np.int16)
-    dL = np.empty((H, W), np.int16)
-    dR = np.empty((H, W), np.int16)
-    out = np.empty((H, W), np.int16)
-    _chk(lib.orc_bm_ranges(kind, _p(L), _p(R), W, H, D, window[0], window[1], census_win[0], census_win[1], tol,
-                           _p(lo), _p(hi), _p(dL), _p(dR), _p(out), nthreads), "orc_bm_ranges")
-    return out, dL, dR
 
 
 def bm_bins(kind, L, R, D, bins, window=(9, 9), census_win=(1, 1), tol=1, nthreads=1):
@@ -338,6 +323,43 @@ def bm1(L, R, P=9, sim_type=0, dmin=0, dmax=64, offset=0, ltor=True, min_delta=0
         ip = None
     _chk(lib.orc_bm1(_p(L), _p(R), W, H, P, sim_type, dmin, dmax, offset, 1 if ltor else 0, min_delta, sim_thresh,
                      grad_thresh, ip, iW, iH, _p(out)), "orc_bm1")
+    return out
+
+
+def _init_args(init):
+    if init is None:
+        return None, 0, 0, None
+    init = np.ascontiguousarray(init, np.uint16)
+    iH, iW = init.shape[:2]
+    return init, iW, iH, _p(init)
+
+
+def ncc_ref(L, R, P=11, dmin=0, dmax=64, offset=0, ltor=True, min_delta=0, ncc_thresh=0.0, grad_thresh=0.0,
+            init=None, fill=0):
+    """The reference's BlockMatcherNCC emulated block by block: (H, W, 4) uint16 pixels
+    (EDisparityId, byte NCC maximum, prev delta, next delta).  Pixels the reference never
+    writes keep `fill` (every channel)."""
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    out = np.full((H, W, 4), fill, np.uint16)
+    init, iW, iH, ip = _init_args(init)
+    _chk(lib.orc_ncc_ref(_p(L), _p(R), W, H, P, dmin, dmax, offset, 1 if ltor else 0, int(min_delta), ncc_thresh,
+                         grad_thresh, ip, iW, iH, _p(out)), "orc_ncc_ref")
+    return out
+
+
+def aw1_ref(L, R, sub=5, sim_type=0, dmin=0, dmax=64, offset=0, ltor=True, min_delta=0.0, sim_thresh=0.0,
+            grad_thresh=0.0, init=None, fill=0):
+    """The reference's BlockMatcherAW1 emulated block by block (sub-patch size 5: Patch_15x15,
+    3: Patch_9x9): (H, W, 4) uint16 pixels as bm1."""
+    L = np.ascontiguousarray(L, np.uint8)
+    R = np.ascontiguousarray(R, np.uint8)
+    H, W = L.shape
+    out = np.full((H, W, 4), fill, np.uint16)
+    init, iW, iH, ip = _init_args(init)
+    _chk(lib.orc_aw1_ref(_p(L), _p(R), W, H, sub, sim_type, dmin, dmax, offset, 1 if ltor else 0, min_delta,
+                         sim_thresh, grad_thresh, ip, iW, iH, _p(out)), "orc_aw1_ref")
     return out
 
 

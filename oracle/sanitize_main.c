@@ -38,20 +38,16 @@ static void run_matchers(int W, int H, int D)
     int16_t* dL = malloc(N * 2);
     int16_t* dR = malloc(N * 2);
     int16_t* out = malloc(N * 2);
-    int16_t* lo = malloc(N * 2);
-    int16_t* hi = malloc(N * 2);
     int32_t* cost = malloc(N * 4);
     int32_t* vol = malloc(N * (size_t)D * 4);
     double* sc = malloc(N * 8);
     double* vd = malloc(N * (size_t)D * 8);
-    for (size_t i = 0; i < N; ++i) { lo[i] = (int16_t)(next() % D); hi[i] = (int16_t)(lo[i] + 1 + next() % 8); }
     EXPECT(orc_census(L, W, H, 9, 7, cl) == 0);
     EXPECT(orc_census(R, W, H, 9, 7, cr) == 0);
     uint8_t* c3 = malloc(N);
     EXPECT(orc_census3x3_ref(L, W, H, c3) == 0);
     EXPECT(orc_bm(ORC_COST_SAD, L, R, W, H, D, 9, 9, 1, 1, 1, dL, dR, out, cost, 3) == 0);
     EXPECT(orc_bm(ORC_COST_CENSUS, cl, cr, W, H, D, 5, 5, 9, 7, 1, dL, dR, out, NULL, 2) == 0);
-    EXPECT(orc_bm_ranges(ORC_COST_CENSUS, cl, cr, W, H, D, 9, 9, 9, 7, 1, lo, hi, dL, dR, out, 2) == 0);
     uint32_t* bins = malloc(N * 4);
     for (size_t i = 0; i < N; ++i) bins[i] = next();
     EXPECT(orc_bm_bins(ORC_COST_SAD, L, R, W, H, D, 5, 5, 1, 1, 1, bins, dL, dR, out, 2) == 0);
@@ -68,7 +64,7 @@ static void run_matchers(int W, int H, int D)
     uint16_t mn = 0, mx = 0;
     EXPECT(orc_crosscheck_ref(a, b, W, H, 1, 1, 0, o, &mn, &mx) == 0);
     EXPECT(orc_crosscheck_ref(a, b, W, H, 2, 0, 3, o, &mn, &mx) == 0);
-    free(L); free(R); free(cl); free(cr); free(dL); free(dR); free(out); free(lo); free(hi); free(cost);
+    free(L); free(R); free(cl); free(cr); free(dL); free(dR); free(out); free(cost);
     free(vol); free(sc); free(vd); free(c3); free(a); free(b); free(o);
 }
 

@@ -142,11 +142,13 @@ static int bm_args_ok(int kind, const void* L, const void* R, int W, int H, int 
     return 1;
 }
 
-/* lo / hi (optional, per left pixel): only candidates lo <= d < hi take part, in
- * both winner-take-alls -- the band search of the learned proposal, whose ranges
- * are per 64 x 56 tile and expanded to pixels by the caller. */
+/* bins (optional, per left pixel): only candidates whose 8-disparity bin d >> 3 is set
+ * take part, in both winner-take-alls -- the band search around an initial map and
+ * of the learned proposal, whose bins are per 64 x 56 tile and expanded to pixels by
+ * the caller.  The right view's candidate d of right pixel xr is the left pixel
+ * xr + d's, so it obeys that left pixel's mask. */
 static int bm_impl(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
-                   int ch, int tol, const int16_t* lo, const int16_t* hi, const uint32_t* bins, int16_t* dL,
+                   int ch, int tol, const uint32_t* bins, int16_t* dL,
                    int16_t* dR, int16_t* out, int32_t* cost_out, int nthreads)
 {
     if (!bm_args_ok(kind, L, R, W, H, D, ww, wh, cw, ch) || !dL || !dR) return -1;
@@ -193,7 +195,6 @@ static int bm_impl(int kind, const void* L, const void* R, int W, int H, int D, 
                     const size_t row = (size_t)y * W;
                     for (int x = xs; x < xe; ++x) {
                         const int A = V[x];
-                        if (lo && (d < lo[row + x] || d >= hi[row + x])) continue;
                         if (bins && !((bins[row + x] >> (d >> 3)) & 1u)) continue;
                         if (A < bestL[row + x]) { bestL[row + x] = A; dL[row + x] = (int16_t)d; }
                         if (A < bestR[row + x - d]) { bestR[row + x - d] = A; dR[row + x - d] = (int16_t)d; }
@@ -215,22 +216,14 @@ static int bm_impl(int kind, const void* L, const void* R, int W, int H, int D, 
 int orc_bm(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
            int ch, int tol, int16_t* dL, int16_t* dR, int16_t* out, int32_t* cost_out, int nthreads)
 {
-    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, NULL, NULL, NULL, dL, dR, out, cost_out, nthreads);
-}
-
-int orc_bm_ranges(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
-                  int ch, int tol, const int16_t* lo, const int16_t* hi, int16_t* dL, int16_t* dR,
-                  int16_t* out, int nthreads)
-{
-    if (!lo || !hi) return -1;
-    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, lo, hi, NULL, dL, dR, out, NULL, nthreads);
+    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, NULL, dL, dR, out, cost_out, nthreads);
 }
 
 int orc_bm_bins(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
                 int ch, int tol, const uint32_t* bins, int16_t* dL, int16_t* dR, int16_t* out, int nthreads)
 {
     if (!bins || D > 256) return -1;
-    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, NULL, NULL, bins, dL, dR, out, NULL, nthreads);
+    return bm_impl(kind, L, R, W, H, D, ww, wh, cw, ch, tol, bins, dL, dR, out, NULL, nthreads);
 }
 
 int orc_bm_brute(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh,
@@ -655,7 +648,7 @@ int orc_disp_denoise(int ref_format, const void* in, int W, int H, int P, float 
 /* The reference launches one 32-thread block per PCX x 4 result origins                 */
 /* (AlgoArraySum<32, 2, P, P>, Kernel.Algo.ArraySum.h:38-60: PCX = ((32 - P + 1) / 8) * 8), */
 /* blocks counted by EvalThreadConfig_ReadDepBlockSize (KernelDriverBase.cpp:166-197)     */
-/* from a 33 x (P + 3) read area.  Which disparities a result sees, and where its test    */
+/* from a 33 x (P + 3) read area, full blocks only.  Which disparities a result sees, and where its test    */
 /* patch lies, is decided per block; this function follows those decisions.              */
 
 /* Similarity of two pixels (EvalSim, BlockMatcher1.cu:184-221). */
@@ -710,7 +703,9 @@ int orc_bm1(const uint8_t* L, const uint8_t* R, int W, int H, int P, int sim_typ
         return -1;
     const int CG = (32 - P + 1) / 8, PCX = CG * 8, PCY = 4, TBX = 33, TBY = P + 3, SDC = 16;
     const unsigned effW = (unsigned)W - (unsigned)(TBX - PCX), effH = (unsigned)H - (unsigned)(TBY - PCY);
-    const int nbx = (int)(effW / PCX + (effW % PCX > 0)), nby = (int)(effH / PCY + (effH % PCY > 0));
+    /* only full blocks: bOnlyFullBlocks defaults to true (KernelDriverBase.h:117-121), so
+       the remainder strips at the right and bottom are never launched */
+    const int nbx = (int)(effW / PCX), nby = (int)(effH / PCY);
     const int NR = PCX * PCY;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int by = 0; by < nby; ++by) {

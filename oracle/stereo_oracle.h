@@ -48,10 +48,6 @@ int orc_bm(int cost_kind, const void* L, const void* R, int W, int H, int D, int
 /* Same specification evaluated by direct 2-D window sums (no running sums).
  * volume (optional, W*H*D int32, index (y*W+x)*D+d) receives every aggregated cost,
  * -1 for invalid (pixel, d) combinations. */
-/* orc_bm restricted to candidates lo[p] <= d < hi[p] of the left pixel p. */
-int orc_bm_ranges(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
-                  int ch, int tol, const int16_t* lo, const int16_t* hi, int16_t* dL, int16_t* dR,
-                  int16_t* out, int nthreads);
 /* orc_bm restricted to candidates whose 8-disparity bin d >> 3 is set in bins[p]
  * of the left pixel p (D <= 256). */
 int orc_bm_bins(int kind, const void* L, const void* R, int W, int H, int D, int ww, int wh, int cw,
@@ -115,6 +111,27 @@ int orc_disp_denoise(int ref_format, const void* in, int W, int H, int P, float 
 int orc_bm1(const uint8_t* L, const uint8_t* R, int W, int H, int P, int sim_type, int dmin, int dmax, int offset,
             int ltor, float min_delta, float sim_thresh, float grad_thresh, const uint16_t* init, int iW, int iH,
             uint16_t* out);
+
+/* The reference's BlockMatcherNCC (BlockMatcherNCC.cu:614-884, and MatcherUseDisparity
+ * :894-1255 when init is given), emulating its block tiling: 32-thread blocks of
+ * (33 - P) / 2 * 2 x 16 result origins from a 33 x (15 + P) read area, P in {5, 9, 11}.
+ * The similarity is the reference's float NCC truncated to a byte; the WTA takes a
+ * strictly larger byte.  out is W*H*4 uint16: EDisparityId code, the byte maximum and the
+ * two neighbour deltas (iMinDeltaThresh applies to them, fNccThresh to max / 255);
+ * unwritten pixels are left as they are.  Needs W >= 48 and H >= 15 + P. */
+int orc_ncc_ref(const uint8_t* L, const uint8_t* R, int W, int H, int P, int dmin, int dmax, int offset, int ltor,
+                int min_delta, float ncc_thresh, float grad_thresh, const uint16_t* init, int iW, int iH,
+                uint16_t* out);
+
+/* The reference's BlockMatcherAW1 (BlockMatcherAW1.cu:657-913, MatcherUseDisparity
+ * :923-1263), emulating its block tiling: sub-patch size s in {5, 3} (EConfig Patch_15x15,
+ * Patch_9x9), blocks of 18 x 6 or 24 x 10 result origins from a 32 x 20 or 32 x 18 read
+ * area, sim_type 0 (255 - |a - b|) or 1 (signed product).  Output as orc_bm1, with the
+ * similarity channels in % of the 3 s^2-pixel window's maximum.  Needs W >= 48 and
+ * H >= 20 (s = 5) or 18 (s = 3). */
+int orc_aw1_ref(const uint8_t* L, const uint8_t* R, int W, int H, int s, int sim_type, int dmin, int dmax, int offset,
+                int ltor, float min_delta, float sim_thresh, float grad_thresh, const uint16_t* init, int iW, int iH,
+                uint16_t* out);
 
 /* RectifyImage (RectifyImage.cu:64-158): target pixel -> source position through the
  * rectified camera (fx, fy, cx, cy), a rotation into the source camera and the source

@@ -161,6 +161,50 @@ def test_bm_matches_numpy_volume(kind, win):
         assert np.array_equal(c, b)
 
 
+@pytest.mark.parametrize("kind", [orc.COST_SAD, orc.COST_CENSUS])
+def test_bm_bins_against_the_cost_volume(kind):
+    """The bins-restricted SAD / census matcher (the checker of the band searches): with
+    every bin set it is orc_bm; with a random mask per left pixel, the left view is the
+    lowest allowed cost of the independent numpy volume, and the right pixel xr takes the
+    lowest cost over the left pixels xr + d whose own mask allows d (lowest d on ties)."""
+    rng = np.random.default_rng(60 + kind)
+    H, W, D = 24, 72, 32
+    L8 = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    R8 = np.roll(L8, -6, axis=1)
+    R8 = np.clip(R8.astype(int) + rng.integers(-9, 10, (H, W)), 0, 255).astype(np.uint8)
+    if kind == orc.COST_SAD:
+        L, R, cwin, win = L8, R8, (1, 1), (5, 5)
+    else:
+        L, R, cwin, win = orc.census(L8, 5, 3), orc.census(R8, 5, 3), (5, 3), (5, 3)
+    full = orc.bm(kind, L, R, D, window=win, census_win=cwin)
+    every = orc.bm_bins(kind, L, R, D, np.full((H, W), 0xF, np.uint32), window=win, census_win=cwin)
+    for a, b in zip(full, every):
+        assert np.array_equal(a, b)
+    vol = np_cost_volume(kind, L, R, D, win[0], win[1], cwin[0], cwin[1])
+    bins = rng.integers(0, 16, (H, W)).astype(np.uint32)
+    out, dl, dr = orc.bm_bins(kind, L, R, D, bins, window=win, census_win=cwin, nthreads=3)
+    d = np.arange(D)
+    allowed = (vol >= 0) & (((bins[:, :, None] >> (d[None, None, :] >> 3)) & 1) == 1)
+    big = np.iinfo(np.int64).max
+    q = np.where(allowed, vol, big)
+    want_l = np.where(allowed.any(-1), q.argmin(-1), -1)
+    assert np.array_equal(dl, want_l)
+    want_r = np.full((H, W), -1)
+    best = np.full((H, W), big)
+    for dd in range(D):
+        xs = np.arange(W - dd)
+        cand = q[:, xs + dd, dd]  # the mask of left pixel xr + d decides
+        better = cand < best[:, xs]
+        best[:, xs] = np.where(better, cand, best[:, xs])
+        want_r[:, xs] = np.where(better, dd, want_r[:, xs])
+    assert np.array_equal(dr, want_r)
+    assert np.array_equal(out, orc.crosscheck_plain(dl, dr, 1))
+    # a mask that differs between the views changes the right map: the rule is not vacuous
+    shifted = np.roll(bins, 5, axis=1)
+    assert not np.array_equal(orc.bm_bins(kind, L, R, D, shifted, window=win, census_win=cwin)[2], dr)
+    assert (dl >= 0).mean() > 0.5
+
+
 def test_bm_noise_free_shift_is_exact():
     # R(x) = L(x + d*): the true disparity has cost 0, and textured windows are unique
     rng = np.random.default_rng(3)

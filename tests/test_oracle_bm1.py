@@ -44,8 +44,8 @@ def numpy_bm1(L, R, P, sim_type, dmin, dmax, off, ltor, min_delta, sim_thr, grad
     H, W = L.shape
     CG = (32 - P + 1) // 8
     PCX = 8 * CG
-    nbx = -(-(W - (33 - PCX)) // PCX)
-    nby = -(-(H - (P - 1)) // 4)
+    nbx = (W - (33 - PCX)) // PCX  # full blocks only
+    nby = (H - (P - 1)) // 4
     out = np.full((H, W, 4), fill, np.uint16)
     boxes = {}
 
@@ -243,6 +243,20 @@ def test_coverage_and_unwritten_pixels():
     assert ys.max() == H - P - 1 + P // 2
 
 
+def test_only_full_blocks_are_launched():
+    """EvalThreadConfig_ReadDepBlockSize keeps only full blocks (bOnlyFullBlocks defaults to
+    true, KernelDriverBase.h:117-121): the remainder strips at the right and the bottom are
+    never written."""
+    W, H, P = 118, 47, 5  # 118 - 9 = 109: 4 full blocks of 24 origins; 47 - 4 = 43: 10 rows of 4
+    L = _textured(9, W, H)
+    out = orc.bm1(L, L, P=P, dmin=0, dmax=16, fill=0x7777)
+    written = (out != 0x7777).any(-1)
+    ys, xs = np.nonzero(written)
+    assert xs.max() == 95 + P // 2  # origins 96..112 lie in the partial fifth block
+    assert ys.max() == 39 + P // 2  # origin rows 40..41 lie in the partial eleventh block row
+    assert np.array_equal(out, numpy_bm1(L, L, P, 0, 0, 16, 0, 1, 0.0, 0.0, 0.0, fill=0x7777))
+
+
 def test_bottom_blocks_are_not_evaluated():
     """A block whose P + 3 read rows pass the bottom evaluates nothing: Unknown."""
     W, H, P = 96, 30, 9  # blocks start at rows 0, 4, .., 20; rows 20 + 12 > 30
@@ -254,12 +268,12 @@ def test_bottom_blocks_are_not_evaluated():
 
 
 def test_flat_patches_cannot_be_evaluated():
-    W, H = 96, 40
+    W, H = 105, 40  # 105 - 9 = 96: four full blocks, centres 2..97
     L = np.full((H, W), 90, np.uint8)
     out = orc.bm1(L, L, P=5, dmin=0, dmax=16, grad_thresh=0.5)
-    written = out[..., 0][2:H - 4, 2:W - 4]
+    written = out[..., 0][2:H - 4, 2:W - 7]
     assert (written == CANNOT).all()
-    assert (out[2:H - 4, 2:W - 4, 1:] == 0).all()
+    assert (out[2:H - 4, 2:W - 7, 1:] == 0).all()
 
 
 def test_saturated_at_the_range_ends():
