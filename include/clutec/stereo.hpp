@@ -256,26 +256,29 @@ public:
 };
 }  // namespace CensusTransform
 
-// Shared implementation of the matcher drivers.
+// Shared implementation of the fused plain-specification matchers: one call runs the
+// left-to-right and right-to-left winner-take-alls and the cross check
+// (CrossCheckDisparityMaps.cu:62-109), writing the checked map as LUM_I16 (-1 invalid) or
+// as RGBA_UInt16 disparity-id pixels, and optionally the two one-sided LUM_I16 maps.
+// Their disparity range is [0, D) with no pixel offset.
 class CMatcherDriver : public CKernelDriverBase
 {
 public:
     using CKernelDriverBase::CKernelDriverBase;
 
-    // disp: LUM_I16 checked map or RGBA_UInt16 disparity-id pixels; lr / rl optional.
     void Process(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R)
     {
         EnsureOK(stereo_matcher_process(m_drv, disp.Raw(), nullptr, nullptr, L.Raw(), R.Raw(), nullptr), m_ctx,
                  "Matcher::Process");
     }
-    void Process(CDeviceImage& disp, CDeviceImage& lr, CDeviceImage& rl, const CDeviceImage& L, const CDeviceImage& R)
+    void ProcessMaps(CDeviceImage& disp, CDeviceImage& lr, CDeviceImage& rl, const CDeviceImage& L, const CDeviceImage& R)
     {
         EnsureOK(stereo_matcher_process(m_drv, disp.Raw(), lr.Raw(), rl.Raw(), L.Raw(), R.Raw(), nullptr), m_ctx,
                  "Matcher::Process");
     }
     // With an initial disparity map (LUM_I16 or RGBA_UInt16 ids, any size up to the
-    // images'): the band search of MatcherUseDisparity (BlockMatcherNCC.cu:894-1255).
-    // The driver must be configured with iUseDispInput.
+    // images'): the band search of MatcherUseDisparity (BlockMatcherNCC.cu:894-1255) over
+    // 8-disparity bins per 64 x 56 tile.  The driver must be configured with iUseDispInput.
     void Process(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R, const CDeviceImage& dispInit)
     {
         EnsureOK(stereo_matcher_process_init(m_drv, disp.Raw(), nullptr, nullptr, L.Raw(), R.Raw(), dispInit.Raw(),
@@ -305,22 +308,63 @@ protected:
         p.lr_tolerance = tol;
         return p;
     }
+    // The fused matchers search [0, D) from the left pixel's own column: a range that does
+    // not start at 0, or a pixel offset, is refused rather than searched differently.
+    static int32_t PlainRange(const _SDisparityConfig& c, const char* who)
+    {
+        if (c.Min() != 0 || c.iOffset != 0)
+            throw CException(STEREO_ERROR_INVALID_CONFIG,
+                             std::string(who) + ": the fused matcher searches disparities [0, D) with no offset; "
+                                                "Min() = " + std::to_string(c.Min()) + ", offset = " +
+                                 std::to_string(c.iOffset) + " (the reference-format BlockMatcherNCC / "
+                                                             "BlockMatcherAW1 drivers take both)");
+        return c.Range();
+    }
+    static void Inactive(bool active, const char* who, const char* what)
+    {
+        if (active)
+            throw CException(STEREO_ERROR_INVALID_CONFIG,
+                             std::string(who) + ": " + what + " is not applied by the fused matcher; pass an inactive value");
+    }
 };
 
+// BlockMatcherSAD1 (BlockMatcherSAD1.h:40-140).  The reference operator does not compile
+// (it reads AlgoSum::PatchCountSplitX, which AlgoArraySum lacks) and is excluded from its
+// build, so no reference results exist; the driver keeps its SParameter::Set signature
+// and runs the fused SAD matcher.  Its only reference window, Patch_16x16, is even and is
+// refused; the fused windows 9x9, 7x7 and 5x5 follow it in the enum.  Thresholds must be
+// inactive: fSadThresh > 255 (a mean SAD always passes), iMinDeltaThresh < 0, fGradThresh <= 0.
 namespace BlockMatcherSAD1
 {
-struct SParameter
+enum class EConfig
 {
-    _SDisparityConfig xDispConfig;
-    int32_t iPatchSize = 9;
-    int32_t iLrTolerance = 1;
+    Patch_16x16 = 0,
+    Patch_9x9,
+    Patch_7x7,
+    Patch_5x5,
+};
+struct _SParameter
+{
+    int32_t iOffset = 0;
+    int32_t iDispRange = 64;
+    float fSadThresh = 1e30f;
+    int32_t iMinDeltaThresh = -1;
+    float fGradThresh = 0.0f;
+};
+struct SParameter : public _SParameter
+{
+    EConfig eConfig = EConfig::Patch_9x9;
+    int32_t iLrTolerance = 1;   // the fused check's |dR - dL| tolerance
     int32_t iUseDispInput = 0;  // 1: Process may take an initial disparity map
-    void Set(const _SDisparityConfig& cfg, int32_t patch = 9, int32_t tol = 1, int32_t useDispInput = 0)
+    void Set(int32_t _iOffset, int32_t _iDispRange, int32_t _iMinDeltaThresh, float _fSadThresh, float _fGradThresh,
+             EConfig _eConfig)
     {
-        xDispConfig = cfg;
-        iPatchSize = patch;
-        iLrTolerance = tol;
-        iUseDispInput = useDispInput;
+        iOffset = _iOffset;
+        iDispRange = _iDispRange;
+        iMinDeltaThresh = _iMinDeltaThresh;
+        fSadThresh = _fSadThresh;
+        fGradThresh = _fGradThresh;
+        eConfig = _eConfig;
     }
 };
 class CDriver : public CMatcherDriver
@@ -329,12 +373,29 @@ public:
     CDriver() : CMatcherDriver("Block Matcher SAD 1") {}
     void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
     {
-        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_SAD, p.iPatchSize, p.iPatchSize, p.xDispConfig.Range(),
-                                                   p.iLrTolerance, 1, 1, p.iUseDispInput));
+        static const int32_t sizes[] = {16, 9, 7, 5};
+        const int i = static_cast<int>(p.eConfig);
+        if (i < 1 || i > 3)
+            throw CException(STEREO_ERROR_INVALID_CONFIG,
+                             "BlockMatcherSAD1: Patch_16x16 (the reference's only window, even) is not built; use "
+                             "Patch_9x9, Patch_7x7 or Patch_5x5");
+        _SDisparityConfig c;
+        c.iMin = 0;
+        c.iMax = p.iDispRange;
+        c.iOffset = p.iOffset;
+        const int32_t D = PlainRange(c, "BlockMatcherSAD1");
+        Inactive(!(p.fSadThresh > 255.0f), "BlockMatcherSAD1", "fSadThresh");
+        Inactive(p.iMinDeltaThresh >= 0, "BlockMatcherSAD1", "iMinDeltaThresh");
+        Inactive(p.fGradThresh > 0.0f, "BlockMatcherSAD1", "fGradThresh");
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_SAD, sizes[i], sizes[i], D, p.iLrTolerance, 1, 1,
+                                                   p.iUseDispInput));
     }
 };
 }  // namespace BlockMatcherSAD1
 
+// The census matcher: CensusDifference (BlockMatcherNCC.cu:153-158) over the fused matcher.
+// The reference has no census block matcher of its own (its call sites are commented
+// out), so the parameters are this engine's.
 namespace BlockMatcherCensus
 {
 struct SParameter
@@ -360,11 +421,36 @@ public:
     void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
     {
         CMatcherDriver::Configure(dev, fmt,
-                                  Params(STEREO_ALGO_CENSUS, p.iPatchSize, p.iPatchSize, p.xDispConfig.Range(), p.iLrTolerance,
-                                         p.iCensusW, p.iCensusH, p.iUseDispInput));
+                                  Params(STEREO_ALGO_CENSUS, p.iPatchSize, p.iPatchSize,
+                                         PlainRange(p.xDispConfig, "BlockMatcherCensus"), p.iLrTolerance, p.iCensusW,
+                                         p.iCensusH, p.iUseDispInput));
     }
 };
 }  // namespace BlockMatcherCensus
+
+// Reference-format drivers of BlockMatcherNCC / BlockMatcherAW1: one direction, the
+// reference's block tiling, decisions and RGBA_UInt16 channels (stereo_ncc_*, stereo_aw1_*).
+class CRefMatcherDriver : public CKernelDriverBase
+{
+public:
+    using CKernelDriverBase::CKernelDriverBase;
+    // The reference's Process (BlockMatcherNCC.h:137-143, BlockMatcherAW1.h:151-157): dispInit
+    // is read when configured with iUseDispInput; the debug surface is not written.
+    void Process(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R, const CDeviceImage& dispInit,
+                 const CDeviceImage& /*debug*/)
+    {
+        Run(disp, L, R, m_bUseInit ? dispInit.Raw() : nullptr);
+    }
+    void Process(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R, const CDeviceImage& dispInit)
+    {
+        Run(disp, L, R, m_bUseInit ? dispInit.Raw() : nullptr);
+    }
+    void Process(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R) { Run(disp, L, R, nullptr); }
+
+protected:
+    virtual void Run(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R, const stereo_image* init) = 0;
+    bool m_bUseInit = false;
+};
 
 namespace BlockMatcherNCC
 {
@@ -374,12 +460,78 @@ enum class EConfig
     Patch_9x9,
     Patch_5x5,
 };
-struct SParameter
+inline int32_t PatchSize(EConfig e)
+{
+    static const int32_t sizes[] = {11, 9, 5};
+    const int i = static_cast<int>(e);
+    return i >= 0 && i < 3 ? sizes[i] : 0;
+}
+// BlockMatcherNCC.h:83-111
+struct _SParameter
+{
+    _SDisparityConfig xDispConfig;
+    int32_t iMinDeltaThresh = 0;
+    float fNccThresh = 0.0f;
+    float fGradThresh = 0.0f;
+    int32_t iUseDispInput = 0;
+};
+struct SParameter : public _SParameter
+{
+    EConfig eConfig = EConfig::Patch_11x11;
+    bool bLeftToRight = true;
+    void Set(const _SDisparityConfig& _xDispConfig, int32_t _iMinDeltaThresh, float _fNccThresh, float _fGradThresh,
+             int32_t _iUseDispInput, bool _bLeftToRight, EConfig _eConfig)
+    {
+        xDispConfig = _xDispConfig;
+        iMinDeltaThresh = _iMinDeltaThresh;
+        fNccThresh = _fNccThresh;
+        fGradThresh = _fGradThresh;
+        iUseDispInput = _iUseDispInput;
+        bLeftToRight = _bLeftToRight;
+        eConfig = _eConfig;
+    }
+};
+class CDriver : public CRefMatcherDriver
+{
+public:
+    CDriver() : CRefMatcherDriver("Block Matcher NCC") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    {
+        const int32_t P = PatchSize(p.eConfig);
+        if (!P) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
+        const stereo_image_format f = fmt.Raw();
+        stereo_ncc_params q{};
+        q.patch = P;
+        q.disp_min = p.xDispConfig.iMin;
+        q.disp_max = p.xDispConfig.iMax;
+        q.disp_offset = p.xDispConfig.iOffset;
+        q.min_delta_thresh = p.iMinDeltaThresh;
+        q.ncc_thresh = p.fNccThresh;
+        q.grad_thresh = p.fGradThresh;
+        q.use_disp_input = p.iUseDispInput ? 1 : 0;
+        q.left_to_right = p.bLeftToRight ? 1 : 0;
+        stereo_driver* d = nullptr;
+        EnsureOK(stereo_ncc_configure(dev.Context(), &d, &f, &q), dev.Context(), "BlockMatcherNCC::Configure");
+        Reset(dev.Context(), d);
+        m_bUseInit = q.use_disp_input != 0;
+    }
+
+protected:
+    void Run(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R, const stereo_image* init) override
+    {
+        EnsureOK(stereo_ncc_process(m_drv, disp.Raw(), L.Raw(), R.Raw(), init, nullptr), m_ctx,
+                 "BlockMatcherNCC::Process");
+    }
+};
+
+// The fused plain-specification NCC matcher: exact integer covariance, both directions and
+// the check in one call (stereo_matcher_* with STEREO_ALGO_NCC).
+struct SFusedParameter
 {
     _SDisparityConfig xDispConfig;
     EConfig eConfig = EConfig::Patch_11x11;
     int32_t iLrTolerance = 1;
-    int32_t iUseDispInput = 0;  // 1: Process may take an initial disparity map (BlockMatcherNCC.h:88)
+    int32_t iUseDispInput = 0;
     void Set(const _SDisparityConfig& cfg, EConfig config, int32_t tol = 1, int32_t useDispInput = 0)
     {
         xDispConfig = cfg;
@@ -388,16 +540,16 @@ struct SParameter
         iUseDispInput = useDispInput;
     }
 };
-class CDriver : public CMatcherDriver
+class CFusedDriver : public CMatcherDriver
 {
 public:
-    CDriver() : CMatcherDriver("Block Matcher NCC") {}
-    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    CFusedDriver() : CMatcherDriver("Block Matcher NCC (fused)") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SFusedParameter& p)
     {
-        const int32_t w = p.eConfig == EConfig::Patch_11x11 ? 11 : p.eConfig == EConfig::Patch_9x9 ? 9 : p.eConfig == EConfig::Patch_5x5 ? 5 : -1;
-        if (w < 0) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
-        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_NCC, w, w, p.xDispConfig.Range(), p.iLrTolerance, 1, 1,
-                                                   p.iUseDispInput));
+        const int32_t w = PatchSize(p.eConfig);
+        if (!w) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_NCC, w, w, PlainRange(p.xDispConfig, "BlockMatcherNCC"),
+                                                   p.iLrTolerance, 1, 1, p.iUseDispInput));
     }
     // winning scores of the last Process (LUM_Single)
     void Scores(CDeviceImage& score, const CDeviceImage& L, const CDeviceImage& R)
@@ -414,12 +566,84 @@ enum class EConfig
     Patch_15x15 = 0,
     Patch_9x9,
 };
-struct SParameter
+enum class ESimType
+{
+    AbsoluteDifference = 0,
+    SignedProduct,
+};
+inline int32_t PatchSize(EConfig e)
+{
+    return e == EConfig::Patch_15x15 ? 15 : e == EConfig::Patch_9x9 ? 9 : 0;
+}
+// BlockMatcherAW1.h:84-124
+struct _SParameter
+{
+    _SDisparityConfig xDispConfig;
+    float fMinDeltaThresh = 0.0f;
+    float fSimThresh = 0.0f;
+    float fGradThresh = 0.0f;
+    int32_t iUseDispInput = 0;
+};
+struct SParameter : public _SParameter
+{
+    EConfig eConfig = EConfig::Patch_15x15;
+    ESimType eSimType = ESimType::AbsoluteDifference;
+    bool bLeftToRight = true;
+    void Set(const _SDisparityConfig& _xDispConfig, float _fMinDeltaThresh, float _fSimThresh, float _fGradThresh,
+             int32_t _iUseDispInput, bool _bLeftToRight, EConfig _eConfig, ESimType _eSimType)
+    {
+        xDispConfig = _xDispConfig;
+        fMinDeltaThresh = _fMinDeltaThresh;
+        fSimThresh = _fSimThresh;
+        fGradThresh = _fGradThresh;
+        iUseDispInput = _iUseDispInput;
+        bLeftToRight = _bLeftToRight;
+        eConfig = _eConfig;
+        eSimType = _eSimType;
+    }
+};
+class CDriver : public CRefMatcherDriver
+{
+public:
+    CDriver() : CRefMatcherDriver("Block Matcher Adaptive Window 1") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    {
+        const int32_t P = PatchSize(p.eConfig);
+        if (!P) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
+        const stereo_image_format f = fmt.Raw();
+        stereo_aw1_params q{};
+        q.patch = P;
+        q.sim_type = static_cast<int32_t>(p.eSimType);
+        q.disp_min = p.xDispConfig.iMin;
+        q.disp_max = p.xDispConfig.iMax;
+        q.disp_offset = p.xDispConfig.iOffset;
+        q.min_delta_thresh = p.fMinDeltaThresh;
+        q.sim_thresh = p.fSimThresh;
+        q.grad_thresh = p.fGradThresh;
+        q.use_disp_input = p.iUseDispInput ? 1 : 0;
+        q.left_to_right = p.bLeftToRight ? 1 : 0;
+        stereo_driver* d = nullptr;
+        EnsureOK(stereo_aw1_configure(dev.Context(), &d, &f, &q), dev.Context(), "BlockMatcherAW1::Configure");
+        Reset(dev.Context(), d);
+        m_bUseInit = q.use_disp_input != 0;
+    }
+
+protected:
+    void Run(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R, const stereo_image* init) override
+    {
+        EnsureOK(stereo_aw1_process(m_drv, disp.Raw(), L.Raw(), R.Raw(), init, nullptr), m_ctx,
+                 "BlockMatcherAW1::Process");
+    }
+};
+
+// The fused plain-specification AW1 matcher (stereo_matcher_* with STEREO_ALGO_AW1):
+// AbsoluteDifference similarity, both directions and the check in one call.
+struct SFusedParameter
 {
     _SDisparityConfig xDispConfig;
     EConfig eConfig = EConfig::Patch_15x15;
     int32_t iLrTolerance = 1;
-    int32_t iUseDispInput = 0;  // 1: Process may take an initial disparity map (BlockMatcherAW1.h:91)
+    int32_t iUseDispInput = 0;
     void Set(const _SDisparityConfig& cfg, EConfig config, int32_t tol = 1, int32_t useDispInput = 0)
     {
         xDispConfig = cfg;
@@ -428,35 +652,83 @@ struct SParameter
         iUseDispInput = useDispInput;
     }
 };
-class CDriver : public CMatcherDriver
+class CFusedDriver : public CMatcherDriver
 {
 public:
-    CDriver() : CMatcherDriver("Block Matcher Adaptive Window 1") {}
-    void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
+    CFusedDriver() : CMatcherDriver("Block Matcher Adaptive Window 1 (fused)") {}
+    void Configure(const CDevice& dev, const SImageFormat& fmt, const SFusedParameter& p)
     {
-        const int32_t s = p.eConfig == EConfig::Patch_15x15 ? 5 : p.eConfig == EConfig::Patch_9x9 ? 3 : -1;
-        if (s < 0) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
-        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_AW1, s, s, p.xDispConfig.Range(), p.iLrTolerance, 1, 1,
-                                                   p.iUseDispInput));
+        const int32_t s = PatchSize(p.eConfig) / 3;
+        if (!s) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_AW1, s, s, PlainRange(p.xDispConfig, "BlockMatcherAW1"),
+                                                   p.iLrTolerance, 1, 1, p.iUseDispInput));
     }
 };
 }  // namespace BlockMatcherAW1
 
+// BlockMatcherAW2 (BlockMatcherAW2.h:40-140).  Excluded from the reference's build, whose
+// band-search branch is empty (BlockMatcherAW2.cu:858-875); the driver keeps its
+// SParameter::Set signature and runs the fused AW2 matcher (SAD, the two smallest
+// neighbours), its band search included.  iOffset must be 0 and the thresholds inactive:
+// fSadThresh > 255, iMinDeltaThresh < 0, fGradThresh <= 0.
 namespace BlockMatcherAW2
 {
 using EConfig = BlockMatcherAW1::EConfig;
-using SParameter = BlockMatcherAW1::SParameter;
+struct _SParameter
+{
+    int32_t iOffset = 0;
+    int32_t iDispRange = 64;
+    int32_t iMinDeltaThresh = -1;
+    float fSadThresh = 1e30f;
+    float fGradThresh = 0.0f;
+    int32_t iUseDispInput = 0;
+};
+struct SParameter : public _SParameter
+{
+    EConfig eConfig = EConfig::Patch_15x15;
+    int32_t iLrTolerance = 1;
+    void Set(int32_t _iOffset, int32_t _iDispRange, int32_t _iMinDeltaThresh, float _fSadThresh, float _fGradThresh,
+             int32_t _iUseDispInput, EConfig _eConfig)
+    {
+        iOffset = _iOffset;
+        iDispRange = _iDispRange;
+        iMinDeltaThresh = _iMinDeltaThresh;
+        fSadThresh = _fSadThresh;
+        fGradThresh = _fGradThresh;
+        iUseDispInput = _iUseDispInput;
+        eConfig = _eConfig;
+    }
+};
 class CDriver : public CMatcherDriver
 {
 public:
     CDriver() : CMatcherDriver("Block Matcher Adaptive Window 2") {}
     void Configure(const CDevice& dev, const SImageFormat& fmt, const SParameter& p)
     {
-        const int32_t s = p.eConfig == EConfig::Patch_15x15 ? 5 : p.eConfig == EConfig::Patch_9x9 ? 3 : -1;
-        if (s < 0) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
-        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_AW2, s, s, p.xDispConfig.Range(), p.iLrTolerance, 1, 1,
-                                                   p.iUseDispInput));
+        const int32_t s = BlockMatcherAW1::PatchSize(p.eConfig) / 3;
+        if (!s) throw CException(STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
+        _SDisparityConfig c;
+        c.iMin = 0;
+        c.iMax = p.iDispRange;
+        c.iOffset = p.iOffset;
+        const int32_t D = PlainRange(c, "BlockMatcherAW2");
+        Inactive(!(p.fSadThresh > 255.0f), "BlockMatcherAW2", "fSadThresh");
+        Inactive(p.iMinDeltaThresh >= 0, "BlockMatcherAW2", "iMinDeltaThresh");
+        Inactive(p.fGradThresh > 0.0f, "BlockMatcherAW2", "fGradThresh");
+        m_bUseInit = p.iUseDispInput != 0;
+        CMatcherDriver::Configure(dev, fmt, Params(STEREO_ALGO_AW2, s, s, D, p.iLrTolerance, 1, 1, p.iUseDispInput));
     }
+    // The reference's Process (BlockMatcherAW2.h:129-134): dispInit is read when configured
+    // with iUseDispInput.
+    void Process(CDeviceImage& disp, const CDeviceImage& L, const CDeviceImage& R, const CDeviceImage& dispInit)
+    {
+        if (m_bUseInit) CMatcherDriver::Process(disp, L, R, dispInit);
+        else CMatcherDriver::Process(disp, L, R);
+    }
+    using CMatcherDriver::Process;
+
+private:
+    bool m_bUseInit = false;
 };
 }  // namespace BlockMatcherAW2
 

@@ -400,6 +400,68 @@ stereo_status stereo_bm1_configure(stereo_context* ctx, stereo_driver** drv, con
 stereo_status stereo_bm1_process(stereo_driver* drv, stereo_image* disp, const stereo_image* left,
                                  const stereo_image* right, const stereo_image* disp_init, stereo_stream stream);
 
+/* ---- BlockMatcherNCC and BlockMatcherAW1 with the reference's results ------------------
+ *
+ * The reference operators themselves, as opposed to the fused plain-specification matcher
+ * above (stereo_matcher_* with STEREO_ALGO_NCC / STEREO_ALGO_AW1): one direction, the
+ * reference's block tiling (32-thread blocks of result origins, full blocks only), its
+ * disparity groups and test-patch placement, its winner rule (a strictly larger value
+ * wins, the lowest disparity on ties), its gates and codes, and its four output channels.
+ * disp is RGBA_U16; pixels the reference does not write are left as they are.  disp_init
+ * (use_disp_input): an RGBA_U16 id map of at most the image size, scale = width / its
+ * width, positions outside it read as Unknown (MatcherUseDisparity). */
+
+/* BlockMatcherNCC::_SParameter (BlockMatcherNCC.h:83-110). */
+typedef struct stereo_ncc_params {
+    int32_t patch;            /* EConfig: 11 (Patch_11x11), 9 or 5 */
+    int32_t disp_min;         /* _SDisparityConfig: disparities disp_min <= d < disp_max (<= 4096) */
+    int32_t disp_max;
+    int32_t disp_offset;      /*   and the pixel offset of MapPixelPos */
+    int32_t min_delta_thresh; /* iMinDeltaThresh: both byte drops around the maximum at least this */
+    float ncc_thresh;         /* fNccThresh: byte maximum / 255 at least this, else NotFound */
+    float grad_thresh;        /* fGradThresh: mean |L(x) - L(x + 1)| below it is CannotEvaluate */
+    int32_t use_disp_input;   /* iUseDispInput: MatcherUseDisparity around an initial map */
+    int32_t left_to_right;    /* bLeftToRight */
+    int32_t reserved[4];      /* must be 0 */
+} stereo_ncc_params;
+
+/* CDriver::Configure (BlockMatcherNCC.cu:1270-1318): fmt is the LUM_U8 pair format, at least
+ * 48 wide and 15 + patch tall. */
+stereo_status stereo_ncc_configure(stereo_context* ctx, stereo_driver** drv, const stereo_image_format* fmt,
+                                   const stereo_ncc_params* params);
+/* CDriver::Process (BlockMatcherNCC.cu:1393-1420; Matcher :614-884, MatcherUseDisparity
+ * :894-1255).  disp channels: EDisparityId (First + d, NotFound, NotSpecific, Saturated,
+ * CannotEvaluate, or Unknown where no byte score exceeds 0), the byte NCC maximum
+ * min(255, 255 max(0, ncc)) of the reference's float formula, and the largest byte drops to
+ * the two disparities before and after the maximum. */
+stereo_status stereo_ncc_process(stereo_driver* drv, stereo_image* disp, const stereo_image* left,
+                                 const stereo_image* right, const stereo_image* disp_init, stereo_stream stream);
+
+/* BlockMatcherAW1::_SParameter (BlockMatcherAW1.h:84-124). */
+typedef struct stereo_aw1_params {
+    int32_t patch;           /* EConfig: 15 (Patch_15x15, sub-patches of 5) or 9 (sub-patches of 3) */
+    int32_t sim_type;        /* ESimType: 0 AbsoluteDifference, 1 SignedProduct */
+    int32_t disp_min;        /* _SDisparityConfig, as stereo_ncc_params */
+    int32_t disp_max;
+    int32_t disp_offset;
+    float min_delta_thresh;  /* fMinDeltaThresh, in % */
+    float sim_thresh;        /* fSimThresh, in % */
+    float grad_thresh;       /* fGradThresh: the window's standard deviation, in % of 255 */
+    int32_t use_disp_input;  /* iUseDispInput */
+    int32_t left_to_right;   /* bLeftToRight */
+    int32_t reserved[4];     /* must be 0 */
+} stereo_aw1_params;
+
+/* CDriver::Configure (BlockMatcherAW1.cu:1270-1300): fmt is the LUM_U8 pair format, at least
+ * 48 wide and 20 (patch 15) or 18 (patch 9) tall. */
+stereo_status stereo_aw1_configure(stereo_context* ctx, stereo_driver** drv, const stereo_image_format* fmt,
+                                   const stereo_aw1_params* params);
+/* CDriver::Process (BlockMatcherAW1.cu:1380-1420; Matcher :657-913, MatcherUseDisparity
+ * :923-1263).  disp channels: EDisparityId as for NCC, then the similarity maximum and the two
+ * neighbour deltas as 16-bit fractions of [-100 %, 100 %] (SimFloatToInteger :172-176). */
+stereo_status stereo_aw1_process(stereo_driver* drv, stereo_image* disp, const stereo_image* left,
+                                 const stereo_image* right, const stereo_image* disp_init, stereo_stream stream);
+
 /* ---- rectification (RectifyImage.h:40-90) ------------------------------------------- */
 
 /* The camera model of the rectification: the rectified camera's pixel becomes a ray,

@@ -86,6 +86,19 @@ class Bm1Params(C.Structure):
                 ("reserved", C.c_int32 * 4)]
 
 
+class NccParams(C.Structure):
+    _fields_ = [("patch", C.c_int32), ("disp_min", C.c_int32), ("disp_max", C.c_int32), ("disp_offset", C.c_int32),
+                ("min_delta_thresh", C.c_int32), ("ncc_thresh", C.c_float), ("grad_thresh", C.c_float),
+                ("use_disp_input", C.c_int32), ("left_to_right", C.c_int32), ("reserved", C.c_int32 * 4)]
+
+
+class Aw1Params(C.Structure):
+    _fields_ = [("patch", C.c_int32), ("sim_type", C.c_int32), ("disp_min", C.c_int32), ("disp_max", C.c_int32),
+                ("disp_offset", C.c_int32), ("min_delta_thresh", C.c_float), ("sim_thresh", C.c_float),
+                ("grad_thresh", C.c_float), ("use_disp_input", C.c_int32), ("left_to_right", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
+
+
 class RectifyParams(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("src_fx", "src_fy", "src_cx", "src_cy", "k1", "k2", "p1", "p2", "k3",
                                          "rect_fx", "rect_fy", "rect_cx", "rect_cy")] + \
@@ -164,6 +177,10 @@ _PROTOS = [
     ("stereo_color_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(ColorParams)]),
     ("stereo_color_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.c_int32, _P]),
     ("stereo_bm1_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
+    ("stereo_ncc_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(NccParams)]),
+    ("stereo_ncc_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
+    ("stereo_aw1_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(Aw1Params)]),
+    ("stereo_aw1_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_denoise_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_scores", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_stage_times", C.c_int, [_P, C.POINTER(C.c_double), C.c_int32]),
@@ -522,6 +539,54 @@ class BlockMatcher1(_Driver):
         di = DeviceImage.from_array(self.ctx, init, PIXEL_RGBA_U16) if init is not None else None
         self.process(out, dl, dr, di)
         return out.download()
+
+
+class _RefMatcher(_Driver):
+    def run(self, L, R, init=None, fill=0):
+        """Host images in, the (H, W, 4) map out; pixels the matcher leaves are `fill`."""
+        dl = DeviceImage.from_array(self.ctx, L, PIXEL_LUM_U8)
+        dr = DeviceImage.from_array(self.ctx, R, PIXEL_LUM_U8)
+        out = DeviceImage.from_array(self.ctx, np.full((self.H, self.W, 4), fill, np.uint16), PIXEL_RGBA_U16)
+        di = DeviceImage.from_array(self.ctx, init, PIXEL_RGBA_U16) if init is not None else None
+        self.process(out, dl, dr, di)
+        return out.download()
+
+
+class BlockMatcherNCC(_RefMatcher):
+    """The reference's BlockMatcherNCC operator (stereo_ncc_*): block-tiled, one direction,
+    byte scores of the reference's float formula; an (h, w, 4) uint16 map of EDisparityId,
+    the byte maximum and the two byte drops around it."""
+
+    def __init__(self, ctx, W, H, patch=11, disp_min=0, disp_max=64, disp_offset=0, min_delta_thresh=0,
+                 ncc_thresh=0.0, grad_thresh=0.0, use_disp_input=False, left_to_right=True):
+        super().__init__(ctx)
+        self.W, self.H = W, H
+        p = NccParams(patch, disp_min, disp_max, disp_offset, int(min_delta_thresh), ncc_thresh, grad_thresh,
+                      1 if use_disp_input else 0, 1 if left_to_right else 0)
+        fmt = ImageFormat(W, H, PIXEL_LUM_U8)
+        _check(lib.stereo_ncc_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(p)), ctx)
+
+    def process(self, disp, left, right, init=None, stream=None):
+        _check(lib.stereo_ncc_process(self.handle, disp.ref, left.ref, right.ref, init.ref if init else None, stream),
+               self.ctx)
+
+
+class BlockMatcherAW1(_RefMatcher):
+    """The reference's BlockMatcherAW1 operator (stereo_aw1_*): block-tiled, one direction;
+    an (h, w, 4) uint16 map of EDisparityId and three similarity channels."""
+
+    def __init__(self, ctx, W, H, patch=15, sim_type=0, disp_min=0, disp_max=64, disp_offset=0,
+                 min_delta_thresh=0.0, sim_thresh=0.0, grad_thresh=0.0, use_disp_input=False, left_to_right=True):
+        super().__init__(ctx)
+        self.W, self.H = W, H
+        p = Aw1Params(patch, sim_type, disp_min, disp_max, disp_offset, min_delta_thresh, sim_thresh, grad_thresh,
+                      1 if use_disp_input else 0, 1 if left_to_right else 0)
+        fmt = ImageFormat(W, H, PIXEL_LUM_U8)
+        _check(lib.stereo_aw1_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(p)), ctx)
+
+    def process(self, disp, left, right, init=None, stream=None):
+        _check(lib.stereo_aw1_process(self.handle, disp.ref, left.ref, right.ref, init.ref if init else None, stream),
+               self.ctx)
 
 
 class Rectify(_Driver):

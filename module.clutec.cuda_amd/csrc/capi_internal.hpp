@@ -137,7 +137,7 @@ struct DriverExt {
 };
 
 struct stereo_driver {
-    enum Kind { CENSUS, MATCHER, CROSSCHECK, BATCH, LEARNED, DENOISE, BM1, RECTIFY, COLOR } kind;
+    enum Kind { CENSUS, MATCHER, CROSSCHECK, BATCH, LEARNED, DENOISE, BM1, RECTIFY, COLOR, REFMATCH } kind;
     stereo_context* ctx = nullptr;
     const char* name = "";
     stereo_image_format fmt{};

@@ -1,0 +1,888 @@
+// The reference's BlockMatcherNCC and BlockMatcherAW1 operators, with their block-tiled
+// results: the same disparities, test patches, decisions and four output channels as
+// the reference's 32-thread blocks (CluTec.Cuda.MultiView/BlockMatcherNCC.cu:614-1255,
+// BlockMatcherAW1.cu:657-1263).
+//
+// Both operators launch one 32-thread block per tile of result origins, full blocks only
+// (EvalThreadConfig_ReadDepBlockSize, KernelDriverBase.cpp:166-197).  A block decides
+// which disparities its results see: groups of 16 from the range start while the group
+// start is below min(Max, MaxDisparity(X0)); a group whose test area lies inside the
+// right image evaluates 16; otherwise, when the block's rows fit, a narrower group at the
+// reference's placement (:789-876).  Each result follows the maximum of its similarity
+// with the values around it, and is written with a code and three channels.
+//   NCC: tile (33 - P) / 2 * 2 x 16 from a 33 x (15 + P) read area.  The similarity is
+//        the reference's float formula over exact integer window sums -- (SIJ / N -
+//        mean_I mean_J) / (sd_I sd_J), 0 where sd_I sd_J < 1 -- clamped at 0 and
+//        truncated to a byte (EvalSim :180-336, EvalMsd :338-373); the gate is the mean
+//        |L(x) - L(x + 1)| (EvalGrad :388-410).
+//   AW1: tile 18 x 6 (sub-patch 5) or 24 x 10 (sub-patch 3) from a 32 x 20 / 32 x 18 read
+//        area; the similarity is the centre sub-patch plus the two largest edge
+//        neighbours of 255 - |a - b| or the signed product (EvalSimAW :246-316); the gate is
+//        the window's standard deviation, truncated to an int, in % (EvalGrad :331-417).
+// Floating point is evaluated as written: contraction off, correctly rounded division and
+// square root, so the byte scores and 16-bit channels equal the oracle's bit for bit.
+//
+// Layout on gfx950: one wave per workgroup.  NCC: a wave takes two vertically adjacent
+// reference blocks of one block column (they share their groups), and lane 32 b + t plays
+// the reference thread t of block b, holding the same CG results.  Per group of 16
+// disparities the right rows sit in LDS with the right windows' mean and deviation; per
+// disparity each lane forms one column's running 16-row sums of L R into LDS, then
+// its results' running row sums and the score.  AW1: a wave takes one reference block;
+// per disparity the column sums, then the sub-patch sums go through LDS.  The band search
+// around an initial map (MatcherUseDisparity) runs one block per wave.
+#include <hip/hip_runtime.h>
+
+#include "capi_internal.hpp"
+
+namespace stereo {
+namespace {
+
+constexpr int R_SDC = 16;
+constexpr int R_FIRST = 1, R_LAST = 0xFFEF, R_CANNOT = 0xFFFF, R_SATURATED = 0xFFFE, R_NOT_FOUND = 0xFFFD,
+              R_NOT_SPECIFIC = 0xFFFC;
+
+struct RefArgs {
+    View L, R, out, init;  // init.ptr == nullptr: the plain matcher
+    int dmin, dmax, offset, ltor;
+    int imin_delta;  // NCC iMinDeltaThresh
+    float min_delta, sim_thresh, grad_thresh;
+    int nby;  // reference block rows
+};
+
+// A group of disparities: labels D0 .. D0 + n - 1; the test origin of result column 0
+// for label D0 + r is base - r (left to right) or base + r; the right columns used are
+// [lo, lo + n + TBX - 1).  X-only: a block whose rows do not fit evaluates nothing.
+struct RefGroup {
+    int n, base, lo;
+};
+
+template <int TBX>
+__device__ __forceinline__ RefGroup ref_group(int X0, int D0, int W, int offset, int ltor, int cap)
+{
+    RefGroup g;
+    const int tx = (ltor ? X0 - D0 - offset : X0 + D0 + offset) - ltor * (R_SDC - 1);
+    if (tx >= 0 && tx + TBX + R_SDC - 1 <= W) {
+        g.n = cap;
+        g.base = tx + ltor * (R_SDC - 1);
+    } else {
+        int dx, sx;
+        if (tx >= 0) {
+            dx = W - tx - TBX;
+            sx = tx;
+        } else {
+            dx = tx + (R_SDC - 1);
+            sx = 0;
+        }
+        g.n = dx >= 0 ? dx + 1 : 0;
+        g.base = sx + ltor * (g.n - 1);
+    }
+    g.lo = ltor ? g.base - (g.n - 1) : g.base;
+    return g;
+}
+
+// The reference thread's initial value of one written result in the band search
+// (:960-1001): the initial id at the result's centre over the scale (outside the map:
+// Unknown), scaled in 16 bits when it is an id, as it is when it is a code.
+__device__ __forceinline__ int ref_init_value(const View& init, int scale, int cx, int cy)
+{
+    const int px = cx / scale, py = cy / scale;
+    const int id = (px < init.w && py < init.h) ? row_ptr<const ushort4>(init, py)[px].x : 0;
+    return id <= R_LAST ? (scale * id) & 0xFFFF : id;
+}
+
+// MinDisp / MaxDisp over the 32 lanes of a half-wave (:418-484): each lane's plain min / max
+// from (Last, First) counts only when valid; lo > hi or none valid: no range.
+__device__ __forceinline__ bool ref_block_range(int tlo, int thi, int& lo, int& hi)
+{
+    int l = (tlo >= R_FIRST && tlo <= R_LAST) ? tlo : 0x7FFFFFFF;
+    int h = (thi >= R_FIRST && thi <= R_LAST) ? thi : -1;
+#pragma unroll
+    for (int m = 16; m >= 1; m >>= 1) {
+        l = min(l, __shfl_xor(l, m, 32));
+        h = max(h, __shfl_xor(h, m, 32));
+    }
+    lo = l;
+    hi = h;
+    return l != 0x7FFFFFFF && h >= 0 && l <= h;
+}
+
+struct Tracker {
+    int disp, mx, prev, prev2, nxt, nxt2, last;
+};
+
+// The peak update (NCC EvalSim :280-333, AW1 EvalSimAW :262-315), branch-free.
+__device__ __forceinline__ void track(Tracker& s, int v, int d, bool upd)
+{
+    const bool act = upd && s.disp <= R_LAST;
+    const bool above = act && v > s.mx;
+    const bool b1 = act && !above && s.disp + 1 == d + R_FIRST;
+    const bool b2 = act && !above && s.disp + 2 == d + R_FIRST;
+    s.prev2 = above ? s.prev : s.prev2;
+    s.prev = above ? s.last : s.prev;
+    s.mx = above ? v : s.mx;
+    s.disp = above ? d + R_FIRST : s.disp;
+    s.nxt = (above || b1) ? v : s.nxt;
+    s.nxt2 = (above || b2) ? v : s.nxt2;
+    s.last = act ? v : s.last;
+}
+
+// ------------------------------------------------------------------------------------
+// NCC
+// ------------------------------------------------------------------------------------
+
+template <int P>
+struct NccShape {
+    static constexpr int CG = (32 - P + 1) / 2;
+    static constexpr int PCX = 2 * CG;
+    static constexpr int TBX = 33;
+    static constexpr int TBY = 16 + P - 1;
+    static constexpr int TW = TBX + R_SDC - 1;     // 48 test columns
+    static constexpr int NC = R_SDC + PCX - 1;     // right window starts a group can use
+};
+
+// NB reference blocks per wave: 2 for the matcher, 1 for the band search.
+template <int P, int NB>
+__global__ __launch_bounds__(64) void ncc_ref_kernel(RefArgs a)
+{
+#pragma clang fp contract(off)
+    using S = NccShape<P>;
+    constexpr int CG = S::CG, PCX = S::PCX, TW = S::TW, NC = S::NC;
+    constexpr int ROWS = 16 * NB, DR = ROWS + P - 1;
+    constexpr float fN = (float)(P * P);
+
+    __shared__ uint8_t sl[DR][S::TBX];
+    __shared__ uint8_t sr[DR][TW];
+    __shared__ float tmean[ROWS][NC + 1], tsd[ROWS][NC + 1];
+    // column sums: of L R per disparity, of the right windows' statistics per group, and
+    // (at the start) of the left windows and gradients in columns 32 .. 63
+    __shared__ int scol[2][ROWS][64];
+
+    const int l = threadIdx.x;
+    const int W = a.L.w, H = a.L.h;
+    const int X0 = blockIdx.x * PCX;
+    const int b = l >> 5, t = l & 31;  // reference block and thread
+    const int by = blockIdx.y * NB + b;
+    const int Y0 = by * 16;
+    const int Y0s = blockIdx.y * NB * 16;
+    const bool lane_ok = b < NB;
+    const bool block_ok = lane_ok && by < a.nby && X0 + P <= W && Y0 + P <= H;
+    const bool rows_fit = Y0 + S::TBY <= H;
+    // this lane's results: row iy, columns ix0 .. ix0 + CG - 1 (AlgoArraySum_W32H16)
+    const int iy = t & 15, ix0 = (t >> 4) * CG;
+    const int row = 16 * b + iy;  // in the strip
+
+    for (int i = l; i < DR * S::TBX; i += 64) {
+        const int r = i / S::TBX, c = i % S::TBX;
+        const int x = X0 + c, y = Y0s + r;
+        sl[r][c] = (x < W && y < H) ? row_ptr<const uint8_t>(a.L, y)[x] : 0;
+    }
+    __syncthreads();
+
+    bool written[CG];
+    Tracker tr[CG];
+    float bmean[CG], bsd[CG];
+#pragma unroll
+    for (int k = 0; k < CG; ++k) {
+        written[k] = block_ok && X0 + ix0 + k + P < W && Y0 + iy + P < H;
+        tr[k] = Tracker{0, 0, 0, 0, 0, 0, 0};
+    }
+
+    // left window sums (EvalMsd) and, for the matcher, the gradient sums (EvalGrad)
+    {
+        const int c = l & 31, r0 = 16 * (lane_ok ? b : 0);
+        int s1 = 0, s2 = 0, sg = 0;
+        for (int j = 0; j < P; ++j) {
+            const int v = sl[r0 + j][c], w = sl[r0 + j][c + 1];
+            s1 += v;
+            s2 += v * v;
+            sg += abs(v - w);
+        }
+        if (lane_ok) {
+            scol[0][r0][c] = s1;
+            scol[1][r0][c] = s2;
+            scol[0][r0][c + 32] = sg;
+        }
+        for (int j = 1; j < 16; ++j) {
+            const int vo = sl[r0 + j - 1][c], wo = sl[r0 + j - 1][c + 1];
+            const int vn = sl[r0 + j + P - 1][c], wn = sl[r0 + j + P - 1][c + 1];
+            s1 += vn - vo;
+            s2 += vn * vn - vo * vo;
+            sg += abs(vn - wn) - abs(vo - wo);
+            if (lane_ok) {
+                scol[0][r0 + j][c] = s1;
+                scol[1][r0 + j][c] = s2;
+                scol[0][r0 + j][c + 32] = sg;
+            }
+        }
+    }
+    __syncthreads();
+    bool cannot[CG];
+    {
+        int s1 = 0, s2 = 0, sg = 0;
+        const int rr = lane_ok ? row : iy;
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            s1 += scol[0][rr][ix0 + i];
+            s2 += scol[1][rr][ix0 + i];
+            sg += scol[0][rr][ix0 + i + 32];
+        }
+#pragma unroll
+        for (int k = 0; k < CG; ++k) {
+            if (k > 0) {
+                s1 += scol[0][rr][ix0 + k + P - 1] - scol[0][rr][ix0 + k - 1];
+                s2 += scol[1][rr][ix0 + k + P - 1] - scol[1][rr][ix0 + k - 1];
+                sg += scol[0][rr][ix0 + k + P - 1 + 32] - scol[0][rr][ix0 + k - 1 + 32];
+            }
+            const float m = (float)s1 / fN, m2 = (float)s2 / fN;
+            const float sq = m * m;
+            bmean[k] = m;
+            bsd[k] = sqrtf(m2 - sq);
+            cannot[k] = (float)sg / fN < a.grad_thresh;
+        }
+    }
+
+    const bool band = NB == 1 && a.init.ptr != nullptr;
+    int d_start, d_stop, cap;
+    bool eval_block;  // the block evaluates disparities
+    if (!band) {
+        // the early-out of a block whose every result is CannotEvaluate (:741-775)
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < CG; ++k) {
+            tr[k].disp = cannot[k] ? R_CANNOT : 0;
+            any = any || !cannot[k];
+        }
+        const uint64_t bal = __ballot(any);
+        eval_block = ((bal >> (32 * b)) & 0xFFFFFFFFull) != 0;
+        const int maxd = a.ltor ? X0 - a.offset + 1 : W - X0 - a.offset;  // MaxDisparity
+        d_start = a.dmin;
+        d_stop = a.dmax < maxd ? a.dmax : maxd;
+        cap = R_SDC;
+    } else {
+        const int scale = W / a.init.w;
+        int tlo = R_LAST, thi = R_FIRST;
+#pragma unroll
+        for (int k = 0; k < CG; ++k) {
+            if (!written[k]) continue;
+            const int v = ref_init_value(a.init, scale, X0 + ix0 + k + P / 2, Y0 + iy + P / 2);
+            tr[k].disp = v;
+            if (v <= R_LAST) {
+                tlo = min(tlo, v);
+                thi = max(thi, v);
+            }
+        }
+        int lo, hi;
+        if (!ref_block_range(tlo, thi, lo, hi)) {
+            // no valid range: every written result keeps its initial value (:1003-1027)
+#pragma unroll
+            for (int k = 0; k < CG; ++k)
+                if (written[k])
+                    row_ptr<ushort4>(a.out, Y0 + iy + P / 2)[X0 + ix0 + k + P / 2] =
+                        make_ushort4((uint16_t)tr[k].disp, 0, 0, 0);
+            return;
+        }
+        d_start = lo < 4 ? 0 : lo - 4;
+        const int d_end = hi + 4;  // inclusive
+        eval_block = d_end <= 32767;  // the reference's 16-bit loop variables wrap beyond
+        d_stop = d_end + 1;
+        cap = min(R_SDC, d_end - d_start + 1);
+    }
+    const bool upd = block_ok && rows_fit && eval_block;
+
+    // this lane's column for the per-disparity sums: the left pixels of its 16 rows
+    uint32_t lreg[16 + P - 1];
+    {
+        const int c = l & 31, r0 = 16 * (lane_ok ? b : 0);
+#pragma unroll
+        for (int j = 0; j < 16 + P - 1; ++j) lreg[j] = sl[r0 + j][c];
+    }
+
+    const bool wave_evaluates = __ballot(upd) != 0;
+    for (int D0 = d_start; wave_evaluates && D0 < d_stop; D0 += R_SDC) {
+        const RefGroup g = ref_group<S::TBX>(X0, D0, W, a.offset, a.ltor, cap);
+        if (g.n <= 0) continue;
+        __syncthreads();  // the previous group's readers of sr, scol and the statistics are done
+        for (int i = l; i < DR * TW; i += 64) {
+            const int r = i / TW, c = i % TW;
+            const int x = g.lo + c, y = Y0s + r;
+            sr[r][c] = (c < g.n + S::TBX - 1 && x >= 0 && x < W && y < H) ? row_ptr<const uint8_t>(a.R, y)[x] : 0;
+        }
+        __syncthreads();
+        // the right windows' statistics at every start column the group uses
+        for (int i = l; i < NB * TW; i += 64) {
+            const int c = i % TW, r0 = 16 * (i / TW);
+            int s1 = 0, s2 = 0;
+            for (int j = 0; j < P; ++j) {
+                const int v = sr[r0 + j][c];
+                s1 += v;
+                s2 += v * v;
+            }
+            scol[0][r0][c] = s1;
+            scol[1][r0][c] = s2;
+            for (int j = 1; j < 16; ++j) {
+                const int vo = sr[r0 + j - 1][c], vn = sr[r0 + j + P - 1][c];
+                s1 += vn - vo;
+                s2 += vn * vn - vo * vo;
+                scol[0][r0 + j][c] = s1;
+                scol[1][r0 + j][c] = s2;
+            }
+        }
+        __syncthreads();
+        for (int i = l; i < ROWS * 2; i += 64) {
+            const int rr = i >> 1, c0 = (i & 1) * ((NC + 1) / 2);
+            const int c1 = min(NC, c0 + (NC + 1) / 2);
+            int s1 = 0, s2 = 0;
+            for (int k = 0; k < P; ++k) {
+                s1 += scol[0][rr][c0 + k];
+                s2 += scol[1][rr][c0 + k];
+            }
+            for (int c = c0; c < c1; ++c) {
+                if (c > c0) {
+                    s1 += scol[0][rr][c + P - 1] - scol[0][rr][c - 1];
+                    s2 += scol[1][rr][c + P - 1] - scol[1][rr][c - 1];
+                }
+                const float m = (float)s1 / fN, m2 = (float)s2 / fN;
+                const float sq = m * m;
+                tmean[rr][c] = m;
+                tsd[rr][c] = sqrtf(m2 - sq);
+            }
+        }
+        for (int r = 0; r < g.n; ++r) {
+            const int d = D0 + r;
+            const int sh = (a.ltor ? g.base - r : g.base + r) - g.lo;  // test origin column in sr
+            __syncthreads();  // the previous disparity's row sums have read scol
+            // column sums of L R over this lane's 16 rows
+            {
+                const int c = l & 31, r0 = 16 * (lane_ok ? b : 0);
+                int q[16 + P - 1];
+#pragma unroll
+                for (int j = 0; j < 16 + P - 1; ++j) q[j] = (int)(lreg[j] * (uint32_t)sr[r0 + j][c + sh]);
+                int s = 0;
+#pragma unroll
+                for (int j = 0; j < P; ++j) s += q[j];
+                if (lane_ok) scol[0][r0][c] = s;
+#pragma unroll
+                for (int j = 1; j < 16; ++j) {
+                    s += q[j + P - 1] - q[j - 1];
+                    if (lane_ok) scol[0][r0 + j][c] = s;
+                }
+            }
+            __syncthreads();
+            // row sums and the score of each result (EvalSim's result lambda)
+            {
+                const int rr = lane_ok ? row : iy;
+                const int(&cr)[64] = scol[0][rr];
+                int s3 = 0;
+#pragma unroll
+                for (int i = 0; i < P; ++i) s3 += cr[ix0 + i];
+#pragma unroll
+                for (int k = 0; k < CG; ++k) {
+                    if (k > 0) s3 += cr[ix0 + k + P - 1] - cr[ix0 + k - 1];
+                    const float cross = (float)s3 / fN;
+                    const float tm = tmean[rr][sh + ix0 + k], ts = tsd[rr][sh + ix0 + k];
+                    const float norm = bsd[k] * ts;
+                    float ncc;
+                    if (norm < 1.0f) {
+                        ncc = 0.0f;
+                    } else {
+                        const float mm = bmean[k] * tm;
+                        ncc = (cross - mm) / norm;
+                    }
+                    ncc = fmaxf(0.0f, ncc);
+                    const int u = (int)fminf(255.0f, ncc * 255.0f);
+                    track(tr[k], u, d, upd);
+                }
+            }
+        }
+    }
+
+    // WriteDisparity (:491-592)
+#pragma unroll
+    for (int k = 0; k < CG; ++k) {
+        if (!written[k]) continue;
+        const Tracker& s = tr[k];
+        ushort4 o;
+        const int dv = s.disp;
+        if (dv > R_LAST || dv == 0) {
+            o = make_ushort4((uint16_t)dv, 0, 0, 0);
+        } else {
+            const float fmax = (float)s.mx / 255.0f;
+            const int diff1 = max(abs(s.prev - s.mx), abs(s.prev2 - s.mx));
+            const int diff2 = max(abs(s.nxt - s.mx), abs(s.nxt2 - s.mx));
+            int code;
+            if (fmax >= a.sim_thresh) {
+                const bool sat = dv <= R_FIRST + a.dmin + 1 || dv >= R_FIRST + a.dmax - 1;
+                code = sat ? R_SATURATED : ((diff1 >= a.imin_delta && diff2 >= a.imin_delta) ? dv : R_NOT_SPECIFIC);
+            } else {
+                code = R_NOT_FOUND;
+            }
+            o = make_ushort4((uint16_t)code, (uint16_t)s.mx, (uint16_t)diff1, (uint16_t)diff2);
+        }
+        row_ptr<ushort4>(a.out, Y0 + iy + P / 2)[X0 + ix0 + k + P / 2] = o;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// AW1
+// ------------------------------------------------------------------------------------
+
+template <int SP>
+struct AwShape {
+    static constexpr int CG = (32 - SP + 1) / 2;
+    static constexpr int SPCX = 2 * CG;              // sub-patch columns
+    static constexpr int P3 = 3 * SP;                // window
+    static constexpr int PCX = SPCX - 2 * SP;        // result columns
+    static constexpr int PCY = 16 - 2 * SP;          // result rows
+    static constexpr int TBX = 32;
+    static constexpr int TBY = PCY + P3 - 1;
+    static constexpr int TW = TBX + R_SDC - 1;       // 47 test columns
+    static constexpr int NR = PCX * PCY;
+    static constexpr int RPL = (NR + 63) / 64;       // results per lane
+    static constexpr int NE = 3 * SP * SP;           // EffectivePatchElementCount
+};
+
+// SimValueToFloat (:155-159) and SimFloatToInteger (:172-176)
+__device__ __forceinline__ float aw1_sim_float(int v, int ne)
+{
+#pragma clang fp contract(off)
+    return ((float)v / (float)ne) * (100.0f / 255.0f);
+}
+
+__device__ __forceinline__ uint32_t aw1_sim_u16(float f)
+{
+#pragma clang fp contract(off)
+    float t = (f / 100.0f + 1.0f) / 2.0f;
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    return (uint32_t)(t * 65535.0f);
+}
+
+template <int SP, int SIM>
+__global__ __launch_bounds__(64) void aw1_ref_kernel(RefArgs a)
+{
+#pragma clang fp contract(off)
+    using S = AwShape<SP>;
+    constexpr int PCX = S::PCX, PCY = S::PCY, TBY = S::TBY, TW = S::TW, NR = S::NR, RPL = S::RPL;
+
+    __shared__ uint8_t sl[TBY][32];
+    __shared__ uint8_t sr[TBY][TW + 1];
+    __shared__ int scol[16][33];
+    __shared__ int ssp[16][33];
+    __shared__ float stab[256];
+
+    const int l = threadIdx.x;
+    const int W = a.L.w, H = a.L.h;
+    const int X0 = blockIdx.x * PCX, Y0 = blockIdx.y * PCY;
+    const bool block_ok = (int)blockIdx.y < a.nby && X0 + S::P3 <= W && Y0 + S::P3 <= H;
+    if (!block_ok) return;  // uniform over the wave
+    const bool rows_fit = Y0 + TBY <= H;
+
+    if (SIM == 1)
+        for (int i = l; i < 256; i += 64) stab[i] = 2.0f * ((float)i / 255.0f - 0.5f);
+    for (int i = l; i < TBY * 32; i += 64) {
+        const int r = i / 32, c = i % 32;
+        const int x = X0 + c, y = Y0 + r;
+        sl[r][c] = (x < W && y < H) ? row_ptr<const uint8_t>(a.L, y)[x] : 0;
+    }
+    __syncthreads();
+
+    // this lane's results: row-major positions l + 64 q of the block's PCX x PCY
+    int rx[RPL], ry[RPL];
+    bool has[RPL], written[RPL];
+    Tracker tr[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+        const int pos = l + 64 * q;
+        has[q] = pos < NR;
+        rx[q] = has[q] ? pos % PCX : 0;
+        ry[q] = has[q] ? pos / PCX : 0;
+        written[q] = has[q] && X0 + rx[q] + S::P3 < W && Y0 + ry[q] + S::P3 < H;
+        tr[q] = Tracker{0, -32767, 0, 0, 0, 0, 0};
+    }
+
+    const bool band = a.init.ptr != nullptr;
+    int d_start, d_stop, cap;
+    bool eval_block;
+    if (!band) {
+        // EvalGrad: the P3 x P3 window's standard deviation, truncated, in % of 255
+        if (l < 32) {
+            int s1 = 0, s2 = 0;
+            for (int j = 0; j < S::P3; ++j) {
+                const int v = sl[j][l];
+                s1 += v;
+                s2 += v * v;
+            }
+            scol[0][l] = s1;
+            ssp[0][l] = s2;
+            for (int j = 1; j < PCY; ++j) {
+                const int vo = sl[j - 1][l], vn = sl[j + S::P3 - 1][l];
+                s1 += vn - vo;
+                s2 += vn * vn - vo * vo;
+                scol[j][l] = s1;
+                ssp[j][l] = s2;
+            }
+        }
+        __syncthreads();
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            if (!has[q]) continue;
+            int s1 = 0, s2 = 0;
+            for (int i = 0; i < S::P3; ++i) {
+                s1 += scol[ry[q]][rx[q] + i];
+                s2 += ssp[ry[q]][rx[q] + i];
+            }
+            const float m = (float)s1 / (float)(S::P3 * S::P3), m2 = (float)s2 / (float)(S::P3 * S::P3);
+            const float sq = m * m;
+            const int isd = (int)sqrtf(fabsf(m2 - sq));
+            const float sd = (float)isd / 255.0f * 100.0f;
+            if (sd < a.grad_thresh) tr[q].disp = R_CANNOT;
+            else any = true;
+        }
+        eval_block = __ballot(any) != 0;
+        const int maxd = a.ltor ? X0 - a.offset + 1 : W - X0 - a.offset;
+        d_start = a.dmin;
+        d_stop = a.dmax < maxd ? a.dmax : maxd;
+        cap = R_SDC;
+    } else {
+        // reference thread t holds the positions 32 r + t: lanes t and t + 32 together
+        const int scale = W / a.init.w;
+        int tlo = R_LAST, thi = R_FIRST;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            if (!written[q]) continue;
+            const int v = ref_init_value(a.init, scale, X0 + rx[q] + S::P3 / 2, Y0 + ry[q] + S::P3 / 2);
+            tr[q].disp = v;
+            if (v <= R_LAST) {
+                tlo = min(tlo, v);
+                thi = max(thi, v);
+            }
+        }
+        tlo = min(tlo, __shfl_xor(tlo, 32));
+        thi = max(thi, __shfl_xor(thi, 32));
+        int lo, hi;
+        if (!ref_block_range(tlo, thi, lo, hi)) {
+#pragma unroll
+            for (int q = 0; q < RPL; ++q)
+                if (written[q])
+                    row_ptr<ushort4>(a.out, Y0 + ry[q] + S::P3 / 2)[X0 + rx[q] + S::P3 / 2] =
+                        make_ushort4((uint16_t)tr[q].disp, 0, 0, 0);
+            return;
+        }
+        d_start = lo < 4 ? 0 : lo - 4;
+        const int d_end = hi + 4;
+        eval_block = d_end <= 32767;
+        d_stop = d_end + 1;
+        cap = min(R_SDC, d_end - d_start + 1);
+    }
+    const bool upd = rows_fit && eval_block;  // uniform
+
+    for (int D0 = d_start; upd && D0 < d_stop; D0 += R_SDC) {
+        const RefGroup g = ref_group<S::TBX>(X0, D0, W, a.offset, a.ltor, cap);
+        if (g.n <= 0) continue;
+        __syncthreads();
+        for (int i = l; i < TBY * TW; i += 64) {
+            const int r = i / TW, c = i % TW;
+            const int x = g.lo + c, y = Y0 + r;
+            sr[r][c] = (c < g.n + S::TBX - 1 && x >= 0 && x < W && y < H) ? row_ptr<const uint8_t>(a.R, y)[x] : 0;
+        }
+        for (int r = 0; r < g.n; ++r) {
+            const int d = D0 + r;
+            const int sh = (a.ltor ? g.base - r : g.base + r) - g.lo;
+            __syncthreads();
+            // column sums of SP rows for the 16 sub-patch top rows and 32 columns
+            {
+                const int c = l & 31, r0 = (l >> 5) * 8;
+                int q[8 + SP - 1];
+#pragma unroll
+                for (int j = 0; j < 8 + SP - 1; ++j) {
+                    const int u = sl[r0 + j][c], v = sr[r0 + j][c + sh];
+                    if (SIM == 0) {
+                        q[j] = 255 - abs(u - v);
+                    } else {
+                        const float f = stab[u] * stab[v];
+                        q[j] = (int)(f * 255.0f);
+                    }
+                }
+                int s = 0;
+#pragma unroll
+                for (int j = 0; j < SP; ++j) s += q[j];
+                scol[r0][c] = s;
+#pragma unroll
+                for (int j = 1; j < 8; ++j) {
+                    s += q[j + SP - 1] - q[j - 1];
+                    scol[r0 + j][c] = s;
+                }
+            }
+            __syncthreads();
+            // sub-patch sums: row l % 16, columns 8 (l / 16) .. + 8 of the 2 CG
+            {
+                const int rr = l & 15, c0 = (l >> 4) * 8;
+                int s = 0;
+#pragma unroll
+                for (int i = 0; i < SP; ++i) s += scol[rr][c0 + i];
+                ssp[rr][c0] = s;
+#pragma unroll
+                for (int i = 1; i < 8; ++i) {
+                    if (c0 + i + SP - 1 < 32) {
+                        s += scol[rr][c0 + i + SP - 1] - scol[rr][c0 + i - 1];
+                        ssp[rr][c0 + i] = s;
+                    }
+                }
+            }
+            __syncthreads();
+            // Two_OutOf_Four (Kernel.Algo.AdaptWnd_W32H16.h:240-285) and the peak update
+#pragma unroll
+            for (int q = 0; q < RPL; ++q) {
+                const int x = rx[q], y = ry[q];
+                int m1 = ssp[y][x + SP], m2 = -32767, v;
+                v = ssp[y + SP][x];
+                m2 = max(m2, min(m1, v));
+                m1 = max(m1, v);
+                v = ssp[y + 2 * SP][x + SP];
+                m2 = max(m2, min(m1, v));
+                m1 = max(m1, v);
+                v = ssp[y + SP][x + 2 * SP];
+                m2 = max(m2, min(m1, v));
+                m1 = max(m1, v);
+                const int aw = (int)(int16_t)(m1 + m2 + ssp[y + SP][x + SP]);
+                track(tr[q], aw, d, has[q]);
+            }
+        }
+    }
+
+    // WriteDisparity (:509-635)
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+        if (!written[q]) continue;
+        const Tracker& s = tr[q];
+        ushort4 o;
+        const int dv = s.disp;
+        if (dv > R_LAST || dv == 0) {
+            o = make_ushort4((uint16_t)dv, 0, 0, 0);
+        } else {
+            const float fmax = aw1_sim_float(s.mx, S::NE);
+            const float dp = fmaxf(fabsf(aw1_sim_float(s.prev, S::NE) - fmax), fabsf(aw1_sim_float(s.prev2, S::NE) - fmax));
+            const float dn = fmaxf(fabsf(aw1_sim_float(s.nxt, S::NE) - fmax), fabsf(aw1_sim_float(s.nxt2, S::NE) - fmax));
+            const bool specific = dp >= a.min_delta && dn >= a.min_delta;
+            int code;
+            if (fmax >= a.sim_thresh) {
+                const bool sat = dv <= R_FIRST + a.dmin + 1 || dv >= R_FIRST + a.dmax - 1;
+                code = sat ? R_SATURATED : (specific ? dv : R_NOT_SPECIFIC);
+            } else {
+                code = R_NOT_FOUND;
+            }
+            o = make_ushort4((uint16_t)code, (uint16_t)aw1_sim_u16(fmax), (uint16_t)aw1_sim_u16(dp),
+                             (uint16_t)aw1_sim_u16(dn));
+        }
+        row_ptr<ushort4>(a.out, Y0 + ry[q] + S::P3 / 2)[X0 + rx[q] + S::P3 / 2] = o;
+    }
+}
+
+struct RefExt : DriverExt {
+    int algorithm = 0;  // STEREO_ALGO_NCC or STEREO_ALGO_AW1
+    stereo_ncc_params ncc{};
+    stereo_aw1_params aw{};
+};
+
+template <int P>
+hipError_t ncc_ref_launch(const RefArgs& a, int nbx, hipStream_t s)
+{
+    if (a.init.ptr)
+        hipLaunchKernelGGL((ncc_ref_kernel<P, 1>), dim3(nbx, a.nby), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((ncc_ref_kernel<P, 2>), dim3(nbx, (a.nby + 1) / 2), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int SP>
+hipError_t aw1_ref_launch(const RefArgs& a, int sim, int nbx, hipStream_t s)
+{
+    if (sim == 0)
+        hipLaunchKernelGGL((aw1_ref_kernel<SP, 0>), dim3(nbx, a.nby), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((aw1_ref_kernel<SP, 1>), dim3(nbx, a.nby), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace stereo
+
+using namespace capi;
+
+namespace {
+
+stereo_status ref_common_checks(stereo_context* ctx, const stereo_image_format* fmt, int dmin, int dmax, int offset,
+                                const int32_t* reserved, int min_w, int min_h, const char* name)
+{
+    if (!format_valid(fmt)) return fail(ctx, STEREO_ERROR_INVALID_FORMAT, "Invalid image format");
+    if (fmt->pixel_type != STEREO_PIXEL_LUM_U8)
+        return fail(ctx, STEREO_ERROR_UNSUPPORTED_PIXEL_TYPE, "Pixel types of given images not supported");
+    for (int i = 0; i < 4; ++i)
+        if (reserved[i]) return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "reserved parameters must be 0");
+    if (dmin < 0 || dmax <= dmin || dmax > 4096 || offset < -4096 || offset > 4096)
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "disparity range [%d, %d) offset %d: need 0 <= min < max <= 4096",
+                    dmin, dmax, offset);
+    if (fmt->width < min_w || fmt->height < min_h || fmt->width > 16384 || fmt->height > 16384)
+        return fail(ctx, STEREO_ERROR_INVALID_FORMAT, "%s: image %dx%d, need %d <= width <= 16384 and %d <= height <= 16384",
+                    name, fmt->width, fmt->height, min_w, min_h);
+    return STEREO_OK;
+}
+
+stereo_status ref_make_driver(stereo_context* ctx, stereo_driver** out, const stereo_image_format* fmt,
+                              stereo::RefExt* x, const char* name)
+{
+    stereo_status st;
+    if ((st = set_device(ctx))) {
+        delete x;
+        return st;
+    }
+    stereo_driver* d = new stereo_driver(stereo_driver::REFMATCH);
+    d->ctx = ctx;
+    d->fmt = *fmt;
+    d->name = name;
+    d->ext = x;
+    if ((st = driver_events(d))) {
+        driver_release(d);
+        delete d;
+        return st;
+    }
+    *out = d;
+    return STEREO_OK;
+}
+
+stereo_status ref_check_io(stereo_driver* d, stereo_image* disp, const stereo_image* left, const stereo_image* right,
+                           const stereo_image* disp_init, int use_init, const char* name)
+{
+    stereo_context* ctx = d->ctx;
+    const int W = d->fmt.width, H = d->fmt.height;
+    stereo_status st;
+    if ((st = check_image(ctx, left, W, H, STEREO_PIXEL_LUM_U8, "left image"))) return st;
+    if ((st = check_image(ctx, right, W, H, STEREO_PIXEL_LUM_U8, "right image"))) return st;
+    if ((st = check_image(ctx, disp, W, H, STEREO_PIXEL_RGBA_U16, "disparity image"))) return st;
+    if (use_init) {
+        if (!disp_init || !disp_init->data)
+            return fail(ctx, STEREO_ERROR_INVALID_ARGUMENT, "%s: use_disp_input needs an initial disparity map", name);
+        const int iw = disp_init->format.width, ih = disp_init->format.height;
+        if (iw < 1 || ih < 1 || iw > W || ih > H)
+            return fail(ctx, STEREO_ERROR_FORMAT_MISMATCH, "initial disparity map %dx%d for a %dx%d image", iw, ih, W, H);
+        if ((st = check_image(ctx, disp_init, iw, ih, STEREO_PIXEL_RGBA_U16, "initial disparity map"))) return st;
+    } else if (disp_init) {
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "%s: configured without use_disp_input", name);
+    }
+    return STEREO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+stereo_status stereo_ncc_configure(stereo_context* ctx, stereo_driver** out, const stereo_image_format* fmt,
+                                   const stereo_ncc_params* p)
+{
+    if (!ctx || !out || !fmt || !p) return STEREO_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    const int P = p->patch;
+    if (P != 11 && P != 9 && P != 5) return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
+    stereo_status st = ref_common_checks(ctx, fmt, p->disp_min, p->disp_max, p->disp_offset, p->reserved, 48, 15 + P,
+                                         "BlockMatcherNCC");
+    if (st) return st;
+    stereo::RefExt* x = new stereo::RefExt();
+    x->algorithm = STEREO_ALGO_NCC;
+    x->ncc = *p;
+    return ref_make_driver(ctx, out, fmt, x, "Block Matcher NCC");
+}
+
+stereo_status stereo_aw1_configure(stereo_context* ctx, stereo_driver** out, const stereo_image_format* fmt,
+                                   const stereo_aw1_params* p)
+{
+    if (!ctx || !out || !fmt || !p) return STEREO_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (p->patch != 15 && p->patch != 9) return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "Invalid algorithm configuration.");
+    if (p->sim_type != 0 && p->sim_type != 1)
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "Given similarity type not supported.");
+    const int sp = p->patch / 3;
+    const int tby = (16 - 2 * sp) + 3 * sp - 1;
+    stereo_status st =
+        ref_common_checks(ctx, fmt, p->disp_min, p->disp_max, p->disp_offset, p->reserved, 48, tby, "BlockMatcherAW1");
+    if (st) return st;
+    stereo::RefExt* x = new stereo::RefExt();
+    x->algorithm = STEREO_ALGO_AW1;
+    x->aw = *p;
+    return ref_make_driver(ctx, out, fmt, x, "Block Matcher Adaptive Window 1");
+}
+
+static stereo_status ref_process(stereo_driver* d, stereo_image* disp, const stereo_image* left,
+                                 const stereo_image* right, const stereo_image* disp_init, stereo_stream stream,
+                                 int algorithm)
+{
+    if (!d || d->kind != stereo_driver::REFMATCH || !disp || !left || !right) return STEREO_ERROR_INVALID_ARGUMENT;
+    const stereo::RefExt* x = static_cast<const stereo::RefExt*>(d->ext);
+    if (x->algorithm != algorithm) return STEREO_ERROR_INVALID_ARGUMENT;
+    stereo_context* ctx = d->ctx;
+    const bool ncc = algorithm == STEREO_ALGO_NCC;
+    const int use_init = ncc ? x->ncc.use_disp_input : x->aw.use_disp_input;
+    stereo_status st;
+    if ((st = ref_check_io(d, disp, left, right, disp_init, use_init, d->name))) return st;
+    if ((st = set_device(ctx))) return st;
+    GUARD(ctx, ncc ? "stereo_ncc_process" : "stereo_aw1_process");
+    const hipStream_t s = (hipStream_t)stream;
+    if ((st = begin(d, s))) return st;
+    stereo::RefArgs a;
+    a.L = view_of(left);
+    a.R = view_of(right);
+    a.out = view_of(disp);
+    a.init = use_init ? view_of(disp_init) : null_view();
+    const int W = d->fmt.width, H = d->fmt.height;
+    hipError_t e;
+    if (ncc) {
+        const stereo_ncc_params& p = x->ncc;
+        a.dmin = p.disp_min;
+        a.dmax = p.disp_max;
+        a.offset = p.disp_offset;
+        a.ltor = p.left_to_right ? 1 : 0;
+        a.imin_delta = p.min_delta_thresh;
+        a.min_delta = 0.0f;
+        a.sim_thresh = p.ncc_thresh;
+        a.grad_thresh = p.grad_thresh;
+        // EvalThreadConfig_ReadDepBlockSize over a 33 x (15 + P) read area, full blocks only
+        const int P = p.patch, PCX = 2 * ((32 - P + 1) / 2);
+        const int nbx = (W - (33 - PCX)) / PCX;
+        a.nby = (H - (P - 1)) / 16;
+        switch (P) {
+        case 11: e = stereo::ncc_ref_launch<11>(a, nbx, s); break;
+        case 9: e = stereo::ncc_ref_launch<9>(a, nbx, s); break;
+        default: e = stereo::ncc_ref_launch<5>(a, nbx, s); break;
+        }
+    } else {
+        const stereo_aw1_params& p = x->aw;
+        a.dmin = p.disp_min;
+        a.dmax = p.disp_max;
+        a.offset = p.disp_offset;
+        a.ltor = p.left_to_right ? 1 : 0;
+        a.imin_delta = 0;
+        a.min_delta = p.min_delta_thresh;
+        a.sim_thresh = p.sim_thresh;
+        a.grad_thresh = p.grad_thresh;
+        const int sp = p.patch / 3, CG = (32 - sp + 1) / 2;
+        const int PCX = 2 * CG - 2 * sp, PCY = 16 - 2 * sp, TBY = PCY + 3 * sp - 1;
+        const int nbx = (W - (32 - PCX)) / PCX;
+        a.nby = (H - (TBY - PCY)) / PCY;
+        e = sp == 5 ? stereo::aw1_ref_launch<5>(a, p.sim_type, nbx, s) : stereo::aw1_ref_launch<3>(a, p.sim_type, nbx, s);
+    }
+    HIP_TRY(ctx, e, ncc ? "BlockMatcherNCC launch" : "BlockMatcherAW1 launch");
+    return end(d, s);
+}
+
+stereo_status stereo_ncc_process(stereo_driver* d, stereo_image* disp, const stereo_image* left,
+                                 const stereo_image* right, const stereo_image* disp_init, stereo_stream stream)
+{
+    return ref_process(d, disp, left, right, disp_init, stream, STEREO_ALGO_NCC);
+}
+
+stereo_status stereo_aw1_process(stereo_driver* d, stereo_image* disp, const stereo_image* left,
+                                 const stereo_image* right, const stereo_image* disp_init, stereo_stream stream)
+{
+    return ref_process(d, disp, left, right, disp_init, stream, STEREO_ALGO_AW1);
+}
+
+}  // extern "C"

@@ -7,10 +7,13 @@
 //   map and RectifyImage of the left image, 5. download, 6. write the maps and report
 //   milliseconds.
 //
-// usage: stereo_run <algo:sad|census|ncc|aw1|aw2|bm1|aw1band|nccband> <W> <H> <D> <seed> <out_prefix> [runs]
+// usage: stereo_run <algo:sad|census|ncc|aw1|aw2|bm1|aw1band|nccband|nccref|aw1ref> <W> <H> <D> <seed> <out_prefix> [runs]
 // aw1band / nccband: the full-range map, taken at half resolution (d / 2), as the initial
 // map of Process(disp, L, R, dispInit) with iUseDispInput; writes <out_prefix>_init.i16 and
 // _band.i16 and stops there.
+// nccref / aw1ref: the reference-format BlockMatcherNCC / BlockMatcherAW1 drivers configured
+// with the reference's SParameter::Set over the disparity range [3, D) with pixel offset 2,
+// Process(disp, L, R, dispInit, debug); writes <out_prefix>_ref.u16x4 and stops there.
 // Otherwise writes <out_prefix>_checked.i16, _lr.i16, _rl.i16 (row-major int16, -1 invalid),
 // _checkid.u16x4 (the cross-check's id map), _denoised.u16 (its de-noised ids),
 // _color.rgba (its Analysis colouring) and _rectified.u8 (the rectified left image)
@@ -38,7 +41,7 @@ static bool write_file(const std::string& path, const void* p, size_t n)
 int main(int argc, char** argv)
 {
     if (argc < 7) {
-        std::fprintf(stderr, "usage: %s <sad|census|ncc|aw1|aw2|bm1|aw1band|nccband> W H D seed out_prefix [runs]\n",
+        std::fprintf(stderr, "usage: %s <sad|census|ncc|aw1|aw2|bm1|aw1band|nccband|nccref|aw1ref> W H D seed out_prefix [runs]\n",
                      argv[0]);
         return 2;
     }
@@ -69,7 +72,12 @@ int main(int argc, char** argv)
         std::vector<double> ms;
 
         auto run = [&](auto& drv) {
-            for (int i = 0; i < runs; ++i) ms.push_back(ProcessKernel(drv, disp, lr, rl, imgL, imgR));
+            for (int i = 0; i < runs; ++i) {
+                drv.Begin();
+                drv.ProcessMaps(disp, lr, rl, imgL, imgR);
+                drv.End();
+                ms.push_back(drv.LastProcessTime());
+            }
             drv.LogLastProcessTime();
         };
         if (algo == "census") {
@@ -79,25 +87,29 @@ int main(int argc, char** argv)
             drv.Configure(dev, fmtIn, p);
             run(drv);
         } else if (algo == "sad") {
+            // the reference's Set(iOffset, iDispRange, iMinDeltaThresh, fSadThresh, fGradThresh,
+            // eConfig), thresholds inactive
             BlockMatcherSAD1::SParameter p;
-            p.Set(dcfg, 9, 1);
+            p.Set(0, D, -1, 1e30f, 0.0f, BlockMatcherSAD1::EConfig::Patch_9x9);
             BlockMatcherSAD1::CDriver drv;
             drv.Configure(dev, fmtIn, p);
             run(drv);
         } else if (algo == "ncc") {
-            BlockMatcherNCC::SParameter p;
+            BlockMatcherNCC::SFusedParameter p;
             p.Set(dcfg, BlockMatcherNCC::EConfig::Patch_11x11, 1);
-            BlockMatcherNCC::CDriver drv;
+            BlockMatcherNCC::CFusedDriver drv;
             drv.Configure(dev, fmtIn, p);
             run(drv);
         } else if (algo == "aw1" || algo == "aw2") {
-            BlockMatcherAW1::SParameter p;
-            p.Set(dcfg, BlockMatcherAW1::EConfig::Patch_15x15, 1);
             if (algo == "aw1") {
-                BlockMatcherAW1::CDriver drv;
+                BlockMatcherAW1::SFusedParameter p;
+                p.Set(dcfg, BlockMatcherAW1::EConfig::Patch_15x15, 1);
+                BlockMatcherAW1::CFusedDriver drv;
                 drv.Configure(dev, fmtIn, p);
                 run(drv);
             } else {
+                BlockMatcherAW2::SParameter p;
+                p.Set(0, D, -1, 1e30f, 0.0f, 0, BlockMatcherAW2::EConfig::Patch_15x15);
                 BlockMatcherAW2::CDriver drv;
                 drv.Configure(dev, fmtIn, p);
                 run(drv);
@@ -106,10 +118,10 @@ int main(int argc, char** argv)
             // MatcherUseDisparity through the driver layer: a full-range pass, then the band
             // search around its map at half resolution
             const bool aw = algo == "aw1band";
-            BlockMatcherAW1::SParameter pa;
-            BlockMatcherNCC::SParameter pn;
-            BlockMatcherAW1::CDriver fa, ba;
-            BlockMatcherNCC::CDriver fn, bn;
+            BlockMatcherAW1::SFusedParameter pa;
+            BlockMatcherNCC::SFusedParameter pn;
+            BlockMatcherAW1::CFusedDriver fa, ba;
+            BlockMatcherNCC::CFusedDriver fn, bn;
             if (aw) {
                 pa.Set(dcfg, BlockMatcherAW1::EConfig::Patch_9x9, 1, 0);
                 fa.Configure(dev, fmtIn, pa);
@@ -145,6 +157,35 @@ int main(int argc, char** argv)
             if (!write_file(out + "_init.i16", init.data(), init.size() * 2) ||
                 !write_file(out + "_band.i16", band.data(), band.size() * 2))
                 return 1;
+            std::sort(ms.begin(), ms.end());
+            std::printf("ms %.4f\n", ms[ms.size() / 2]);
+            return 0;
+        } else if (algo == "nccref" || algo == "aw1ref") {
+            // the reference operators through their own SParameter::Set, a range that does not
+            // start at 0 and a pixel offset
+            _SDisparityConfig rcfg;
+            rcfg.iMin = 3;
+            rcfg.iMax = D;
+            rcfg.iOffset = 2;
+            CDeviceImage ids(dev, fmtId), unused(dev, fmtId);
+            std::vector<uint16_t> hids((size_t)W * H * 4, 0x5A5A);
+            ids.CopyFrom(hids.data(), (size_t)W * 8);
+            if (algo == "nccref") {
+                BlockMatcherNCC::SParameter p;
+                p.Set(rcfg, 2, 0.5f, 2.0f, 0, true, BlockMatcherNCC::EConfig::Patch_9x9);
+                BlockMatcherNCC::CDriver drv;
+                drv.Configure(dev, fmtIn, p);
+                for (int i = 0; i < runs; ++i) ms.push_back(ProcessKernel(drv, ids, imgL, imgR, unused, unused));
+            } else {
+                BlockMatcherAW1::SParameter p;
+                p.Set(rcfg, 0.5f, 60.0f, 2.0f, 0, true, BlockMatcherAW1::EConfig::Patch_15x15,
+                      BlockMatcherAW1::ESimType::AbsoluteDifference);
+                BlockMatcherAW1::CDriver drv;
+                drv.Configure(dev, fmtIn, p);
+                for (int i = 0; i < runs; ++i) ms.push_back(ProcessKernel(drv, ids, imgL, imgR, unused, unused));
+            }
+            ids.CopyInto(hids.data(), (size_t)W * 8);
+            if (!write_file(out + "_ref.u16x4", hids.data(), hids.size() * 2)) return 1;
             std::sort(ms.begin(), ms.end());
             std::printf("ms %.4f\n", ms[ms.size() / 2]);
             return 0;

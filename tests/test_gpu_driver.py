@@ -88,3 +88,27 @@ def test_stereo_run_band_search_matches_oracle(tmp_path, algo):
         want = orc.ncc_bins(L, R, D, bins, window=(9, 9), tol=1, nthreads=8)[0]
         assert (got == want).mean() >= 0.999
     assert (got >= 0).mean() > 0.5
+
+
+@pytest.mark.parametrize("algo", ["nccref", "aw1ref"])
+def test_stereo_run_reference_operators_with_min_and_offset(tmp_path, algo):
+    """BlockMatcherNCC / BlockMatcherAW1::CDriver through the C++ layer, configured with the
+    reference's own SParameter::Set arguments over _SDisparityConfig [3, D) with pixel
+    offset 2 and Process(disp, L, R, dispInit, debug): bit-exact against the emulations."""
+    W, H, D, seed = 320, 160, 64, 9
+    prefix = str(tmp_path / algo)
+    print(f"stereo_run {algo} {W}x{H} D={D}", flush=True)
+    r = subprocess.run([EXE, algo, str(W), str(H), str(D), str(seed), prefix, "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(prefix + "_ref.u16x4", np.uint16).reshape(H, W, 4)
+    L, R = cs.synth_pair(seed, W, H, 0, 8, D - 1, min(8, D - 1))
+    if algo == "nccref":
+        want = orc.ncc_ref(L, R, P=9, dmin=3, dmax=D, offset=2, ltor=True, min_delta=2, ncc_thresh=0.5,
+                           grad_thresh=2.0, fill=0x5A5A)
+    else:
+        want = orc.aw1_ref(L, R, sub=5, sim_type=0, dmin=3, dmax=D, offset=2, ltor=True, min_delta=0.5,
+                           sim_thresh=60.0, grad_thresh=2.0, fill=0x5A5A)
+    assert np.array_equal(got, want)
+    ids = got[..., 0]
+    assert ((ids >= 1) & (ids <= 0xFFEF)).mean() > 0.3

@@ -135,12 +135,24 @@ def test_cpp_driver_layer_builds_and_reports_errors(tmp_path):
     prints its usage and turns a missing device into a CException exit."""
     import subprocess
     src = tmp_path / "t.cpp"
-    src.write_text('#include "clutec/stereo.hpp"\nint main(){ Clu::Cuda::BlockMatcherNCC::SParameter p; '
-                   'p.Set(Clu::Cuda::_SDisparityConfig{}, Clu::Cuda::BlockMatcherNCC::EConfig::Patch_9x9); '
-                   'p.Set(Clu::Cuda::_SDisparityConfig{}, Clu::Cuda::BlockMatcherNCC::EConfig::Patch_9x9, 1, 1); '
-                   'Clu::Cuda::BlockMatcherAW2::SParameter q; '
-                   'q.Set(Clu::Cuda::_SDisparityConfig{}, Clu::Cuda::BlockMatcherAW1::EConfig::Patch_9x9, 1, 1); '
-                   'return p.iUseDispInput + q.iUseDispInput - 2; }\n')
+    # the reference's own SParameter::Set calls (BlockMatcherNCC.h:95, BlockMatcherAW1.h:106,
+    # BlockMatcherSAD1.h:86, BlockMatcherAW2.h:95) and Process shapes compile unchanged
+    src.write_text(
+        '#include "clutec/stereo.hpp"\n'
+        'using namespace Clu::Cuda;\n'
+        'int main(){ _SDisparityConfig c; c.iMin = 3; c.iMax = 64; c.iOffset = 2;\n'
+        ' BlockMatcherNCC::SParameter p; p.Set(c, 2, 0.5f, 2.0f, 1, true, BlockMatcherNCC::EConfig::Patch_9x9);\n'
+        ' BlockMatcherAW1::SParameter a; a.Set(c, 0.5f, 60.0f, 2.0f, 0, false, BlockMatcherAW1::EConfig::Patch_15x15,\n'
+        '   BlockMatcherAW1::ESimType::SignedProduct);\n'
+        ' BlockMatcherSAD1::SParameter s; s.Set(0, 64, -1, 1e30f, 0.0f, BlockMatcherSAD1::EConfig::Patch_9x9);\n'
+        ' BlockMatcherAW2::SParameter q; q.Set(0, 64, -1, 1e30f, 0.0f, 1, BlockMatcherAW2::EConfig::Patch_9x9);\n'
+        ' if (p.iUseDispInput + q.iUseDispInput == 7) { CDevice dev(0); CDeviceImage img;\n'
+        '   BlockMatcherNCC::CDriver n; n.Configure(dev, img.Format(), p); n.Process(img, img, img, img, img);\n'
+        '   BlockMatcherAW1::CDriver w; w.Configure(dev, img.Format(), a); w.Process(img, img, img, img, img);\n'
+        '   BlockMatcherSAD1::CDriver d1; d1.Configure(dev, img.Format(), s); d1.Process(img, img, img);\n'
+        '   BlockMatcherAW2::CDriver d2; d2.Configure(dev, img.Format(), q); d2.Process(img, img, img, img);\n'
+        '   ProcessKernel(n, img, img, img, img, img); }\n'
+        ' return p.iUseDispInput + q.iUseDispInput - 2; }\n')
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
