@@ -165,6 +165,8 @@ _PROTOS = [
     ("stereo_census_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_validate", C.c_int, [C.POINTER(ImageFormat), C.POINTER(MatcherParams), C.c_char_p, C.c_size_t]),
     ("stereo_matcher_configure", C.c_int, [_P, C.POINTER(_P), C.POINTER(ImageFormat), C.POINTER(MatcherParams)]),
+    ("stereo_matcher_process_noalloc", C.c_int, [_P, C.POINTER(MatcherParams), C.POINTER(Image), C.POINTER(Image),
+                                                 C.POINTER(Image), C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_process", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image),
                                           C.POINTER(Image), C.POINTER(Image), _P]),
     ("stereo_matcher_process_init", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(Image),

@@ -50,6 +50,11 @@ stereo_status stereo_create(stereo_context** out, int device)
         delete ctx;
         return fail(nullptr, STEREO_ERROR_INVALID_DEVICE, "hipSetDevice: %s", hipGetErrorString(e));
     }
+    e = stereo::noalloc_setup(device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, STEREO_ERROR_HIP, "scratch array of the no-allocation matcher: %s", hipGetErrorString(e));
+    }
     *out = ctx;
     return STEREO_OK;
 }

@@ -50,6 +50,10 @@ void bm_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len);
 hipError_t launch_bm(int kind, int ww, const BmParams& p, hipStream_t s);
 hipError_t launch_bm_finish(const BmFinishParams& f, hipStream_t s);
 
+// Device address of the no-allocation entry point's scratch array on the current device,
+// looked up once per device (csrc/noalloc.hip); called by stereo_create.
+hipError_t noalloc_setup(int device);
+
 // ---- work-list matcher: many frames, per-tile disparity ranges ---------------------
 // A frame is one stereo pair; its tiles are 64 x BM_TILE_H pixels; every tile has a
 // disparity range [lo, hi) cut into items (one workgroup's work each).  Item i keeps
