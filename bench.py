@@ -341,6 +341,47 @@ def bm1_leg(cs, np, ctx, steps):
     return res
 
 
+def reference_operators_leg(cs, np, ctx, steps):
+    """The reference-format BlockMatcherNCC (11x11, the 1080p pair, D=256) and
+    BlockMatcherAW1 (15x15, AbsoluteDifference, the 1280x720 slanted-plane pair, D=128):
+    one direction, the reference's block tiling, decisions and four channels.  Evaluations
+    are counted as W*H*D per frame; kernel_ms from HIP events around the one launch."""
+    res = {}
+    cases = (("ncc_ref_11x11_1920x1080_d256", "ncc", 1920, 1080, 256, (1, 0, 24, 255, 8)),
+             ("aw1_ref_15x15_1280x720_d128", "aw1", 1280, 720, 128, (2, 1, 6, 127, 8)))
+    for name, kind, w, h, d, (seed, skind, nreg, dmax, dbg) in cases:
+        L, R = cs.synth_pair(seed, w, h, skind, nreg, dmax, dbg)
+        if kind == "ncc":
+            m = cs.BlockMatcherNCC(ctx, w, h, patch=11, disp_min=0, disp_max=d, min_delta_thresh=2, ncc_thresh=0.5,
+                                   grad_thresh=2.0)
+        else:
+            m = cs.BlockMatcherAW1(ctx, w, h, patch=15, disp_min=0, disp_max=d, min_delta_thresh=0.5, sim_thresh=60.0,
+                                   grad_thresh=2.0)
+        dl = cs.DeviceImage.from_array(ctx, L, cs.PIXEL_LUM_U8)
+        dr = cs.DeviceImage.from_array(ctx, R, cs.PIXEL_LUM_U8)
+        out = cs.DeviceImage(ctx, w, h, cs.PIXEL_RGBA_U16)
+        for _ in range(2):
+            m.process(out, dl, dr)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.process(out, dl, dr)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        ctx.set_timing(True)
+        kms = []
+        for _ in range(3):
+            m.process(out, dl, dr)
+            kms.append(m.last_time_ms())
+        ctx.set_timing(False)
+        c = out.download()[..., 0]
+        res[name] = {"ms_per_frame": round(ms, 4), "kernel_ms": round(statistics.median(kms), 4),
+                     "M_evals_per_s": round(w * h * d / (ms * 1e-3) / 1e6, 1),
+                     "disparity_fraction": round(float(((c >= 1) & (c <= 0xFFEF)).mean()), 4)}
+        m.close()
+    return res
+
+
 def post_filters_leg(cs, np, ctx, disp, steps):
     """The steps around the matcher at 1080p, inputs resident in HBM: de-noise of the
     census map (5x5 window, plain int16 map and the 4 x uint16 id map), five disparity
@@ -518,6 +559,7 @@ def main():
         result["other_matchers"] = other_matchers_leg(cs, np, ctx, max(5, min(args.steps, 20)))
         result["post_filters"] = post_filters_leg(cs, np, ctx, disp, max(20, args.steps))
         result["block_matcher1"] = bm1_leg(cs, np, ctx, max(5, min(args.steps, 10)))
+        result["reference_operators"] = reference_operators_leg(cs, np, ctx, max(5, min(args.steps, 10)))
 
     if not args.no_cpu:
         threads = host_cpus()[0]  # every core the process may use

@@ -271,11 +271,25 @@ stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** drv, i
 stereo_status stereo_batch_process(stereo_driver* drv, stereo_image* disp, stereo_image* disp_lr,
                                    stereo_image* disp_rl, const stereo_image* left, const stereo_image* right,
                                    stereo_stream stream);
-/* Times every candidate tuning on the given images and keeps the fastest; returns
- * it in *best (may be NULL).  Blocks. */
+/* Times the candidate tunings on the given images and keeps the fastest: one work-item
+ * length (32, 64, 128, 256) for every pair with the grid (1, 2, 3 workgroups per CU) and the
+ * item order, then the item length of each class of pairs (disparity count x tile count,
+ * as in stereo_batch_tuning_table), largest classes first.  The tile is 64 x 56 for every
+ * class.  Returns the overall tuning in *best (may be NULL).  Blocks. */
 stereo_status stereo_batch_autotune(stereo_driver* drv, const stereo_image* left, const stereo_image* right,
                                     stereo_batch_tuning* best);
 stereo_status stereo_batch_get_tuning(const stereo_driver* drv, stereo_batch_tuning* tuning);
+/* The tuning as a JSON table for a later process: {"matcher", "window", "grid", "order",
+ * "use_graph", "chunk", "classes": [{"disparities", "tiles", "tile", "chunk"}, ...]}, one class
+ * per disparity count and tile count ("small" below 64 tiles of 64 x 56, "medium" below 256,
+ * "large") among the driver's pairs, each with its work-item length.  buf may be NULL;
+ * *needed (may be NULL) receives the text's length + 1. */
+stereo_status stereo_batch_tuning_table(const stereo_driver* drv, char* buf, size_t n, size_t* needed);
+/* Applies a table written by stereo_batch_tuning_table: grid, order, use_graph, the default
+ * work-item length, and the length of every listed class the driver's pairs have (the others
+ * take the default).  Malformed tables and lengths outside 32..256 are
+ * STEREO_ERROR_INVALID_CONFIG, and the previous tuning stays. */
+stereo_status stereo_batch_apply_tuning_table(stereo_driver* drv, const char* json);
 
 /* ---- learned disparity proposal ----------------------------------------------
  *

@@ -11,6 +11,7 @@ kernels.
     disp = m.run(left_u8, right_u8)          # numpy in, numpy int16 out
 """
 import ctypes as C
+import json
 import os
 
 import numpy as np
@@ -195,6 +196,8 @@ _PROTOS = [
                                         C.POINTER(Image), _P]),
     ("stereo_batch_autotune", C.c_int, [_P, C.POINTER(Image), C.POINTER(Image), C.POINTER(BatchTuning)]),
     ("stereo_batch_get_tuning", C.c_int, [_P, C.POINTER(BatchTuning)]),
+    ("stereo_batch_tuning_table", C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("stereo_batch_apply_tuning_table", C.c_int, [_P, C.c_char_p]),
     ("stereo_cnn_create", C.c_int, [_P, C.POINTER(_P), C.c_int32, C.c_int32, C.c_int32]),
     ("stereo_cnn_destroy", None, [_P]),
     ("stereo_cnn_set_params", C.c_int, [_P, _P, C.c_size_t]),
@@ -726,10 +729,31 @@ class BatchMatcher(_Driver):
         o, a, b, l, r = self.dev
         _check(lib.stereo_batch_process(self.handle, o, a, b, l, r, stream), self.ctx)
 
-    def autotune(self):
+    def autotune(self, table_path=None):
+        """Times the candidate tunings on the uploaded pairs and keeps the fastest; with
+        `table_path`, also writes the per-class table (tuning_table()) there as JSON."""
         t = BatchTuning()
         _check(lib.stereo_batch_autotune(self.handle, self.dev[3], self.dev[4], C.byref(t)), self.ctx)
+        if table_path is not None:
+            with open(table_path, "w") as f:
+                json.dump(self.tuning_table(), f, indent=1)
         return t.as_dict()
+
+    def tuning_table(self):
+        """{"matcher", "window", "grid", "order", "use_graph", "chunk", "classes": [...]},
+        one class per disparity count and tile count with its work-item length."""
+        need = C.c_size_t(0)
+        _check(lib.stereo_batch_tuning_table(self.handle, None, 0, C.byref(need)), self.ctx)
+        buf = C.create_string_buffer(need.value)
+        _check(lib.stereo_batch_tuning_table(self.handle, buf, need.value, None), self.ctx)
+        return json.loads(buf.value.decode())
+
+    def apply_tuning_table(self, table):
+        """A table from tuning_table() or a JSON file written by autotune(table_path)."""
+        if isinstance(table, (str, os.PathLike)):
+            with open(table) as f:
+                table = json.load(f)
+        _check(lib.stereo_batch_apply_tuning_table(self.handle, json.dumps(table).encode()), self.ctx)
 
     def tuning(self):
         t = BatchTuning()

@@ -1,7 +1,9 @@
 // C ABI of the batched matcher: many stereo pairs of uneven size and disparity range
 // in one persistent work-list launch, captured into a HIP graph and replayed.
 #include <algorithm>
+#include <cstring>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "capi_internal.hpp"
@@ -19,6 +21,11 @@ struct BatchExt : DriverExt {
     int kind = 0;
     size_t elem = 1;
     stereo_batch_tuning tuning{};
+    // work-item length per pair (0: tuning.chunk); set per class by the autotuner or a
+    // tuning table
+    std::vector<int> pair_chunk;
+    hipEvent_t ev_done = nullptr;  // recorded after every process call's last launch
+    bool done_recorded = false;
     std::vector<uint8_t*> padL, padR;
     std::vector<stereo::BmFrame> frames;  // host copy of the device frame table
     stereo::BmFrame* frames_dev = nullptr;
@@ -53,6 +60,7 @@ struct BatchExt : DriverExt {
         if (cap) (void)hipStreamDestroy(cap);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_done) (void)hipEventDestroy(ev_done);
         for (uint8_t* p : padL)
             if (p) (void)hipFree(p);
         for (uint8_t* p : padR)
@@ -71,17 +79,52 @@ size_t rpart_need(const std::vector<stereo::BmFrame>& frames, int chunk)
     return n;
 }
 
+// Tuning classes of the pairs: the disparity count and the tile count (small below 64
+// tiles of 64 x 56, medium below 256, large), the two sizes that decide how a pair's
+// work splits into items.
+const char* const kSizeClass[3] = {"small", "medium", "large"};
+int size_class(const stereo::BmFrame& f)
+{
+    const int t = f.ntx * f.nty;
+    return t < 64 ? 0 : (t < 256 ? 1 : 2);
+}
+
+int chunk_of(const BatchExt* x, int f)
+{
+    const int c = x->pair_chunk.empty() ? 0 : x->pair_chunk[f];
+    return c ? c : x->tuning.chunk;
+}
+
+int max_chunk(const BatchExt* x)
+{
+    int m = 0;
+    for (int f = 0; f < x->npairs; ++f) m = std::max(m, chunk_of(x, f));
+    return m;
+}
+
+size_t rpart_need_mixed(const BatchExt* x)
+{
+    const int stride = stereo::bm_list_rp_stride(max_chunk(x));
+    size_t n = 0;
+    for (int f = 0; f < x->npairs; ++f) {
+        const stereo::BmFrame& fr = x->frames[f];
+        const int c = chunk_of(x, f);
+        n += (size_t)fr.ntx * fr.nty * ((fr.D + c - 1) / c) * stereo::BM_TILE_H * stride;
+    }
+    return n;
+}
+
 // Items of every tile of every pair, tile by tile (the slots the merge reads), the
 // tile table, and the schedule: 0 = longest items of the largest frames first,
 // 1 = table order.
 stereo_status upload_work(BatchExt* x)
 {
     HIP_TRY(x->ctx, hipDeviceSynchronize(), "hipDeviceSynchronize");  // no launch of the old tables in flight
-    const int chunk = x->tuning.chunk;
     std::vector<stereo::BmItem> items;
     std::vector<stereo::BmTile> tiles;
     for (int f = 0; f < x->npairs; ++f) {
         const stereo::BmFrame& fr = x->frames[f];
+        const int chunk = chunk_of(x, f);
         for (int ty = 0; ty < fr.nty; ++ty)
             for (int tx = 0; tx < fr.ntx; ++tx) {
                 stereo::BmTile t;
@@ -103,8 +146,9 @@ stereo_status upload_work(BatchExt* x)
             }
     }
     const int n = (int)items.size();
-    if (n > x->max_items || rpart_need(x->frames, chunk) > x->rpart_elems)
-        return fail(x->ctx, STEREO_ERROR_INVALID_CONFIG, "batch: chunk %d needs more workspace than configured", chunk);
+    if (n > x->max_items || rpart_need_mixed(x) > x->rpart_elems)
+        return fail(x->ctx, STEREO_ERROR_INVALID_CONFIG, "batch: work items of up to %d disparities need more workspace "
+                    "than configured", max_chunk(x));
     std::vector<int> sched(n);
     std::iota(sched.begin(), sched.end(), 0);
     if (x->tuning.order == 0)
@@ -154,7 +198,7 @@ stereo_status enqueue(stereo_driver* d, BatchExt* x, const stereo_image* left, c
     lp.counter = x->counter_dev;
     lp.lkey = x->lkey;
     lp.rpart = x->rpart;
-    lp.rp_stride = stereo::bm_list_rp_stride(x->tuning.chunk);
+    lp.rp_stride = stereo::bm_list_rp_stride(max_chunk(x));
     lp.nframes = x->npairs;
     lp.total_rows = x->total_rows;
     const int grid = std::max(1, std::min(x->tuning.grid, x->nitems));
@@ -236,7 +280,7 @@ stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** out, i
     if ((st = driver_events(d))) return cleanup(st);
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&x->cap, hipStreamNonBlocking)) || (e = hipEventCreate(&x->ev_fork)) ||
-        (e = hipEventCreate(&x->ev_join)))
+        (e = hipEventCreate(&x->ev_join)) || (e = hipEventCreate(&x->ev_done)))
         return cleanup(hip_fail(ctx, e, "hipStreamCreate"));
     d->bx = params->window_w / 2 + (x->kind == stereo::BM_COST_CENSUS ? params->census_w / 2 : 0);
     d->by = params->window_h / 2 + (x->kind == stereo::BM_COST_CENSUS ? params->census_h / 2 : 0);
@@ -278,6 +322,7 @@ stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** out, i
         return cleanup(fail(ctx, STEREO_ERROR_INVALID_FORMAT, "batch: width above %d", stereo::BM_LIST_MAX_WIDTH));
     x->total_rows = rows;
     x->max_items = (int)max_items;
+    x->pair_chunk.assign(npairs, 0);
     for (int c = kMinChunk; c <= 256; c += 8) x->rpart_elems = std::max(x->rpart_elems, rpart_need(x->frames, c));
     if ((e = hipMalloc((void**)&x->frames_dev, npairs * sizeof(stereo::BmFrame))) ||
         (e = hipMalloc((void**)&x->tiles_dev, tiles * sizeof(stereo::BmTile))) ||
@@ -317,7 +362,9 @@ stereo_status stereo_batch_process(stereo_driver* d, stereo_image* disp, stereo_
     GUARD(ctx, "stereo_batch_process");
     const hipStream_t s = (hipStream_t)stream;
     if (set_outputs(x, disp, disp_lr, disp_rl)) {
-        // the device frame table is read by earlier work still in flight: wait for it
+        // the device frame table is read by the previous call's launches, which may still
+        // run on another stream: wait for their end before rewriting it
+        if (x->done_recorded) HIP_TRY(ctx, hipEventSynchronize(x->ev_done), "hipEventSynchronize");
         HIP_TRY(ctx, hipStreamSynchronize(s), "hipStreamSynchronize");
         HIP_TRY(ctx, hipMemcpy(x->frames_dev, x->frames.data(), x->npairs * sizeof(stereo::BmFrame), hipMemcpyHostToDevice),
                 "hipMemcpy");
@@ -326,6 +373,8 @@ stereo_status stereo_batch_process(stereo_driver* d, stereo_image* disp, stereo_
     if ((st = begin(d, s))) return st;
     if (!x->tuning.use_graph) {
         if ((st = enqueue(d, x, left, right, s))) return st;
+        HIP_TRY(ctx, hipEventRecord(x->ev_done, s), "hipEventRecord");
+        x->done_recorded = true;
         return end(d, s);
     }
     std::vector<const void*> key;
@@ -352,6 +401,8 @@ stereo_status stereo_batch_process(stereo_driver* d, stereo_image* disp, stereo_
         x->graph_key = key;
     }
     HIP_TRY(ctx, hipGraphLaunch(x->exec, s), "hipGraphLaunch");
+    HIP_TRY(ctx, hipEventRecord(x->ev_done, s), "hipEventRecord");
+    x->done_recorded = true;
     return end(d, s);
 }
 
@@ -381,6 +432,7 @@ stereo_status stereo_batch_autotune(stereo_driver* d, const stereo_image* left, 
     size_t spitch = ((size_t)x->max_width * 2 + 255) & ~(size_t)255;
     int maxh = 0;
     for (const stereo::BmFrame& f : x->frames) maxh = std::max(maxh, f.H);
+    HIP_TRY(ctx, hipDeviceSynchronize(), "hipDeviceSynchronize");  // earlier calls may still read the frame table
     HIP_TRY(ctx, hipMalloc((void**)&scratch, spitch * maxh), "hipMalloc");
     for (stereo::BmFrame& f : x->frames) {
         f.disp.ptr = reinterpret_cast<uint8_t*>(scratch);
@@ -404,7 +456,23 @@ stereo_status stereo_batch_autotune(stereo_driver* d, const stereo_image* left, 
         cus = prop.multiProcessorCount;
     const int chunks[] = {32, 64, 128, 256};
     const int grids[] = {cus, 2 * cus, 3 * cus};
+    const std::vector<int> keep_chunks = x->pair_chunk;
+    std::fill(x->pair_chunk.begin(), x->pair_chunk.end(), 0);
     st = STEREO_OK;
+    // time the current work list: warm-up, then three runs
+    auto time_it = [&](float* ms) -> stereo_status {
+        stereo_status s2 = upload_work(x);
+        if (s2) return s2;
+        if ((s2 = enqueue(d, x, left, right, nullptr))) return s2;
+        hipError_t e = hipEventRecord(e0, nullptr);
+        for (int r = 0; r < 3 && e == hipSuccess && !s2; ++r) s2 = enqueue(d, x, left, right, nullptr);
+        if (s2) return s2;
+        if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+        return e == hipSuccess ? STEREO_OK : hip_fail(ctx, e, "autotune timing");
+    };
+    // one item length for every pair, with the grid size and the item order
     for (int c : chunks)
         for (int g : grids)
             for (int o = 0; o < 2 && !st; ++o) {
@@ -412,20 +480,59 @@ stereo_status stereo_batch_autotune(stereo_driver* d, const stereo_image* left, 
                 x->tuning.grid = g;
                 x->tuning.order = o;
                 x->tuning.use_graph = 0;
-                if ((st = upload_work(x))) break;
-                if ((st = enqueue(d, x, left, right, nullptr))) break;  // warm-up
-                hipError_t e = hipEventRecord(e0, nullptr);
-                for (int r = 0; r < 3 && e == hipSuccess && !st; ++r) st = enqueue(d, x, left, right, nullptr);
-                if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
-                if (e == hipSuccess) e = hipEventSynchronize(e1);
                 float ms = 0.f;
-                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-                if (e != hipSuccess) st = hip_fail(ctx, e, "autotune timing");
-                if (!st && ms < win_ms) {
+                if ((st = time_it(&ms))) break;
+                if (ms < win_ms) {
                     win_ms = ms;
                     win = x->tuning;
                 }
             }
+    // then the item length of each class of pairs (disparity count x tile count), the
+    // classes with the most evaluations first; a length whose partial keys do not fit the
+    // workspace is skipped
+    std::vector<int> win_chunks(x->npairs, 0);
+    if (!st) {
+        x->tuning = win;
+        x->tuning.use_graph = 0;
+        std::vector<std::pair<long long, std::pair<int, int>>> classes;  // (evaluations, (D, size class))
+        for (int f = 0; f < x->npairs; ++f) {
+            const stereo::BmFrame& fr = x->frames[f];
+            const std::pair<int, int> key(fr.D, size_class(fr));
+            auto it = std::find_if(classes.begin(), classes.end(), [&](const auto& c) { return c.second == key; });
+            if (it == classes.end()) classes.push_back({0, key}), it = classes.end() - 1;
+            it->first += (long long)fr.W * fr.H * fr.D;
+        }
+        std::sort(classes.begin(), classes.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        for (const auto& cl : classes) {
+            int best_c = 0;
+            float best_ms = win_ms;
+            for (int c : chunks) {
+                if (c == win.chunk || c > std::max(32, cl.second.first)) continue;
+                for (int f = 0; f < x->npairs; ++f) {
+                    const stereo::BmFrame& fr = x->frames[f];
+                    if (fr.D == cl.second.first && size_class(fr) == cl.second.second) x->pair_chunk[f] = c;
+                }
+                if (rpart_need_mixed(x) <= x->rpart_elems) {
+                    float ms = 0.f;
+                    if ((st = time_it(&ms))) break;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best_c = c;
+                    }
+                }
+                for (int f = 0; f < x->npairs; ++f) x->pair_chunk[f] = win_chunks[f];
+            }
+            if (st) break;
+            if (best_c) {
+                for (int f = 0; f < x->npairs; ++f) {
+                    const stereo::BmFrame& fr = x->frames[f];
+                    if (fr.D == cl.second.first && size_class(fr) == cl.second.second) win_chunks[f] = best_c;
+                }
+                win_ms = best_ms;
+            }
+            x->pair_chunk = win_chunks;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipDeviceSynchronize();
@@ -433,12 +540,129 @@ stereo_status stereo_batch_autotune(stereo_driver* d, const stereo_image* left, 
     x->frames = saved;
     x->tuning = st ? keep : win;
     x->tuning.use_graph = keep.use_graph;
+    x->pair_chunk = st ? keep_chunks : win_chunks;
     HIP_TRY(ctx, hipMemcpy(x->frames_dev, x->frames.data(), x->npairs * sizeof(stereo::BmFrame), hipMemcpyHostToDevice),
             "hipMemcpy");
     const stereo_status st2 = upload_work(x);
     if (st) return st;
     if (st2) return st2;
     if (best) *best = x->tuning;
+    return STEREO_OK;
+}
+
+stereo_status stereo_batch_tuning_table(const stereo_driver* d, char* buf, size_t n, size_t* needed)
+{
+    if (!d || d->kind != stereo_driver::BATCH) return STEREO_ERROR_INVALID_ARGUMENT;
+    const BatchExt* x = static_cast<const BatchExt*>(d->ext);
+    std::string t = "{\"matcher\": \"";
+    t += x->kind == stereo::BM_COST_CENSUS ? "census" : "sad";
+    char tmp[256];
+    std::snprintf(tmp, sizeof(tmp),
+                  "\", \"window\": %d, \"grid\": %d, \"order\": %d, \"use_graph\": %d, \"chunk\": %d, \"classes\": [",
+                  x->mp.window_w, x->tuning.grid, x->tuning.order, x->tuning.use_graph, x->tuning.chunk);
+    t += tmp;
+    std::vector<std::pair<int, int>> seen;
+    for (int f = 0; f < x->npairs; ++f) {
+        const std::pair<int, int> key(x->frames[f].D, size_class(x->frames[f]));
+        if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+        seen.push_back(key);
+        std::snprintf(tmp, sizeof(tmp), "%s{\"disparities\": %d, \"tiles\": \"%s\", \"tile\": \"%dx%d\", \"chunk\": %d}",
+                      seen.size() > 1 ? ", " : "", key.first, kSizeClass[key.second], stereo::BM_TILE_W, stereo::BM_TILE_H,
+                      chunk_of(x, f));
+        t += tmp;
+    }
+    t += "]}";
+    if (needed) *needed = t.size() + 1;
+    if (buf && n) {
+        const size_t k = std::min(n - 1, t.size());
+        std::memcpy(buf, t.data(), k);
+        buf[k] = 0;
+        if (k < t.size()) return fail(d->ctx, STEREO_ERROR_INVALID_ARGUMENT, "tuning table: buffer of %zu bytes, %zu needed",
+                                      n, t.size() + 1);
+    }
+    return STEREO_OK;
+}
+
+namespace {
+
+// The integer after "key": in [from, to); false when absent.
+bool json_int(const char* from, const char* to, const char* key, int* v)
+{
+    const std::string k = std::string("\"") + key + "\":";
+    const char* p = std::search(from, to, k.begin(), k.end());
+    if (p == to) return false;
+    return std::sscanf(p + k.size(), "%d", v) == 1;
+}
+
+bool json_word(const char* from, const char* to, const char* key, char* out, size_t n)
+{
+    const std::string k = std::string("\"") + key + "\":";
+    const char* p = std::search(from, to, k.begin(), k.end());
+    if (p == to) return false;
+    p += k.size();
+    while (p < to && *p == ' ') ++p;
+    if (p >= to || *p != '"') return false;
+    size_t i = 0;
+    for (++p; p < to && *p != '"' && i + 1 < n; ++p) out[i++] = *p;
+    out[i] = 0;
+    return p < to && *p == '"';
+}
+
+}  // namespace
+
+stereo_status stereo_batch_apply_tuning_table(stereo_driver* d, const char* json)
+{
+    if (!d || d->kind != stereo_driver::BATCH || !json) return STEREO_ERROR_INVALID_ARGUMENT;
+    BatchExt* x = static_cast<BatchExt*>(d->ext);
+    stereo_context* ctx = d->ctx;
+    // the top-level keys are read from the text without the class list
+    const char* cls = std::strstr(json, "\"classes\"");
+    const char* cls_end = cls ? std::strchr(cls, ']') : nullptr;
+    if (cls && !cls_end) return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "tuning table: unterminated class list");
+    const std::string top = cls ? std::string(json, cls) + std::string(cls_end + 1) : std::string(json);
+    const char* t0 = top.c_str();
+    const char* t1 = t0 + top.size();
+    stereo_batch_tuning t = x->tuning;
+    int v;
+    if (json_int(t0, t1, "grid", &v)) t.grid = v;
+    if (json_int(t0, t1, "order", &v)) t.order = v;
+    if (json_int(t0, t1, "use_graph", &v)) t.use_graph = v;
+    if (json_int(t0, t1, "chunk", &v)) t.chunk = v;
+    auto chunk_ok = [](int c) { return c >= kMinChunk && c <= 256 && c % 8 == 0; };
+    if (!chunk_ok(t.chunk) || t.grid < 1 || t.grid > 65535 || t.order < 0 || t.order > 1 || t.use_graph < 0 ||
+        t.use_graph > 1)
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "tuning table: grid %d, order %d, use_graph %d, chunk %d", t.grid,
+                    t.order, t.use_graph, t.chunk);
+    std::vector<int> chunks(x->npairs, 0);
+    for (const char* p = cls ? std::strchr(cls, '{') : nullptr; p && p < cls_end; p = std::strchr(p + 1, '{')) {
+        const char* q = std::strchr(p, '}');
+        if (!q || q > cls_end) return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "tuning table: unterminated class entry");
+        int D, c;
+        char size[16];
+        if (!json_int(p, q, "disparities", &D) || !json_int(p, q, "chunk", &c) || !json_word(p, q, "tiles", size, 16))
+            return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "tuning table: a class needs disparities, tiles and chunk");
+        if (!chunk_ok(c)) return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "tuning table: chunk %d", c);
+        int sc = -1;
+        for (int i = 0; i < 3; ++i)
+            if (!std::strcmp(size, kSizeClass[i])) sc = i;
+        if (sc < 0) return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "tuning table: tiles \"%s\"", size);
+        for (int f = 0; f < x->npairs; ++f)
+            if (x->frames[f].D == D && size_class(x->frames[f]) == sc) chunks[f] = c;
+    }
+    stereo_status st = set_device(ctx);
+    if (st) return st;
+    if (x->done_recorded) HIP_TRY(ctx, hipEventSynchronize(x->ev_done), "hipEventSynchronize");
+    const stereo_batch_tuning keep = x->tuning;
+    const std::vector<int> keep_chunks = x->pair_chunk;
+    x->tuning = t;
+    x->pair_chunk = chunks;
+    st = upload_work(x);
+    if (st) {
+        x->tuning = keep;
+        x->pair_chunk = keep_chunks;
+        (void)upload_work(x);
+        return st;
+    }
     return STEREO_OK;
 }
 

@@ -303,8 +303,11 @@ struct BandExt : DriverExt {
     uint32_t* lkey = nullptr;
     uint32_t* rpart = nullptr;
     int ntx = 0, nty = 0, max_items = 0;
+    hipEvent_t ev_done = nullptr;  // recorded after every list finish launch
+    bool done_recorded = false;
     ~BandExt() override
     {
+        if (ev_done) (void)hipEventDestroy(ev_done);
         for (void* p : {(void*)frame_dev, (void*)tiles, (void*)items, (void*)nitems, (void*)counter, (void*)tile_evals,
                         (void*)tile_mask, (void*)lkey, (void*)rpart})
             if (p) (void)hipFree(p);
@@ -338,7 +341,8 @@ stereo_status band_alloc(stereo_driver* d)
         (e = hipMalloc((void**)&x->tile_mask, nt * sizeof(uint32_t))) ||
         (e = hipMalloc((void**)&x->lkey, stereo::bm_list_lkey_elems(x->max_items) * 4)) ||
         (e = hipMalloc((void**)&x->rpart, stereo::bm_list_rpart_elems(x->max_items, kBandChunk) * 4)) ||
-        (e = hipMemset(x->counter, 0, sizeof(int))) || (e = hipDeviceSynchronize()))
+        (e = hipMemset(x->counter, 0, sizeof(int))) || (e = hipEventCreate(&x->ev_done)) ||
+        (e = hipDeviceSynchronize()))
         return hip_fail(d->ctx, e, "hipMalloc");
     stereo::BmFrame& f = x->frame;
     std::memset(&f, 0, sizeof(f));
@@ -668,6 +672,8 @@ stereo_status stereo_matcher_process_init(stereo_driver* d, stereo_image* disp, 
     f.disp_l = disp_lr ? view_of(disp_lr) : null_view();
     f.disp_r = disp_rl ? view_of(disp_rl) : null_view();
     if (std::memcmp(&f, &x->frame, sizeof(f)) != 0) {
+        // the previous call's launches read the table and may still run on another stream
+        if (x->done_recorded) HIP_TRY(ctx, hipEventSynchronize(x->ev_done), "hipEventSynchronize");
         HIP_TRY(ctx, hipStreamSynchronize(s), "hipStreamSynchronize");
         HIP_TRY(ctx, hipMemcpy(x->frame_dev, &f, sizeof(f), hipMemcpyHostToDevice), "hipMemcpy");
         x->frame = f;
@@ -713,6 +719,8 @@ stereo_status stereo_matcher_process_init(stereo_driver* d, stereo_image* disp, 
             "list matcher launch");
     if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[1], s), "hipEventRecord");
     HIP_TRY(ctx, stereo::launch_bm_list_finish(lp, W, s), "list finish launch");
+    HIP_TRY(ctx, hipEventRecord(x->ev_done, s), "hipEventRecord");
+    x->done_recorded = true;
     return end(d, s);
 }
 

@@ -592,6 +592,98 @@ __global__ __launch_bounds__(1024) void band_items_kernel(CnnBandArgs a)
     }
 }
 
+
+// Inference through the five 16-channel layers in one launch.  A workgroup of 8 waves
+// owns an output tile of FZ_TW x FZ_TH quarter pixels and keeps every intermediate
+// activation of its tile in LDS: the input with the network's whole halo (the dilations
+// add up to 10), then each layer's output over the region the remaining layers read
+// (halo 9, 7, 3, 1, 0), ping-ponging between two staged buffers, so that nothing but
+// the input and the final FP32 output crosses HBM.  Each layer is computed exactly as
+// conv3x3_kernel computes it -- the same MFMA operands and tap order, the same epilogue
+// (bias, ReLU, bf16 rounding) -- and activations outside the image are set to zero, the
+// padding the next layer would see; the output therefore equals the layer-by-layer
+// launches bit for bit.
+constexpr int FZ_TW = 32, FZ_TH = 16, FZ_PS = 40;  // staged pixel stride: 16 bf16 channels + 8 bytes
+constexpr int FZ_AW = FZ_TW + 20, FZ_AH = FZ_TH + 20;  // buffer A: the input region (halo 10)
+constexpr int FZ_BW = FZ_TW + 18, FZ_BH = FZ_TH + 18;  // buffer B: layer 0's output region (halo 9)
+
+__global__ __launch_bounds__(512) void cnn_fused_forward_kernel(CnnFusedArgs a)
+{
+    __shared__ __align__(16) uint8_t bufA[FZ_AW * FZ_AH * FZ_PS];
+    __shared__ __align__(16) uint8_t bufB[FZ_BW * FZ_BH * FZ_PS];
+    const int lane = threadIdx.x & 63;
+    const int col = lane & 15, grp = lane >> 4;
+    const int wv = threadIdx.x >> 6;
+    constexpr int NW = 8;
+    const int ntx = (a.W + FZ_TW - 1) / FZ_TW, nty = (a.H + FZ_TH - 1) / FZ_TH;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = b % ntx, r = b / ntx;
+    const int ty = r % nty, n = r / nty;
+    const int x0 = tx * FZ_TW, y0 = ty * FZ_TH;
+    const uint16_t* img = a.x + (size_t)n * a.H * a.W * 16;
+
+    // stage the input region (halo 10), zero outside the image: 4 x 8-byte chunks per pixel
+    for (int i = threadIdx.x; i < FZ_AW * FZ_AH * 4; i += 512) {
+        const int px = i >> 2, q = i & 3;
+        const int gx = x0 - 10 + px % FZ_AW, gy = y0 - 10 + px / FZ_AW;
+        uint2 v = make_uint2(0u, 0u);
+        if (gx >= 0 && gx < a.W && gy >= 0 && gy < a.H)
+            v = *reinterpret_cast<const uint2*>(img + ((size_t)gy * a.W + gx) * 16 + 4 * q);
+        *reinterpret_cast<uint2*>(bufA + px * FZ_PS + 8 * q) = v;
+    }
+    __syncthreads();
+
+    constexpr int grow[6] = {10, 9, 7, 3, 1, 0};  // halo of each layer's input; grow[5]: the tile
+#pragma unroll 1
+    for (int l = 0; l < 5; ++l) {
+        const int gi = grow[l], go = grow[l + 1];
+        const int dil = gi - go;
+        const int rwi = FZ_TW + 2 * gi;                 // input region row length
+        const int rwo = FZ_TW + 2 * go, rho = FZ_TH + 2 * go;
+        const uint8_t* in = (l & 1) ? bufB : bufA;
+        uint8_t* out = (l & 1) ? bufA : bufB;
+        v4s wr[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+            wr[t] = *reinterpret_cast<const v4s*>(a.w[l] + ((size_t)col * 9 + t) * 16 + 4 * grp);
+        float bias[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bias[i] = a.bias[l][4 * grp + i];
+        const int nbx = (rwo + 15) >> 4;
+        for (int item = wv; item < nbx * rho; item += NW) {
+            const int oy = item / nbx, ox = (item % nbx) * 16 + col;
+            const int oxr = min(ox, rwo - 1);  // lanes past the region read its last pixel
+            const int ix = oxr + dil, iy = oy + dil;
+            v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const int sy = iy + (t / 3 - 1) * dil, sx = ix + (t % 3 - 1) * dil;
+                const v4s bv = *reinterpret_cast<const v4s*>(in + (sy * rwi + sx) * FZ_PS + 8 * grp);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[t], bv, acc, 0, 0, 0);
+            }
+            // D: lane (col, grp) holds pixel ox, channels 4 grp .. 4 grp + 3
+            const int gx = x0 - go + ox, gy = y0 - go + oy;
+            const bool inside = gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
+            float v[4] = {acc[0], acc[1], acc[2], acc[3]};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] += bias[i];
+            if (l < 4) {
+                if (ox >= rwo) continue;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[i] = (inside && v[i] > 0.f) ? v[i] : 0.f;
+                uint2 pk;
+                pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                *reinterpret_cast<uint2*>(out + (oy * rwo + ox) * FZ_PS + 8 * grp) = pk;
+            } else if (ox < rwo && inside) {
+                *reinterpret_cast<float4*>(a.out + (((size_t)n * a.H + gy) * a.W + gx) * 16 + 4 * grp) =
+                    make_float4(v[0], v[1], v[2], v[3]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace cnn
 
 hipError_t launch_cnn_conv(int cin, int cout, int epi, const CnnConvArgs& a, hipStream_t s)
@@ -673,6 +765,15 @@ hipError_t launch_cnn_adam(float* param, const float* grad, float* m, float* v, 
 {
     hipLaunchKernelGGL(cnn::adam_kernel, dim3((n + 255) / 256), dim3(256), 0, s, param, grad, m, v, n, cin, cout, lr, b1,
                        b2, eps, c1, c2, wf, wb);
+    return hipGetLastError();
+}
+
+hipError_t launch_cnn_fused_forward(const CnnFusedArgs& a, hipStream_t s)
+{
+    if (a.N < 1 || a.H < 1 || a.W < 1) return hipErrorInvalidValue;
+    const long ntiles = (long)a.N * ((a.H + cnn::FZ_TH - 1) / cnn::FZ_TH) * ((a.W + cnn::FZ_TW - 1) / cnn::FZ_TW);
+    if (ntiles > 0x7fffffffL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cnn::cnn_fused_forward_kernel, dim3((int)ntiles), dim3(512), 0, s, a);
     return hipGetLastError();
 }
 

@@ -199,6 +199,16 @@ struct CnnBandArgs {
 };
 
 hipError_t launch_cnn_conv(int cin, int cout, int epi, const CnnConvArgs& a, hipStream_t s);
+// Inference of the whole 16-channel network (dilations 1, 2, 4, 2, 1) in one launch:
+// x [N][H][W][16] bf16 in, out [N][H][W][16] FP32 (bias added, no ReLU on the last layer).
+struct CnnFusedArgs {
+    const uint16_t* x;
+    const uint16_t* w[5];  // bf16 [16][9][16] per layer
+    const float* bias[5];  // [16] per layer
+    float* out;
+    int N, H, W;
+};
+hipError_t launch_cnn_fused_forward(const CnnFusedArgs& a, hipStream_t s);
 int cnn_dw_parts(const CnnDwArgs& a);
 hipError_t launch_cnn_dw(int cin, int cout, const CnnDwArgs& a, float* out, hipStream_t s);
 // clabel: per-pixel confidence target (0 or 1, NaN: from |e| <= tau where label is set)

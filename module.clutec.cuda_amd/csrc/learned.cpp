@@ -89,8 +89,25 @@ stereo_status refresh_weights(stereo_cnn* c, float lr, hipStream_t s)
     return STEREO_OK;
 }
 
-stereo_status cnn_forward(stereo_cnn* c, int n, hipStream_t s)
+// Forward over slots [0, n).  Training keeps every layer's activation for the backward
+// pass and runs the layers one launch each; inference (fused) runs the whole network in
+// one launch whose output is the same, bit for bit.
+stereo_status cnn_forward(stereo_cnn* c, int n, hipStream_t s, bool fused = false)
 {
+    if (fused) {
+        stereo::CnnFusedArgs a;
+        a.x = c->act[0];
+        for (int l = 0; l < kNumLayers; ++l) {
+            a.w[l] = c->wf + layer_off(l);
+            a.bias[l] = c->param + layer_off(l) + layer_w(l);
+        }
+        a.out = c->out;
+        a.N = n;
+        a.H = c->H4;
+        a.W = c->W4;
+        HIP_TRY(c->ctx, stereo::launch_cnn_fused_forward(a, s), "fused network launch");
+        return STEREO_OK;
+    }
     for (int l = 0; l < kNumLayers; ++l) {
         const size_t o = layer_off(l);
         stereo::CnnConvArgs a;
@@ -339,7 +356,7 @@ stereo_status stereo_cnn_forward(stereo_cnn* c, int32_t n, float* out, stereo_st
     if (!c) return STEREO_ERROR_INVALID_ARGUMENT;
     if (n < 1 || n > c->maxn) return fail(c->ctx, STEREO_ERROR_INVALID_ARGUMENT, "frame count %d outside [1, %d]", n, c->maxn);
     const hipStream_t s = (hipStream_t)stream;
-    stereo_status st = cnn_forward(c, n, s);
+    stereo_status st = cnn_forward(c, n, s, true);
     if (st) return st;
     if (out) {
         const size_t P = (size_t)n * c->W4 * c->H4;

@@ -272,7 +272,12 @@ __global__ __launch_bounds__(64) void ncc_ref_kernel(RefArgs a)
             }
         }
         int lo, hi;
-        if (!ref_block_range(tlo, thi, lo, hi)) {
+        // lanes 0 .. 31 hold the block's reference threads; every lane takes their range,
+        // so that the whole wave goes on (or stops) together and shares the group loads
+        const bool valid = ref_block_range(tlo, thi, lo, hi);
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        hi = __builtin_amdgcn_readfirstlane(hi);
+        if (!__builtin_amdgcn_readfirstlane((int)valid)) {
             // no valid range: every written result keeps its initial value (:1003-1027)
 #pragma unroll
             for (int k = 0; k < CG; ++k)

@@ -212,3 +212,38 @@ def test_training_reduces_loss(ctx):
     print(f"loss {first} -> {last}", flush=True)
     assert last[0] + last[1] < 0.8 * (first[0] + first[1])
     assert np.all(np.isfinite(net.params()))
+
+
+def test_fused_inference_equals_layer_launches(ctx):
+    """stereo_cnn_forward runs the whole network in one launch (activations in LDS); its
+    output equals five layer launches of stereo_conv3x3_bf16 on the same bf16 input and
+    weights bit for bit, on frames whose size is not a multiple of the fused tile."""
+    W, H, D = 536, 212, 64  # quarter resolution 134 x 53
+    W4, H4 = W // 4, H // 4
+    net = cs.ProposalNet(ctx, W4, H4, max_frames=2)
+    p = cs.cnn_init_params(11)
+    p[-2:] = [0.3, -0.2]
+    net.set_params(p)
+    xs = []
+    for i, seed in enumerate((61, 62)):
+        (dl, dr, dc, _), x0, _, _ = _frame(ctx, seed, W, H, D)
+        net.set_input(i, dl, dr, dc)
+        xs.append(x0)
+    out = cs.DeviceBuffer(ctx, 2 * H4 * W4 * 2 * 4)
+    net.forward_ptr(2, out.ptr)
+    cs._check(cs.lib.stereo_synchronize(ctx.handle, None), ctx)
+    got = out.download(np.float32, (2, H4, W4, 2))
+    x = np.zeros((2, H4, W4, 16), np.float32)
+    x[..., :2] = np.stack(xs)
+    k = 0
+    for l, (ci, co, dil) in enumerate(cs.CNN_LAYERS):
+        w = np.zeros((16, 3, 3, 16), np.float32)
+        w[:co, :, :, :ci] = p[k:k + co * 9 * ci].reshape(co, 3, 3, ci)
+        k += co * 9 * ci
+        b = np.zeros(16, np.float32)
+        b[:co] = p[k:k + co]
+        k += co
+        _, _, x = _conv(ctx, x, w, b, dil, relu=l < 4)
+    assert k == cs.CNN_PARAMS
+    assert np.array_equal(got, x[..., :2]), int((got != x[..., :2]).sum())
+    net.close()

@@ -80,20 +80,68 @@ def test_batch_tunings_agree(ctx, tuning):
             assert np.array_equal(a, c)
 
 
-def test_batch_autotune(ctx):
+def test_batch_autotune(ctx, tmp_path):
+    """The autotuner's per-class table is persisted, applied to a second driver of the
+    same pairs, and both give the one-pair results."""
     pairs = _pairs(900)
-    b = cs.BatchMatcher(ctx, [(w, h) for w, h, _ in SIZES], [d for _, _, d in SIZES], algorithm=cs.ALGO_CENSUS,
-                        window=(9, 9), census=(9, 7))
+    sizes, ds = [(w, h) for w, h, _ in SIZES], [d for _, _, d in SIZES]
+    b = cs.BatchMatcher(ctx, sizes, ds, algorithm=cs.ALGO_CENSUS, window=(9, 9), census=(9, 7))
     b.upload(pairs)
-    t = b.autotune()
-    print(f"autotuned {t}", flush=True)
+    path = tmp_path / "tuning.json"
+    t = b.autotune(table_path=str(path))
+    table = b.tuning_table()
+    print(f"autotuned {t}, table {table}", flush=True)
     assert t["chunk"] in (32, 64, 128, 256) and t["order"] in (0, 1) and t == b.tuning()
+    assert table["matcher"] == "census" and table["chunk"] == t["chunk"] and table["grid"] == t["grid"]
+    # every pair here has under 64 tiles: one class per disparity count
+    assert sorted(c["disparities"] for c in table["classes"]) == sorted(set(ds))
+    assert all(c["tiles"] == "small" and c["tile"] == "64x56" and c["chunk"] in (32, 64, 128, 256)
+               for c in table["classes"])
     b.process()
     got = b.results()
     want = _singles(ctx, pairs, cs.ALGO_CENSUS, (9, 9), (9, 7))
     for g, w in zip(got, want):
         for a, c in zip(g, w):
             assert np.array_equal(a, c)
+    b2 = cs.BatchMatcher(ctx, sizes, ds, algorithm=cs.ALGO_CENSUS, window=(9, 9), census=(9, 7))
+    b2.apply_tuning_table(str(path))
+    assert b2.tuning_table() == table
+    for g, w in zip(b2.run(pairs), want):
+        for a, c in zip(g, w):
+            assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("mix", [{64: 64, 128: 128, 192: 96, 256: 128}, {64: 32, 128: 128, 192: 192, 256: 256}])
+def test_batch_per_class_items(ctx, mix):
+    """Pairs of one batch cut into work items of different lengths per disparity count:
+    bit-exact against the one-pair matcher."""
+    algo, window, census = cs.ALGO_SAD, (7, 7), (1, 1)
+    pairs = _pairs(1000)
+    b = cs.BatchMatcher(ctx, [(w, h) for w, h, _ in SIZES], [d for _, _, d in SIZES], algorithm=algo, window=window,
+                        census=census, tuning={"chunk": 64, "grid": 512, "order": 0, "use_graph": 1})
+    table = b.tuning_table()
+    table["classes"] = [{"disparities": d, "tiles": "small", "chunk": c} for d, c in mix.items()]
+    b.apply_tuning_table(table)
+    got_table = b.tuning_table()
+    assert {c["disparities"]: c["chunk"] for c in got_table["classes"]} == mix
+    got = b.run(pairs)
+    want = _singles(ctx, pairs, algo, window, census)
+    for i, (g, w) in enumerate(zip(got, want)):
+        for a, c in zip(g, w):
+            assert np.array_equal(a, c), f"pair {i}"
+
+
+def test_batch_refuses_bad_tables(ctx):
+    b = cs.BatchMatcher(ctx, [(320, 240), (256, 160)], [64, 128], algorithm=cs.ALGO_SAD, window=(7, 7))
+    keep = b.tuning_table()
+    bad = [dict(keep, chunk=12), dict(keep, classes=[{"disparities": 64, "tiles": "huge", "chunk": 64}]),
+           dict(keep, classes=[{"disparities": 64, "chunk": 64}]),
+           dict(keep, classes=[{"disparities": 64, "tiles": "small", "chunk": 300}]), dict(keep, grid=0)]
+    for t in bad:
+        with pytest.raises(cs.StereoError) as e:
+            b.apply_tuning_table(t)
+        assert e.value.status == cs.ERROR_INVALID_CONFIG
+        assert b.tuning_table() == keep
 
 
 def test_batch_rejects_bad_config(ctx):
