@@ -219,13 +219,22 @@ def batch_leg(cs, np, ctx, steps):
         b = cs.BatchMatcher(ctx, sizes, disps, algorithm=algo, window=win, census=cen)
         b.upload(pairs)
         tuning = b.autotune()
-        b.process()
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
+        table = b.tuning_table()
+
+        def timed():
             b.process()
-        ctx.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / steps
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                b.process()
+            ctx.synchronize()
+            return (time.perf_counter() - t0) * 1e3 / steps
+
+        # the autotuned per-class item lengths against the one length it found for all pairs
+        b.apply_tuning_table(dict(table, classes=[]))
+        ms_one_length = timed()
+        b.apply_tuning_table(table)
+        ms = timed()
         got = b.results()
         # one pair at a time, same images resident
         singles, exact = [], 0
@@ -243,6 +252,8 @@ def batch_leg(cs, np, ctx, steps):
             exact += int(np.array_equal(o.download(), got[i][0]))
         out[name] = {"ms_per_batch": round(ms, 3), "M_evals_per_s": round(evals / (ms * 1e-3) / 1e6, 1),
                      "pairs_per_s": round(64 / (ms * 1e-3), 1), "tuning": tuning,
+                     "tuning_classes": {f'{c["disparities"]}/{c["tiles"]}': c["chunk"] for c in table["classes"]},
+                     "tile": "64x56", "ms_one_item_length": round(ms_one_length, 3),
                      "one_at_a_time_ms": round(t_single, 3), "speedup_vs_one_at_a_time": round(t_single / ms, 3),
                      "pairs_bit_exact_vs_one_at_a_time": f"{exact}/64"}
     return out

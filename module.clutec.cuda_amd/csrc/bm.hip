@@ -309,17 +309,20 @@ __global__ __launch_bounds__(256, Cost::LIST_WG) void bm_list_kernel(BmListParam
     }
 }
 
+// TPS: the most tiles a segment's left pixels xr + ds .. xr + de - 1 can span,
+// (seg_len - 1) / 64 + 2, at most 5 (seg_len <= 256).
+template <int TPS>
 __device__ __forceinline__ uint32_t rkey_at(const BmFinishParams& f, int xr, int y)
 {
     // right pixel xr meets left pixels xr+d; segment s covers d in [ds, de), and the
     // tile holding left pixel x keeps the partial under index xr - x0 + 255.  A
-    // segment spans at most 5 tiles (seg_len <= 256).  Two segments' loads are issued
+    // segment spans at most TPS tiles.  Two segments' loads are issued
     // together and unconditionally: past the segment's last tile a load repeats that
     // tile (the max is unchanged), and a segment with no tile reads a fixed address
     // whose value is discarded.
     uint32_t best = 0;
     for (int s0 = 0; s0 < f.nseg; s0 += 2) {
-        uint32_t v[2][5];
+        uint32_t v[2][TPS];
         bool any[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -331,7 +334,7 @@ __device__ __forceinline__ uint32_t rkey_at(const BmFinishParams& f, int xr, int
             any[q] = tlo <= thi;
             const uint32_t* base = f.rpart + ((size_t)s * f.ntx * f.H + y) * BM_NPART + (xr + 255);
 #pragma unroll
-            for (int j = 0; j < 5; ++j) {
+            for (int j = 0; j < TPS; ++j) {
                 const int t = min(tlo + j, thi);
                 const uint32_t* a = any[q] ? base + ((size_t)t * f.H * BM_NPART - 64 * t) : f.rpart;
                 v[q][j] = *a;
@@ -340,7 +343,7 @@ __device__ __forceinline__ uint32_t rkey_at(const BmFinishParams& f, int xr, int
 #pragma unroll
         for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int j = 0; j < 5; ++j) best = (any[q] && v[q][j] > best) ? v[q][j] : best;
+            for (int j = 0; j < TPS; ++j) best = (any[q] && v[q][j] > best) ? v[q][j] : best;
     }
     return best;
 }
@@ -396,6 +399,7 @@ __device__ __forceinline__ uint32_t lkey_at(const BmFinishParams& f, int x, int 
     return lk;
 }
 
+template <int TPS>
 __global__ __launch_bounds__(256) void bm_finish_kernel(BmFinishParams f)
 {
     extern __shared__ uint32_t rrow[];
@@ -437,7 +441,7 @@ __global__ __launch_bounds__(256) void bm_finish_kernel(BmFinishParams f)
 #pragma unroll 2
     for (int x = threadIdx.x; x < W; x += 256) {
         const bool col_ok = x >= f.bx && x < W - f.bx;
-        const uint32_t r = rkey_at(f, x, y);
+        const uint32_t r = rkey_at<TPS>(f, x, y);
         rrow[x] = (row_ok && col_ok) ? r : 0u;
     }
     __syncthreads();
@@ -634,10 +638,22 @@ void bm_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len)
     *ntx = (W + bm::TW - 1) / bm::TW;
     *nty = (H + bm::TH - 1) / bm::TH;
     // Split the disparity range so that the launch holds enough equal workgroups for
-    // the 256 CUs (3 resident each) to finish together: about 1200.  Every extra
-    // segment costs one more set of partial keys, so no more than that.  Each
-    // segment is a multiple of 8 disparities.
+    // the 256 CUs (3 resident each) to finish together, while every extra segment
+    // costs the merge one more plane of left keys and partial keys.  Segments of whole
+    // 64-disparity blocks (the kernel parks its right-view emissions in blocks of 64):
+    // the shortest such length whose launch stays within 4800 workgroups, when that
+    // launch has at least 1200 (1080p D=256: 4 x 64; 1080p D=128: 2 x 64; 2160p
+    // D=256: 2 x 128).  Smaller frames: about 1200 workgroups in segments of a
+    // multiple of 8 disparities.
     const int tiles = *ntx * *nty;
+    for (int len = 64; len <= 256 && D >= 64; len *= 2) {
+        const long wg = (long)tiles * ((D + len - 1) / len);
+        if (wg > 4800 && len < 256 && len < D) continue;
+        if (wg < 1200) break;
+        *seg_len = len;
+        *nseg = (D + len - 1) / len;
+        return;
+    }
     int n = (1200 + tiles - 1) / tiles;
     if (n > D / 8) n = D / 8;
     if (n > BM_MAX_SEG) n = BM_MAX_SEG;
@@ -666,8 +682,13 @@ hipError_t launch_bm(int kind, int ww, const BmParams& p, hipStream_t s)
 
 hipError_t launch_bm_finish(const BmFinishParams& f, hipStream_t s)
 {
-    if (f.W > BM_MAX_WIDTH) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(bm::bm_finish_kernel, dim3(f.H), dim3(256), (size_t)f.W * sizeof(uint32_t), s, f);
+    if (f.W > BM_MAX_WIDTH || f.seg_len < 1 || f.seg_len > 256) return hipErrorInvalidValue;
+    const int tps = (f.seg_len - 1) / bm::TW + 2;
+    const size_t lds = (size_t)f.W * sizeof(uint32_t);
+    if (tps <= 2) hipLaunchKernelGGL(bm::bm_finish_kernel<2>, dim3(f.H), dim3(256), lds, s, f);
+    else if (tps == 3) hipLaunchKernelGGL(bm::bm_finish_kernel<3>, dim3(f.H), dim3(256), lds, s, f);
+    else if (tps == 4) hipLaunchKernelGGL(bm::bm_finish_kernel<4>, dim3(f.H), dim3(256), lds, s, f);
+    else hipLaunchKernelGGL(bm::bm_finish_kernel<5>, dim3(f.H), dim3(256), lds, s, f);
     return hipGetLastError();
 }
 
