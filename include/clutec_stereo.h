@@ -210,11 +210,11 @@ stereo_status stereo_matcher_stage_times(stereo_driver* drv, double* ms, int32_t
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
  * stereo_matcher_process for the SAD and census matchers with no configured driver and no
  * allocation.  params as for stereo_matcher_configure (algorithm SAD or CENSUS,
- * use_disp_input 0; the frame size is the left image's).  The padded inputs, left keys and
- * right-view partial keys live in a library-owned device array of
- * STEREO_NOALLOC_SCRATCH_BYTES per GPU, set up by stereo_create and held until the process
- * ends; frames whose workspace exceeds it (1920x1080 census, for instance; 1280x720 census
- * and 1920x1080 SAD fit at D = 256) are STEREO_ERROR_INVALID_FORMAT.  Outputs equal
+ * use_disp_input 0; the frame size is the left image's).  The padded inputs and the two key
+ * planes live in a library-owned device array of STEREO_NOALLOC_SCRATCH_BYTES per GPU, set
+ * up by stereo_create and held until the process ends; frames whose workspace exceeds it
+ * (2560x1440 census, for instance; 1920x1080 census and 3200x1800 SAD fit, at any D) are
+ * STEREO_ERROR_INVALID_FORMAT.  Outputs equal
  * stereo_matcher_process's.  Every call on one GPU shares the array: calls that overlap on
  * one GPU (two streams, two threads) race; calls on one stream are ordered. */
 #define STEREO_NOALLOC_SCRATCH_BYTES (64u << 20)

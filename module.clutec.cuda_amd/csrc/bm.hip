@@ -75,13 +75,21 @@ struct CostSad {
 };
 
 // One workgroup's work: a 64 x 56 tile of one frame and a disparity range [ds, de).
+// Keys go out in one of two ways:
+//   partials  right pixel xr's partial key of tile row r at rp[r * rp_stride + xr - x0 +
+//             255 - rp_kofs], the left key of tile pixel (c, r) at lk[r * lk_stride + c];
+//             a later merge takes the maximum over the items and segments;
+//   planes    (ATOM) one W x H plane of right keys (rp = its row y0, rp_stride = W) and one
+//             of left keys (lk = pixel (x0, y0), lk_stride = W), merged in place by
+//             atomicMax: every segment of a tile folds its keys into the same planes,
+//             which the merge kernel reads and clears for the next frame.
 struct Job {
     View L, R;
     int W, H, bx;
     int x0, y0, ds, de;
-    uint32_t* rp;   // partial keys of tile row 0: right pixel xr at rp[row * rp_stride + xr - x0 + 255 - rp_kofs]
+    uint32_t* rp;
     int rp_stride, rp_kofs;
-    uint32_t* lk;   // left key of tile pixel (0, 0) (row stride lk_stride)
+    uint32_t* lk;
     int lk_stride;
 };
 
@@ -91,7 +99,7 @@ __device__ __forceinline__ bool job_is_edge(const Job& j)
 }
 
 // tid: the thread index (threadIdx.x, or an opaque copy of it, see bm_list_kernel)
-template <class Cost, int AX, int AY, bool EDGE>
+template <class Cost, int AX, int AY, bool EDGE, bool ATOM>
 __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64],
                                         int tid)
 {
@@ -119,7 +127,7 @@ __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * H
     const int gw = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x = x0 + lane;
     const int rows_here = H - (y0 + gw * RG);  // output rows of this wave inside the image
-    uint32_t* const rp = p.rp + (size_t)gw * RG * p.rp_stride - p.rp_kofs;
+    uint32_t* const rp = ATOM ? p.rp + (size_t)gw * RG * p.rp_stride : p.rp + (size_t)gw * RG * p.rp_stride - p.rp_kofs;
     uint32_t* const eb = &ebuf[gw][0][0];
     const bool lane_ok = x >= p.bx && x < W - p.bx;
     const int dlim = x - p.bx;
@@ -201,16 +209,35 @@ __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * H
                 const int base = dl & ~63;
                 if (lane <= (dl & 63) && base + lane >= ds) {
                     const int k = 318 - (base + lane);
+                    if (ATOM) {
+                        // right pixel x0 + k - 255, inside the image
+                        const int xr = x0 + k - 255;
+                        if (xr >= 0 && xr < W) {
 #pragma unroll
-                    for (int i = 0; i < RG; ++i)
-                        if (i < rows_here) rp[i * p.rp_stride + k] = eb[i * 64 + lane];
+                            for (int i = 0; i < RG; ++i) {
+                                const uint32_t v = eb[i * 64 + lane];
+                                if (i < rows_here && v) atomicMax(&rp[(size_t)i * p.rp_stride + xr], v);
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < RG; ++i)
+                            if (i < rows_here) rp[i * p.rp_stride + k] = eb[i * 64 + lane];
+                    }
                 }
             }
         }
     }
 
     // what is still in flight belongs to right pixels x0+lane-(de-1)
-    {
+    if (ATOM) {
+        const int xr = x0 + lane - (de - 1);
+        if (xr >= 0 && xr < W) {
+#pragma unroll
+            for (int i = 0; i < RG; ++i)
+                if (i < rows_here && racc[i]) atomicMax(&rp[(size_t)i * p.rp_stride + xr], racc[i]);
+        }
+    } else {
         const int k = lane + 255 - (de - 1);
 #pragma unroll
         for (int i = 0; i < RG; ++i)
@@ -219,19 +246,25 @@ __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * H
     if (x < W) {
         uint32_t* lk = p.lk + (size_t)gw * RG * p.lk_stride + lane;
 #pragma unroll
-        for (int i = 0; i < RG; ++i)
-            if (i < rows_here) lk[(size_t)i * p.lk_stride] = lacc[i];
+        for (int i = 0; i < RG; ++i) {
+            if (i >= rows_here) continue;
+            if (ATOM) {
+                if (lacc[i]) atomicMax(&lk[(size_t)i * p.lk_stride], lacc[i]);
+            } else {
+                lk[(size_t)i * p.lk_stride] = lacc[i];
+            }
+        }
     }
 }
 
-template <class Cost, int AX, int AY>
+template <class Cost, int AX, int AY, bool ATOM>
 __device__ __forceinline__ void run_job(const Job& j, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64],
                                         int tid)
 {
     if (job_is_edge(j))
-        bm_tile<Cost, AX, AY, true>(j, hbuf, ebuf, tid);
+        bm_tile<Cost, AX, AY, true, ATOM>(j, hbuf, ebuf, tid);
     else
-        bm_tile<Cost, AX, AY, false>(j, hbuf, ebuf, tid);
+        bm_tile<Cost, AX, AY, false, ATOM>(j, hbuf, ebuf, tid);
 }
 
 // Grid launch over one frame: one workgroup per (tile x, tile y, disparity segment),
@@ -256,12 +289,13 @@ __global__ __launch_bounds__(256, 2) void bm_fused_kernel(BmParams p)
     j.y0 = ty * TH;
     j.ds = seg * p.seg_len;
     j.de = min(p.D, j.ds + p.seg_len);
-    j.rp = p.rpart + (((size_t)seg * p.ntx + tx) * p.H + j.y0) * BM_NPART;
-    j.rp_stride = BM_NPART;
+    // the key planes: every segment of the tile folds into the same rows
+    j.rp = p.rkey + (size_t)j.y0 * p.W;
+    j.rp_stride = p.W;
     j.rp_kofs = 0;
-    j.lk = p.lkey + (size_t)seg * p.W * p.H + (size_t)j.y0 * p.W + j.x0;
+    j.lk = p.lkey + (size_t)j.y0 * p.W + j.x0;
     j.lk_stride = p.W;
-    run_job<Cost, AX, AY>(j, hbuf, ebuf, threadIdx.x);
+    run_job<Cost, AX, AY, true>(j, hbuf, ebuf, threadIdx.x);
 }
 
 // Persistent launch over a work list: each workgroup takes items blockIdx.x,
@@ -305,7 +339,7 @@ __global__ __launch_bounds__(256, Cost::LIST_WG) void bm_list_kernel(BmListParam
         // registers that stay live through the whole item
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        run_job<Cost, AX, AY>(j, hbuf, ebuf, tid);
+        run_job<Cost, AX, AY, false>(j, hbuf, ebuf, tid);
     }
 }
 
@@ -457,6 +491,59 @@ __global__ __launch_bounds__(256) void bm_finish_kernel(BmFinishParams f)
         write_pixel(f.disp, f.disp_l, f.disp_r, f.dispid, x, y, inside, inside ? lkey_at(f, x, y) : 0u, rrow, f.tol,
                     f.value_base);
     }
+}
+
+// Check of the key planes that the grid matcher folds its segments into with
+// atomicMax: one workgroup per image row reads the row's right keys into LDS and every
+// left pixel looks up the right winner at x - dL.  Each key is cleared by the thread
+// that read it, so the planes start the next frame at zero.
+__global__ __launch_bounds__(256) void bm_finish_planes_kernel(BmFinishParams f)
+{
+    extern __shared__ uint32_t rrow[];
+    const int y = blockIdx.x;
+    const int W = f.W;
+    const bool row_ok = y >= f.by && y < f.H - f.by;
+    uint32_t* const lrow = f.lplane + (size_t)y * W;
+    uint32_t* const rrowg = f.rplane + (size_t)y * W;
+    uint32_t lpre[FIN_PF];
+#pragma unroll
+    for (int k = 0; k < FIN_PF; ++k) {
+        const int x = threadIdx.x + 256 * k;
+        lpre[k] = 0u;
+        if (x < W) {
+            lpre[k] = lrow[x];
+            lrow[x] = 0u;
+        }
+    }
+    for (int x = threadIdx.x; x < W; x += 256) {
+        const bool col_ok = x >= f.bx && x < W - f.bx;
+        const uint32_t r = rrowg[x];
+        rrowg[x] = 0u;
+        rrow[x] = (row_ok && col_ok) ? r : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < FIN_PF; ++k) {
+        const int x = threadIdx.x + 256 * k;
+        if (x < W) {
+            const bool inside = row_ok && x >= f.bx && x < W - f.bx;
+            write_pixel(f.disp, f.disp_l, f.disp_r, f.dispid, x, y, inside, inside ? lpre[k] : 0u, rrow, f.tol,
+                        f.value_base);
+        }
+    }
+    for (int x = threadIdx.x + 256 * FIN_PF; x < W; x += 256) {
+        const bool inside = row_ok && x >= f.bx && x < W - f.bx;
+        const uint32_t lk = lrow[x];
+        lrow[x] = 0u;
+        write_pixel(f.disp, f.disp_l, f.disp_r, f.dispid, x, y, inside, inside ? lk : 0u, rrow, f.tol, f.value_base);
+    }
+}
+
+// Clears n words (the key planes, where a caller cannot rely on the last frame's merge
+// having cleared them).
+__global__ __launch_bounds__(256) void bm_clear_kernel(uint32_t* p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0u;
 }
 
 // Merge and check for the work-list matcher: one workgroup per frame row.  A right
@@ -680,11 +767,24 @@ hipError_t launch_bm(int kind, int ww, const BmParams& p, hipStream_t s)
     return launch_cost<bm::CostSad>(p, ww, s);
 }
 
+hipError_t launch_bm_clear(uint32_t* p, size_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    const size_t blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+    hipLaunchKernelGGL(bm::bm_clear_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_bm_finish(const BmFinishParams& f, hipStream_t s)
 {
     if (f.W > BM_MAX_WIDTH || f.seg_len < 1 || f.seg_len > 256) return hipErrorInvalidValue;
-    const int tps = (f.seg_len - 1) / bm::TW + 2;
     const size_t lds = (size_t)f.W * sizeof(uint32_t);
+    if (f.lplane) {
+        if (!f.rplane) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(bm::bm_finish_planes_kernel, dim3(f.H), dim3(256), lds, s, f);
+        return hipGetLastError();
+    }
+    const int tps = (f.seg_len - 1) / bm::TW + 2;
     if (tps <= 2) hipLaunchKernelGGL(bm::bm_finish_kernel<2>, dim3(f.H), dim3(256), lds, s, f);
     else if (tps == 3) hipLaunchKernelGGL(bm::bm_finish_kernel<3>, dim3(f.H), dim3(256), lds, s, f);
     else if (tps == 4) hipLaunchKernelGGL(bm::bm_finish_kernel<4>, dim3(f.H), dim3(256), lds, s, f);

@@ -462,9 +462,18 @@ stereo_status stereo_matcher_configure(stereo_context* ctx, stereo_driver** out,
     hipError_t e;
     if ((e = hipMalloc((void**)&d->padL, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
     if ((e = hipMalloc((void**)&d->padR, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
-    if ((e = hipMalloc((void**)&d->lkey, (size_t)d->nseg * W * H * 4)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
-    if ((e = hipMalloc((void**)&d->rpart, (size_t)d->nseg * d->ntx * H * stereo::BM_NPART * 4)) != hipSuccess)
-        return cleanup(hip_fail(ctx, e, "hipMalloc"));
+    if (ncc || aw) {
+        if ((e = hipMalloc((void**)&d->lkey, (size_t)d->nseg * W * H * 4)) != hipSuccess)
+            return cleanup(hip_fail(ctx, e, "hipMalloc"));
+        if ((e = hipMalloc((void**)&d->rpart, (size_t)d->nseg * d->ntx * H * stereo::BM_NPART * 4)) != hipSuccess)
+            return cleanup(hip_fail(ctx, e, "hipMalloc"));
+    } else {
+        // the key planes start at zero; every frame's merge leaves them so
+        if ((e = hipMalloc((void**)&d->lkey, W * H * 4)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
+        if ((e = hipMalloc((void**)&d->rkey, W * H * 4)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
+        if ((e = hipMemset(d->lkey, 0, W * H * 4)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMemset"));
+        if ((e = hipMemset(d->rkey, 0, W * H * 4)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMemset"));
+    }
     if (ncc) {
         if ((e = hipMalloc((void**)&d->lstats, W * H * sizeof(int2))) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
         if ((e = hipMalloc((void**)&d->rstats, W * H * sizeof(int2))) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
@@ -550,11 +559,17 @@ static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool disp
         ap.bins_nty = bins_nty;
         HIP_TRY(ctx, stereo::launch_aw(d->mp.window_w, ap, s), "adaptive-window launch");
     } else {
+        if (d->planes_dirty) {
+            // an earlier frame's merge did not run: clear what its matcher folded in
+            HIP_TRY(ctx, stereo::launch_bm_clear(d->lkey, (size_t)W * H, s), "clear launch");
+            HIP_TRY(ctx, stereo::launch_bm_clear(d->rkey, (size_t)W * H, s), "clear launch");
+            d->planes_dirty = false;
+        }
         stereo::BmParams bp;
         bp.L = pl;
         bp.R = pr;
         bp.lkey = d->lkey;
-        bp.rpart = d->rpart;
+        bp.rkey = d->rkey;
         bp.W = W;
         bp.H = H;
         bp.D = d->mp.num_disparities;
@@ -563,6 +578,8 @@ static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool disp
         bp.ntx = d->ntx;
         bp.nseg = d->nseg;
         bp.seg_len = d->seg_len;
+        // from the matcher launch on, the planes hold keys until the merge has run
+        d->planes_dirty = true;
         HIP_TRY(ctx, stereo::launch_bm(census ? stereo::BM_COST_CENSUS : stereo::BM_COST_SAD, d->mp.window_w, bp, s),
                 "matcher launch");
     }
@@ -570,6 +587,10 @@ static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool disp
     stereo::BmFinishParams f;
     f.lkey = d->lkey;
     f.rpart = d->rpart;
+    if (d->rkey) {
+        f.lplane = d->lkey;
+        f.rplane = d->rkey;
+    }
     f.disp = (disp && !dispid) ? view_of(disp) : null_view();
     f.dispid = (disp && dispid) ? view_of(disp) : null_view();
     f.disp_l = disp_lr ? view_of(disp_lr) : null_view();
@@ -585,6 +606,7 @@ static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool disp
     f.tol = d->mp.lr_tolerance;
     f.value_base = d->mp.algorithm == STEREO_ALGO_AW1 ? 255 * 3 * d->mp.window_w * d->mp.window_w : 0;
     HIP_TRY(ctx, stereo::launch_bm_finish(f, s), "finish launch");
+    if (d->rkey) d->planes_dirty = false;
     return end(d, s);
 }
 

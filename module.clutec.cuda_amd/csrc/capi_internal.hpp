@@ -154,8 +154,13 @@ struct stereo_driver {
     uint8_t* padR = nullptr;
     size_t pad_pitch = 0;
     size_t pad_elem = 0;
+    // keys: per disparity segment for NCC and the adaptive windows (lkey, rpart); one
+    // plane each for SAD and census (lkey, rkey), cleared by every frame's merge --
+    // planes_dirty when a frame's merge did not run after its matcher
     uint32_t* lkey = nullptr;
     uint32_t* rpart = nullptr;
+    uint32_t* rkey = nullptr;
+    bool planes_dirty = false;
     int2* lstats = nullptr;  // NCC window statistics
     int2* rstats = nullptr;
     // crosscheck
@@ -192,6 +197,7 @@ inline void driver_release(stereo_driver* d)
     if (d->padR) (void)hipFree(d->padR);
     if (d->lkey) (void)hipFree(d->lkey);
     if (d->rpart) (void)hipFree(d->rpart);
+    if (d->rkey) (void)hipFree(d->rkey);
     if (d->lstats) (void)hipFree(d->lstats);
     if (d->rstats) (void)hipFree(d->rstats);
     if (d->range_dev) (void)hipFree(d->range_dev);

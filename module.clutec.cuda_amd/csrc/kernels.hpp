@@ -25,16 +25,20 @@ constexpr int BM_MAX_WIDTH = 16384; // one image row of right-view keys fits in 
 
 struct BmParams {
     View L, R;         // uint8 grey (SAD) or uint64 census (CENSUS)
-    uint32_t* lkey;    // nseg*W*H best left keys per disparity segment (pitch W)
-    uint32_t* rpart;   // nseg*ntx*H*BM_NPART right-view partial keys
+    uint32_t* lkey;    // W*H best left keys (pitch W), all zero at the launch
+    uint32_t* rkey;    // W*H best right keys (pitch W), all zero at the launch
     int W, H, D;
     int bx, by;        // matchable border
     int ntx, nseg, seg_len;
 };
 
 struct BmFinishParams {
+    // the NCC and adaptive-window matchers' keys: per disparity segment
     const uint32_t* lkey;   // nseg*W*H
     const uint32_t* rpart;  // nseg*ntx*H*BM_NPART
+    // the SAD / census matcher's key planes (lplane != nullptr): read, then cleared
+    uint32_t* lplane = nullptr;  // W*H
+    uint32_t* rplane = nullptr;  // W*H
     View disp;     // int16 checked map (optional: ptr == nullptr)
     View disp_l;   // int16 left map (optional)
     View disp_r;   // int16 right map (optional)
@@ -49,6 +53,8 @@ bool bm_supported(int kind, int ww, int wh, int D);
 void bm_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len);
 hipError_t launch_bm(int kind, int ww, const BmParams& p, hipStream_t s);
 hipError_t launch_bm_finish(const BmFinishParams& f, hipStream_t s);
+// Clears n words of key planes (on a stream; safe under graph capture).
+hipError_t launch_bm_clear(uint32_t* p, size_t n, hipStream_t s);
 
 // Device address of the no-allocation entry point's scratch array on the current device,
 // looked up once per device (csrc/noalloc.hip); called by stereo_create.

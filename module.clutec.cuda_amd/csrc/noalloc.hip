@@ -2,18 +2,16 @@
 // under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
 //
 // stereo_matcher_process runs on a configured driver whose workspace -- the padded
-// inputs, one left winner key per pixel and disparity segment, and the right-view
-// partial keys of every tile -- was allocated by stereo_matcher_configure.  This entry
-// point takes the same parameters per call and places that workspace in `scratch`, a
-// file-scope device array of STEREO_NOALLOC_SCRATCH_BYTES that every GPU the library
-// runs on holds until the process ends.  stereo_create looks up the array's device
-// address once per device (hipGetSymbolAddress with that device current); host code
-// uses that address, never the array's name.  The disparity range is cut into as many
-// segments as the driver would use, or fewer until the workspace fits; the maps do not
-// depend on the segmentation.  Nothing in the workspace needs to start at zero: every
-// left key and partial key is written by the matching launch before the merge reads it,
-// and the pad columns of the inputs only meet masked candidates.  All calls on one GPU
-// share the array, so calls that overlap on one GPU race; calls on one stream are ordered.
+// inputs and the two key planes (the best left and right keys of every pixel) -- was
+// allocated by stereo_matcher_configure.  This entry point takes the same parameters
+// per call and places that workspace in `scratch`, a file-scope device array of
+// STEREO_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until the process
+// ends.  stereo_create looks up the array's device address once per device
+// (hipGetSymbolAddress with that device current); host code uses that address, never the
+// array's name.  The key planes must start at zero, and the array keeps whatever the last
+// call of any size left, so every call clears them with a kernel first; the pad columns
+// of the inputs only meet masked candidates.  All calls on one GPU share the array, so
+// calls that overlap on one GPU race; calls on one stream are ordered.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -30,27 +28,21 @@ void* g_scratch[kMaxDevices] = {};
 
 struct Layout {
     int ntx, nty, nseg, seg_len;
-    size_t pad_elem, pad_pitch, pad_bytes, lkey_bytes, rpart_bytes;
-    size_t total() const { return 2 * pad_bytes + lkey_bytes + rpart_bytes; }
+    size_t pad_elem, pad_pitch, pad_bytes, plane_bytes;
+    size_t total() const { return 2 * pad_bytes + 2 * plane_bytes; }
 };
 
-Layout layout_for(int W, int H, int D, bool census, int nseg_cap)
+// The driver's segmentation and workspace for a frame: two padded inputs, then the left
+// and the right key plane.
+Layout layout_for(int W, int H, int D, bool census)
 {
     Layout l{};
     stereo::bm_grid(W, H, D, &l.ntx, &l.nty, &l.nseg, &l.seg_len);
-    if (nseg_cap < l.nseg) {
-        // fewer, longer segments (multiples of 8 disparities)
-        int len = (D + nseg_cap - 1) / nseg_cap;
-        len = (len + 7) & ~7;
-        l.seg_len = len;
-        l.nseg = (D + len - 1) / len;
-    }
     l.pad_elem = census ? 8 : 1;
     const size_t cols = (size_t)stereo::BM_PAD_LEFT + W + stereo::BM_PAD_RIGHT;
     l.pad_pitch = (cols * l.pad_elem + 255) & ~(size_t)255;
     l.pad_bytes = l.pad_pitch * H;
-    l.lkey_bytes = ((size_t)l.nseg * W * H * 4 + 255) & ~(size_t)255;
-    l.rpart_bytes = (size_t)l.nseg * l.ntx * H * stereo::BM_NPART * 4;
+    l.plane_bytes = ((size_t)W * H * 4 + 255) & ~(size_t)255;
     return l;
 }
 
@@ -104,8 +96,7 @@ stereo_status stereo_matcher_process_noalloc(stereo_context* ctx, const stereo_m
     if (disp_rl && (st = check_image(ctx, disp_rl, W, H, STEREO_PIXEL_LUM_I16, "right-to-left output"))) return st;
 
     const bool census = p->algorithm == STEREO_ALGO_CENSUS;
-    Layout lay = layout_for(W, H, D, census, 1 << 30);
-    while (lay.total() > STEREO_NOALLOC_SCRATCH_BYTES && lay.nseg > 1) lay = layout_for(W, H, D, census, lay.nseg - 1);
+    const Layout lay = layout_for(W, H, D, census);
     if (lay.total() > STEREO_NOALLOC_SCRATCH_BYTES)
         return fail(ctx, STEREO_ERROR_INVALID_FORMAT,
                     "stereo_matcher_process_noalloc: a %dx%d %s frame needs %zu bytes of workspace, more than the %u "
@@ -126,7 +117,9 @@ stereo_status stereo_matcher_process_noalloc(stereo_context* ctx, const stereo_m
     uint8_t* padL = base;
     uint8_t* padR = padL + lay.pad_bytes;
     uint32_t* lkey = reinterpret_cast<uint32_t*>(padR + lay.pad_bytes);
-    uint32_t* rpart = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(lkey) + lay.lkey_bytes);
+    uint32_t* rkey = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(lkey) + lay.plane_bytes);
+    // both planes at once: they are contiguous
+    HIP_TRY(ctx, stereo::launch_bm_clear(lkey, 2 * lay.plane_bytes / 4, s), "clear launch");
     View pl, pr;
     pl.pitch = pr.pitch = lay.pad_pitch;
     pl.w = pr.w = W;
@@ -148,7 +141,7 @@ stereo_status stereo_matcher_process_noalloc(stereo_context* ctx, const stereo_m
     bp.L = pl;
     bp.R = pr;
     bp.lkey = lkey;
-    bp.rpart = rpart;
+    bp.rkey = rkey;
     bp.W = W;
     bp.H = H;
     bp.D = D;
@@ -160,8 +153,10 @@ stereo_status stereo_matcher_process_noalloc(stereo_context* ctx, const stereo_m
     HIP_TRY(ctx, stereo::launch_bm(census ? stereo::BM_COST_CENSUS : stereo::BM_COST_SAD, p->window_w, bp, s),
             "matcher launch");
     stereo::BmFinishParams f;
-    f.lkey = lkey;
-    f.rpart = rpart;
+    f.lkey = nullptr;
+    f.rpart = nullptr;
+    f.lplane = lkey;
+    f.rplane = rkey;
     f.disp = (disp && !dispid) ? view_of(disp) : null_view();
     f.dispid = (disp && dispid) ? view_of(disp) : null_view();
     f.disp_l = disp_lr ? view_of(disp_lr) : null_view();

@@ -44,7 +44,7 @@ def _configured(ctx, algo, window, census, D, L, R, dispid=False):
 
 @pytest.mark.parametrize("algo,window,census", [(cs.ALGO_CENSUS, (9, 9), (9, 7)), (cs.ALGO_SAD, (9, 9), (1, 1))])
 def test_noalloc_equals_configured_driver_on_two_calls(ctx, algo, window, census):
-    for k, (W, H, D, seed, dispid) in enumerate([(1280, 720, 256, 11, False), (333, 197, 64, 12, True)]):
+    for k, (W, H, D, seed, dispid) in enumerate([(1920, 1080, 256, 11, False), (333, 197, 64, 12, True)]):
         print(f"noalloc algo {algo} call {k}: {W}x{H} D={D}", flush=True)
         L, R = cs.synth_pair(seed, W, H, 0, 8, D - 1, 8)
         params = cs.matcher_params(algo, window, census, D, 1, False)
@@ -56,7 +56,7 @@ def test_noalloc_equals_configured_driver_on_two_calls(ctx, algo, window, census
 
 
 def test_noalloc_refuses_what_does_not_fit(ctx):
-    W, H, D = 1920, 1080, 256
+    W, H, D = 2560, 1440, 256
     L = np.zeros((H, W), np.uint8)
     params = cs.matcher_params(cs.ALGO_CENSUS, (9, 9), (9, 7), D, 1, False)
     dl = cs.DeviceImage.from_array(ctx, L, cs.PIXEL_LUM_U8)
