@@ -274,11 +274,12 @@ __global__ __launch_bounds__(256, 2) void bm_fused_kernel(BmParams p)
 {
     __shared__ uint32_t hbuf[2][HROWS * HSTRIDE];
     __shared__ uint32_t ebuf[4][RG][64];
-    // one-dimensional grid of ntx * nty * nseg workgroups, tile x fastest
-    const int nty = (p.H + TH - 1) / TH;
+    // one-dimensional grid of ntx * nty * nseg workgroups
     const int w = xcd_remap(blockIdx.x, gridDim.x);
-    const int tx = w % p.ntx, rest = w / p.ntx;
-    const int ty = rest % nty, seg = rest / nty;
+    // segment fastest: the segments of a tile run together on one XCD, so their atomics
+    // on the same key-plane lines meet in its L2, as do the census rows they read
+    const int seg = w % p.nseg, rest = w / p.nseg;
+    const int tx = rest % p.ntx, ty = rest / p.ntx;
     Job j;
     j.L = p.L;
     j.R = p.R;
@@ -724,29 +725,16 @@ void bm_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len)
 {
     *ntx = (W + bm::TW - 1) / bm::TW;
     *nty = (H + bm::TH - 1) / bm::TH;
-    // Split the disparity range so that the launch holds enough equal workgroups for
-    // the 256 CUs (3 resident each) to finish together, while every extra segment
-    // costs the merge one more plane of left keys and partial keys.  Segments of whole
-    // 64-disparity blocks (the kernel parks its right-view emissions in blocks of 64):
-    // the shortest such length whose launch stays within 4800 workgroups, when that
-    // launch has at least 1200 (1080p D=256: 4 x 64; 1080p D=128: 2 x 64; 2160p
-    // D=256: 2 x 128).  Smaller frames: about 1200 workgroups in segments of a
-    // multiple of 8 disparities.
+    // Split the disparity range so that the launch holds enough workgroups for the 256
+    // CUs (3 resident each) to finish together.  Every segment folds into the same key
+    // planes, so the merge does not grow with the count; what a segment costs is its
+    // set-up and its atomics.  Measured best: whole 64-disparity blocks (the kernel parks
+    // its right-view emissions in blocks of 64) when they give at least 1200 workgroups
+    // (1080p D=256: 4 x 64, 2160p), else segments of 16 (640x480 D=64, 720p D=128), where
+    // 8 already costs more in set-up than it gains in balance.
     const int tiles = *ntx * *nty;
-    for (int len = 64; len <= 256 && D >= 64; len *= 2) {
-        const long wg = (long)tiles * ((D + len - 1) / len);
-        if (wg > 4800 && len < 256 && len < D) continue;
-        if (wg < 1200) break;
-        *seg_len = len;
-        *nseg = (D + len - 1) / len;
-        return;
-    }
-    int n = (1200 + tiles - 1) / tiles;
-    if (n > D / 8) n = D / 8;
-    if (n > BM_MAX_SEG) n = BM_MAX_SEG;
-    if (n < 1) n = 1;
-    int len = (D + n - 1) / n;
-    len = (len + 7) & ~7;
+    int len = (long)tiles * ((D + 63) / 64) >= 1200 ? 64 : 16;
+    if (len > D) len = D;
     *seg_len = len;
     *nseg = (D + len - 1) / len;
 }
