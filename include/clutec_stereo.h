@@ -262,7 +262,7 @@ typedef struct stereo_batch_tuning {
 
 /* params: algorithm SAD or CENSUS, windows and tolerance as for the matcher;
  * params->num_disparities is ignored, num_disparities[i] gives pair i's D.  Widths up
- * to 11264 (STEREO_ERROR_INVALID_FORMAT above). */
+ * to 16384 (STEREO_ERROR_INVALID_FORMAT above). */
 stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** drv, int32_t npairs,
                                      const stereo_image_format* fmts, const int32_t* num_disparities,
                                      const stereo_matcher_params* params, const stereo_batch_tuning* tuning);
@@ -367,7 +367,7 @@ typedef struct stereo_learned_params {
 } stereo_learned_params;
 
 /* The learned-proposal census matcher for LUM_U8 pairs of fmt (width and height
- * multiples of 4, width up to 11264); cnn_params = STEREO_CNN_PARAMS trained values. */
+ * multiples of 4, width up to 16384); cnn_params = STEREO_CNN_PARAMS trained values. */
 stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** drv, const stereo_image_format* fmt,
                                        const stereo_learned_params* params, const float* cnn_params, size_t n);
 /* Outputs as stereo_matcher_process. */

@@ -35,9 +35,12 @@ struct BatchExt : DriverExt {
     int* counter_dev = nullptr;  // item counter of the list matcher (reset by its finish)
     int* nitems_dev = nullptr;
     int ntiles = 0, nitems = 0, max_items = 0;
-    uint32_t* lkey = nullptr;
-    uint32_t* rpart = nullptr;
-    size_t rpart_elems = 0;
+    // key planes of every pair (left, right), zero at every matcher launch: cleared at
+    // configure and by every merge; planes_dirty when a matcher ran without its merge
+    uint32_t* lplane = nullptr;
+    uint32_t* rplane = nullptr;
+    size_t plane_elems = 0;
+    bool planes_dirty = false;
     int total_rows = 0, max_width = 0;
     hipStream_t cap = nullptr;  // capture stream
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -66,18 +69,10 @@ struct BatchExt : DriverExt {
         for (uint8_t* p : padR)
             if (p) (void)hipFree(p);
         for (void* p : {(void*)frames_dev, (void*)tiles_dev, (void*)items_dev, (void*)sched_dev, (void*)counter_dev, (void*)nitems_dev,
-                        (void*)lkey, (void*)rpart})
+                        (void*)lplane, (void*)rplane})
             if (p) (void)hipFree(p);
     }
 };
-
-size_t rpart_need(const std::vector<stereo::BmFrame>& frames, int chunk)
-{
-    size_t n = 0;
-    for (const stereo::BmFrame& f : frames)
-        n += (size_t)f.ntx * f.nty * ((f.D + chunk - 1) / chunk) * stereo::BM_TILE_H * stereo::bm_list_rp_stride(chunk);
-    return n;
-}
 
 // Tuning classes of the pairs: the disparity count and the tile count (small below 64
 // tiles of 64 x 56, medium below 256, large), the two sizes that decide how a pair's
@@ -93,25 +88,6 @@ int chunk_of(const BatchExt* x, int f)
 {
     const int c = x->pair_chunk.empty() ? 0 : x->pair_chunk[f];
     return c ? c : x->tuning.chunk;
-}
-
-int max_chunk(const BatchExt* x)
-{
-    int m = 0;
-    for (int f = 0; f < x->npairs; ++f) m = std::max(m, chunk_of(x, f));
-    return m;
-}
-
-size_t rpart_need_mixed(const BatchExt* x)
-{
-    const int stride = stereo::bm_list_rp_stride(max_chunk(x));
-    size_t n = 0;
-    for (int f = 0; f < x->npairs; ++f) {
-        const stereo::BmFrame& fr = x->frames[f];
-        const int c = chunk_of(x, f);
-        n += (size_t)fr.ntx * fr.nty * ((fr.D + c - 1) / c) * stereo::BM_TILE_H * stride;
-    }
-    return n;
 }
 
 // Items of every tile of every pair, tile by tile (the slots the merge reads), the
@@ -146,9 +122,9 @@ stereo_status upload_work(BatchExt* x)
             }
     }
     const int n = (int)items.size();
-    if (n > x->max_items || rpart_need_mixed(x) > x->rpart_elems)
-        return fail(x->ctx, STEREO_ERROR_INVALID_CONFIG, "batch: work items of up to %d disparities need more workspace "
-                    "than configured", max_chunk(x));
+    if (n > x->max_items)
+        return fail(x->ctx, STEREO_ERROR_INVALID_CONFIG, "batch: %d work items, more than the %d configured", n,
+                    x->max_items);
     std::vector<int> sched(n);
     std::iota(sched.begin(), sched.end(), 0);
     if (x->tuning.order == 0)
@@ -196,14 +172,26 @@ stereo_status enqueue(stereo_driver* d, BatchExt* x, const stereo_image* left, c
     lp.nitems = x->nitems_dev;
     lp.sched = x->sched_dev;
     lp.counter = x->counter_dev;
-    lp.lkey = x->lkey;
-    lp.rpart = x->rpart;
-    lp.rp_stride = stereo::bm_list_rp_stride(max_chunk(x));
+    lp.lplane = x->lplane;
+    lp.rplane = x->rplane;
     lp.nframes = x->npairs;
     lp.total_rows = x->total_rows;
     const int grid = std::max(1, std::min(x->tuning.grid, x->nitems));
+    x->planes_dirty = true;  // until the merge is enqueued behind the matcher
     HIP_TRY(ctx, stereo::launch_bm_list(x->kind, x->mp.window_w, lp, grid, s), "list matcher launch");
     HIP_TRY(ctx, stereo::launch_bm_list_finish(lp, x->max_width, s), "list finish launch");
+    x->planes_dirty = false;
+    return STEREO_OK;
+}
+
+// A matcher launch without its merge leaves keys in the planes: clear them (a kernel, so
+// that the call may also run under a caller's graph capture).
+stereo_status clear_if_dirty(BatchExt* x, hipStream_t s)
+{
+    if (!x->planes_dirty) return STEREO_OK;
+    HIP_TRY(x->ctx, stereo::launch_bm_clear(x->lplane, x->plane_elems, s), "clear launch");
+    HIP_TRY(x->ctx, stereo::launch_bm_clear(x->rplane, x->plane_elems, s), "clear launch");
+    x->planes_dirty = false;
     return STEREO_OK;
 }
 
@@ -312,6 +300,8 @@ stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** out, i
         f.nty = (H + stereo::BM_TILE_H - 1) / stereo::BM_TILE_H;
         f.tile_base = tiles;
         f.row_base = rows;
+        f.plane_off = x->plane_elems;
+        x->plane_elems += (size_t)W * H;
         tiles += f.ntx * f.nty;
         rows += H;
         max_items += (size_t)f.ntx * f.nty * ((f.D + kMinChunk - 1) / kMinChunk);
@@ -323,16 +313,15 @@ stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** out, i
     x->total_rows = rows;
     x->max_items = (int)max_items;
     x->pair_chunk.assign(npairs, 0);
-    for (int c = kMinChunk; c <= 256; c += 8) x->rpart_elems = std::max(x->rpart_elems, rpart_need(x->frames, c));
     if ((e = hipMalloc((void**)&x->frames_dev, npairs * sizeof(stereo::BmFrame))) ||
         (e = hipMalloc((void**)&x->tiles_dev, tiles * sizeof(stereo::BmTile))) ||
         (e = hipMalloc((void**)&x->items_dev, max_items * sizeof(stereo::BmItem))) ||
         (e = hipMalloc((void**)&x->sched_dev, max_items * sizeof(int))) ||
         (e = hipMalloc((void**)&x->nitems_dev, sizeof(int))) ||
         (e = hipMalloc((void**)&x->counter_dev, sizeof(int))) ||
-        (e = hipMalloc((void**)&x->lkey, stereo::bm_list_lkey_elems((int)max_items) * 4)) ||
-        (e = hipMalloc((void**)&x->rpart, x->rpart_elems * 4)) || (e = hipMemset(x->counter_dev, 0, sizeof(int))) ||
-        (e = hipDeviceSynchronize()))
+        (e = hipMalloc((void**)&x->lplane, x->plane_elems * 4)) || (e = hipMalloc((void**)&x->rplane, x->plane_elems * 4)) ||
+        (e = hipMemset(x->lplane, 0, x->plane_elems * 4)) || (e = hipMemset(x->rplane, 0, x->plane_elems * 4)) ||
+        (e = hipMemset(x->counter_dev, 0, sizeof(int))) || (e = hipDeviceSynchronize()))
         return cleanup(hip_fail(ctx, e, "hipMalloc"));
     if ((e = hipMemcpy(x->frames_dev, x->frames.data(), npairs * sizeof(stereo::BmFrame), hipMemcpyHostToDevice)))
         return cleanup(hip_fail(ctx, e, "hipMemcpy"));
@@ -370,6 +359,7 @@ stereo_status stereo_batch_process(stereo_driver* d, stereo_image* disp, stereo_
                 "hipMemcpy");
         x->drop_graph();
     }
+    if ((st = clear_if_dirty(x, s))) return st;
     if ((st = begin(d, s))) return st;
     if (!x->tuning.use_graph) {
         if ((st = enqueue(d, x, left, right, s))) return st;
@@ -463,6 +453,7 @@ stereo_status stereo_batch_autotune(stereo_driver* d, const stereo_image* left, 
     auto time_it = [&](float* ms) -> stereo_status {
         stereo_status s2 = upload_work(x);
         if (s2) return s2;
+        if ((s2 = clear_if_dirty(x, nullptr))) return s2;
         if ((s2 = enqueue(d, x, left, right, nullptr))) return s2;
         hipError_t e = hipEventRecord(e0, nullptr);
         for (int r = 0; r < 3 && e == hipSuccess && !s2; ++r) s2 = enqueue(d, x, left, right, nullptr);
@@ -488,8 +479,7 @@ stereo_status stereo_batch_autotune(stereo_driver* d, const stereo_image* left, 
                 }
             }
     // then the item length of each class of pairs (disparity count x tile count), the
-    // classes with the most evaluations first; a length whose partial keys do not fit the
-    // workspace is skipped
+    // classes with the most evaluations first
     std::vector<int> win_chunks(x->npairs, 0);
     if (!st) {
         x->tuning = win;
@@ -512,13 +502,11 @@ stereo_status stereo_batch_autotune(stereo_driver* d, const stereo_image* left, 
                     const stereo::BmFrame& fr = x->frames[f];
                     if (fr.D == cl.second.first && size_class(fr) == cl.second.second) x->pair_chunk[f] = c;
                 }
-                if (rpart_need_mixed(x) <= x->rpart_elems) {
-                    float ms = 0.f;
-                    if ((st = time_it(&ms))) break;
-                    if (ms < best_ms) {
-                        best_ms = ms;
-                        best_c = c;
-                    }
+                float ms = 0.f;
+                if ((st = time_it(&ms))) break;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best_c = c;
                 }
                 for (int f = 0; f < x->npairs; ++f) x->pair_chunk[f] = win_chunks[f];
             }

@@ -329,18 +329,18 @@ __global__ __launch_bounds__(256, Cost::LIST_WG) void bm_list_kernel(BmListParam
         j.y0 = it.ty * TH;
         j.ds = it.ds;
         j.de = it.de;
-        // the item keeps only its partial columns k in [256 - de, 318 - ds]
-        j.rp = p.rpart + (size_t)i * TH * p.rp_stride;
-        j.rp_stride = p.rp_stride;
-        j.rp_kofs = 256 - it.de;
-        j.lk = p.lkey + (size_t)i * TH * TW;
-        j.lk_stride = TW;
+        // the frame's key planes
+        j.rp = p.rplane + f.plane_off + (size_t)j.y0 * f.W;
+        j.rp_stride = f.W;
+        j.rp_kofs = 0;
+        j.lk = p.lplane + f.plane_off + (size_t)j.y0 * f.W + j.x0;
+        j.lk_stride = f.W;
         // an opaque copy of the thread index per item: the addresses derived from it are
         // recomputed for each item instead of being hoisted out of this loop into
         // registers that stay live through the whole item
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        run_job<Cost, AX, AY, false>(j, hbuf, ebuf, tid);
+        run_job<Cost, AX, AY, true>(j, hbuf, ebuf, tid);
     }
 }
 
@@ -494,18 +494,14 @@ __global__ __launch_bounds__(256) void bm_finish_kernel(BmFinishParams f)
     }
 }
 
-// Check of the key planes that the grid matcher folds its segments into with
-// atomicMax: one workgroup per image row reads the row's right keys into LDS and every
-// left pixel looks up the right winner at x - dL.  Each key is cleared by the thread
-// that read it, so the planes start the next frame at zero.
-__global__ __launch_bounds__(256) void bm_finish_planes_kernel(BmFinishParams f)
+// Check of one image row from the key planes that the matchers fold their keys into
+// with atomicMax: the workgroup reads the row's right keys into LDS and every left pixel
+// looks up the right winner at x - dL.  Each key is cleared by the thread that read it,
+// so the planes start the next frame at zero.
+__device__ __forceinline__ void planes_row(const View& disp, const View& disp_l, const View& disp_r, const View& dispid,
+                                           int W, int y, bool row_ok, int bx, int tol, int value_base, uint32_t* lrow,
+                                           uint32_t* rrowg, uint32_t* rrow)
 {
-    extern __shared__ uint32_t rrow[];
-    const int y = blockIdx.x;
-    const int W = f.W;
-    const bool row_ok = y >= f.by && y < f.H - f.by;
-    uint32_t* const lrow = f.lplane + (size_t)y * W;
-    uint32_t* const rrowg = f.rplane + (size_t)y * W;
     uint32_t lpre[FIN_PF];
 #pragma unroll
     for (int k = 0; k < FIN_PF; ++k) {
@@ -517,7 +513,7 @@ __global__ __launch_bounds__(256) void bm_finish_planes_kernel(BmFinishParams f)
         }
     }
     for (int x = threadIdx.x; x < W; x += 256) {
-        const bool col_ok = x >= f.bx && x < W - f.bx;
+        const bool col_ok = x >= bx && x < W - bx;
         const uint32_t r = rrowg[x];
         rrowg[x] = 0u;
         rrow[x] = (row_ok && col_ok) ? r : 0u;
@@ -527,17 +523,26 @@ __global__ __launch_bounds__(256) void bm_finish_planes_kernel(BmFinishParams f)
     for (int k = 0; k < FIN_PF; ++k) {
         const int x = threadIdx.x + 256 * k;
         if (x < W) {
-            const bool inside = row_ok && x >= f.bx && x < W - f.bx;
-            write_pixel(f.disp, f.disp_l, f.disp_r, f.dispid, x, y, inside, inside ? lpre[k] : 0u, rrow, f.tol,
-                        f.value_base);
+            const bool inside = row_ok && x >= bx && x < W - bx;
+            write_pixel(disp, disp_l, disp_r, dispid, x, y, inside, inside ? lpre[k] : 0u, rrow, tol, value_base);
         }
     }
     for (int x = threadIdx.x + 256 * FIN_PF; x < W; x += 256) {
-        const bool inside = row_ok && x >= f.bx && x < W - f.bx;
+        const bool inside = row_ok && x >= bx && x < W - bx;
         const uint32_t lk = lrow[x];
         lrow[x] = 0u;
-        write_pixel(f.disp, f.disp_l, f.disp_r, f.dispid, x, y, inside, inside ? lk : 0u, rrow, f.tol, f.value_base);
+        write_pixel(disp, disp_l, disp_r, dispid, x, y, inside, inside ? lk : 0u, rrow, tol, value_base);
     }
+}
+
+// SAD / census grid launch: one workgroup per image row.
+__global__ __launch_bounds__(256) void bm_finish_planes_kernel(BmFinishParams f)
+{
+    extern __shared__ uint32_t rrow[];
+    const int y = blockIdx.x;
+    const bool row_ok = y >= f.by && y < f.H - f.by;
+    planes_row(f.disp, f.disp_l, f.disp_r, f.dispid, f.W, y, row_ok, f.bx, f.tol, f.value_base,
+               f.lplane + (size_t)y * f.W, f.rplane + (size_t)y * f.W, rrow);
 }
 
 // Clears n words (the key planes, where a caller cannot rely on the last frame's merge
@@ -547,145 +552,26 @@ __global__ __launch_bounds__(256) void bm_clear_kernel(uint32_t* p, size_t n)
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0u;
 }
 
-// Merge and check for the work-list matcher: one workgroup per frame row.  A right
-// pixel xr of tile row ty collects the partial keys of every item whose tile and
-// disparity range reach it (x0 - (de-1) <= xr <= x0 + 63 - ds); a left pixel merges
-// the left keys of its tile's items.  The items of the tile row are staged in LDS
-// first (up to BM_LIST_ROW_ITEMS).  The right-view merge then walks the row's item
-// segments as one flat range of elements: each thread issues the loads of eight
-// elements together and folds them into the LDS row with ds_max_u32, so that the
-// loads of a row are in flight at once rather than one per loop step.  Rows with
-// more items than fit read the item table from memory, pixel by pixel.
-constexpr int BM_LIST_ROW_ITEMS = 1024;
-constexpr int BM_LIST_LEFT_UNROLL = 8;  // left keys of a tile's first items loaded together
-
-__global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p, int max_width)
+// Merge and check for the work-list matcher: one workgroup per row of every frame, from
+// the frame's key planes (row r of the launch is row r - row_base of the frame whose rows
+// it falls in).  Also returns the item counter to 0 for the next matcher launch.
+__global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p)
 {
-    extern __shared__ uint32_t smem[];
-    uint32_t* rrow = smem;
+    extern __shared__ uint32_t rrow[];
     const int r = blockIdx.x;
     if (r == 0 && threadIdx.x == 0) *p.counter = 0;  // the matcher launch before this one has ended
     int fi = 0;
     while (fi + 1 < p.nframes && p.frames[fi + 1].row_base <= r) ++fi;
     const BmFrame& f = p.frames[fi];
     const int y = r - f.row_base;
-    const int W = f.W;
-    const int ntx = f.ntx;
-    const int ty = y / TH, ry = y - ty * TH;
-    const BmTile* trow = p.tiles + f.tile_base + ty * ntx;
     const bool row_ok = y >= f.by && y < f.H - f.by;
-
-    // tile -> [toff[tx], toff[tx+1]) in the staged item table (slot, ds, de, tile x)
-    int* toff = reinterpret_cast<int*>(smem + ((max_width + 1) & ~1));  // even offsets keep irng 8-byte aligned
-    int* islot = toff + (BM_MAX_WIDTH / TW + 2);
-    int2* irng = reinterpret_cast<int2*>(islot + BM_LIST_ROW_ITEMS);
-    int* itx = reinterpret_cast<int*>(irng + BM_LIST_ROW_ITEMS);
-    __shared__ int staged;
-    if (threadIdx.x == 0) {
-        int n = 0;
-        for (int tx = 0; tx < ntx; ++tx) {
-            toff[tx] = n;
-            n += trow[tx].nitems;
-        }
-        toff[ntx] = n;
-        staged = n <= BM_LIST_ROW_ITEMS;
-    }
-    for (int x = threadIdx.x; x < W; x += 256) rrow[x] = 0u;
-    __syncthreads();
-    if (staged) {
-        for (int tx = threadIdx.x; tx < ntx; tx += 256) {
-            const BmTile t = trow[tx];
-            for (int k = 0; k < t.nitems; ++k) {
-                const int j = t.first_item + k;
-                islot[toff[tx] + k] = j;
-                irng[toff[tx] + k] = make_int2(p.items[j].ds, p.items[j].de);
-                itx[toff[tx] + k] = tx;
-            }
-        }
-        __syncthreads();
-        if (row_ok) {
-            // element (k, c): column c of staged item k, right pixel x0 - (de - 1) + c
-            const int tot = toff[ntx] * p.rp_stride;
-            for (int f0 = threadIdx.x; f0 < tot; f0 += 8 * 256) {
-                // all eight loads are issued before any is used: an element past the end
-                // or outside the item's columns reads a fixed address, and is discarded
-                uint32_t v[8];
-                int xr[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int e = f0 + u * 256;
-                    const int ec = min(e, tot - 1);
-                    const int k = ec / p.rp_stride, c = ec - k * p.rp_stride;
-                    const int2 rg = irng[k];
-                    const int x = itx[k] * TW - (rg.y - 1) + c;
-                    const bool ok = e < tot && c < TW - 1 + rg.y - rg.x && x >= f.bx && x < W - f.bx;
-                    const uint32_t* a = ok ? p.rpart + ((size_t)islot[k] * TH + ry) * p.rp_stride + c : p.rpart;
-                    v[u] = *a;
-                    xr[u] = ok ? x : -1;
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (xr[u] >= 0) atomicMax(&rrow[xr[u]], v[u]);
-            }
-        }
-    } else {
-        for (int x = threadIdx.x; x < W; x += 256) {
-            uint32_t best = 0;
-            if (row_ok && x >= f.bx && x < W - f.bx) {
-                const int tlo = x / TW;  // the first tile whose pixels reach x
-                const int thi = min(ntx - 1, (x + f.D - 1) / 64);
-                for (int tx = tlo; tx <= thi; ++tx) {
-                    const int x0 = tx * TW;
-                    const BmTile t = trow[tx];
-                    for (int j = t.first_item; j < t.first_item + t.nitems; ++j) {
-                        const int ds = p.items[j].ds, de = p.items[j].de;
-                        if (x >= x0 - (de - 1) && x <= x0 + 63 - ds) {
-                            const uint32_t v = p.rpart[((size_t)j * TH + ry) * p.rp_stride + (x - x0 + de - 1)];
-                            best = v > best ? v : best;
-                        }
-                    }
-                }
-            }
-            rrow[x] = best;
-        }
-    }
-    __syncthreads();
-    for (int x = threadIdx.x; x < W; x += 256) {
-        const bool inside = row_ok && x >= f.bx && x < W - f.bx;
-        uint32_t lk = 0;
-        if (inside) {
-            const int tx = x / TW;
-            const int kb = staged ? toff[tx] : trow[tx].first_item;
-            const int ke = staged ? toff[tx + 1] : trow[tx].first_item + trow[tx].nitems;
-            uint32_t lv[BM_LIST_LEFT_UNROLL];
-#pragma unroll
-            for (int q = 0; q < BM_LIST_LEFT_UNROLL; ++q) {
-                // unconditional: past the tile's items a load reads a fixed address
-                const int k = kb + q;
-                const int j = k < ke ? (staged ? islot[k] : k) : 0;
-                const uint32_t* a = k < ke ? p.lkey + ((size_t)j * TH + ry) * TW + (x & (TW - 1)) : p.lkey;
-                lv[q] = *a;
-            }
-#pragma unroll
-            for (int q = 0; q < BM_LIST_LEFT_UNROLL; ++q) lk = (kb + q < ke && lv[q] > lk) ? lv[q] : lk;
-            for (int k = kb + BM_LIST_LEFT_UNROLL; k < ke; ++k) {
-                const int j = staged ? islot[k] : k;
-                const uint32_t v = p.lkey[((size_t)j * TH + ry) * TW + (x & (TW - 1))];
-                lk = v > lk ? v : lk;
-            }
-        }
-        write_pixel(f.disp, f.disp_l, f.disp_r, f.dispid, x, y, inside, lk, rrow, f.tol, f.value_base);
-    }
+    const size_t row = f.plane_off + (size_t)y * f.W;
+    planes_row(f.disp, f.disp_l, f.disp_r, f.dispid, f.W, y, row_ok, f.bx, f.tol, f.value_base, p.lplane + row,
+               p.rplane + row, rrow);
 }
 
 }  // namespace bm
 
-size_t bm_list_lkey_elems(int max_items) { return (size_t)max_items * bm::TH * bm::TW; }
-int bm_list_rp_stride(int chunk_max) { return 64 + chunk_max; }
-size_t bm_list_rpart_elems(int max_items, int chunk_max)
-{
-    return (size_t)max_items * bm::TH * bm_list_rp_stride(chunk_max);
-}
 
 template <class Cost>
 static hipError_t launch_list_cost(const BmListParams& p, int ww, int grid, hipStream_t s)
@@ -698,18 +584,16 @@ static hipError_t launch_list_cost(const BmListParams& p, int ww, int grid, hipS
 
 hipError_t launch_bm_list(int kind, int ww, const BmListParams& p, int grid, hipStream_t s)
 {
-    if (grid < 1 || !p.counter) return hipErrorInvalidValue;
+    if (grid < 1 || !p.counter || !p.lplane || !p.rplane) return hipErrorInvalidValue;
     if (kind == BM_COST_CENSUS) return launch_list_cost<bm::CostCensus>(p, ww, grid, s);
     return launch_list_cost<bm::CostSad>(p, ww, grid, s);
 }
 
 hipError_t launch_bm_list_finish(const BmListParams& p, int max_width, hipStream_t s)
 {
-    if (max_width > BM_LIST_MAX_WIDTH || p.total_rows < 1 || !p.counter) return hipErrorInvalidValue;
-    const size_t lds = ((size_t)(max_width + 1) & ~(size_t)1) * 4 + (BM_MAX_WIDTH / bm::TW + 2) * 4 +
-                       (size_t)bm::BM_LIST_ROW_ITEMS * 16;
-    if (lds > 65536) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(bm::bm_list_finish_kernel, dim3(p.total_rows), dim3(256), lds, s, p, max_width);
+    if (max_width < 1 || max_width > BM_LIST_MAX_WIDTH || p.total_rows < 1 || !p.counter || !p.lplane || !p.rplane)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(bm::bm_list_finish_kernel, dim3(p.total_rows), dim3(256), (size_t)max_width * 4, s, p);
     return hipGetLastError();
 }
 

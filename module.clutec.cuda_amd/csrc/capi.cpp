@@ -300,8 +300,12 @@ struct BandExt : DriverExt {
     int* counter = nullptr;
     int* tile_evals = nullptr;
     uint32_t* tile_mask = nullptr;
-    uint32_t* lkey = nullptr;
-    uint32_t* rpart = nullptr;
+    // key planes of the SAD / census band search (left, right; W x H), zero at every
+    // matcher launch: cleared at allocation and by every merge; planes_dirty when a
+    // matcher ran without it
+    uint32_t* lplane = nullptr;
+    uint32_t* rplane = nullptr;
+    bool planes_dirty = false;
     int ntx = 0, nty = 0, max_items = 0;
     hipEvent_t ev_done = nullptr;  // recorded after every list finish launch
     bool done_recorded = false;
@@ -309,7 +313,7 @@ struct BandExt : DriverExt {
     {
         if (ev_done) (void)hipEventDestroy(ev_done);
         for (void* p : {(void*)frame_dev, (void*)tiles, (void*)items, (void*)nitems, (void*)counter, (void*)tile_evals,
-                        (void*)tile_mask, (void*)lkey, (void*)rpart})
+                        (void*)tile_mask, (void*)lplane, (void*)rplane})
             if (p) (void)hipFree(p);
     }
 };
@@ -339,8 +343,8 @@ stereo_status band_alloc(stereo_driver* d)
         (e = hipMalloc((void**)&x->nitems, sizeof(int))) || (e = hipMalloc((void**)&x->counter, sizeof(int))) ||
         (e = hipMalloc((void**)&x->tile_evals, nt * sizeof(int))) ||
         (e = hipMalloc((void**)&x->tile_mask, nt * sizeof(uint32_t))) ||
-        (e = hipMalloc((void**)&x->lkey, stereo::bm_list_lkey_elems(x->max_items) * 4)) ||
-        (e = hipMalloc((void**)&x->rpart, stereo::bm_list_rpart_elems(x->max_items, kBandChunk) * 4)) ||
+        (e = hipMalloc((void**)&x->lplane, (size_t)W * H * 4)) || (e = hipMalloc((void**)&x->rplane, (size_t)W * H * 4)) ||
+        (e = hipMemset(x->lplane, 0, (size_t)W * H * 4)) || (e = hipMemset(x->rplane, 0, (size_t)W * H * 4)) ||
         (e = hipMemset(x->counter, 0, sizeof(int))) || (e = hipEventCreate(&x->ev_done)) ||
         (e = hipDeviceSynchronize()))
         return hip_fail(d->ctx, e, "hipMalloc");
@@ -388,8 +392,6 @@ stereo_status stereo_matcher_validate(const stereo_image_format* fmt, const ster
         p->algorithm != STEREO_ALGO_NCC && p->algorithm != STEREO_ALGO_AW1 && p->algorithm != STEREO_ALGO_AW2)
         return say(STEREO_ERROR_INVALID_CONFIG, "the band search around an initial disparity map is built for the SAD, "
                                                 "census, NCC and adaptive-window matchers");
-    if (p->use_disp_input && fmt->width > stereo::BM_LIST_MAX_WIDTH)
-        return say(STEREO_ERROR_INVALID_FORMAT, "the band search takes widths up to 11264");
     if (p->lr_tolerance < 0) return say(STEREO_ERROR_INVALID_CONFIG, "lr_tolerance must be >= 0");
     if (p->num_disparities < 8 || p->num_disparities > 256 || p->num_disparities % 8)
         return say(STEREO_ERROR_INVALID_CONFIG, "num_disparities must be a multiple of 8 in [8, 256]");
@@ -731,16 +733,22 @@ stereo_status stereo_matcher_process_init(stereo_driver* d, stereo_image* disp, 
     lp.nitems = x->nitems;
     lp.sched = nullptr;
     lp.counter = x->counter;
-    lp.lkey = x->lkey;
-    lp.rpart = x->rpart;
-    lp.rp_stride = stereo::bm_list_rp_stride(kBandChunk);
+    lp.lplane = x->lplane;
+    lp.rplane = x->rplane;
     lp.nframes = 1;
     lp.total_rows = H;
     const int grid = x->max_items < 768 ? x->max_items : 768;
+    if (x->planes_dirty) {
+        // an earlier call's matcher ran without its merge: clear its keys
+        HIP_TRY(ctx, stereo::launch_bm_clear(x->lplane, (size_t)W * H, s), "clear launch");
+        HIP_TRY(ctx, stereo::launch_bm_clear(x->rplane, (size_t)W * H, s), "clear launch");
+    }
+    x->planes_dirty = true;  // until the merge is enqueued behind the matcher
     HIP_TRY(ctx, stereo::launch_bm_list(census ? stereo::BM_COST_CENSUS : stereo::BM_COST_SAD, d->mp.window_w, lp, grid, s),
             "list matcher launch");
     if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[1], s), "hipEventRecord");
     HIP_TRY(ctx, stereo::launch_bm_list_finish(lp, W, s), "list finish launch");
+    x->planes_dirty = false;
     HIP_TRY(ctx, hipEventRecord(x->ev_done, s), "hipEventRecord");
     x->done_recorded = true;
     return end(d, s);

@@ -62,11 +62,11 @@ hipError_t noalloc_setup(int device);
 
 // ---- work-list matcher: many frames, per-tile disparity ranges ---------------------
 // A frame is one stereo pair; its tiles are 64 x BM_TILE_H pixels; every tile has a
-// disparity range [lo, hi) cut into items (one workgroup's work each).  Item i keeps
-// its left keys at lkey + i*BM_TILE_H*64 (row stride 64) and its right-view partial
-// keys at rpart + i*BM_TILE_H*rp_stride (row stride rp_stride = 64 + the longest
-// item; right pixel xr of tile x0 at column xr - x0 + de - 1).  The matcher launch is
-// persistent: a fixed grid walks the item list, whose length may be produced on the
+// disparity range [lo, hi) cut into items (one workgroup's work each).  Every item folds
+// its left winners and right-view partial maxima into its frame's two key planes (W x H
+// each, at plane_off in lplane and rplane) with atomicMax; the merge launch reads the
+// planes and clears them, so they are zero at every matcher launch.  The matcher launch
+// is persistent: a fixed grid walks the item list, whose length may be produced on the
 // device.
 constexpr int BM_TILE_W = 64;
 constexpr int BM_TILE_H = 56;
@@ -80,6 +80,7 @@ struct BmFrame {
     int ntx, nty;
     int tile_base;      // index of tile (0, 0) in the tile table
     int row_base;       // first finish row of this frame
+    size_t plane_off;   // element offset of the frame's key planes (row stride W)
 };
 
 struct BmTile {
@@ -100,20 +101,16 @@ struct BmListParams {
     const int* sched;   // order in which the persistent grid takes the items (nullptr: 0, 1, ...)
     int* counter;       // next item to hand out: 0 before the matcher launch; the finish launch
                         // resets it, so a matcher launch is always followed by its finish
-    uint32_t* lkey;
-    uint32_t* rpart;
-    int rp_stride;      // bm_list_rp_stride(longest item)
+    uint32_t* lplane;   // every frame's left-key plane, zero at the matcher launch
+    uint32_t* rplane;   // every frame's right-key plane, zero at the matcher launch
     int nframes;
     int total_rows;     // sum of the frames' heights
 };
 
 hipError_t launch_bm_list(int kind, int ww, const BmListParams& p, int grid, hipStream_t s);
 hipError_t launch_bm_list_finish(const BmListParams& p, int max_width, hipStream_t s);
-// widest frame the work-list merge takes (its LDS row and item table stay within 64 KiB)
-constexpr int BM_LIST_MAX_WIDTH = 11264;
-size_t bm_list_lkey_elems(int max_items);
-int bm_list_rp_stride(int chunk_max);
-size_t bm_list_rpart_elems(int max_items, int chunk_max);
+// widest frame the work-list merge takes (one LDS row of right keys)
+constexpr int BM_LIST_MAX_WIDTH = BM_MAX_WIDTH;
 
 // ---- reference-format cross check ------------------------------------------------
 // range[0..1] receive min / max of the kept ids (device memory, reset by the launch).

@@ -529,8 +529,11 @@ struct LearnedExt : DriverExt {
     int* tile_evals = nullptr;
     uint32_t* tile_mask = nullptr;
     int* counter = nullptr;  // item counter of the list matcher (reset by its finish)
-    uint32_t* lkey = nullptr;
-    uint32_t* rpart = nullptr;
+    // the band search's key planes (left, right; W x H), zero at every matcher launch:
+    // cleared at configure and by every merge; planes_dirty when a matcher ran without it
+    uint32_t* lplane = nullptr;
+    uint32_t* rplane = nullptr;
+    bool planes_dirty = false;
     int ntx = 0, nty = 0, max_items = 0;
     hipEvent_t ev[6] = {};
     bool timed = false;
@@ -558,7 +561,7 @@ struct LearnedExt : DriverExt {
         for (stereo_image* im : {&L4, &R4, &dc})
             if (im->data) (void)hipFree(im->data);
         for (void* p : {(void*)padL, (void*)padR, (void*)frame_dev, (void*)tiles, (void*)items, (void*)nitems,
-                        (void*)tile_evals, (void*)tile_mask, (void*)counter, (void*)lkey, (void*)rpart})
+                        (void*)tile_evals, (void*)tile_mask, (void*)counter, (void*)lplane, (void*)rplane})
             if (p) (void)hipFree(p);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -647,10 +650,10 @@ stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** out,
         (e = hipMalloc((void**)&x->tile_mask, nt * sizeof(uint32_t))) ||
         (e = hipMalloc((void**)&x->counter, sizeof(int))) ||
         (e = hipStreamCreateWithFlags(&x->cap, hipStreamNonBlocking)) ||
-        (e = hipMalloc((void**)&x->lkey, stereo::bm_list_lkey_elems(x->max_items) * 4)) ||
-        (e = hipMalloc((void**)&x->rpart, stereo::bm_list_rpart_elems(x->max_items, p->chunk) * 4)))
+        (e = hipMalloc((void**)&x->lplane, (size_t)W * H * 4)) || (e = hipMalloc((void**)&x->rplane, (size_t)W * H * 4)))
         return cleanup(hip_fail(ctx, e, "hipMalloc"));
     if ((e = hipMemset(x->padL, 0, x->pad_pitch * H)) || (e = hipMemset(x->padR, 0, x->pad_pitch * H)) ||
+        (e = hipMemset(x->lplane, 0, (size_t)W * H * 4)) || (e = hipMemset(x->rplane, 0, (size_t)W * H * 4)) ||
         (e = hipMemset(x->counter, 0, sizeof(int))) || (e = hipDeviceSynchronize()))
         return cleanup(hip_fail(ctx, e, "hipMemset"));
     stereo::BmFrame& f = x->frame;
@@ -720,15 +723,16 @@ stereo_status learned_enqueue(stereo_driver* d, LearnedExt* x, const stereo_imag
     lp.nitems = x->nitems;
     lp.sched = nullptr;
     lp.counter = x->counter;
-    lp.lkey = x->lkey;
-    lp.rpart = x->rpart;
-    lp.rp_stride = stereo::bm_list_rp_stride(x->lp.chunk);
+    lp.lplane = x->lplane;
+    lp.rplane = x->rplane;
     lp.nframes = 1;
     lp.total_rows = H;
     const int grid = x->max_items < 768 ? x->max_items : 768;
+    x->planes_dirty = true;  // until the merge is enqueued behind the matcher
     HIP_TRY(ctx, stereo::launch_bm_list(stereo::BM_COST_CENSUS, d->mp.window_w, lp, grid, s), "list matcher launch");
     if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[4], s), "hipEventRecord");
     HIP_TRY(ctx, stereo::launch_bm_list_finish(lp, W, s), "list finish launch");
+    x->planes_dirty = false;
     if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[5], s), "hipEventRecord");
     return STEREO_OK;
 }
@@ -762,6 +766,14 @@ stereo_status stereo_learned_process(stereo_driver* d, stereo_image* disp, stere
         HIP_TRY(ctx, hipStreamSynchronize(s), "hipStreamSynchronize");
         HIP_TRY(ctx, hipMemcpy(x->frame_dev, &f, sizeof(f), hipMemcpyHostToDevice), "hipMemcpy");
         x->frame = f;
+    }
+
+    if (x->planes_dirty) {
+        // an earlier frame's band search ran without its merge: clear its keys
+        const size_t n = (size_t)W * H;
+        HIP_TRY(ctx, stereo::launch_bm_clear(x->lplane, n, s), "clear launch");
+        HIP_TRY(ctx, stereo::launch_bm_clear(x->rplane, n, s), "clear launch");
+        x->planes_dirty = false;
     }
 
     // timed calls run the stages directly with events between them; otherwise the

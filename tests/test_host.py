@@ -60,14 +60,14 @@ def test_matcher_validate(algo, win, cen, D, expect):
 
 
 def test_matcher_validate_band_search():
-    # the band search around an initial map is configured for every matcher; its tile
-    # table limits the width
+    # the band search around an initial map is configured for every matcher, up to the
+    # matchers' own width limit
     for algo, win in [(cs.ALGO_SAD, (9, 9)), (cs.ALGO_CENSUS, (9, 9)), (cs.ALGO_NCC, (11, 11)),
                       (cs.ALGO_AW1, (5, 5)), (cs.ALGO_AW2, (3, 3))]:
         p = cs.matcher_params(algo, win, (9, 7), 128, 1, use_disp_input=True)
         st, msg = cs.validate_matcher(640, 480, p)
         assert st == cs.OK, msg
-        assert cs.validate_matcher(11264 + 64, 480, p)[0] == cs.ERROR_INVALID_FORMAT
+        assert cs.validate_matcher(16384 + 64, 480, p)[0] == cs.ERROR_INVALID_FORMAT
     p = cs.matcher_params(cs.ALGO_AW1, (5, 5), (9, 7), 128, 1)
     p.use_disp_input = 2
     assert cs.validate_matcher(640, 480, p)[0] == cs.ERROR_INVALID_CONFIG
