@@ -536,6 +536,8 @@ struct LearnedExt : DriverExt {
     bool planes_dirty = false;
     int ntx = 0, nty = 0, max_items = 0;
     hipEvent_t ev[6] = {};
+    hipEvent_t ev_done = nullptr;  // recorded after every process call's last launch
+    bool done_recorded = false;
     bool timed = false;
     // the launch sequence captured for a given pair of input images
     hipStream_t cap = nullptr;
@@ -565,6 +567,7 @@ struct LearnedExt : DriverExt {
             if (p) (void)hipFree(p);
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (ev_done) (void)hipEventDestroy(ev_done);
     }
 };
 
@@ -649,7 +652,7 @@ stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** out,
         (e = hipMalloc((void**)&x->nitems, sizeof(int))) || (e = hipMalloc((void**)&x->tile_evals, nt * sizeof(int))) ||
         (e = hipMalloc((void**)&x->tile_mask, nt * sizeof(uint32_t))) ||
         (e = hipMalloc((void**)&x->counter, sizeof(int))) ||
-        (e = hipStreamCreateWithFlags(&x->cap, hipStreamNonBlocking)) ||
+        (e = hipStreamCreateWithFlags(&x->cap, hipStreamNonBlocking)) || (e = hipEventCreate(&x->ev_done)) ||
         (e = hipMalloc((void**)&x->lplane, (size_t)W * H * 4)) || (e = hipMalloc((void**)&x->rplane, (size_t)W * H * 4)))
         return cleanup(hip_fail(ctx, e, "hipMalloc"));
     if ((e = hipMemset(x->padL, 0, x->pad_pitch * H)) || (e = hipMemset(x->padR, 0, x->pad_pitch * H)) ||
@@ -763,6 +766,8 @@ stereo_status stereo_learned_process(stereo_driver* d, stereo_image* disp, stere
     f.disp_r = disp_rl ? view_of(disp_rl) : null_view();
     f.dispid = null_view();
     if (std::memcmp(&f, &x->frame, sizeof(f)) != 0) {
+        // the previous call's launches read the table and may still run on another stream
+        if (x->done_recorded) HIP_TRY(ctx, hipEventSynchronize(x->ev_done), "hipEventSynchronize");
         HIP_TRY(ctx, hipStreamSynchronize(s), "hipStreamSynchronize");
         HIP_TRY(ctx, hipMemcpy(x->frame_dev, &f, sizeof(f), hipMemcpyHostToDevice), "hipMemcpy");
         x->frame = f;
@@ -779,8 +784,12 @@ stereo_status stereo_learned_process(stereo_driver* d, stereo_image* disp, stere
     // timed calls run the stages directly with events between them; otherwise the
     // sequence is captured once per pair of input images and replayed as a graph
     x->timed = ctx->timing != 0;
-    if (x->timed) return learned_enqueue(d, x, left, right, s, true);
-    if (x->lp.direct_launch) return learned_enqueue(d, x, left, right, s, false);
+    if (x->timed || x->lp.direct_launch) {
+        if ((st = learned_enqueue(d, x, left, right, s, x->timed))) return st;
+        HIP_TRY(ctx, hipEventRecord(x->ev_done, s), "hipEventRecord");
+        x->done_recorded = true;
+        return STEREO_OK;
+    }
     if (!x->exec || x->graph_key[0] != left->data || x->graph_key[1] != right->data) {
         x->drop_graph();
         HIP_TRY(ctx, hipStreamBeginCapture(x->cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -798,6 +807,8 @@ stereo_status stereo_learned_process(stereo_driver* d, stereo_image* disp, stere
         x->graph_key[1] = right->data;
     }
     HIP_TRY(ctx, hipGraphLaunch(x->exec, s), "hipGraphLaunch");
+    HIP_TRY(ctx, hipEventRecord(x->ev_done, s), "hipEventRecord");
+    x->done_recorded = true;
     return STEREO_OK;
 }
 

@@ -130,6 +130,20 @@ __device__ __forceinline__ void track(Tracker& s, int v, int d, bool upd)
 // NCC
 // ------------------------------------------------------------------------------------
 
+// x / N correctly rounded, for an integer 0 <= x <= N * 65025 (a window sum of bytes or
+// of products of two bytes over N = P * P pixels): the product with the rounded reciprocal
+// and one residual step, x - q0 N being exact in the fused multiply-add.  Equal to the
+// IEEE division for every such x and P = 5, 7, 9, 11, 15 (checked exhaustively on the CPU).
+template <int N>
+__device__ __forceinline__ float div_window(int x)
+{
+    constexpr float R = 1.0f / (float)N;
+    const float xf = (float)x;
+    const float q0 = xf * R;
+    const float r = __builtin_fmaf(-q0, (float)N, xf);
+    return __builtin_fmaf(r, R, q0);
+}
+
 template <int P>
 struct NccShape {
     static constexpr int CG = (32 - P + 1) / 2;
@@ -148,14 +162,30 @@ __global__ __launch_bounds__(64) void ncc_ref_kernel(RefArgs a)
     using S = NccShape<P>;
     constexpr int CG = S::CG, PCX = S::PCX, TW = S::TW, NC = S::NC;
     constexpr int ROWS = 16 * NB, DR = ROWS + P - 1;
-    constexpr float fN = (float)(P * P);
 
-    __shared__ uint8_t sl[DR][S::TBX];
-    __shared__ uint8_t sr[DR][TW];
-    __shared__ float tmean[ROWS][NC + 1], tsd[ROWS][NC + 1];
-    // column sums: of L R per disparity, of the right windows' statistics per group, and
-    // (at the start) of the left windows and gradients in columns 32 .. 63
-    __shared__ int scol[2][ROWS][64];
+    // The left pixels are needed only until the left statistics and the lanes' column
+    // registers are formed; the right windows' statistics take their place.  Column sums:
+    // of L R per disparity, of the right windows' sums and squares per group, and (at the
+    // start) of the left windows and gradients (columns 32 .. 63), one plane used in turn;
+    // rows of 66 words (2 mod 32), so that the 16 rows and two column groups of a
+    // half-wave's row-sum reads fall in 32 different banks.
+    constexpr int SC = 66;
+    struct Lds {
+        union {
+            uint8_t sl[DR][S::TBX];
+            struct {
+                float tmean[ROWS][NC + 1], tsd[ROWS][NC + 1];
+            } st;
+        } u;
+        uint8_t sr[DR][TW];
+        int scol[ROWS][SC];
+    };
+    __shared__ Lds lds;
+    auto& sl = lds.u.sl;
+    auto& sr = lds.sr;
+    auto& tmean = lds.u.st.tmean;
+    auto& tsd = lds.u.st.tsd;
+    auto& scol = lds.scol;
 
     const int l = threadIdx.x;
     const int W = a.L.w, H = a.L.h;
@@ -187,57 +217,78 @@ __global__ __launch_bounds__(64) void ncc_ref_kernel(RefArgs a)
         tr[k] = Tracker{0, 0, 0, 0, 0, 0, 0};
     }
 
-    // left window sums (EvalMsd) and, for the matcher, the gradient sums (EvalGrad)
+    // left window sums (EvalMsd) and, for the matcher, the gradient sums (EvalGrad): the
+    // sums and gradients first, then the sums of squares in the same plane
+    bool cannot[CG];
     {
         const int c = l & 31, r0 = 16 * (lane_ok ? b : 0);
-        int s1 = 0, s2 = 0, sg = 0;
+        int s1 = 0, sg = 0;
         for (int j = 0; j < P; ++j) {
             const int v = sl[r0 + j][c], w = sl[r0 + j][c + 1];
             s1 += v;
-            s2 += v * v;
             sg += abs(v - w);
         }
         if (lane_ok) {
-            scol[0][r0][c] = s1;
-            scol[1][r0][c] = s2;
-            scol[0][r0][c + 32] = sg;
+            scol[r0][c] = s1;
+            scol[r0][c + 32] = sg;
         }
         for (int j = 1; j < 16; ++j) {
             const int vo = sl[r0 + j - 1][c], wo = sl[r0 + j - 1][c + 1];
             const int vn = sl[r0 + j + P - 1][c], wn = sl[r0 + j + P - 1][c + 1];
             s1 += vn - vo;
-            s2 += vn * vn - vo * vo;
             sg += abs(vn - wn) - abs(vo - wo);
             if (lane_ok) {
-                scol[0][r0 + j][c] = s1;
-                scol[1][r0 + j][c] = s2;
-                scol[0][r0 + j][c + 32] = sg;
+                scol[r0 + j][c] = s1;
+                scol[r0 + j][c + 32] = sg;
             }
         }
     }
     __syncthreads();
-    bool cannot[CG];
     {
-        int s1 = 0, s2 = 0, sg = 0;
+        int s1 = 0, sg = 0;
         const int rr = lane_ok ? row : iy;
 #pragma unroll
         for (int i = 0; i < P; ++i) {
-            s1 += scol[0][rr][ix0 + i];
-            s2 += scol[1][rr][ix0 + i];
-            sg += scol[0][rr][ix0 + i + 32];
+            s1 += scol[rr][ix0 + i];
+            sg += scol[rr][ix0 + i + 32];
         }
 #pragma unroll
         for (int k = 0; k < CG; ++k) {
             if (k > 0) {
-                s1 += scol[0][rr][ix0 + k + P - 1] - scol[0][rr][ix0 + k - 1];
-                s2 += scol[1][rr][ix0 + k + P - 1] - scol[1][rr][ix0 + k - 1];
-                sg += scol[0][rr][ix0 + k + P - 1 + 32] - scol[0][rr][ix0 + k - 1 + 32];
+                s1 += scol[rr][ix0 + k + P - 1] - scol[rr][ix0 + k - 1];
+                sg += scol[rr][ix0 + k + P - 1 + 32] - scol[rr][ix0 + k - 1 + 32];
             }
-            const float m = (float)s1 / fN, m2 = (float)s2 / fN;
+            bmean[k] = div_window<P * P>(s1);
+            cannot[k] = div_window<P * P>(sg) < a.grad_thresh;
+        }
+    }
+    __syncthreads();
+    {
+        const int c = l & 31, r0 = 16 * (lane_ok ? b : 0);
+        int s2 = 0;
+        for (int j = 0; j < P; ++j) {
+            const int v = sl[r0 + j][c];
+            s2 += v * v;
+        }
+        if (lane_ok) scol[r0][c] = s2;
+        for (int j = 1; j < 16; ++j) {
+            const int vo = sl[r0 + j - 1][c], vn = sl[r0 + j + P - 1][c];
+            s2 += vn * vn - vo * vo;
+            if (lane_ok) scol[r0 + j][c] = s2;
+        }
+    }
+    __syncthreads();
+    {
+        int s2 = 0;
+        const int rr = lane_ok ? row : iy;
+#pragma unroll
+        for (int i = 0; i < P; ++i) s2 += scol[rr][ix0 + i];
+#pragma unroll
+        for (int k = 0; k < CG; ++k) {
+            if (k > 0) s2 += scol[rr][ix0 + k + P - 1] - scol[rr][ix0 + k - 1];
+            const float m = bmean[k], m2 = div_window<P * P>(s2);
             const float sq = m * m;
-            bmean[k] = m;
             bsd[k] = sqrtf(m2 - sq);
-            cannot[k] = (float)sg / fN < a.grad_thresh;
         }
     }
 
@@ -313,43 +364,41 @@ __global__ __launch_bounds__(64) void ncc_ref_kernel(RefArgs a)
             sr[r][c] = (c < g.n + S::TBX - 1 && x >= 0 && x < W && y < H) ? row_ptr<const uint8_t>(a.R, y)[x] : 0;
         }
         __syncthreads();
-        // the right windows' statistics at every start column the group uses
-        for (int i = l; i < NB * TW; i += 64) {
-            const int c = i % TW, r0 = 16 * (i / TW);
-            int s1 = 0, s2 = 0;
-            for (int j = 0; j < P; ++j) {
-                const int v = sr[r0 + j][c];
-                s1 += v;
-                s2 += v * v;
-            }
-            scol[0][r0][c] = s1;
-            scol[1][r0][c] = s2;
-            for (int j = 1; j < 16; ++j) {
-                const int vo = sr[r0 + j - 1][c], vn = sr[r0 + j + P - 1][c];
-                s1 += vn - vo;
-                s2 += vn * vn - vo * vo;
-                scol[0][r0 + j][c] = s1;
-                scol[1][r0 + j][c] = s2;
-            }
-        }
-        __syncthreads();
-        for (int i = l; i < ROWS * 2; i += 64) {
-            const int rr = i >> 1, c0 = (i & 1) * ((NC + 1) / 2);
-            const int c1 = min(NC, c0 + (NC + 1) / 2);
-            int s1 = 0, s2 = 0;
-            for (int k = 0; k < P; ++k) {
-                s1 += scol[0][rr][c0 + k];
-                s2 += scol[1][rr][c0 + k];
-            }
-            for (int c = c0; c < c1; ++c) {
-                if (c > c0) {
-                    s1 += scol[0][rr][c + P - 1] - scol[0][rr][c - 1];
-                    s2 += scol[1][rr][c + P - 1] - scol[1][rr][c - 1];
+        // the right windows' statistics at every start column the group uses: the means
+        // from the column sums, then the deviations from the column sums of squares (a
+        // thread finishes the same columns in both passes, so it reads its own means)
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass) __syncthreads();  // the means' readers of the plane are done
+            for (int i = l; i < NB * TW; i += 64) {
+                const int c = i % TW, r0 = 16 * (i / TW);
+                int s = 0;
+                for (int j = 0; j < P; ++j) {
+                    const int v = sr[r0 + j][c];
+                    s += pass ? v * v : v;
                 }
-                const float m = (float)s1 / fN, m2 = (float)s2 / fN;
-                const float sq = m * m;
-                tmean[rr][c] = m;
-                tsd[rr][c] = sqrtf(m2 - sq);
+                scol[r0][c] = s;
+                for (int j = 1; j < 16; ++j) {
+                    const int vo = sr[r0 + j - 1][c], vn = sr[r0 + j + P - 1][c];
+                    s += pass ? vn * vn - vo * vo : vn - vo;
+                    scol[r0 + j][c] = s;
+                }
+            }
+            __syncthreads();
+            for (int i = l; i < ROWS * 2; i += 64) {
+                const int rr = i >> 1, c0 = (i & 1) * ((NC + 1) / 2);
+                const int c1 = min(NC, c0 + (NC + 1) / 2);
+                int s = 0;
+                for (int k = 0; k < P; ++k) s += scol[rr][c0 + k];
+                for (int c = c0; c < c1; ++c) {
+                    if (c > c0) s += scol[rr][c + P - 1] - scol[rr][c - 1];
+                    if (!pass) {
+                        tmean[rr][c] = div_window<P * P>(s);
+                    } else {
+                        const float m = tmean[rr][c], m2 = div_window<P * P>(s);
+                        const float sq = m * m;
+                        tsd[rr][c] = sqrtf(m2 - sq);
+                    }
+                }
             }
         }
         for (int r = 0; r < g.n; ++r) {
@@ -365,25 +414,25 @@ __global__ __launch_bounds__(64) void ncc_ref_kernel(RefArgs a)
                 int s = 0;
 #pragma unroll
                 for (int j = 0; j < P; ++j) s += q[j];
-                if (lane_ok) scol[0][r0][c] = s;
+                if (lane_ok) scol[r0][c] = s;
 #pragma unroll
                 for (int j = 1; j < 16; ++j) {
                     s += q[j + P - 1] - q[j - 1];
-                    if (lane_ok) scol[0][r0 + j][c] = s;
+                    if (lane_ok) scol[r0 + j][c] = s;
                 }
             }
             __syncthreads();
             // row sums and the score of each result (EvalSim's result lambda)
             {
                 const int rr = lane_ok ? row : iy;
-                const int(&cr)[64] = scol[0][rr];
+                const int* const cr = scol[rr];
                 int s3 = 0;
 #pragma unroll
                 for (int i = 0; i < P; ++i) s3 += cr[ix0 + i];
 #pragma unroll
                 for (int k = 0; k < CG; ++k) {
                     if (k > 0) s3 += cr[ix0 + k + P - 1] - cr[ix0 + k - 1];
-                    const float cross = (float)s3 / fN;
+                    const float cross = div_window<P * P>(s3);
                     const float tm = tmean[rr][sh + ix0 + k], ts = tsd[rr][sh + ix0 + k];
                     const float norm = bsd[k] * ts;
                     float ncc;
