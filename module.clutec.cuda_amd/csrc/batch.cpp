@@ -26,6 +26,11 @@ struct BatchExt : DriverExt {
     std::vector<int> pair_chunk;
     hipEvent_t ev_done = nullptr;  // recorded after every process call's last launch
     bool done_recorded = false;
+    // the inputs of one call: left and right image of every pair into its padded buffer,
+    // as one batched census or copy launch (base: census tiles or rows before the image)
+    std::vector<stereo::ImageJob> jobs;
+    stereo::ImageJob* jobs_dev = nullptr;
+    int job_units = 0;  // census tiles or rows of all images
     std::vector<uint8_t*> padL, padR;
     std::vector<stereo::BmFrame> frames;  // host copy of the device frame table
     stereo::BmFrame* frames_dev = nullptr;
@@ -69,7 +74,7 @@ struct BatchExt : DriverExt {
         for (uint8_t* p : padR)
             if (p) (void)hipFree(p);
         for (void* p : {(void*)frames_dev, (void*)tiles_dev, (void*)items_dev, (void*)sched_dev, (void*)counter_dev, (void*)nitems_dev,
-                        (void*)lplane, (void*)rplane})
+                        (void*)lplane, (void*)rplane, (void*)jobs_dev})
             if (p) (void)hipFree(p);
     }
 };
@@ -148,23 +153,43 @@ stereo_status upload_work(BatchExt* x)
     return STEREO_OK;
 }
 
+// The job table of the call's input images; rewritten (after the previous call's
+// launches, which read it) when the images change.  Before any capture or launch.
+stereo_status upload_jobs(BatchExt* x, const stereo_image* left, const stereo_image* right)
+{
+    std::vector<stereo::ImageJob> jobs(2 * x->npairs);
+    int base = 0;
+    for (int i = 0; i < x->npairs; ++i) {
+        const stereo::BmFrame& f = x->frames[i];
+        for (int k = 0; k < 2; ++k) {
+            stereo::ImageJob& j = jobs[2 * i + k];
+            j.src = view_of(k ? &right[i] : &left[i]);
+            j.dst = k ? f.R : f.L;
+            j.base = base;
+            base += x->kind == stereo::BM_COST_CENSUS ? stereo::census_tiles(f.W, f.H) : f.H;
+        }
+    }
+    const bool same = jobs.size() == x->jobs.size() &&
+                      std::memcmp(jobs.data(), x->jobs.data(), jobs.size() * sizeof(stereo::ImageJob)) == 0;
+    if (same) return STEREO_OK;
+    if (x->done_recorded) HIP_TRY(x->ctx, hipEventSynchronize(x->ev_done), "hipEventSynchronize");
+    HIP_TRY(x->ctx, hipMemcpy(x->jobs_dev, jobs.data(), jobs.size() * sizeof(stereo::ImageJob), hipMemcpyHostToDevice),
+            "hipMemcpy");
+    x->jobs = jobs;
+    x->job_units = base;
+    return STEREO_OK;
+}
+
 stereo_status enqueue(stereo_driver* d, BatchExt* x, const stereo_image* left, const stereo_image* right, hipStream_t s)
 {
     stereo_context* ctx = d->ctx;
-    for (int i = 0; i < x->npairs; ++i) {
-        const stereo::BmFrame& f = x->frames[i];
-        if (x->kind == stereo::BM_COST_CENSUS) {
-            const View vl = view_of(&left[i]), vr = view_of(&right[i]);
-            HIP_TRY(ctx, stereo::launch_census_pair(vl, f.L, &vr, &f.R, x->mp.census_w, x->mp.census_h, s), "census launch");
-        } else {
-            HIP_TRY(ctx,
-                    hipMemcpy2DAsync(f.L.ptr, f.L.pitch, left[i].data, left[i].pitch, f.W, f.H, hipMemcpyDeviceToDevice, s),
-                    "hipMemcpy2DAsync");
-            HIP_TRY(ctx,
-                    hipMemcpy2DAsync(f.R.ptr, f.R.pitch, right[i].data, right[i].pitch, f.W, f.H, hipMemcpyDeviceToDevice, s),
-                    "hipMemcpy2DAsync");
-        }
-    }
+    (void)left;
+    (void)right;  // read through the job table
+    if (x->kind == stereo::BM_COST_CENSUS)
+        HIP_TRY(ctx, stereo::launch_census_batch(x->jobs_dev, 2 * x->npairs, x->job_units, x->mp.census_w, x->mp.census_h, s),
+                "census launch");
+    else
+        HIP_TRY(ctx, stereo::launch_pad_copy_batch(x->jobs_dev, 2 * x->npairs, x->job_units, s), "copy launch");
     stereo::BmListParams lp;
     lp.frames = x->frames_dev;
     lp.tiles = x->tiles_dev;
@@ -320,6 +345,7 @@ stereo_status stereo_batch_configure(stereo_context* ctx, stereo_driver** out, i
         (e = hipMalloc((void**)&x->nitems_dev, sizeof(int))) ||
         (e = hipMalloc((void**)&x->counter_dev, sizeof(int))) ||
         (e = hipMalloc((void**)&x->lplane, x->plane_elems * 4)) || (e = hipMalloc((void**)&x->rplane, x->plane_elems * 4)) ||
+        (e = hipMalloc((void**)&x->jobs_dev, 2 * npairs * sizeof(stereo::ImageJob))) ||
         (e = hipMemset(x->lplane, 0, x->plane_elems * 4)) || (e = hipMemset(x->rplane, 0, x->plane_elems * 4)) ||
         (e = hipMemset(x->counter_dev, 0, sizeof(int))) || (e = hipDeviceSynchronize()))
         return cleanup(hip_fail(ctx, e, "hipMalloc"));
@@ -360,6 +386,7 @@ stereo_status stereo_batch_process(stereo_driver* d, stereo_image* disp, stereo_
         x->drop_graph();
     }
     if ((st = clear_if_dirty(x, s))) return st;
+    if ((st = upload_jobs(x, left, right))) return st;
     if ((st = begin(d, s))) return st;
     if (!x->tuning.use_graph) {
         if ((st = enqueue(d, x, left, right, s))) return st;
@@ -448,6 +475,14 @@ stereo_status stereo_batch_autotune(stereo_driver* d, const stereo_image* left, 
     const int grids[] = {cus, 2 * cus, 3 * cus};
     const std::vector<int> keep_chunks = x->pair_chunk;
     std::fill(x->pair_chunk.begin(), x->pair_chunk.end(), 0);
+    if ((st = upload_jobs(x, left, right))) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(scratch);
+        x->frames = saved;
+        (void)hipMemcpy(x->frames_dev, x->frames.data(), x->npairs * sizeof(stereo::BmFrame), hipMemcpyHostToDevice);
+        return st;
+    }
     st = STEREO_OK;
     // time the current work list: warm-up, then three runs
     auto time_it = [&](float* ms) -> stereo_status {

@@ -511,10 +511,7 @@ static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool disp
         const View vl = view_of(left), vr = view_of(right);
         HIP_TRY(ctx, stereo::launch_census_pair(vl, pl, &vr, &pr, d->mp.census_w, d->mp.census_h, s), "census launch");
     } else {
-        HIP_TRY(ctx, hipMemcpy2DAsync(pl.ptr, pl.pitch, left->data, left->pitch, W, H, hipMemcpyDeviceToDevice, s),
-                "hipMemcpy2DAsync");
-        HIP_TRY(ctx, hipMemcpy2DAsync(pr.ptr, pr.pitch, right->data, right->pitch, W, H, hipMemcpyDeviceToDevice, s),
-                "hipMemcpy2DAsync");
+        HIP_TRY(ctx, stereo::launch_pad_copy_pair(view_of(left), pl, view_of(right), pr, s), "copy launch");
     }
     if (ncc) {
         HIP_TRY(ctx, stereo::launch_ncc_stats(view_of(left), d->lstats, W, d->mp.window_w, d->mp.window_h, 8388608.0f, s),
@@ -708,10 +705,7 @@ stereo_status stereo_matcher_process_init(stereo_driver* d, stereo_image* disp, 
         const View vl = view_of(left), vr = view_of(right);
         HIP_TRY(ctx, stereo::launch_census_pair(vl, f.L, &vr, &f.R, d->mp.census_w, d->mp.census_h, s), "census launch");
     } else {
-        HIP_TRY(ctx, hipMemcpy2DAsync(f.L.ptr, f.L.pitch, left->data, left->pitch, W, H, hipMemcpyDeviceToDevice, s),
-                "hipMemcpy2DAsync");
-        HIP_TRY(ctx, hipMemcpy2DAsync(f.R.ptr, f.R.pitch, right->data, right->pitch, W, H, hipMemcpyDeviceToDevice, s),
-                "hipMemcpy2DAsync");
+        HIP_TRY(ctx, stereo::launch_pad_copy_pair(view_of(left), f.L, view_of(right), f.R, s), "copy launch");
     }
     if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[0], s), "hipEventRecord");
     stereo::CnnBandArgs ba{};

@@ -30,17 +30,21 @@ constexpr int CT_H = 32;
 constexpr int CT_STRIP = 8;
 
 template <int CW, int CH>
-__global__ __launch_bounds__(256) void census_wide_kernel(View src0, View dst0, View src1, View dst1, View q0, View q1)
+struct CensusShape {
+    static constexpr int RX = CW / 2, RY = CH / 2;
+    static constexpr int LW = CT_W + 2 * RX, LH = CT_H + 2 * RY;
+};
+
+// One 64 x 32 tile of one image (origin x0, y0); q: its 4 x 4 box means, or q.ptr == nullptr.
+template <int CW, int CH>
+__device__ __forceinline__ void census_tile(const View& src, const View& dst, const View& q, int x0, int y0,
+                                            uint32_t (*tile)[CensusShape<CW, CH>::LW + 1])
 {
     constexpr int RX = CW / 2, RY = CH / 2, NB = CW * CH;
-    constexpr int LW = CT_W + 2 * RX, LH = CT_H + 2 * RY;
+    constexpr int LW = CT_W + 2 * RX;
+    constexpr int LH = CT_H + 2 * RY;
     constexpr int NHI = NB > 32 ? NB - 32 : 0;  // bits landing in the high word
-    __shared__ uint32_t tile[LH][LW + 1];
-
-    const View src = blockIdx.z ? src1 : src0;
-    const View dst = blockIdx.z ? dst1 : dst0;
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * CT_W, y0 = blockIdx.y * CT_H;
     for (int i = tid; i < LH * LW; i += 256) {
         const int r = i / LW, c = i - r * LW;
         const int gy = clampi(y0 - RY + r, 0, src.h - 1);
@@ -106,7 +110,6 @@ __global__ __launch_bounds__(256) void census_wide_kernel(View src0, View dst0, 
     }
     // quarter resolution: 16 x 8 box means of the tile (the image size is a
     // multiple of 4, so every 4 x 4 block lies inside the image)
-    const View q = blockIdx.z ? q1 : q0;
     if (q.ptr && tid < (CT_W / 4) * (CT_H / 4)) {
         const int qx = tid & 15, qy = tid >> 4;
         const int X = x0 / 4 + qx, Y = y0 / 4 + qy;
@@ -146,6 +149,80 @@ __global__ __launch_bounds__(256) void census3x3_ref_kernel(View src, View dst)
 }
 
 template <int CW, int CH>
+__global__ __launch_bounds__(256) void census_wide_kernel(View src0, View dst0, View src1, View dst1, View q0, View q1)
+{
+    using S = CensusShape<CW, CH>;
+    __shared__ uint32_t tile[S::LH][S::LW + 1];
+    census_tile<CW, CH>(blockIdx.z ? src1 : src0, blockIdx.z ? dst1 : dst0, blockIdx.z ? q1 : q0, blockIdx.x * CT_W,
+                        blockIdx.y * CT_H, tile);
+}
+
+// Every image of a batch in one launch: workgroup b takes tile b - jobs[i].base of the
+// image i whose tiles hold it (the jobs in order of base).
+__device__ __forceinline__ int job_of(const ImageJob* jobs, int njobs, int b)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].base <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int CW, int CH>
+__global__ __launch_bounds__(256) void census_batch_kernel(const ImageJob* jobs, int njobs)
+{
+    using S = CensusShape<CW, CH>;
+    __shared__ uint32_t tile[S::LH][S::LW + 1];
+    const int i = job_of(jobs, njobs, blockIdx.x);
+    const ImageJob j = jobs[i];
+    const int t = blockIdx.x - j.base;
+    const int ntx = (j.src.w + CT_W - 1) / CT_W;
+    const View none{nullptr, 0, 0, 0};
+    census_tile<CW, CH>(j.src, j.dst, none, (t % ntx) * CT_W, (t / ntx) * CT_H, tile);
+}
+
+// Rows of every image of a batch into the padded matcher inputs: workgroup b copies row
+// b - jobs[i].base of image i.
+__global__ __launch_bounds__(256) void pad_copy_batch_kernel(const ImageJob* jobs, int njobs)
+{
+    const ImageJob j = jobs[job_of(jobs, njobs, blockIdx.x)];
+    const int y = blockIdx.x - j.base;
+    const uint8_t* sp = row_ptr<const uint8_t>(j.src, y);
+    uint8_t* dp = row_ptr<uint8_t>(j.dst, y);
+    const int W = j.src.w;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(sp) | reinterpret_cast<uintptr_t>(dp)) & 3) == 0;
+    for (int x = threadIdx.x * 4; x < W; x += 256 * 4) {
+        if (aligned && x + 4 <= W) {
+            *reinterpret_cast<uint32_t*>(dp + x) = *reinterpret_cast<const uint32_t*>(sp + x);
+        } else {
+            for (int k = 0; k < 4 && x + k < W; ++k) dp[x + k] = sp[x + k];
+        }
+    }
+}
+
+// The bytes of two W x H grey images into the padded matcher inputs in one launch: one
+// workgroup row per image row, 4 bytes per lane when both rows are 4-byte aligned.
+__global__ __launch_bounds__(256) void pad_copy_kernel(View s0, View d0, View s1, View d1)
+{
+    const View& src = blockIdx.z ? s1 : s0;
+    const View& dst = blockIdx.z ? d1 : d0;
+    const int y = blockIdx.y;
+    const uint8_t* sp = row_ptr<const uint8_t>(src, y);
+    uint8_t* dp = row_ptr<uint8_t>(dst, y);
+    const int W = src.w;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(sp) | reinterpret_cast<uintptr_t>(dp)) & 3) == 0;
+    for (int x = (blockIdx.x * 256 + threadIdx.x) * 4; x < W; x += gridDim.x * 256 * 4) {
+        if (aligned && x + 4 <= W) {
+            *reinterpret_cast<uint32_t*>(dp + x) = *reinterpret_cast<const uint32_t*>(sp + x);
+        } else {
+            for (int k = 0; k < 4 && x + k < W; ++k) dp[x + k] = sp[x + k];
+        }
+    }
+}
+
+template <int CW, int CH>
 hipError_t launch_wide(const View& src0, const View& dst0, const View* src1, const View* dst1, const View* q0,
                        const View* q1, hipStream_t s)
 {
@@ -173,6 +250,38 @@ hipError_t launch_census(const View& src, const View& dst, int cw, int ch, int r
         return hipGetLastError();
     }
     return launch_census_pair(src, dst, nullptr, nullptr, cw, ch, s);
+}
+
+hipError_t launch_pad_copy_pair(const View& src0, const View& dst0, const View& src1, const View& dst1, hipStream_t s)
+{
+    if (src0.w != src1.w || src0.h != src1.h || dst0.w < src0.w || dst1.w < src1.w || dst0.h < src0.h ||
+        dst1.h < src1.h || src0.w < 1 || src0.h < 1)
+        return hipErrorInvalidValue;
+    dim3 grid((src0.w + 1023) / 1024, src0.h, 2);
+    hipLaunchKernelGGL(pad_copy_kernel, grid, dim3(256), 0, s, src0, dst0, src1, dst1);
+    return hipGetLastError();
+}
+
+int census_tiles(int w, int h) { return ((w + CT_W - 1) / CT_W) * ((h + CT_H - 1) / CT_H); }
+
+hipError_t launch_census_batch(const ImageJob* jobs, int njobs, int total_tiles, int cw, int ch, hipStream_t s)
+{
+    if (njobs < 1 || total_tiles < 1) return hipErrorInvalidValue;
+    const dim3 grid(total_tiles), block(256);
+    if (cw == 9 && ch == 7) hipLaunchKernelGGL((census_batch_kernel<9, 7>), grid, block, 0, s, jobs, njobs);
+    else if (cw == 7 && ch == 9) hipLaunchKernelGGL((census_batch_kernel<7, 9>), grid, block, 0, s, jobs, njobs);
+    else if (cw == 5 && ch == 5) hipLaunchKernelGGL((census_batch_kernel<5, 5>), grid, block, 0, s, jobs, njobs);
+    else if (cw == 7 && ch == 7) hipLaunchKernelGGL((census_batch_kernel<7, 7>), grid, block, 0, s, jobs, njobs);
+    else if (cw == 3 && ch == 3) hipLaunchKernelGGL((census_batch_kernel<3, 3>), grid, block, 0, s, jobs, njobs);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_pad_copy_batch(const ImageJob* jobs, int njobs, int total_rows, hipStream_t s)
+{
+    if (njobs < 1 || total_rows < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pad_copy_batch_kernel, dim3(total_rows), dim3(256), 0, s, jobs, njobs);
+    return hipGetLastError();
 }
 
 hipError_t launch_census_pair(const View& src0, const View& dst0, const View* src1, const View* dst1, int cw, int ch,

@@ -12,6 +12,19 @@ bool census_supported(int cw, int ch, int ref3x3);
 hipError_t launch_census(const View& src, const View& dst, int cw, int ch, int ref3x3, hipStream_t s);
 // both images of a pair in one launch (src1/dst1 may be null for a single image);
 // q0/q1, when given, receive the 4 x 4 box means (sum + 8) >> 4 of the images
+// One image of a batch: source, padded destination, and its first workgroup in the
+// batched launch (census tiles of 64 x 32 or rows, counted over the images before it).
+struct ImageJob {
+    View src, dst;
+    int base;
+};
+int census_tiles(int w, int h);
+// Census of every image of a batch in one launch; jobs in device memory, in order of base.
+hipError_t launch_census_batch(const ImageJob* jobs, int njobs, int total_tiles, int cw, int ch, hipStream_t s);
+// Copies every grey image of a batch into its padded input in one launch (one workgroup per row).
+hipError_t launch_pad_copy_batch(const ImageJob* jobs, int njobs, int total_rows, hipStream_t s);
+// Copies two grey images of one size into padded matcher inputs (one launch).
+hipError_t launch_pad_copy_pair(const View& src0, const View& dst0, const View& src1, const View& dst1, hipStream_t s);
 hipError_t launch_census_pair(const View& src0, const View& dst0, const View* src1, const View* dst1, int cw, int ch,
                               hipStream_t s, const View* q0 = nullptr, const View* q1 = nullptr);
 

@@ -130,10 +130,7 @@ stereo_status stereo_matcher_process_noalloc(stereo_context* ctx, const stereo_m
         const View vl = view_of(left), vr = view_of(right);
         HIP_TRY(ctx, stereo::launch_census_pair(vl, pl, &vr, &pr, p->census_w, p->census_h, s), "census launch");
     } else {
-        HIP_TRY(ctx, hipMemcpy2DAsync(pl.ptr, pl.pitch, left->data, left->pitch, W, H, hipMemcpyDeviceToDevice, s),
-                "hipMemcpy2DAsync");
-        HIP_TRY(ctx, hipMemcpy2DAsync(pr.ptr, pr.pitch, right->data, right->pitch, W, H, hipMemcpyDeviceToDevice, s),
-                "hipMemcpy2DAsync");
+        HIP_TRY(ctx, stereo::launch_pad_copy_pair(view_of(left), pl, view_of(right), pr, s), "copy launch");
     }
     const int bx = p->window_w / 2 + (census ? p->census_w / 2 : 0);
     const int by = p->window_h / 2 + (census ? p->census_h / 2 : 0);
