@@ -732,6 +732,224 @@ __global__ __launch_bounds__(64) void aw1_ref_kernel(RefArgs a)
     }
 }
 
+// AW1 over NB vertically stacked reference blocks of one block column, 256 lanes: the
+// blocks share their disparity groups (they depend on the block column only), and the
+// similarity column sums and sub-patch sums are functions of the position and the
+// disparity, so the stack forms them once per disparity for all its blocks' results
+// (the one-block kernel forms 16 sub-patch rows for 6 or 10 result rows).  Decisions,
+// tracker and write-out per result as in aw1_ref_kernel; the full-range search only.
+template <int SP, int SIM, int NB>
+__global__ __launch_bounds__(256) void aw1_ref_stack_kernel(RefArgs a)
+{
+#pragma clang fp contract(off)
+    using S = AwShape<SP>;
+    constexpr int PCX = S::PCX, PCY = S::PCY, TW = S::TW, P3 = S::P3, SPCX = S::SPCX;
+    constexpr int RROWS = PCY * NB;        // result rows of the stack
+    constexpr int SROWS = RROWS + 2 * SP;  // sub-patch top rows its results read
+    constexpr int IROWS = SROWS + SP - 1;  // image rows
+    constexpr int NRES = PCX * RROWS;
+    constexpr int RPT = (NRES + 255) / 256;       // results per thread
+    constexpr int CR = (SROWS + 7) / 8;           // column-sum rows per thread
+    constexpr int GR = (RROWS + 7) / 8;           // gate column-sum rows per thread
+    constexpr int CPG = (SPCX + 3) / 4;           // sub-patch columns per thread
+    static_assert(IROWS == RROWS + P3 - 1, "the gate and the sub-patches read the same rows");
+    static_assert(4 * SROWS <= 256, "one thread per sub-patch row and column group");
+
+    __shared__ uint8_t sl[IROWS][32];
+    __shared__ uint8_t sr[IROWS][TW + 1];
+    __shared__ int scol[SROWS][33];
+    __shared__ int ssp[SROWS][33];
+    __shared__ float stab[256];
+    __shared__ int sflag[NB];
+    __shared__ int sany;
+
+    const int l = threadIdx.x;
+    const int W = a.L.w, H = a.L.h;
+    const int X0 = blockIdx.x * PCX;
+    const int by0 = blockIdx.y * NB;
+    const int Y0 = by0 * PCY;
+
+    if (SIM == 1) stab[l] = 2.0f * ((float)l / 255.0f - 0.5f);
+    for (int i = l; i < IROWS * 32; i += 256) {
+        const int r = i / 32, c = i % 32;
+        const int x = X0 + c, y = Y0 + r;
+        sl[r][c] = (x < W && y < H) ? row_ptr<const uint8_t>(a.L, y)[x] : 0;
+    }
+    if (l < NB) sflag[l] = 0;
+    if (l == 0) sany = 0;
+
+    // this thread's results: positions l + 256 q of the stack's PCX x RROWS, row-major
+    int rx[RPT], ry[RPT], rb[RPT];
+    bool has[RPT], written[RPT], upd[RPT];
+    Tracker tr[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int pos = l + 256 * q;
+        rx[q] = pos < NRES ? pos % PCX : 0;
+        ry[q] = pos < NRES ? pos / PCX : 0;
+        rb[q] = ry[q] / PCY;
+        const int yb = Y0 + rb[q] * PCY;  // the block's origin row
+        const bool block_ok = by0 + rb[q] < a.nby && X0 + P3 <= W && yb + P3 <= H;
+        has[q] = pos < NRES && block_ok;
+        written[q] = has[q] && X0 + rx[q] + P3 < W && Y0 + ry[q] + P3 < H;
+        tr[q] = Tracker{0, -32767, 0, 0, 0, 0, 0};
+    }
+    __syncthreads();
+
+    // EvalGrad: each result's P3 x P3 window standard deviation, truncated, in % of 255
+    {
+        const int c = l & 31, r0 = (l >> 5) * GR;
+        int s1 = 0, s2 = 0;
+        for (int j = 0; j < P3; ++j) {
+            const int v = r0 + j < IROWS ? sl[r0 + j][c] : 0;
+            s1 += v;
+            s2 += v * v;
+        }
+        if (r0 < RROWS) {
+            scol[r0][c] = s1;
+            ssp[r0][c] = s2;
+        }
+        for (int j = 1; j < GR; ++j) {
+            if (r0 + j >= RROWS) break;
+            const int vo = sl[r0 + j - 1][c], vn = sl[r0 + j + P3 - 1][c];
+            s1 += vn - vo;
+            s2 += vn * vn - vo * vo;
+            scol[r0 + j][c] = s1;
+            ssp[r0 + j][c] = s2;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        if (!has[q]) continue;
+        int s1 = 0, s2 = 0;
+        for (int i = 0; i < P3; ++i) {
+            s1 += scol[ry[q]][rx[q] + i];
+            s2 += ssp[ry[q]][rx[q] + i];
+        }
+        const float m = (float)s1 / (float)(P3 * P3), m2 = (float)s2 / (float)(P3 * P3);
+        const float sq = m * m;
+        const int isd = (int)sqrtf(fabsf(m2 - sq));
+        const float sd = (float)isd / 255.0f * 100.0f;
+        if (sd < a.grad_thresh) tr[q].disp = R_CANNOT;
+        else sflag[rb[q]] = 1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        // the block's rows fit and one of its results evaluates (:741-775)
+        const bool rows_fit = Y0 + rb[q] * PCY + S::TBY <= H;
+        upd[q] = has[q] && rows_fit && sflag[rb[q]] != 0;
+        if (upd[q]) sany = 1;
+    }
+    __syncthreads();
+    const bool evaluate = sany != 0;
+
+    const int maxd = a.ltor ? X0 - a.offset + 1 : W - X0 - a.offset;
+    const int d_stop = a.dmax < maxd ? a.dmax : maxd;
+    for (int D0 = a.dmin; evaluate && D0 < d_stop; D0 += R_SDC) {
+        const RefGroup g = ref_group<S::TBX>(X0, D0, W, a.offset, a.ltor, R_SDC);
+        if (g.n <= 0) continue;
+        __syncthreads();
+        for (int i = l; i < IROWS * TW; i += 256) {
+            const int r = i / TW, c = i % TW;
+            const int x = g.lo + c, y = Y0 + r;
+            sr[r][c] = (c < g.n + S::TBX - 1 && x >= 0 && x < W && y < H) ? row_ptr<const uint8_t>(a.R, y)[x] : 0;
+        }
+        for (int r = 0; r < g.n; ++r) {
+            const int d = D0 + r;
+            const int sh = (a.ltor ? g.base - r : g.base + r) - g.lo;
+            __syncthreads();
+            // column sums of SP rows: the SROWS sub-patch top rows x 32 columns
+            {
+                const int c = l & 31, r0 = (l >> 5) * CR;
+                int qv[CR + SP - 1];
+#pragma unroll
+                for (int j = 0; j < CR + SP - 1; ++j) {
+                    const int rr = r0 + j < IROWS ? r0 + j : IROWS - 1;
+                    const int u = sl[rr][c], v = sr[rr][c + sh];
+                    if (SIM == 0) {
+                        qv[j] = 255 - abs(u - v);
+                    } else {
+                        const float f = stab[u] * stab[v];
+                        qv[j] = (int)(f * 255.0f);
+                    }
+                }
+                int sum = 0;
+#pragma unroll
+                for (int j = 0; j < SP; ++j) sum += qv[j];
+                if (r0 < SROWS) scol[r0][c] = sum;
+#pragma unroll
+                for (int j = 1; j < CR; ++j) {
+                    sum += qv[j + SP - 1] - qv[j - 1];
+                    if (r0 + j < SROWS) scol[r0 + j][c] = sum;
+                }
+            }
+            __syncthreads();
+            // sub-patch sums: row l / 4, columns CPG (l % 4) .. + CPG
+            if (l < 4 * SROWS) {
+                const int rr = l >> 2, c0 = (l & 3) * CPG;
+                int sum = 0;
+#pragma unroll
+                for (int i = 0; i < SP; ++i) sum += scol[rr][c0 + i];
+                if (c0 < SPCX) ssp[rr][c0] = sum;
+#pragma unroll
+                for (int i = 1; i < CPG; ++i) {
+                    if (c0 + i < SPCX) {
+                        sum += scol[rr][c0 + i + SP - 1] - scol[rr][c0 + i - 1];
+                        ssp[rr][c0 + i] = sum;
+                    }
+                }
+            }
+            __syncthreads();
+            // Two_OutOf_Four (Kernel.Algo.AdaptWnd_W32H16.h:240-285) and the peak update
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int x = rx[q], y = ry[q];
+                int m1 = ssp[y][x + SP], m2 = -32767, v;
+                v = ssp[y + SP][x];
+                m2 = max(m2, min(m1, v));
+                m1 = max(m1, v);
+                v = ssp[y + 2 * SP][x + SP];
+                m2 = max(m2, min(m1, v));
+                m1 = max(m1, v);
+                v = ssp[y + SP][x + 2 * SP];
+                m2 = max(m2, min(m1, v));
+                m1 = max(m1, v);
+                const int aw = (int)(int16_t)(m1 + m2 + ssp[y + SP][x + SP]);
+                track(tr[q], aw, d, upd[q]);
+            }
+        }
+    }
+
+    // WriteDisparity (:509-635)
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        if (!written[q]) continue;
+        const Tracker& t = tr[q];
+        ushort4 o;
+        const int dv = t.disp;
+        if (dv > R_LAST || dv == 0) {
+            o = make_ushort4((uint16_t)dv, 0, 0, 0);
+        } else {
+            const float fmax = aw1_sim_float(t.mx, S::NE);
+            const float dp = fmaxf(fabsf(aw1_sim_float(t.prev, S::NE) - fmax), fabsf(aw1_sim_float(t.prev2, S::NE) - fmax));
+            const float dn = fmaxf(fabsf(aw1_sim_float(t.nxt, S::NE) - fmax), fabsf(aw1_sim_float(t.nxt2, S::NE) - fmax));
+            const bool specific = dp >= a.min_delta && dn >= a.min_delta;
+            int code;
+            if (fmax >= a.sim_thresh) {
+                const bool sat = dv <= R_FIRST + a.dmin + 1 || dv >= R_FIRST + a.dmax - 1;
+                code = sat ? R_SATURATED : (specific ? dv : R_NOT_SPECIFIC);
+            } else {
+                code = R_NOT_FOUND;
+            }
+            o = make_ushort4((uint16_t)code, (uint16_t)aw1_sim_u16(fmax), (uint16_t)aw1_sim_u16(dp),
+                             (uint16_t)aw1_sim_u16(dn));
+        }
+        row_ptr<ushort4>(a.out, Y0 + ry[q] + P3 / 2)[X0 + rx[q] + P3 / 2] = o;
+    }
+}
+
 struct RefExt : DriverExt {
     int algorithm = 0;  // STEREO_ALGO_NCC or STEREO_ALGO_AW1
     stereo_ncc_params ncc{};
@@ -748,13 +966,23 @@ hipError_t ncc_ref_launch(const RefArgs& a, int nbx, hipStream_t s)
     return hipGetLastError();
 }
 
+// blocks stacked per workgroup in the full-range search: 8 x 6 result rows (sub-patch 5),
+// 5 x 10 (sub-patch 3)
+template <int SP>
+constexpr int aw1_stack() { return SP == 5 ? 8 : 5; }
+
 template <int SP>
 hipError_t aw1_ref_launch(const RefArgs& a, int sim, int nbx, hipStream_t s)
 {
-    if (sim == 0)
-        hipLaunchKernelGGL((aw1_ref_kernel<SP, 0>), dim3(nbx, a.nby), dim3(64), 0, s, a);
-    else
-        hipLaunchKernelGGL((aw1_ref_kernel<SP, 1>), dim3(nbx, a.nby), dim3(64), 0, s, a);
+    constexpr int NB = aw1_stack<SP>();
+    const dim3 sgrid(nbx, (a.nby + NB - 1) / NB);
+    if (a.init.ptr) {
+        if (sim == 0) hipLaunchKernelGGL((aw1_ref_kernel<SP, 0>), dim3(nbx, a.nby), dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((aw1_ref_kernel<SP, 1>), dim3(nbx, a.nby), dim3(64), 0, s, a);
+    } else {
+        if (sim == 0) hipLaunchKernelGGL((aw1_ref_stack_kernel<SP, 0, NB>), sgrid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((aw1_ref_stack_kernel<SP, 1, NB>), sgrid, dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 
