@@ -577,6 +577,11 @@ stereo_status stereo_denoise_process(stereo_driver* drv, stereo_image* out, cons
  * v_bcnt_u32_b32); the roofline of the integer matchers uses the lower of this and
  * the specified peak.  No counterpart in the reference.  Synchronous. */
 stereo_status stereo_valu_peak(stereo_context* ctx, int32_t op, double* tera_ops);
+/* The number of float bit patterns (NaNs excepted) whose bf16 from the device's packed
+ * conversion instruction, which the one-launch network uses, differs from integer
+ * round-to-nearest-even, which its other kernels and CPU reference use; 0 on gfx950.
+ * No counterpart in the reference.  Synchronous. */
+stereo_status stereo_bf16_rounding_check(stereo_context* ctx, uint64_t* mismatches);
 
 /* ---- synthetic workloads (host only, no device needed) ------------------- */
 

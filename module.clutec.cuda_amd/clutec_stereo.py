@@ -224,6 +224,7 @@ _PROTOS = [
     ("stereo_learned_tile_ranges", C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32]),
     ("stereo_learned_tile_masks", C.c_int, [_P, C.POINTER(C.c_uint32), C.c_int32]),
     ("stereo_valu_peak", C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    ("stereo_bf16_rounding_check", C.c_int, [_P, C.POINTER(C.c_uint64)]),
     ("stereo_synth_pair", C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     _P, _P, _P]),
 ]

@@ -606,6 +606,98 @@ __global__ __launch_bounds__(1024) void band_items_kernel(CnnBandArgs a)
 constexpr int FZ_TW = 32, FZ_TH = 16, FZ_PS = 40;  // staged pixel stride: 16 bf16 channels + 8 bytes
 constexpr int FZ_AW = FZ_TW + 20, FZ_AH = FZ_TH + 20;  // buffer A: the input region (halo 10)
 constexpr int FZ_BW = FZ_TW + 18, FZ_BH = FZ_TH + 18;  // buffer B: layer 0's output region (halo 9)
+constexpr int FZ_IL = 4;  // blocks of 16 output pixels a wave computes at once
+constexpr int FZ_NW = 8;  // waves per workgroup
+
+// The halo of each layer's input (grow[5]: the tile itself).
+constexpr int fz_grow(int l) { return l == 0 ? 10 : l == 1 ? 9 : l == 2 ? 7 : l == 3 ? 3 : l == 4 ? 1 : 0; }
+
+typedef __bf16 fz_bf2 __attribute__((ext_vector_type(2)));
+typedef float fz_f2 __attribute__((ext_vector_type(2)));
+
+// Two floats to packed bf16, rounded to nearest even with v_cvt_pk_bf16_f32; equal to
+// f2bf for every float but NaN (all 2^32 patterns compared on the device).
+__device__ __forceinline__ uint32_t fz_pack_bf16(float lo, float hi)
+{
+    const fz_f2 v = {lo, hi};
+    const fz_bf2 b = __builtin_convertvector(v, fz_bf2);
+    return __builtin_bit_cast(uint32_t, b);
+}
+
+struct FzWeights {
+    v4s w[9];
+    float bias[4];
+};
+
+__device__ __forceinline__ void fz_load_weights(const CnnFusedArgs& a, int l, int col, int grp, FzWeights& wt)
+{
+#pragma unroll
+    for (int t = 0; t < 9; ++t) wt.w[t] = *reinterpret_cast<const v4s*>(a.w[l] + ((size_t)col * 9 + t) * 16 + 4 * grp);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) wt.bias[i] = a.bias[l][4 * grp + i];
+}
+
+// Layer L of the tile: every output region size and the dilation are constants.
+template <int L>
+__device__ __forceinline__ void fz_layer(const CnnFusedArgs& a, const uint8_t* in, uint8_t* out, const FzWeights& wt,
+                                         int col, int grp, int wv, int x0, int y0, int n)
+{
+    constexpr int gi = fz_grow(L), go = fz_grow(L + 1), dil = gi - go;
+    constexpr int rwi = FZ_TW + 2 * gi;  // input region row length
+    constexpr int rwo = FZ_TW + 2 * go, rho = FZ_TH + 2 * go;
+    constexpr int nbx = (rwo + 15) >> 4;
+    constexpr int nitems = nbx * rho;
+    // FZ_IL blocks of 16 output pixels per wave at once, each with its own accumulator,
+    // their taps interleaved (a block past the end repeats the first one's reads and
+    // writes nothing)
+    for (int item0 = wv; item0 < nitems; item0 += FZ_NW * FZ_IL) {
+        int oy[FZ_IL], ox[FZ_IL], base[FZ_IL];
+        bool live[FZ_IL];
+        v4f acc[FZ_IL];
+#pragma unroll
+        for (int u = 0; u < FZ_IL; ++u) {
+            const int item = item0 + u * FZ_NW;
+            live[u] = item < nitems;
+            const int it = live[u] ? item : item0;
+            oy[u] = it / nbx;
+            ox[u] = (it % nbx) * 16 + col;
+            const int oxr = min(ox[u], rwo - 1);  // lanes past the region read its last pixel
+            base[u] = ((oy[u] + dil) * rwi + oxr + dil) * FZ_PS + 8 * grp;
+            acc[u] = v4f{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int toff = ((t / 3 - 1) * dil * rwi + (t % 3 - 1) * dil) * FZ_PS;
+            v4s bv[FZ_IL];
+#pragma unroll
+            for (int u = 0; u < FZ_IL; ++u) bv[u] = *reinterpret_cast<const v4s*>(in + base[u] + toff);
+#pragma unroll
+            for (int u = 0; u < FZ_IL; ++u)
+                acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wt.w[t], bv[u], acc[u], 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < FZ_IL; ++u) {
+            if (!live[u] || ox[u] >= rwo) continue;
+            // D: lane (col, grp) holds pixel ox, channels 4 grp .. 4 grp + 3
+            const int gx = x0 - go + ox[u], gy = y0 - go + oy[u];
+            const bool inside = gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
+            float v[4] = {acc[u][0], acc[u][1], acc[u][2], acc[u][3]};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] += wt.bias[i];
+            if (L < 4) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[i] = (inside && v[i] > 0.f) ? v[i] : 0.f;
+                uint2 pk;
+                pk.x = fz_pack_bf16(v[0], v[1]);
+                pk.y = fz_pack_bf16(v[2], v[3]);
+                *reinterpret_cast<uint2*>(out + (oy[u] * rwo + ox[u]) * FZ_PS + 8 * grp) = pk;
+            } else if (inside) {
+                *reinterpret_cast<float4*>(a.out + (((size_t)n * a.H + gy) * a.W + gx) * 16 + 4 * grp) =
+                    make_float4(v[0], v[1], v[2], v[3]);
+            }
+        }
+    }
+}
 
 __global__ __launch_bounds__(512) void cnn_fused_forward_kernel(CnnFusedArgs a)
 {
@@ -614,7 +706,6 @@ __global__ __launch_bounds__(512) void cnn_fused_forward_kernel(CnnFusedArgs a)
     const int lane = threadIdx.x & 63;
     const int col = lane & 15, grp = lane >> 4;
     const int wv = threadIdx.x >> 6;
-    constexpr int NW = 8;
     const int ntx = (a.W + FZ_TW - 1) / FZ_TW, nty = (a.H + FZ_TH - 1) / FZ_TH;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = b % ntx, r = b / ntx;
@@ -622,6 +713,9 @@ __global__ __launch_bounds__(512) void cnn_fused_forward_kernel(CnnFusedArgs a)
     const int x0 = tx * FZ_TW, y0 = ty * FZ_TH;
     const uint16_t* img = a.x + (size_t)n * a.H * a.W * 16;
 
+    // a layer's weights are loaded while the layer before it computes
+    FzWeights w0, w1;
+    fz_load_weights(a, 0, col, grp, w0);
     // stage the input region (halo 10), zero outside the image: 4 x 8-byte chunks per pixel
     for (int i = threadIdx.x; i < FZ_AW * FZ_AH * 4; i += 512) {
         const int px = i >> 2, q = i & 3;
@@ -632,56 +726,19 @@ __global__ __launch_bounds__(512) void cnn_fused_forward_kernel(CnnFusedArgs a)
         *reinterpret_cast<uint2*>(bufA + px * FZ_PS + 8 * q) = v;
     }
     __syncthreads();
-
-    constexpr int grow[6] = {10, 9, 7, 3, 1, 0};  // halo of each layer's input; grow[5]: the tile
-#pragma unroll 1
-    for (int l = 0; l < 5; ++l) {
-        const int gi = grow[l], go = grow[l + 1];
-        const int dil = gi - go;
-        const int rwi = FZ_TW + 2 * gi;                 // input region row length
-        const int rwo = FZ_TW + 2 * go, rho = FZ_TH + 2 * go;
-        const uint8_t* in = (l & 1) ? bufB : bufA;
-        uint8_t* out = (l & 1) ? bufA : bufB;
-        v4s wr[9];
-#pragma unroll
-        for (int t = 0; t < 9; ++t)
-            wr[t] = *reinterpret_cast<const v4s*>(a.w[l] + ((size_t)col * 9 + t) * 16 + 4 * grp);
-        float bias[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) bias[i] = a.bias[l][4 * grp + i];
-        const int nbx = (rwo + 15) >> 4;
-        for (int item = wv; item < nbx * rho; item += NW) {
-            const int oy = item / nbx, ox = (item % nbx) * 16 + col;
-            const int oxr = min(ox, rwo - 1);  // lanes past the region read its last pixel
-            const int ix = oxr + dil, iy = oy + dil;
-            v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                const int sy = iy + (t / 3 - 1) * dil, sx = ix + (t % 3 - 1) * dil;
-                const v4s bv = *reinterpret_cast<const v4s*>(in + (sy * rwi + sx) * FZ_PS + 8 * grp);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[t], bv, acc, 0, 0, 0);
-            }
-            // D: lane (col, grp) holds pixel ox, channels 4 grp .. 4 grp + 3
-            const int gx = x0 - go + ox, gy = y0 - go + oy;
-            const bool inside = gx >= 0 && gx < a.W && gy >= 0 && gy < a.H;
-            float v[4] = {acc[0], acc[1], acc[2], acc[3]};
-#pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] += bias[i];
-            if (l < 4) {
-                if (ox >= rwo) continue;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) v[i] = (inside && v[i] > 0.f) ? v[i] : 0.f;
-                uint2 pk;
-                pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-                pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
-                *reinterpret_cast<uint2*>(out + (oy * rwo + ox) * FZ_PS + 8 * grp) = pk;
-            } else if (ox < rwo && inside) {
-                *reinterpret_cast<float4*>(a.out + (((size_t)n * a.H + gy) * a.W + gx) * 16 + 4 * grp) =
-                    make_float4(v[0], v[1], v[2], v[3]);
-            }
-        }
-        __syncthreads();
-    }
+    fz_load_weights(a, 1, col, grp, w1);
+    fz_layer<0>(a, bufA, bufB, w0, col, grp, wv, x0, y0, n);
+    __syncthreads();
+    fz_load_weights(a, 2, col, grp, w0);
+    fz_layer<1>(a, bufB, bufA, w1, col, grp, wv, x0, y0, n);
+    __syncthreads();
+    fz_load_weights(a, 3, col, grp, w1);
+    fz_layer<2>(a, bufA, bufB, w0, col, grp, wv, x0, y0, n);
+    __syncthreads();
+    fz_load_weights(a, 4, col, grp, w0);
+    fz_layer<3>(a, bufB, bufA, w1, col, grp, wv, x0, y0, n);
+    __syncthreads();
+    fz_layer<4>(a, bufA, bufB, w0, col, grp, wv, x0, y0, n);
 }
 
 }  // namespace cnn

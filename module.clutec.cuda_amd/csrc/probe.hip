@@ -40,6 +40,28 @@ __global__ __launch_bounds__(256) void valu_peak_kernel(uint32_t* out, int iters
     out[blockIdx.x * 256 + threadIdx.x] = s;  // keeps the chains live
 }
 
+typedef __bf16 probe_bf2 __attribute__((ext_vector_type(2)));
+typedef float probe_f2 __attribute__((ext_vector_type(2)));
+
+// Counts the float bit patterns, NaNs excepted, whose bf16 from v_cvt_pk_bf16_f32 (the
+// one-launch network's epilogue) differs from the integer round-to-nearest-even of the
+// network's other kernels and of its CPU reference; all 2^32 patterns, grid-stride.
+__global__ __launch_bounds__(256) void bf16_rounding_kernel(unsigned long long* bad)
+{
+    unsigned long long cnt = 0;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * 256 + threadIdx.x; i < (1ull << 32);
+         i += (unsigned long long)gridDim.x * 256) {
+        const uint32_t u = (uint32_t)i;
+        if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) continue;
+        const float f = __uint_as_float(u);
+        const probe_f2 v = {f, f};
+        const uint32_t hw = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, probe_bf2)) & 0xFFFFu;
+        const uint32_t sw = (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+        cnt += hw != sw;
+    }
+    if (cnt) atomicAdd(bad, cnt);
+}
+
 }  // namespace
 }  // namespace stereo
 
@@ -80,5 +102,26 @@ extern "C" stereo_status stereo_valu_peak(stereo_context* ctx, int32_t op, doubl
     HIP_TRY(ctx, e, "valu probe");
     const double ops = (double)stereo::PROBE_BLOCKS * 256 * iters * stereo::UNROLL * stereo::CHAINS;
     *tera_ops = best > 0.f ? ops / (best * 1e-3) / 1e12 : 0.0;
+    return STEREO_OK;
+}
+
+extern "C" stereo_status stereo_bf16_rounding_check(stereo_context* ctx, uint64_t* mismatches)
+{
+    if (!ctx || !mismatches) return STEREO_ERROR_INVALID_ARGUMENT;
+    stereo_status st;
+    if ((st = set_device(ctx))) return st;
+    GUARD(ctx, "stereo_bf16_rounding_check");
+    unsigned long long* d = nullptr;
+    unsigned long long h = 0;
+    hipError_t e = hipMalloc((void**)&d, sizeof(h));
+    if (e == hipSuccess) e = hipMemset(d, 0, sizeof(h));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(stereo::bf16_rounding_kernel, dim3(4096), dim3(256), 0, nullptr, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+    if (d) (void)hipFree(d);
+    HIP_TRY(ctx, e, "bf16 rounding check");
+    *mismatches = h;
     return STEREO_OK;
 }

@@ -5,6 +5,8 @@ FP32 outputs agree with the FP64 oracle to FP32 accumulation error (checked
 relative to the sum of absolute products); bf16 outputs agree to one bf16 rounding.
 Network gradients are checked layer by layer against the oracle run with the same
 bf16 rounding points."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -212,6 +214,15 @@ def test_training_reduces_loss(ctx):
     print(f"loss {first} -> {last}", flush=True)
     assert last[0] + last[1] < 0.8 * (first[0] + first[1])
     assert np.all(np.isfinite(net.params()))
+
+
+def test_packed_bf16_conversion_rounds_like_the_integer_rule(ctx):
+    """The one-launch network packs activations with the device's bf16 conversion
+    instruction; the layer kernels and the CPU reference round with the integer
+    nearest-even rule.  Every float bit pattern but NaN gives the same bf16."""
+    n = C.c_uint64(123)
+    assert cs.lib.stereo_bf16_rounding_check(ctx.handle, C.byref(n)) == 0
+    assert n.value == 0
 
 
 def test_fused_inference_equals_layer_launches(ctx):
