@@ -755,10 +755,13 @@ __global__ __launch_bounds__(256) void aw1_ref_stack_kernel(RefArgs a)
     static_assert(IROWS == RROWS + P3 - 1, "the gate and the sub-patches read the same rows");
     static_assert(4 * SROWS <= 256, "one thread per sub-patch row and column group");
 
+    // row stride of the column and sub-patch sums: 50 (18 mod 32) puts the two rows of 18
+    // results a half-wave reads in different banks (sub-patch 5); 33 otherwise
+    constexpr int SS = SP == 5 ? 50 : 33;
     __shared__ uint8_t sl[IROWS][32];
     __shared__ uint8_t sr[IROWS][TW + 1];
-    __shared__ int scol[SROWS][33];
-    __shared__ int ssp[SROWS][33];
+    __shared__ int scol[SROWS][SS];
+    __shared__ int ssp[SROWS][SS];
     __shared__ float stab[256];
     __shared__ int sflag[NB];
     __shared__ int sany;
