@@ -72,21 +72,9 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
         m1 = p.bins[tb1 * p.ntx + tx];
         ysplit = (tb0 + 1) * BM_TILE_H;
         const int b0 = ds >> 3, b1 = (de - 1) >> 3;
-        if (((m0 | m1) & (uint32_t)((2ull << b1) - (1ull << b0))) == 0u) {
-            // nothing to search in this segment: every right-view partial the loop below
-            // would write (slots 256 - de .. 318 - ds) and every left key is the empty key
-            const int lane = tid & 63, gw = tid >> 6;
-            const int rows_here = min(RG, H - (y0 + gw * RG));
-            uint32_t* const rp = p.rpart + (((size_t)seg * p.ntx + tx) * H + y0 + gw * RG) * BM_NPART;
-            for (int kk = 256 - de + lane; kk <= 318 - ds; kk += 64)
-                for (int i = 0; i < rows_here; ++i) rp[i * BM_NPART + kk] = 0u;
-            const int x = x0 + lane;
-            if (x < W) {
-                uint32_t* lk = p.lkey + (size_t)seg * W * H + (size_t)(y0 + gw * RG) * W + x;
-                for (int i = 0; i < rows_here; ++i) lk[(size_t)i * W] = 0u;
-            }
-            return;
-        }
+        // nothing to search in this segment: its keys would all be empty, which the key
+        // planes already hold
+        if (((m0 | m1) & (uint32_t)((2ull << b1) - (1ull << b0))) == 0u) return;
     }
 
     // row phase identity: H row hr is image row y0 - B + hr
@@ -105,7 +93,8 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
     const int gw = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x = x0 + lane;
     const int rows_here = H - (y0 + gw * RG);
-    uint32_t* const rp = p.rpart + (((size_t)seg * p.ntx + tx) * H + y0 + gw * RG) * BM_NPART;
+    // the wave's first row of the right-key plane; every segment folds into it
+    uint32_t* const rp = p.rkey + (size_t)(y0 + gw * RG) * W;
     uint32_t* const eb = &ebuf[gw][0][0];
     const bool lane_ok = x >= p.bx && x < W - p.bx;
     const int dlim = x - p.bx;
@@ -118,11 +107,14 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
     auto flush = [&](int dl) {
         if ((dl & 63) == 63 || dl == de - 1) {
             const int base = dl & ~63;
-            if (lane <= (dl & 63) && base + lane >= ds) {
-                const int k = 318 - (base + lane);
+            // right pixel x0 + 63 - (base + lane), inside the image
+            const int xr = x0 + 63 - (base + lane);
+            if (lane <= (dl & 63) && base + lane >= ds && xr >= 0 && xr < W) {
 #pragma unroll
-                for (int i = 0; i < RG; ++i)
-                    if (i < rows_here) rp[i * BM_NPART + k] = eb[i * 64 + lane];
+                for (int i = 0; i < RG; ++i) {
+                    const uint32_t v = eb[i * 64 + lane];
+                    if (i < rows_here && v) atomicMax(&rp[(size_t)i * W + xr], v);
+                }
             }
         }
     };
@@ -256,16 +248,19 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
         }
     }
     {
-        const int k = lane + 255 - (de - 1);
+        // what is still in flight belongs to right pixels x0 + lane - (de - 1)
+        const int xr = x0 + lane - (de - 1);
+        if (xr >= 0 && xr < W) {
 #pragma unroll
-        for (int i = 0; i < RG; ++i)
-            if (i < rows_here) rp[i * BM_NPART + k] = racc[i];
+            for (int i = 0; i < RG; ++i)
+                if (i < rows_here && racc[i]) atomicMax(&rp[(size_t)i * W + xr], racc[i]);
+        }
     }
     if (x < W) {
-        uint32_t* lk = p.lkey + (size_t)seg * W * H + (size_t)(y0 + gw * RG) * W + x;
+        uint32_t* lk = p.lkey + (size_t)(y0 + gw * RG) * W + x;
 #pragma unroll
         for (int i = 0; i < RG; ++i)
-            if (i < rows_here) lk[(size_t)i * W] = lacc[i];
+            if (i < rows_here && lacc[i]) atomicMax(&lk[(size_t)i * W], lacc[i]);
     }
 }
 

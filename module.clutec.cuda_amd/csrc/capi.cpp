@@ -464,7 +464,8 @@ stereo_status stereo_matcher_configure(stereo_context* ctx, stereo_driver** out,
     hipError_t e;
     if ((e = hipMalloc((void**)&d->padL, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
     if ((e = hipMalloc((void**)&d->padR, d->pad_pitch * H)) != hipSuccess) return cleanup(hip_fail(ctx, e, "hipMalloc"));
-    if (ncc || aw) {
+    if (ncc) {
+        // per segment: stereo_matcher_scores reads the left keys after the frame
         if ((e = hipMalloc((void**)&d->lkey, (size_t)d->nseg * W * H * 4)) != hipSuccess)
             return cleanup(hip_fail(ctx, e, "hipMalloc"));
         if ((e = hipMalloc((void**)&d->rpart, (size_t)d->nseg * d->ntx * H * stereo::BM_NPART * 4)) != hipSuccess)
@@ -541,11 +542,17 @@ static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool disp
         np.bins_nty = bins_nty;
         HIP_TRY(ctx, stereo::launch_ncc(d->mp.window_w, np, s), "ncc launch");
     } else if (aw) {
+        if (d->planes_dirty) {
+            // an earlier frame's merge did not run: clear what its matcher folded in
+            HIP_TRY(ctx, stereo::launch_bm_clear(d->lkey, (size_t)W * H, s), "clear launch");
+            HIP_TRY(ctx, stereo::launch_bm_clear(d->rkey, (size_t)W * H, s), "clear launch");
+            d->planes_dirty = false;
+        }
         stereo::AwParams ap;
         ap.L = pl;
         ap.R = pr;
         ap.lkey = d->lkey;
-        ap.rpart = d->rpart;
+        ap.rkey = d->rkey;
         ap.W = W;
         ap.H = H;
         ap.D = d->mp.num_disparities;
@@ -556,6 +563,7 @@ static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool disp
         ap.seg_len = d->seg_len;
         ap.bins = bins;
         ap.bins_nty = bins_nty;
+        d->planes_dirty = true;  // until the merge has run
         HIP_TRY(ctx, stereo::launch_aw(d->mp.window_w, ap, s), "adaptive-window launch");
     } else {
         if (d->planes_dirty) {

@@ -154,9 +154,10 @@ struct stereo_driver {
     uint8_t* padR = nullptr;
     size_t pad_pitch = 0;
     size_t pad_elem = 0;
-    // keys: per disparity segment for NCC and the adaptive windows (lkey, rpart); one
-    // plane each for SAD and census (lkey, rkey), cleared by every frame's merge --
-    // planes_dirty when a frame's merge did not run after its matcher
+    // keys: per disparity segment for NCC (lkey, rpart; stereo_matcher_scores reads the
+    // left keys after the frame); one plane each for SAD, census and the adaptive windows
+    // (lkey, rkey), cleared by every frame's merge -- planes_dirty when a frame's merge did
+    // not run after its matcher
     uint32_t* lkey = nullptr;
     uint32_t* rpart = nullptr;
     uint32_t* rkey = nullptr;

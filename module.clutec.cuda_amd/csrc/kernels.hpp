@@ -166,8 +166,8 @@ hipError_t launch_ncc_score(const NccScoreParams& f, hipStream_t s);
 // ---- adaptive-window matchers -------------------------------------------------------
 struct AwParams {
     View L, R;             // uint8 grey, padded views like the SAD matcher's inputs
-    uint32_t* lkey;        // nseg*W*H
-    uint32_t* rpart;       // nseg*ntx*H*BM_NPART
+    uint32_t* lkey;        // W*H left-key plane, zero at the launch (every segment folds in)
+    uint32_t* rkey;        // W*H right-key plane, zero at the launch
     int W, H, D;
     int bx, by;
     int ntx, nseg, seg_len;
