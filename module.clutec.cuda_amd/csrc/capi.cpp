@@ -591,6 +591,10 @@ static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool disp
                 "matcher launch");
     }
     if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[1], s), "hipEventRecord");
+    // a failure here leaves the key planes holding this frame's keys (the next frame
+    // clears them first); STEREO_FAIL_AT can select it
+    if (injected_failure())
+        return fail(ctx, STEREO_ERROR_HIP, "%s: injected failure between the matcher and the merge", d->name);
     stereo::BmFinishParams f;
     f.lkey = d->lkey;
     f.rpart = d->rpart;

@@ -57,6 +57,66 @@ def test_injected_failure_is_reported_and_recovers(fail_at):
     assert len(oks) == 3 and len({l.split()[-1] for l in oks}) == 1 and int(oks[0].split()[-1]) > 0
 
 
+PLANES_CHILD = r"""
+import sys, hashlib
+sys.path[:0] = [{pkg!r}]
+import clutec_stereo as cs
+ctx = cs.Context(0)
+pairs = [cs.synth_pair(6, 320, 240, 1, 5, 63, 8), cs.synth_pair(5, 320, 240, 0, 8, 63, 8)]
+for algo, win, cen in ((cs.ALGO_CENSUS, (9, 9), (9, 7)), (cs.ALGO_SAD, (7, 7), (1, 1)), (cs.ALGO_AW1, (5, 5), (1, 1))):
+    m = None
+    for _ in range(2):  # an injected failure may hit the configure
+        try:
+            m = cs.Matcher(ctx, 320, 240, algorithm=algo, window=win, census=cen, num_disp=64, tol=1)
+            break
+        except cs.StereoError as e:
+            print(algo, "configure error", str(e), flush=True)
+    for attempt in range(3):
+        L, R = pairs[min(attempt, 1)]  # frame 0: another pair than frames 1 and 2
+        try:
+            out = m.run(L, R, maps=True)
+        except cs.StereoError as e:
+            print(algo, "attempt", attempt, "error", str(e), flush=True)
+            continue
+        h = hashlib.md5(b"".join(o.tobytes() for o in out)).hexdigest()
+        print(algo, "attempt", attempt, "ok", h, flush=True)
+"""
+
+
+def test_frame_after_an_interrupted_merge_is_clean():
+    """The census, SAD and adaptive-window matchers fold their keys into two planes that
+    every frame's merge clears.  A frame whose merge does not run (a failure injected
+    between the matcher and the merge launch) leaves its keys there; the next frame, on
+    another pair, must clear them and equal that pair's frame in a run without failures
+    (stale keys of the other pair would win some of its atomicMax and change the maps)."""
+    code = PLANES_CHILD.format(pkg=os.path.join(ROOT, "module.clutec.cuda_amd"))
+
+    def run(env_extra):
+        env = dict(os.environ, **env_extra)
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr[-2000:]
+        return r.stdout.strip().splitlines()
+
+    clean = {l.split()[0]: l.split()[-1] for l in run({}) if " ok " in l and " attempt 1 " in l}
+    assert len(clean) == 3
+    checked = set()
+    for fail_at in range(1, 60):
+        print(f"child with STEREO_FAIL_AT={fail_at}", flush=True)
+        lines = run({"STEREO_FAIL_AT": str(fail_at)})
+        for l in lines:
+            if "between the matcher and the merge" not in l:
+                continue
+            algo, _, k = l.split()[:3]
+            nxt = [m for m in lines if m.startswith(f"{algo} attempt {int(k) + 1} ")]
+            if int(k) == 0:
+                # the frame after the interrupted one, on the other pair
+                assert nxt and nxt[0].split()[3] == "ok" and nxt[0].split()[-1] == clean[algo], lines
+                checked.add(algo)
+        if len(checked) == 3:
+            break
+    assert len(checked) == 3, checked
+
+
 def test_cnn_state_round_trip_resumes_training(ctx):
     W, H, D = 256, 128, 64
     frames = [_frame(ctx, 300 + i, W, H, D)[0] for i in range(2)]
