@@ -110,6 +110,42 @@ struct Tracker {
     int disp, mx, prev, prev2, nxt, nxt2, last;
 };
 
+// Copies the ROWS x COLS bytes of img at (x0, y0) into dst[r][c] (row pitch DP), zero
+// outside the image or at columns c >= cmax, with NT threads: a thread issues its loads
+// FB at a time before storing them, so that FB are in flight together (a load-then-store
+// loop keeps one).
+template <int ROWS, int COLS, int DP, int NT>
+__device__ __forceinline__ void fill_window(uint8_t* dst, const View& img, int x0, int y0, int cmax, int tid_in)
+{
+    constexpr int N = ROWS * COLS, K = (N + NT - 1) / NT, FB = 8;
+    // an opaque copy of the thread index: the element positions derived from it are
+    // recomputed per call instead of being hoisted out of the caller's loops into
+    // registers that stay live through them
+    int tid = tid_in;
+    asm volatile("" : "+v"(tid));
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += FB) {
+        uint32_t v[FB];
+#pragma unroll
+        for (int k = 0; k < FB; ++k) {
+            const int i = tid + NT * (k0 + k);
+            const int r = i / COLS, c = i - r * COLS;
+            const int x = x0 + c, y = y0 + r;
+            v[k] = (k0 + k < K && i < N && c < cmax && x >= 0 && x < img.w && y >= 0 && y < img.h)
+                       ? row_ptr<const uint8_t>(img, y)[x]
+                       : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < FB; ++k) {
+            const int i = tid + NT * (k0 + k);
+            if (k0 + k < K && i < N) {
+                const int r = i / COLS, c = i - r * COLS;
+                dst[r * DP + c] = (uint8_t)v[k];
+            }
+        }
+    }
+}
+
 // The peak update (NCC EvalSim :280-333, AW1 EvalSimAW :262-315), branch-free.
 __device__ __forceinline__ void track(Tracker& s, int v, int d, bool upd)
 {
@@ -201,11 +237,7 @@ __global__ __launch_bounds__(64) void ncc_ref_kernel(RefArgs a)
     const int iy = t & 15, ix0 = (t >> 4) * CG;
     const int row = 16 * b + iy;  // in the strip
 
-    for (int i = l; i < DR * S::TBX; i += 64) {
-        const int r = i / S::TBX, c = i % S::TBX;
-        const int x = X0 + c, y = Y0s + r;
-        sl[r][c] = (x < W && y < H) ? row_ptr<const uint8_t>(a.L, y)[x] : 0;
-    }
+    fill_window<DR, S::TBX, S::TBX, 64>(&sl[0][0], a.L, X0, Y0s, S::TBX, l);
     __syncthreads();
 
     bool written[CG];
@@ -358,11 +390,7 @@ __global__ __launch_bounds__(64) void ncc_ref_kernel(RefArgs a)
         const RefGroup g = ref_group<S::TBX>(X0, D0, W, a.offset, a.ltor, cap);
         if (g.n <= 0) continue;
         __syncthreads();  // the previous group's readers of sr, scol and the statistics are done
-        for (int i = l; i < DR * TW; i += 64) {
-            const int r = i / TW, c = i % TW;
-            const int x = g.lo + c, y = Y0s + r;
-            sr[r][c] = (c < g.n + S::TBX - 1 && x >= 0 && x < W && y < H) ? row_ptr<const uint8_t>(a.R, y)[x] : 0;
-        }
+        fill_window<DR, TW, TW, 64>(&sr[0][0], a.R, g.lo, Y0s, g.n + S::TBX - 1, l);
         __syncthreads();
         // the right windows' statistics at every start column the group uses: the means
         // from the column sums, then the deviations from the column sums of squares (a
