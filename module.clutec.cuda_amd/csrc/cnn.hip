@@ -593,7 +593,7 @@ __global__ __launch_bounds__(1024) void band_items_kernel(CnnBandArgs a)
 }
 
 
-// Inference through the five 16-channel layers in one launch.  A workgroup of 8 waves
+// Inference through the five 16-channel layers in one launch.  A workgroup of 16 waves
 // owns an output tile of FZ_TW x FZ_TH quarter pixels and keeps every intermediate
 // activation of its tile in LDS: the input with the network's whole halo (the dilations
 // add up to 10), then each layer's output over the region the remaining layers read
@@ -607,7 +607,7 @@ constexpr int FZ_TW = 32, FZ_TH = 16, FZ_PS = 40;  // staged pixel stride: 16 bf
 constexpr int FZ_AW = FZ_TW + 20, FZ_AH = FZ_TH + 20;  // buffer A: the input region (halo 10)
 constexpr int FZ_BW = FZ_TW + 18, FZ_BH = FZ_TH + 18;  // buffer B: layer 0's output region (halo 9)
 constexpr int FZ_IL = 4;  // blocks of 16 output pixels a wave computes at once
-constexpr int FZ_NW = 8;  // waves per workgroup
+constexpr int FZ_NW = 16;  // waves per workgroup: four per SIMD, the one workgroup a CU holds
 
 // The halo of each layer's input (grow[5]: the tile itself).
 constexpr int fz_grow(int l) { return l == 0 ? 10 : l == 1 ? 9 : l == 2 ? 7 : l == 3 ? 3 : l == 4 ? 1 : 0; }
@@ -699,7 +699,7 @@ __device__ __forceinline__ void fz_layer(const CnnFusedArgs& a, const uint8_t* i
     }
 }
 
-__global__ __launch_bounds__(512) void cnn_fused_forward_kernel(CnnFusedArgs a)
+__global__ __launch_bounds__(64 * FZ_NW) void cnn_fused_forward_kernel(CnnFusedArgs a)
 {
     __shared__ __align__(16) uint8_t bufA[FZ_AW * FZ_AH * FZ_PS];
     __shared__ __align__(16) uint8_t bufB[FZ_BW * FZ_BH * FZ_PS];
@@ -717,7 +717,7 @@ __global__ __launch_bounds__(512) void cnn_fused_forward_kernel(CnnFusedArgs a)
     FzWeights w0, w1;
     fz_load_weights(a, 0, col, grp, w0);
     // stage the input region (halo 10), zero outside the image: 4 x 8-byte chunks per pixel
-    for (int i = threadIdx.x; i < FZ_AW * FZ_AH * 4; i += 512) {
+    for (int i = threadIdx.x; i < FZ_AW * FZ_AH * 4; i += 64 * FZ_NW) {
         const int px = i >> 2, q = i & 3;
         const int gx = x0 - 10 + px % FZ_AW, gy = y0 - 10 + px / FZ_AW;
         uint2 v = make_uint2(0u, 0u);
@@ -830,7 +830,7 @@ hipError_t launch_cnn_fused_forward(const CnnFusedArgs& a, hipStream_t s)
     if (a.N < 1 || a.H < 1 || a.W < 1) return hipErrorInvalidValue;
     const long ntiles = (long)a.N * ((a.H + cnn::FZ_TH - 1) / cnn::FZ_TH) * ((a.W + cnn::FZ_TW - 1) / cnn::FZ_TW);
     if (ntiles > 0x7fffffffL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(cnn::cnn_fused_forward_kernel, dim3((int)ntiles), dim3(512), 0, s, a);
+    hipLaunchKernelGGL(cnn::cnn_fused_forward_kernel, dim3((int)ntiles), dim3(64 * cnn::FZ_NW), 0, s, a);
     return hipGetLastError();
 }
 
