@@ -18,6 +18,10 @@
 // Both images of a pair go through one launch (blockIdx.z).  On request the same
 // launch writes the 4 x 4 box means (sum + 8) >> 4 of both images (the quarter-
 // resolution pair of the learned proposal), read from the staged tile.
+//
+// The matchers take their descriptors from census_tile_pk instead: the same comparisons
+// at one VALU operation per bit, in a bit order of its own (see there).  The operator
+// (launch_census) keeps the raster order above.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -124,6 +128,145 @@ __device__ __forceinline__ void census_tile(const View& src, const View& dst, co
     }
 }
 
+// The matchers' own descriptors.  They compare every neighbour of the window with the
+// centre by the same rule as census_tile, but lay the bits out in an order made for
+// packed arithmetic; the matchers only ever take the Hamming distance of two
+// descriptors of one layout, which does not depend on the order.  The staged tile
+// holds, per word, pixels c and c + 1 of a row as two 16-bit halves, so one LDS read
+// brings two neighbours and one v_pk_sub_i16 compares both with the centre
+// (replicated in both halves): a half is negative exactly when its neighbour is
+// greater.  A v_perm_b32 with the sign-replicating selectors turns the signs of two
+// such results into four 0x00 / 0xFF bytes, and one v_and_or_b32 keeps bit s of each
+// byte: four descriptor bits for four VALU operations, against eight in census_tile.
+// A row's neighbour pairs go dx = (0, 1), (2, 3), ...; with an odd width the last
+// neighbour of two consecutive rows forms a pair, and the last row's alone is paired
+// with 0 (never greater than the centre, so that bit is always 0).  The results are
+// taken in window-row order, two at a time, into bit s of the bytes of the low word
+// (first 8 pairs of results), then of the high word.  The centre's own comparison
+// gives 0, as in census_tile.
+typedef short census_s2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t census_pk_gt(uint32_t cen2, uint32_t pair)
+{
+    const census_s2 c = __builtin_bit_cast(census_s2, cen2);
+    const census_s2 n = __builtin_bit_cast(census_s2, pair);
+    return __builtin_bit_cast(uint32_t, (census_s2)(c - n));
+}
+
+// (s & m) | w as one v_and_or_b32 (the compiler otherwise pairs the ORs of two groups into
+// a v_or3_b32 after two v_and_b32: three operations for two groups instead of two)
+__device__ __forceinline__ uint32_t census_and_or(uint32_t s, uint32_t m, uint32_t w)
+{
+    uint32_t r;
+    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(s), "s"(m), "v"(w));
+    return r;
+}
+
+template <int CW, int CH>
+__device__ __forceinline__ void census_tile_pk(const View& src, const View& dst, const View& q, int x0, int y0,
+                                               uint32_t (*tile)[CensusShape<CW, CH>::LW + 1])
+{
+    constexpr int RX = CW / 2, RY = CH / 2;
+    constexpr int LW = CT_W + 2 * RX;
+    constexpr int LH = CT_H + 2 * RY;
+    constexpr int NP = CW / 2;        // neighbour pairs of a window row
+    constexpr bool ODD = (CW & 1) != 0;  // a last neighbour at dx = CW - 1
+    static_assert((CW * CH + 3) / 4 <= 16, "descriptor longer than 64 bits");
+    const int tid = threadIdx.x;
+    for (int i = tid; i < LH * LW; i += 256) {
+        const int r = i / LW, c = i - r * LW;
+        const uint8_t* row = row_ptr<const uint8_t>(src, clampi(y0 - RY + r, 0, src.h - 1));
+        const int gx = x0 - RX + c;
+        tile[r][c] = (uint32_t)row[clampi(gx, 0, src.w - 1)] | ((uint32_t)row[clampi(gx + 1, 0, src.w - 1)] << 16);
+    }
+    __syncthreads();
+
+    const int cx = tid & 63;
+    const int strip = tid >> 6;
+    const int x = x0 + cx;
+    const uint32_t* col = &tile[strip * CT_STRIP][cx];
+    uint32_t acc[CT_STRIP][2], pend[CT_STRIP], cen2[CT_STRIP];
+    bool has[CT_STRIP];
+    int slot[CT_STRIP];
+#pragma unroll
+    for (int k = 0; k < CT_STRIP; ++k) {
+        cen2[k] = __builtin_amdgcn_perm(0u, col[(k + RY) * (LW + 1) + RX], 0x01000100u);
+        acc[k][0] = acc[k][1] = 0u;
+        pend[k] = 0u;
+        has[k] = false;
+        slot[k] = 0;
+    }
+    // one result into pixel k's descriptor: every second one completes a group of four
+    auto push = [&](int k, uint32_t t) {
+        if (!has[k]) {
+            pend[k] = t;
+            has[k] = true;
+            return;
+        }
+        const uint32_t s = __builtin_amdgcn_perm(pend[k], t, 0x0B0A0908u);
+        uint32_t& w = acc[k][slot[k] >> 3];
+        w = census_and_or(s, 0x01010101u << (slot[k] & 7), w);
+        ++slot[k];
+        has[k] = false;
+    };
+    constexpr int NR = CT_STRIP + CH - 1;
+    constexpr int NV = NP + (ODD ? 1 : 0);
+    uint32_t v[NV], vn[NV], sprev = 0u;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] = col[2 * j];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        if (r + 1 < NR) {
+#pragma unroll
+            for (int j = 0; j < NV; ++j) vn[j] = col[(r + 1) * (LW + 1) + 2 * j];
+        }
+        // the last neighbours of this row and the one before, and this row's alone
+        const uint32_t qpair = ODD ? __builtin_amdgcn_perm(v[NV - 1], sprev, 0x05040100u) : 0u;
+        const uint32_t qlast = ODD ? (v[NV - 1] & 0xFFFFu) : 0u;
+#pragma unroll
+        for (int k = 0; k < CT_STRIP; ++k) {
+            const int dy = r - k;
+            if (dy < 0 || dy >= CH) continue;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) push(k, census_pk_gt(cen2[k], v[j]));
+            if (ODD) {
+                if (dy & 1) push(k, census_pk_gt(cen2[k], qpair));
+                else if (dy == CH - 1) push(k, census_pk_gt(cen2[k], qlast));
+            }
+        }
+        if (ODD) sprev = v[NV - 1];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] = vn[j];
+        asm volatile("" ::: "memory");
+    }
+#pragma unroll
+    for (int k = 0; k < CT_STRIP; ++k)
+        if (has[k]) push(k, 0u);
+#pragma unroll
+    for (int k = 0; k < CT_STRIP; ++k) asm volatile("" : "+v"(acc[k][0]), "+v"(acc[k][1]));
+#pragma unroll
+    for (int k = 0; k < CT_STRIP; ++k) {
+        const int y = y0 + strip * CT_STRIP + k;
+        const bool inside = x >= RX && x < src.w - RX && y >= RY && y < src.h - RY;
+        if (x < dst.w && y < dst.h) {
+            const uint64_t d = inside ? (((uint64_t)acc[k][1] << 32) | acc[k][0]) : 0ull;
+            row_ptr<uint64_t>(dst, y)[x] = d;
+        }
+    }
+    if (q.ptr && tid < (CT_W / 4) * (CT_H / 4)) {
+        const int qx = tid & 15, qy = tid >> 4;
+        const int X = x0 / 4 + qx, Y = y0 / 4 + qy;
+        if (X < q.w && Y < q.h) {
+            uint32_t sum = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sum += tile[RY + 4 * qy + j][RX + 4 * qx + i] & 0xFFFFu;
+            row_ptr<uint8_t>(q, Y)[X] = (uint8_t)((sum + 8) >> 4);
+        }
+    }
+}
+
 // Reference 3x3 census into uint8 (CensusTransform.cu:103-119 bit layout).
 __global__ __launch_bounds__(256) void census3x3_ref_kernel(View src, View dst)
 {
@@ -148,13 +291,18 @@ __global__ __launch_bounds__(256) void census3x3_ref_kernel(View src, View dst)
     row_ptr<uint8_t>(dst, y)[x] = (uint8_t)v;
 }
 
-template <int CW, int CH>
+// PK: the matchers' bit order (census_tile_pk), else the raster order of the operator.
+template <int CW, int CH, bool PK>
 __global__ __launch_bounds__(256) void census_wide_kernel(View src0, View dst0, View src1, View dst1, View q0, View q1)
 {
     using S = CensusShape<CW, CH>;
     __shared__ uint32_t tile[S::LH][S::LW + 1];
-    census_tile<CW, CH>(blockIdx.z ? src1 : src0, blockIdx.z ? dst1 : dst0, blockIdx.z ? q1 : q0, blockIdx.x * CT_W,
-                        blockIdx.y * CT_H, tile);
+    if (PK)
+        census_tile_pk<CW, CH>(blockIdx.z ? src1 : src0, blockIdx.z ? dst1 : dst0, blockIdx.z ? q1 : q0,
+                               blockIdx.x * CT_W, blockIdx.y * CT_H, tile);
+    else
+        census_tile<CW, CH>(blockIdx.z ? src1 : src0, blockIdx.z ? dst1 : dst0, blockIdx.z ? q1 : q0,
+                            blockIdx.x * CT_W, blockIdx.y * CT_H, tile);
 }
 
 // Every image of a batch in one launch: workgroup b takes tile b - jobs[i].base of the
@@ -180,7 +328,7 @@ __global__ __launch_bounds__(256) void census_batch_kernel(const ImageJob* jobs,
     const int t = blockIdx.x - j.base;
     const int ntx = (j.src.w + CT_W - 1) / CT_W;
     const View none{nullptr, 0, 0, 0};
-    census_tile<CW, CH>(j.src, j.dst, none, (t % ntx) * CT_W, (t / ntx) * CT_H, tile);
+    census_tile_pk<CW, CH>(j.src, j.dst, none, (t % ntx) * CT_W, (t / ntx) * CT_H, tile);
 }
 
 // Rows of every image of a batch into the padded matcher inputs: workgroup b copies row
@@ -222,15 +370,34 @@ __global__ __launch_bounds__(256) void pad_copy_kernel(View s0, View d0, View s1
     }
 }
 
-template <int CW, int CH>
+template <int CW, int CH, bool PK>
 hipError_t launch_wide(const View& src0, const View& dst0, const View* src1, const View* dst1, const View* q0,
                        const View* q1, hipStream_t s)
 {
     dim3 grid((src0.w + CT_W - 1) / CT_W, (src0.h + CT_H - 1) / CT_H, src1 ? 2 : 1);
     const View none{nullptr, 0, 0, 0};
-    hipLaunchKernelGGL((census_wide_kernel<CW, CH>), grid, dim3(256), 0, s, src0, dst0, src1 ? *src1 : src0,
+    hipLaunchKernelGGL((census_wide_kernel<CW, CH, PK>), grid, dim3(256), 0, s, src0, dst0, src1 ? *src1 : src0,
                        dst1 ? *dst1 : dst0, q0 ? *q0 : none, q1 ? *q1 : none);
     return hipGetLastError();
+}
+
+template <bool PK>
+hipError_t launch_wide_any(const View& src0, const View& dst0, const View* src1, const View* dst1, int cw, int ch,
+                           hipStream_t s, const View* q0, const View* q1)
+{
+    if (src1 && (src1->w != src0.w || src1->h != src0.h)) return hipErrorInvalidValue;
+    if (q0 || q1) {
+        // quarter-resolution outputs: sizes a multiple of 4, one per image
+        if ((src0.w & 3) || (src0.h & 3) || (q1 && !src1)) return hipErrorInvalidValue;
+        if (q0 && (q0->w != src0.w / 4 || q0->h != src0.h / 4)) return hipErrorInvalidValue;
+        if (q1 && (q1->w != src0.w / 4 || q1->h != src0.h / 4)) return hipErrorInvalidValue;
+    }
+    if (cw == 9 && ch == 7) return launch_wide<9, 7, PK>(src0, dst0, src1, dst1, q0, q1, s);
+    if (cw == 7 && ch == 9) return launch_wide<7, 9, PK>(src0, dst0, src1, dst1, q0, q1, s);
+    if (cw == 5 && ch == 5) return launch_wide<5, 5, PK>(src0, dst0, src1, dst1, q0, q1, s);
+    if (cw == 7 && ch == 7) return launch_wide<7, 7, PK>(src0, dst0, src1, dst1, q0, q1, s);
+    if (cw == 3 && ch == 3) return launch_wide<3, 3, PK>(src0, dst0, src1, dst1, q0, q1, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -249,7 +416,7 @@ hipError_t launch_census(const View& src, const View& dst, int cw, int ch, int r
         hipLaunchKernelGGL(census3x3_ref_kernel, grid, dim3(256), 0, s, src, dst);
         return hipGetLastError();
     }
-    return launch_census_pair(src, dst, nullptr, nullptr, cw, ch, s);
+    return launch_wide_any<false>(src, dst, nullptr, nullptr, cw, ch, s, nullptr, nullptr);
 }
 
 hipError_t launch_pad_copy_pair(const View& src0, const View& dst0, const View& src1, const View& dst1, hipStream_t s)
@@ -287,19 +454,7 @@ hipError_t launch_pad_copy_batch(const ImageJob* jobs, int njobs, int total_rows
 hipError_t launch_census_pair(const View& src0, const View& dst0, const View* src1, const View* dst1, int cw, int ch,
                               hipStream_t s, const View* q0, const View* q1)
 {
-    if (src1 && (src1->w != src0.w || src1->h != src0.h)) return hipErrorInvalidValue;
-    if (q0 || q1) {
-        // quarter-resolution outputs: sizes a multiple of 4, one per image
-        if ((src0.w & 3) || (src0.h & 3) || (q1 && !src1)) return hipErrorInvalidValue;
-        if (q0 && (q0->w != src0.w / 4 || q0->h != src0.h / 4)) return hipErrorInvalidValue;
-        if (q1 && (q1->w != src0.w / 4 || q1->h != src0.h / 4)) return hipErrorInvalidValue;
-    }
-    if (cw == 9 && ch == 7) return launch_wide<9, 7>(src0, dst0, src1, dst1, q0, q1, s);
-    if (cw == 7 && ch == 9) return launch_wide<7, 9>(src0, dst0, src1, dst1, q0, q1, s);
-    if (cw == 5 && ch == 5) return launch_wide<5, 5>(src0, dst0, src1, dst1, q0, q1, s);
-    if (cw == 7 && ch == 7) return launch_wide<7, 7>(src0, dst0, src1, dst1, q0, q1, s);
-    if (cw == 3 && ch == 3) return launch_wide<3, 3>(src0, dst0, src1, dst1, q0, q1, s);
-    return hipErrorInvalidValue;
+    return launch_wide_any<true>(src0, dst0, src1, dst1, cw, ch, s, q0, q1);
 }
 
 }  // namespace stereo

@@ -25,6 +25,8 @@ hipError_t launch_census_batch(const ImageJob* jobs, int njobs, int total_tiles,
 hipError_t launch_pad_copy_batch(const ImageJob* jobs, int njobs, int total_rows, hipStream_t s);
 // Copies two grey images of one size into padded matcher inputs (one launch).
 hipError_t launch_pad_copy_pair(const View& src0, const View& dst0, const View& src1, const View& dst1, hipStream_t s);
+// The matchers' census of one or two images (and optionally their 4 x 4 box means), in the
+// matchers' own bit order: only Hamming distances between such descriptors are meaningful.
 hipError_t launch_census_pair(const View& src0, const View& dst0, const View* src1, const View* dst1, int cw, int ch,
                               hipStream_t s, const View* q0 = nullptr, const View* q1 = nullptr);
 
