@@ -49,11 +49,23 @@ __device__ __forceinline__ void census_tile(const View& src, const View& dst, co
     constexpr int LH = CT_H + 2 * RY;
     constexpr int NHI = NB > 32 ? NB - 32 : 0;  // bits landing in the high word
     const int tid = threadIdx.x;
-    for (int i = tid; i < LH * LW; i += 256) {
-        const int r = i / LW, c = i - r * LW;
-        const int gy = clampi(y0 - RY + r, 0, src.h - 1);
-        const int gx = clampi(x0 - RX + c, 0, src.w - 1);
-        tile[r][c] = row_ptr<const uint8_t>(src, gy)[gx];
+    {
+        // every load of the thread issued before the first store: one load latency for
+        // the staging instead of one per position (a position past the tile repeats the
+        // last one and is not stored)
+        constexpr int NS = LH * LW, NIT = (NS + 255) / 256;
+        uint32_t a[NIT];
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int i = min(tid + 256 * k, NS - 1);
+            const int r = i / LW, c = i - r * LW;
+            a[k] = row_ptr<const uint8_t>(src, clampi(y0 - RY + r, 0, src.h - 1))[clampi(x0 - RX + c, 0, src.w - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int i = tid + 256 * k;
+            if (i < NS) tile[i / LW][i % LW] = a[k];
+        }
     }
     __syncthreads();
 
@@ -173,11 +185,24 @@ __device__ __forceinline__ void census_tile_pk(const View& src, const View& dst,
     constexpr bool ODD = (CW & 1) != 0;  // a last neighbour at dx = CW - 1
     static_assert((CW * CH + 3) / 4 <= 16, "descriptor longer than 64 bits");
     const int tid = threadIdx.x;
-    for (int i = tid; i < LH * LW; i += 256) {
-        const int r = i / LW, c = i - r * LW;
-        const uint8_t* row = row_ptr<const uint8_t>(src, clampi(y0 - RY + r, 0, src.h - 1));
-        const int gx = x0 - RX + c;
-        tile[r][c] = (uint32_t)row[clampi(gx, 0, src.w - 1)] | ((uint32_t)row[clampi(gx + 1, 0, src.w - 1)] << 16);
+    {
+        // every load of the thread issued before the first store, as in census_tile
+        constexpr int NS = LH * LW, NIT = (NS + 255) / 256;
+        uint32_t a[NIT], b[NIT];
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int i = min(tid + 256 * k, NS - 1);
+            const int r = i / LW, c = i - r * LW;
+            const uint8_t* row = row_ptr<const uint8_t>(src, clampi(y0 - RY + r, 0, src.h - 1));
+            const int gx = x0 - RX + c;
+            a[k] = row[clampi(gx, 0, src.w - 1)];
+            b[k] = row[clampi(gx + 1, 0, src.w - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int i = tid + 256 * k;
+            if (i < NS) tile[i / LW][i % LW] = a[k] | (b[k] << 16);
+        }
     }
     __syncthreads();
 
