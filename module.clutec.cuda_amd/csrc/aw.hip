@@ -278,6 +278,10 @@ __global__ __launch_bounds__(256, 2) void aw_fused_kernel(AwParams p)
     const int ty = rest % nty, seg = rest / nty;
     const int x0 = tx * TW;
     const int de = min(p.D, (seg + 1) * p.seg_len);
+    // no pixel of the tile has a candidate in the segment (left pixel x takes d when
+    // bx <= x < W - bx and d <= x - bx): the job would only fold zero keys
+    const int xmax = min(x0 + TW - 1, p.W - p.bx - 1);
+    if (xmax < x0 || xmax - p.bx < seg * p.seg_len) return;
     const bool edge = (x0 < p.bx + de - 1) || (x0 + TW > p.W - p.bx);
     if (edge)
         aw_tile<SP, true, BAND>(p, hbuf, ebuf, tx, ty, seg);

@@ -98,6 +98,16 @@ __device__ __forceinline__ bool job_is_edge(const Job& j)
     return (j.x0 < j.bx + j.de - 1) || (j.x0 + TW > j.W - j.bx);
 }
 
+// Whether any pixel of the tile has a candidate in [ds, de): left pixel x takes d when
+// bx <= x < W - bx and d <= x - bx.  Near the left edge a whole tile's segment can fall
+// short (1080p, D = 256: the last segment on the first three tile columns); such a job
+// would only fold zero keys, which the key planes skip.
+__device__ __forceinline__ bool job_has_candidates(const Job& j)
+{
+    const int xmax = min(j.x0 + TW - 1, j.W - j.bx - 1);
+    return xmax >= j.x0 && xmax - j.bx >= j.ds;
+}
+
 // tid: the thread index (threadIdx.x, or an opaque copy of it, see bm_list_kernel)
 template <class Cost, int AX, int AY, bool EDGE, bool ATOM>
 __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64],
@@ -296,6 +306,7 @@ __global__ __launch_bounds__(256, 2) void bm_fused_kernel(BmParams p)
     j.rp_kofs = 0;
     j.lk = p.lkey + (size_t)j.y0 * p.W + j.x0;
     j.lk_stride = p.W;
+    if (!job_has_candidates(j)) return;
     run_job<Cost, AX, AY, true>(j, hbuf, ebuf, threadIdx.x);
 }
 
@@ -335,6 +346,7 @@ __global__ __launch_bounds__(256, Cost::LIST_WG) void bm_list_kernel(BmListParam
         j.rp_kofs = 0;
         j.lk = p.lplane + f.plane_off + (size_t)j.y0 * f.W + j.x0;
         j.lk_stride = f.W;
+        if (!job_has_candidates(j)) continue;
         // an opaque copy of the thread index per item: the addresses derived from it are
         // recomputed for each item instead of being hoisted out of this loop into
         // registers that stay live through the whole item
