@@ -29,9 +29,9 @@ W, H, D = 1920, 1080, 256
 OPS_PER_EVAL = 12
 # HBM bytes per launch of the fused census kernel on this workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 256 MiB device copy
-# (profiles/bm_census_1080p_d256.md): 18,898 KiB / 0.5 + 94,298 KiB / 1.0; the writes
+# (profiles/bm_census_1080p_d256.md): 18,044 KiB / 0.5 + 94,298 KiB / 1.0; the writes
 # are the key planes' atomics
-TRAFFIC_BYTES = (18898 * 2 + 94298) * 1024
+TRAFFIC_BYTES = (18044 * 2 + 94298) * 1024
 # MI355X: 256 CUs x 64 lanes x 2.4 GHz = 39.3 T int32 lane-ops/s
 PEAK_INT_TOPS = 256 * 64 * 2.4e9 / 1e12
 PEAK_HBM_GBS = 8000.0
