@@ -115,7 +115,7 @@ def learned_leg(cs, np, ctx, L, R, dl, dr, full_disp, full_ms, steps):
     """The learned-proposal census matcher (c4) on the same frame: speedup over the
     full-range pipeline and the fraction of pixels equal to its output.  Also the
     same bands without the network's fallback (every tile on its coarse band), and
-    the pipeline launched stage by stage instead of as one graph."""
+    the pipeline replayed as one captured graph instead of launched stage by stage."""
     wdir = os.path.join(ROOT, "module.clutec.cuda_amd", "weights")
     wpath = os.path.join(wdir, "proposal_cnn.bin")
     if not os.path.exists(wpath):
@@ -132,9 +132,9 @@ def learned_leg(cs, np, ctx, L, R, dl, dr, full_disp, full_ms, steps):
     centre_residual = bool(meta.get("centre_residual", 0))
     chunk = int(meta.get("chunk", 32))
 
-    def run(t, direct=False, stages=True):
+    def run(t, graph=False, stages=True):
         lm = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, band=band, chunk=chunk, logit_thresh=t,
-                               centre_residual=centre_residual, direct_launch=direct)
+                               centre_residual=centre_residual, graph_replay=graph)
         out = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_I16)
         for _ in range(3):
             lm.process(out, None, None, dl, dr)
@@ -165,7 +165,7 @@ def learned_leg(cs, np, ctx, L, R, dl, dr, full_disp, full_ms, steps):
     res = run(thr)
     res.update({"band": band, "chunk": chunk, "logit_thresh": thr, "centre_residual": int(centre_residual),
                 "trained": {k: meta.get(k) for k in ("frames", "size", "steps", "batch") if k in meta}})
-    res["direct_launch"] = run(thr, direct=True, stages=False)
+    res["graph_replay"] = run(thr, graph=True, stages=False)
     # the same bands with no tile sent to the full range: what the network's confidence adds
     res["without_fallback"] = run(-1e30)
     # the reference's own band search (MatcherUseDisparity) around the census match at

@@ -133,7 +133,7 @@ class BatchTuning(C.Structure):
 class LearnedParams(C.Structure):
     _fields_ = [("matcher", MatcherParams), ("coarse_disparities", C.c_int32), ("band", C.c_int32),
                 ("chunk", C.c_int32), ("logit_thresh", C.c_float), ("centre_residual", C.c_int32),
-                ("direct_launch", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("graph_replay", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 CNN_PARAMS = 7554
@@ -850,7 +850,7 @@ class LearnedMatcher(_Driver):
     """Census matcher with the learned band proposal (stereo_learned_*)."""
 
     def __init__(self, ctx, W, H, cnn_params, window=(9, 9), census=(9, 7), num_disp=256, tol=1, band=8, chunk=64,
-                 logit_thresh=0.0, coarse_disp=0, centre_residual=False, direct_launch=False):
+                 logit_thresh=0.0, coarse_disp=0, centre_residual=False, graph_replay=False):
         super().__init__(ctx)
         self.W, self.H = W, H
         fmt = ImageFormat(W, H, PIXEL_LUM_U8)
@@ -861,7 +861,7 @@ class LearnedMatcher(_Driver):
         lp.chunk = chunk
         lp.logit_thresh = logit_thresh
         lp.centre_residual = 1 if centre_residual else 0
-        lp.direct_launch = 1 if direct_launch else 0
+        lp.graph_replay = 1 if graph_replay else 0
         p = np.ascontiguousarray(cnn_params, np.float32)
         _check(lib.stereo_learned_configure(ctx.handle, C.byref(self.handle), C.byref(fmt), C.byref(lp),
                                             p.ctypes.data, p.size), ctx)

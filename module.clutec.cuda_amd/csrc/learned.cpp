@@ -599,9 +599,9 @@ stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** out,
     if (Dc % 8 || Dc > 256 || p->band < 8 || p->band % 8 || p->chunk < 8 || p->chunk % 8)
         return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "learned matcher: coarse D %d, band %d, chunk %d must be multiples of 8",
                     Dc, p->band, p->chunk);
-    if ((p->centre_residual != 0 && p->centre_residual != 1) || (p->direct_launch != 0 && p->direct_launch != 1) ||
+    if ((p->centre_residual != 0 && p->centre_residual != 1) || (p->graph_replay != 0 && p->graph_replay != 1) ||
         p->reserved[0] || p->reserved[1])
-        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "learned matcher: centre_residual and direct_launch are 0 or 1, "
+        return fail(ctx, STEREO_ERROR_INVALID_CONFIG, "learned matcher: centre_residual and graph_replay are 0 or 1, "
                     "reserved fields 0");
     if ((st = set_device(ctx))) return st;
 
@@ -781,10 +781,10 @@ stereo_status stereo_learned_process(stereo_driver* d, stereo_image* disp, stere
         x->planes_dirty = false;
     }
 
-    // timed calls run the stages directly with events between them; otherwise the
-    // sequence is captured once per pair of input images and replayed as a graph
+    // the stages run directly (timed calls with events between them); with graph_replay
+    // the sequence is captured once per pair of input images and replayed as a graph
     x->timed = ctx->timing != 0;
-    if (x->timed || x->lp.direct_launch) {
+    if (x->timed || !x->lp.graph_replay) {
         if ((st = learned_enqueue(d, x, left, right, s, x->timed))) return st;
         HIP_TRY(ctx, hipEventRecord(x->ev_done, s), "hipEventRecord");
         x->done_recorded = true;

@@ -243,11 +243,12 @@ def test_learned_masks_from_coarse_map(ctx):
 
 
 def test_learned_graph_replays(ctx):
-    """Untimed calls replay a captured graph; replays with new content in the same
-    input buffers, and a timed direct run, all give the same maps as fresh runs."""
+    """With graph_replay, untimed calls replay a captured graph; replays with new content
+    in the same input buffers, a timed run and the default direct launch all give the
+    same maps as fresh runs."""
     W, H, D = 384, 224, 128
     params = cs.cnn_init_params(3)
-    m = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, logit_thresh=-2.0)
+    m = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, logit_thresh=-2.0, graph_replay=True)
     dl = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_U8)
     dr = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_U8)
     out = cs.DeviceImage(ctx, W, H, cs.PIXEL_LUM_I16)
@@ -262,5 +263,5 @@ def test_learned_graph_replays(ctx):
         want = fresh.run(L, R)
         ctx.set_timing(False)
         assert np.array_equal(got, want), f"replay {rep}"
-        direct = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, logit_thresh=-2.0, direct_launch=True)
+        direct = cs.LearnedMatcher(ctx, W, H, params, num_disp=D, logit_thresh=-2.0)
         assert np.array_equal(direct.run(L, R), want), f"direct launch {rep}"
