@@ -572,8 +572,15 @@ __global__ __launch_bounds__(256) void bm_list_finish_kernel(BmListParams p)
     extern __shared__ uint32_t rrow[];
     const int r = blockIdx.x;
     if (r == 0 && threadIdx.x == 0) *p.counter = 0;  // the matcher launch before this one has ended
-    int fi = 0;
-    while (fi + 1 < p.nframes && p.frames[fi + 1].row_base <= r) ++fi;
+    // the frame whose rows hold r: the last one with row_base <= r (frames in row order),
+    // by binary search, so the dependent loads before the row's work do not grow with the
+    // batch (64 frames: 6 instead of up to 63)
+    int fi = 0, hi = p.nframes - 1;
+    while (fi < hi) {
+        const int mid = (fi + hi + 1) >> 1;
+        if (p.frames[mid].row_base <= r) fi = mid;
+        else hi = mid - 1;
+    }
     const BmFrame& f = p.frames[fi];
     const int y = r - f.row_base;
     const bool row_ok = y >= f.by && y < f.H - f.by;
