@@ -61,6 +61,10 @@ CASES = [
     (cs.ALGO_CENSUS, 419, 133, 192, (9, 9), (9, 7)),
     (cs.ALGO_CENSUS, 300, 90, 256, (7, 7), (7, 7)),
     (cs.ALGO_CENSUS, 130, 60, 16, (5, 5), (5, 5)),
+    # every census window the matchers take: the matchers' descriptors pair neighbours
+    # differently for each width (odd widths pair the last column across rows)
+    (cs.ALGO_CENSUS, 250, 110, 96, (9, 9), (7, 9)),
+    (cs.ALGO_CENSUS, 170, 75, 48, (5, 5), (3, 3)),
 ]
 
 
