@@ -268,8 +268,12 @@ def other_matchers_leg(cs, np, ctx, steps):
     res = {}
     cases = [("sad9x9_640x480_d64", cs.ALGO_SAD, (9, 9), 640, 480, 64, (0, 0, 8, 63, 8), 10),
              ("ncc11x11_1920x1080_d256", cs.ALGO_NCC, (11, 11), 1920, 1080, 256, (1, 0, 24, 255, 8), 11),
-             ("aw1_15x15_1280x720_d128", cs.ALGO_AW1, (5, 5), 1280, 720, 128, (2, 1, 6, 127, 8), None),
-             ("aw2_15x15_1280x720_d128", cs.ALGO_AW2, (5, 5), 1280, 720, 128, (2, 1, 6, 127, 8), None)]
+             ("aw1_15x15_1280x720_d128", cs.ALGO_AW1, (5, 5), 1280, 720, 128, (2, 1, 6, 127, 8), 16),
+             ("aw2_15x15_1280x720_d128", cs.ALGO_AW2, (5, 5), 1280, 720, 128, (2, 1, 6, 127, 8), 16)]
+    # operations per evaluation: the algorithmic count for SAD and NCC; for the adaptive
+    # windows the VALU operations the kernel issues (SQ_INSTS_VALU x 64 / evaluations,
+    # profiles/ncc_aw_sad_matchers.md), so their fraction is the issue utilisation
+    ops_basis = {cs.ALGO_SAD: "algorithmic", cs.ALGO_NCC: "algorithmic", cs.ALGO_AW1: "issued", cs.ALGO_AW2: "issued"}
     for name, algo, win, w, h, d, (seed, kind, nreg, dmax, dbg), ops in cases:
         L, R = cs.synth_pair(seed, w, h, kind, nreg, dmax, dbg)
         m = cs.Matcher(ctx, w, h, algorithm=algo, window=win, num_disp=d, tol=1)
@@ -296,6 +300,7 @@ def other_matchers_leg(cs, np, ctx, steps):
         if ops:
             r["kernel_valu_frac"] = round(w * h * d * ops / (kms * 1e-3) / (PEAK_INT_TOPS * 1e12), 4)
             r["ops_per_eval"] = ops
+            r["ops_basis"] = ops_basis[algo]
         res[name] = r
         if algo in (cs.ALGO_NCC, cs.ALGO_AW1):
             # the band search around an initial map (MatcherUseDisparity): the full-range
