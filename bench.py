@@ -470,7 +470,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(a frame shards into row bands with a 7-row halo, see DESIGN.md)", file=sys.stderr)
+              "(across GPUs a frame would shard into row bands with a 7-row halo; that path is not built)",
+              file=sys.stderr)
         return 2
 
     import numpy as np
