@@ -176,7 +176,7 @@ __device__ __forceinline__ uint32_t census_and_or(uint32_t s, uint32_t m, uint32
 
 template <int CW, int CH>
 __device__ __forceinline__ void census_tile_pk(const View& src, const View& dst, const View& q, int x0, int y0,
-                                               uint32_t (*tile)[CensusShape<CW, CH>::LW + 1])
+                                               uint32_t (*tile)[CensusShape<CW, CH>::LW + 4])
 {
     constexpr int RX = CW / 2, RY = CH / 2;
     constexpr int LW = CT_W + 2 * RX;
@@ -184,8 +184,41 @@ __device__ __forceinline__ void census_tile_pk(const View& src, const View& dst,
     constexpr int NP = CW / 2;        // neighbour pairs of a window row
     constexpr bool ODD = (CW & 1) != 0;  // a last neighbour at dx = CW - 1
     static_assert((CW * CH + 3) / 4 <= 16, "descriptor longer than 64 bits");
+    constexpr int LS = LW + 4;  // LDS row stride: a multiple of 4 words for 16-byte stores
     const int tid = threadIdx.x;
-    {
+    // Interior tiles of 4-byte-aligned images with a 9-wide window (the staged columns
+    // x0 - 4 .. x0 + 71 all inside the row and the first of them on a dword boundary): a
+    // thread loads dwords D, D' of a row and builds the four words of D's pixels with
+    // v_perm_b32, pixel k in the low half and pixel k + 1 (the next dword's first for
+    // k = 3) in the high half.  Elsewhere every word takes two clamped byte loads.
+    const bool dwords = RX == 4 && x0 >= RX && x0 - RX + 4 * (LW / 4 + 1) <= src.w &&
+                        ((reinterpret_cast<uintptr_t>(src.ptr) | src.pitch) & 3) == 0;
+    if (dwords) {
+        constexpr int NG = LW / 4, NS = LH * NG, NIT = (NS + 255) / 256;
+        uint32_t d0[NIT], d1[NIT];
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int i = min(tid + 256 * k, NS - 1);
+            const int r = i / NG, g = i - r * NG;
+            const uint32_t* row = reinterpret_cast<const uint32_t*>(
+                row_ptr<const uint8_t>(src, clampi(y0 - RY + r, 0, src.h - 1)) + x0 - RX) + g;
+            d0[k] = row[0];
+            d1[k] = row[1];
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int i = tid + 256 * k;
+            if (i < NS) {
+                const int r = i / NG, g = i - r * NG;
+                uint4 w;
+                w.x = __builtin_amdgcn_perm(d1[k], d0[k], 0x0C010C00u);
+                w.y = __builtin_amdgcn_perm(d1[k], d0[k], 0x0C020C01u);
+                w.z = __builtin_amdgcn_perm(d1[k], d0[k], 0x0C030C02u);
+                w.w = __builtin_amdgcn_perm(d1[k], d0[k], 0x0C040C03u);
+                *reinterpret_cast<uint4*>(&tile[r][4 * g]) = w;
+            }
+        }
+    } else {
         // every load of the thread issued before the first store, as in census_tile
         constexpr int NS = LH * LW, NIT = (NS + 255) / 256;
         uint32_t a[NIT], b[NIT];
@@ -215,7 +248,7 @@ __device__ __forceinline__ void census_tile_pk(const View& src, const View& dst,
     int slot[CT_STRIP];
 #pragma unroll
     for (int k = 0; k < CT_STRIP; ++k) {
-        cen2[k] = __builtin_amdgcn_perm(0u, col[(k + RY) * (LW + 1) + RX], 0x01000100u);
+        cen2[k] = __builtin_amdgcn_perm(0u, col[(k + RY) * LS + RX], 0x01000100u);
         acc[k][0] = acc[k][1] = 0u;
         pend[k] = 0u;
         has[k] = false;
@@ -243,7 +276,7 @@ __device__ __forceinline__ void census_tile_pk(const View& src, const View& dst,
     for (int r = 0; r < NR; ++r) {
         if (r + 1 < NR) {
 #pragma unroll
-            for (int j = 0; j < NV; ++j) vn[j] = col[(r + 1) * (LW + 1) + 2 * j];
+            for (int j = 0; j < NV; ++j) vn[j] = col[(r + 1) * LS + 2 * j];
         }
         // the last neighbours of this row and the one before, and this row's alone
         const uint32_t qpair = ODD ? __builtin_amdgcn_perm(v[NV - 1], sprev, 0x05040100u) : 0u;
@@ -321,13 +354,15 @@ template <int CW, int CH, bool PK>
 __global__ __launch_bounds__(256) void census_wide_kernel(View src0, View dst0, View src1, View dst1, View q0, View q1)
 {
     using S = CensusShape<CW, CH>;
-    __shared__ uint32_t tile[S::LH][S::LW + 1];
-    if (PK)
+    if (PK) {
+        __shared__ __align__(16) uint32_t tile[S::LH][S::LW + 4];
         census_tile_pk<CW, CH>(blockIdx.z ? src1 : src0, blockIdx.z ? dst1 : dst0, blockIdx.z ? q1 : q0,
                                blockIdx.x * CT_W, blockIdx.y * CT_H, tile);
-    else
+    } else {
+        __shared__ uint32_t tile[S::LH][S::LW + 1];
         census_tile<CW, CH>(blockIdx.z ? src1 : src0, blockIdx.z ? dst1 : dst0, blockIdx.z ? q1 : q0,
                             blockIdx.x * CT_W, blockIdx.y * CT_H, tile);
+    }
 }
 
 // Every image of a batch in one launch: workgroup b takes tile b - jobs[i].base of the
@@ -347,7 +382,7 @@ template <int CW, int CH>
 __global__ __launch_bounds__(256) void census_batch_kernel(const ImageJob* jobs, int njobs)
 {
     using S = CensusShape<CW, CH>;
-    __shared__ uint32_t tile[S::LH][S::LW + 1];
+    __shared__ __align__(16) uint32_t tile[S::LH][S::LW + 4];
     const int i = job_of(jobs, njobs, blockIdx.x);
     const ImageJob j = jobs[i];
     const int t = blockIdx.x - j.base;
