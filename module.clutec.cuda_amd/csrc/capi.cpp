@@ -515,9 +515,8 @@ static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool disp
         HIP_TRY(ctx, stereo::launch_pad_copy_pair(view_of(left), pl, view_of(right), pr, s), "copy launch");
     }
     if (ncc) {
-        HIP_TRY(ctx, stereo::launch_ncc_stats(view_of(left), d->lstats, W, d->mp.window_w, d->mp.window_h, 8388608.0f, s),
-                "ncc stats launch");
-        HIP_TRY(ctx, stereo::launch_ncc_stats(view_of(right), d->rstats, W, d->mp.window_w, d->mp.window_h, 1.0f, s),
+        HIP_TRY(ctx, stereo::launch_ncc_stats_pair(view_of(left), d->lstats, 8388608.0f, view_of(right), d->rstats, 1.0f, W,
+                                                   d->mp.window_w, d->mp.window_h, s),
                 "ncc stats launch");
     }
     if (ctx->timing) HIP_TRY(ctx, hipEventRecord(d->ev_stage[0], s), "hipEventRecord");

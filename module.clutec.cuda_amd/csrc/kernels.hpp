@@ -162,6 +162,9 @@ struct NccScoreParams {
 bool ncc_supported(int ww, int wh, int D);
 void ncc_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len);
 hipError_t launch_ncc_stats(const View& img, int2* out, int pitch_px, int ww, int wh, float scale, hipStream_t s);
+// Both images of a pair (one size) in one launch.
+hipError_t launch_ncc_stats_pair(const View& img0, int2* out0, float scale0, const View& img1, int2* out1, float scale1,
+                                 int pitch_px, int ww, int wh, hipStream_t s);
 hipError_t launch_ncc(int ww, const NccParams& p, hipStream_t s);
 hipError_t launch_ncc_score(const NccScoreParams& f, hipStream_t s);
 

@@ -43,11 +43,15 @@ constexpr int SCOLS = TW + SDC - 1;
 // variance is below N^2, and 0 outside the matchable area).  A workgroup does a
 // 64 x 32 block: the input rows it needs go to LDS, horizontal sums of v and v^2
 // over 2 ax + 1 columns go to LDS, and each lane slides the vertical sums down 8
-// rows of one column.
+// rows of one column.  Both images of a pair go through one launch (blockIdx.z).
 constexpr int ST_W = 64, ST_H = 32, ST_MAXA = 6;
 
-__global__ __launch_bounds__(256) void ncc_stats_kernel(View img, int2* out, int pitch_px, int ax, int ay, float scale)
+__global__ __launch_bounds__(256) void ncc_stats_kernel(View img0, int2* out0, float scale0, View img1, int2* out1,
+                                                        float scale1, int pitch_px, int ax, int ay)
 {
+    const View img = blockIdx.z ? img1 : img0;
+    int2* const out = blockIdx.z ? out1 : out0;
+    const float scale = blockIdx.z ? scale1 : scale0;
     __shared__ uint8_t pix[ST_H + 2 * ST_MAXA][ST_W + 2 * ST_MAXA];
     __shared__ int hs[ST_H + 2 * ST_MAXA][ST_W];
     __shared__ int hss[ST_H + 2 * ST_MAXA][ST_W];
@@ -451,8 +455,20 @@ void ncc_grid(int W, int H, int D, int* ntx, int* nty, int* nseg, int* seg_len)
 hipError_t launch_ncc_stats(const View& img, int2* out, int pitch_px, int ww, int wh, float scale, hipStream_t s)
 {
     if (ww / 2 > ncc::ST_MAXA || wh / 2 > ncc::ST_MAXA) return hipErrorInvalidValue;
-    dim3 grid((img.w + ncc::ST_W - 1) / ncc::ST_W, (img.h + ncc::ST_H - 1) / ncc::ST_H);
-    hipLaunchKernelGGL(ncc::ncc_stats_kernel, grid, dim3(256), 0, s, img, out, pitch_px, ww / 2, wh / 2, scale);
+    dim3 grid((img.w + ncc::ST_W - 1) / ncc::ST_W, (img.h + ncc::ST_H - 1) / ncc::ST_H, 1);
+    hipLaunchKernelGGL(ncc::ncc_stats_kernel, grid, dim3(256), 0, s, img, out, scale, img, out, scale, pitch_px, ww / 2,
+                       wh / 2);
+    return hipGetLastError();
+}
+
+hipError_t launch_ncc_stats_pair(const View& img0, int2* out0, float scale0, const View& img1, int2* out1, float scale1,
+                                 int pitch_px, int ww, int wh, hipStream_t s)
+{
+    if (ww / 2 > ncc::ST_MAXA || wh / 2 > ncc::ST_MAXA || img0.w != img1.w || img0.h != img1.h)
+        return hipErrorInvalidValue;
+    dim3 grid((img0.w + ncc::ST_W - 1) / ncc::ST_W, (img0.h + ncc::ST_H - 1) / ncc::ST_H, 2);
+    hipLaunchKernelGGL(ncc::ncc_stats_kernel, grid, dim3(256), 0, s, img0, out0, scale0, img1, out1, scale1, pitch_px,
+                       ww / 2, wh / 2);
     return hipGetLastError();
 }
 
