@@ -25,11 +25,13 @@
 //                 with DPP row shifts, and the sums H go to LDS (double buffered).
 //   column phase  lane = output column, wave = 14-row group.  Vertical running
 //                 sums of H give the aggregated cost A of 14 pixels; the key
-//                 ((65535-A) << 8 | 255-d) is built with one v_perm_b32.  The left
-//                 winner is a v_max_u32 per pixel.  The right winner for right
-//                 pixel x-d is carried along the wave: each step shifts the running
-//                 maxima one lane up with a DPP wave shift fused into v_max_u32, so
-//                 lane x always holds right pixel x-d.  What leaves lane 63 is a
+//                 (0xFF << 24 | A << 8 | d, see min_i32) is built with one
+//                 v_perm_b32.  The left winner is a signed minimum per pixel.  The
+//                 right winner for right pixel x-d is carried along the wave: each
+//                 step shifts the running minima one lane up with a DPP wave shift
+//                 fused into v_min_i32, so lane x always holds right pixel x-d.
+//                 Running sums slide with one v_sad_u32 each (slide).  The keys
+//                 leave the tile complemented, as (65535-A) << 8 | 255-d.  What leaves lane 63 is a
 //                 partial result for the tile to the right; lane 63 parks it in
 //                 LDS and the wave stores 64 steps at a time, coalesced.
 #include "common.hpp"
@@ -108,6 +110,21 @@ __device__ __forceinline__ bool job_has_candidates(const Job& j)
     return xmax >= j.x0 && xmax - j.bx >= j.ds;
 }
 
+// s - out + in on packed u16 pairs where each half of s is at least out's half (a running
+// sum and a term inside it): the 32-bit difference then borrows nothing between the halves,
+// and the pattern below is one v_sad_u32 (|s - out| + in).
+__device__ __forceinline__ uint32_t slide(uint32_t s, uint32_t out, uint32_t in)
+{
+    return (max(s, out) - min(s, out)) + in;
+}
+
+// Keys inside the tile: 0xFF000000 | A << 8 | d for aggregated cost A and disparity d,
+// negative as int32, so that the signed minimum picks the least cost and, among equal
+// costs, the least d, and 0 stands for no candidate.  Their complement is the plane key
+// (65535 - A) << 8 | (255 - d), folded with atomicMax where 0 is again no candidate.
+__device__ __forceinline__ uint32_t min_i32(uint32_t a, uint32_t b) { return (int32_t)a < (int32_t)b ? a : b; }
+__device__ __forceinline__ uint32_t plane_key(uint32_t k) { return k ? ~k : 0u; }
+
 // tid: the thread index (threadIdx.x, or an opaque copy of it, see bm_list_kernel)
 template <class Cost, int AX, int AY, bool EDGE, bool ATOM>
 __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * HSTRIDE], uint32_t (*ebuf)[RG][64],
@@ -172,9 +189,9 @@ __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * H
             uint2* hrow = reinterpret_cast<uint2*>(hb + hr * HSTRIDE + RUN * run);
 #pragma unroll
             for (int i = 0; i < RUN; i += 2) {
-                if (i > 0) h = h + e[i + 2 * AX] - e[i - 1];
+                if (i > 0) h = slide(h, e[i - 1], e[i + 2 * AX]);
                 const uint32_t h0 = h;
-                h = h + e[i + 1 + 2 * AX] - e[i];
+                h = slide(h, e[i], e[i + 1 + 2 * AX]);
                 hrow[i / 2] = make_uint2(h0, h);
             }
             __syncthreads();
@@ -184,31 +201,30 @@ __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * H
             const uint32_t* hcol = hb + gw * RG * HSTRIDE + lane + 4;
 #pragma unroll
             for (int i = 0; i < RG + 2 * AY; ++i) hv[i] = hcol[i * HSTRIDE];
-            uint32_t abar = 0xFFFFFFFFu;
+            uint32_t a = 0;
 #pragma unroll
-            for (int i = 0; i <= 2 * AY; ++i) abar -= hv[i];
-            const uint32_t ddv = (uint32_t)(255 - d) | ((uint32_t)(254 - d) << 8);
+            for (int i = 0; i <= 2 * AY; ++i) a += hv[i];
+            const uint32_t ddv = (uint32_t)d | ((uint32_t)(d + 1) << 8);
             bool ok0 = true, ok1 = true;
             if (EDGE) { ok0 = lane_ok && d <= dlim; ok1 = lane_ok && d + 1 <= dlim; }
             uint32_t k0[RG], k1[RG];
 #pragma unroll
             for (int i = 0; i < RG; ++i) {
-                if (i > 0) abar = abar + hv[i - 1] - hv[i + 2 * AY];
-                k0[i] = __builtin_amdgcn_perm(ddv, abar, 0x0C010004u);
-                k1[i] = __builtin_amdgcn_perm(ddv, abar, 0x0C030205u);
+                if (i > 0) a = slide(a, hv[i - 1], hv[i + 2 * AY]);
+                k0[i] = __builtin_amdgcn_perm(ddv, a, 0x0D010004u);
+                k1[i] = __builtin_amdgcn_perm(ddv, a, 0x0D030205u);
                 if (EDGE) { k0[i] = ok0 ? k0[i] : 0u; k1[i] = ok1 ? k1[i] : 0u; }
-                const uint32_t kk = k0[i] > k1[i] ? k0[i] : k1[i];
-                lacc[i] = lacc[i] > kk ? lacc[i] : kk;
+                lacc[i] = min_i32(lacc[i], min_i32(k0[i], k1[i]));
             }
             // right-view chain: one lane per step; all rows first so that no DPP read
             // waits on the VALU write just before it, then lane 63 parks what leaves
             static_assert(RG == 14, "the right-view chain block is written for 14 rows");
-            max_shift_in14(racc, k0);
+            min_shift_in14(racc, k0);
             if (lane == 63) {
 #pragma unroll
                 for (int i = 0; i < RG; ++i) eb[i * 64 + (d & 63)] = racc[i];
             }
-            max_shift_in14(racc, k1);
+            min_shift_in14(racc, k1);
             if (lane == 63) {
 #pragma unroll
                 for (int i = 0; i < RG; ++i) eb[i * 64 + ((d + 1) & 63)] = racc[i];
@@ -226,13 +242,13 @@ __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * H
 #pragma unroll
                             for (int i = 0; i < RG; ++i) {
                                 const uint32_t v = eb[i * 64 + lane];
-                                if (i < rows_here && v) atomicMax(&rp[(size_t)i * p.rp_stride + xr], v);
+                                if (i < rows_here && v) atomicMax(&rp[(size_t)i * p.rp_stride + xr], ~v);
                             }
                         }
                     } else {
 #pragma unroll
                         for (int i = 0; i < RG; ++i)
-                            if (i < rows_here) rp[i * p.rp_stride + k] = eb[i * 64 + lane];
+                            if (i < rows_here) rp[i * p.rp_stride + k] = plane_key(eb[i * 64 + lane]);
                     }
                 }
             }
@@ -245,13 +261,13 @@ __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * H
         if (xr >= 0 && xr < W) {
 #pragma unroll
             for (int i = 0; i < RG; ++i)
-                if (i < rows_here && racc[i]) atomicMax(&rp[(size_t)i * p.rp_stride + xr], racc[i]);
+                if (i < rows_here && racc[i]) atomicMax(&rp[(size_t)i * p.rp_stride + xr], ~racc[i]);
         }
     } else {
         const int k = lane + 255 - (de - 1);
 #pragma unroll
         for (int i = 0; i < RG; ++i)
-            if (i < rows_here) rp[i * p.rp_stride + k] = racc[i];
+            if (i < rows_here) rp[i * p.rp_stride + k] = plane_key(racc[i]);
     }
     if (x < W) {
         uint32_t* lk = p.lk + (size_t)gw * RG * p.lk_stride + lane;
@@ -259,9 +275,9 @@ __device__ __forceinline__ void bm_tile(const Job& p, uint32_t (*hbuf)[HROWS * H
         for (int i = 0; i < RG; ++i) {
             if (i >= rows_here) continue;
             if (ATOM) {
-                if (lacc[i]) atomicMax(&lk[(size_t)i * p.lk_stride], lacc[i]);
+                if (lacc[i]) atomicMax(&lk[(size_t)i * p.lk_stride], ~lacc[i]);
             } else {
-                lk[(size_t)i * p.lk_stride] = lacc[i];
+                lk[(size_t)i * p.lk_stride] = plane_key(lacc[i]);
             }
         }
     }

@@ -92,6 +92,29 @@ __device__ __forceinline__ void max_shift_in14(uint32_t (&acc)[14], const uint32
                  : "v"(key[0]), "v"(key[1]), "v"(key[2]), "v"(key[3]), "v"(key[4]), "v"(key[5]), "v"(key[6]), "v"(key[7]), "v"(key[8]), "v"(key[9]), "v"(key[10]), "v"(key[11]), "v"(key[12]), "v"(key[13]));
 }
 
+// The same chains with a signed minimum, for keys that are all negative as int32: the 0
+// that lane 0 reads is then the identity.
+__device__ __forceinline__ void min_shift_in14(uint32_t (&acc)[14], const uint32_t (&key)[14])
+{
+    asm volatile("s_nop 1\n"
+                 "v_min_i32_dpp %0, %0, %14 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %1, %1, %15 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %2, %2, %16 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %3, %3, %17 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %4, %4, %18 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %5, %5, %19 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %6, %6, %20 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %7, %7, %21 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %8, %8, %22 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %9, %9, %23 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %10, %10, %24 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %11, %11, %25 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %12, %12, %26 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 "v_min_i32_dpp %13, %13, %27 wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+                 : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]), "+v"(acc[6]), "+v"(acc[7]), "+v"(acc[8]), "+v"(acc[9]), "+v"(acc[10]), "+v"(acc[11]), "+v"(acc[12]), "+v"(acc[13])
+                 : "v"(key[0]), "v"(key[1]), "v"(key[2]), "v"(key[3]), "v"(key[4]), "v"(key[5]), "v"(key[6]), "v"(key[7]), "v"(key[8]), "v"(key[9]), "v"(key[10]), "v"(key[11]), "v"(key[12]), "v"(key[13]));
+}
+
 __device__ __forceinline__ void max_shift_in13(uint32_t (&acc)[13], const uint32_t (&key)[13])
 {
     asm volatile("s_nop 1\n"
