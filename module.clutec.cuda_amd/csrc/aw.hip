@@ -170,9 +170,9 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
             uint2* hrow = reinterpret_cast<uint2*>(hb + hr * HSTRIDE + RUN * run);
 #pragma unroll
             for (int i = 0; i < RUN; i += 2) {
-                if (i > 0) h = h + e[i + 2 * R] - e[i - 1];
+                if (i > 0) h = slide(h, e[i - 1], e[i + 2 * R]);
                 const uint32_t h0 = h;
-                h = h + e[i + 1 + 2 * R] - e[i];
+                h = slide(h, e[i], e[i + 1 + 2 * R]);
                 hrow[i / 2] = make_uint2(h0, h);
             }
             __syncthreads();
@@ -188,7 +188,7 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
                 sc[0] = v;
 #pragma unroll
                 for (int k = 1; k < NC; ++k) {
-                    v = v + hc[(k + 2 * R) * HSTRIDE] - hc[(k - 1) * HSTRIDE];
+                    v = slide(v, hc[(k - 1) * HSTRIDE], hc[(k + 2 * R) * HSTRIDE]);
                     sc[k] = v;
                 }
             }
@@ -204,8 +204,8 @@ __device__ __forceinline__ void aw_tile(const AwParams& p, uint32_t (*hbuf)[HROW
                 sr[0] = vr;
 #pragma unroll
                 for (int k = 1; k < RG; ++k) {
-                    vl = vl + hl[(k + 2 * R) * HSTRIDE] - hl[(k - 1) * HSTRIDE];
-                    vr = vr + hq[(k + 2 * R) * HSTRIDE] - hq[(k - 1) * HSTRIDE];
+                    vl = slide(vl, hl[(k - 1) * HSTRIDE], hl[(k + 2 * R) * HSTRIDE]);
+                    vr = slide(vr, hq[(k - 1) * HSTRIDE], hq[(k + 2 * R) * HSTRIDE]);
                     sl[k] = vl;
                     sr[k] = vr;
                 }

@@ -110,14 +110,6 @@ __device__ __forceinline__ bool job_has_candidates(const Job& j)
     return xmax >= j.x0 && xmax - j.bx >= j.ds;
 }
 
-// s - out + in on packed u16 pairs where each half of s is at least out's half (a running
-// sum and a term inside it): the 32-bit difference then borrows nothing between the halves,
-// and the pattern below is one v_sad_u32 (|s - out| + in).
-__device__ __forceinline__ uint32_t slide(uint32_t s, uint32_t out, uint32_t in)
-{
-    return (max(s, out) - min(s, out)) + in;
-}
-
 // Keys inside the tile: 0xFF000000 | A << 8 | d for aggregated cost A and disparity d,
 // negative as int32, so that the signed minimum picks the least cost and, among equal
 // costs, the least d, and 0 stands for no candidate.  Their complement is the plane key

@@ -22,6 +22,15 @@ __device__ __forceinline__ T* row_ptr(const View& v, int y)
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// One step of a running sum, s - out + in, where s is a sum of non-negative terms that
+// include out (so s >= out), as one v_sad_u32 (|s - out| + in, the pattern below).  On
+// packed u16 pairs it holds when each half of s is at least out's half: the 32-bit
+// difference then borrows nothing between the halves.  Every operand stays below 2^31.
+__device__ __forceinline__ uint32_t slide(uint32_t s, uint32_t out, uint32_t in)
+{
+    return (max(s, out) - min(s, out)) + in;
+}
+
 // Winner-take-all key shared by every integer matcher.  A larger key is better:
 //   key = ((65535 - cost) << 8) | (255 - d)
 // so the maximum key has the smallest aggregated cost and, among equal costs,

@@ -311,7 +311,7 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
             uint32_t* hrow = hb + hr * HSTRIDE + RUN * run;
 #pragma unroll
             for (int i = 0; i < RUN; ++i) {
-                if (i > 0) h = h + e[i + 2 * AX] - e[i - 1];
+                if (i > 0) h = slide(h, e[i - 1], e[i + 2 * AX]);
                 hrow[i] = h;
             }
             __syncthreads();
@@ -333,7 +333,7 @@ __device__ __forceinline__ void ncc_tile(const NccParams& p, uint32_t* hb, int16
             uint32_t key[RG];
 #pragma unroll
             for (int i = 0; i < RG; ++i) {
-                if (i > 0) sij = sij + hv[i + 2 * AY] - hv[i - 1];
+                if (i > 0) sij = slide(sij, hv[i - 1], hv[i + 2 * AY]);
                 const int sj = srow_s[i * SCOLS];
                 const int cov = __mul24(nsi[i], sj) + (int)__umul24((uint32_t)N, sij);
                 const float sc = (float)cov * rfi[i] * srow_r[i * SCOLS];
