@@ -276,7 +276,9 @@ __global__ __launch_bounds__(256) void bm1_kernel(Bm1Args a)
                 sv[buf][r0][c] = base - s;
 #pragma unroll
                 for (int j = 1; j < SEG; ++j) {
-                    s += t[j + P - 1] - t[j - 1];
+                    // AbsoluteDifference terms are non-negative: one v_sad_u32 per step
+                    s = SIM == 0 ? (int)slide((uint32_t)s, (uint32_t)t[j - 1], (uint32_t)t[j + P - 1])
+                                 : s + t[j + P - 1] - t[j - 1];
                     sv[buf][r0 + j][c] = base - s;
                 }
             }
@@ -299,7 +301,9 @@ __global__ __launch_bounds__(256) void bm1_kernel(Bm1Args a)
                 for (int i = 0; i < P; ++i) s += vrow[ix0 + i];
 #pragma unroll
                 for (int k = 0; k < RPT; ++k) {
-                    if (k > 0) s += vrow[ix0 + k + P - 1] - vrow[ix0 + k - 1];
+                    if (k > 0)
+                        s = SIM == 0 ? (int)slide((uint32_t)s, (uint32_t)vrow[ix0 + k - 1], (uint32_t)vrow[ix0 + k + P - 1])
+                                     : s + vrow[ix0 + k + P - 1] - vrow[ix0 + k - 1];
                     // branch-free: every state value is a select
                     const int v = (int)(int16_t)s;
                     const bool act = upd && disp[k] <= B1_LAST;

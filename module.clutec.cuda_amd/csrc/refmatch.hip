@@ -445,7 +445,7 @@ __global__ __launch_bounds__(64) void ncc_ref_kernel(RefArgs a)
                 if (lane_ok) scol[r0][c] = s;
 #pragma unroll
                 for (int j = 1; j < 16; ++j) {
-                    s += q[j + P - 1] - q[j - 1];
+                    s = (int)slide((uint32_t)s, (uint32_t)q[j - 1], (uint32_t)q[j + P - 1]);
                     if (lane_ok) scol[r0 + j][c] = s;
                 }
             }
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(64) void ncc_ref_kernel(RefArgs a)
                 for (int i = 0; i < P; ++i) s3 += cr[ix0 + i];
 #pragma unroll
                 for (int k = 0; k < CG; ++k) {
-                    if (k > 0) s3 += cr[ix0 + k + P - 1] - cr[ix0 + k - 1];
+                    if (k > 0) s3 = (int)slide((uint32_t)s3, (uint32_t)cr[ix0 + k - 1], (uint32_t)cr[ix0 + k + P - 1]);
                     const float cross = div_window<P * P>(s3);
                     const float tm = tmean[rr][sh + ix0 + k], ts = tsd[rr][sh + ix0 + k];
                     const float norm = bsd[k] * ts;
@@ -691,7 +691,9 @@ __global__ __launch_bounds__(64) void aw1_ref_kernel(RefArgs a)
                 scol[r0][c] = s;
 #pragma unroll
                 for (int j = 1; j < 8; ++j) {
-                    s += q[j + SP - 1] - q[j - 1];
+                    // AbsoluteDifference terms are non-negative: one v_sad_u32 per step
+                    s = SIM == 0 ? (int)slide((uint32_t)s, (uint32_t)q[j - 1], (uint32_t)q[j + SP - 1])
+                                 : s + q[j + SP - 1] - q[j - 1];
                     scol[r0 + j][c] = s;
                 }
             }
@@ -706,7 +708,8 @@ __global__ __launch_bounds__(64) void aw1_ref_kernel(RefArgs a)
 #pragma unroll
                 for (int i = 1; i < 8; ++i) {
                     if (c0 + i + SP - 1 < 32) {
-                        s += scol[rr][c0 + i + SP - 1] - scol[rr][c0 + i - 1];
+                        s = SIM == 0 ? (int)slide((uint32_t)s, (uint32_t)scol[rr][c0 + i - 1], (uint32_t)scol[rr][c0 + i + SP - 1])
+                                     : s + scol[rr][c0 + i + SP - 1] - scol[rr][c0 + i - 1];
                         ssp[rr][c0 + i] = s;
                     }
                 }
@@ -912,7 +915,8 @@ __global__ __launch_bounds__(256) void aw1_ref_stack_kernel(RefArgs a)
                 if (r0 < SROWS) scol[r0][c] = sum;
 #pragma unroll
                 for (int j = 1; j < CR; ++j) {
-                    sum += qv[j + SP - 1] - qv[j - 1];
+                    sum = SIM == 0 ? (int)slide((uint32_t)sum, (uint32_t)qv[j - 1], (uint32_t)qv[j + SP - 1])
+                                   : sum + qv[j + SP - 1] - qv[j - 1];
                     if (r0 + j < SROWS) scol[r0 + j][c] = sum;
                 }
             }
@@ -927,7 +931,8 @@ __global__ __launch_bounds__(256) void aw1_ref_stack_kernel(RefArgs a)
 #pragma unroll
                 for (int i = 1; i < CPG; ++i) {
                     if (c0 + i < SPCX) {
-                        sum += scol[rr][c0 + i + SP - 1] - scol[rr][c0 + i - 1];
+                        sum = SIM == 0 ? (int)slide((uint32_t)sum, (uint32_t)scol[rr][c0 + i - 1], (uint32_t)scol[rr][c0 + i + SP - 1])
+                                       : sum + scol[rr][c0 + i + SP - 1] - scol[rr][c0 + i - 1];
                         ssp[rr][c0 + i] = sum;
                     }
                 }
