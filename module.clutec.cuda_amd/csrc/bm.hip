@@ -327,14 +327,10 @@ __global__ __launch_bounds__(256, Cost::LIST_WG) void bm_list_kernel(BmListParam
     __shared__ uint32_t ebuf[4][RG][64];
     __shared__ int next;
     const int n = *p.nitems;
-    for (;;) {
-        // items are handed out in schedule order by a device counter (reset to 0 by
-        // the finish kernel), so that a workgroup takes the next item when it is free
-        __syncthreads();  // the previous item is done with the LDS buffers and `next`
-        if (threadIdx.x == 0) next = atomicAdd(p.counter, 1);
-        __syncthreads();
-        const int k = next;
-        if (k >= n) break;
+    // Items go out in schedule order: workgroup b takes item b first, with no counter
+    // traffic at the launch; after that a device counter (reset to 0 by the finish
+    // kernel) hands item gridDim.x + c to the c-th workgroup that comes back free.
+    for (int k = blockIdx.x; k < n;) {
         const int i = p.sched ? p.sched[k] : k;
         const BmItem it = p.items[i];
         const BmFrame& f = p.frames[it.frame];
@@ -354,13 +350,18 @@ __global__ __launch_bounds__(256, Cost::LIST_WG) void bm_list_kernel(BmListParam
         j.rp_kofs = 0;
         j.lk = p.lplane + f.plane_off + (size_t)j.y0 * f.W + j.x0;
         j.lk_stride = f.W;
-        if (!job_has_candidates(j)) continue;
-        // an opaque copy of the thread index per item: the addresses derived from it are
-        // recomputed for each item instead of being hoisted out of this loop into
-        // registers that stay live through the whole item
-        int tid = threadIdx.x;
-        asm volatile("" : "+v"(tid));
-        run_job<Cost, AX, AY, true>(j, hbuf, ebuf, tid);
+        if (job_has_candidates(j)) {
+            // an opaque copy of the thread index per item: the addresses derived from it
+            // are recomputed for each item instead of being hoisted out of this loop into
+            // registers that stay live through the whole item
+            int tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            run_job<Cost, AX, AY, true>(j, hbuf, ebuf, tid);
+        }
+        __syncthreads();  // the item is done with the LDS buffers, and `next` is free
+        if (threadIdx.x == 0) next = (int)gridDim.x + atomicAdd(p.counter, 1);
+        __syncthreads();
+        k = next;
     }
 }
 

@@ -114,8 +114,9 @@ struct BmListParams {
     const BmItem* items;
     const int* nitems;  // device-side item count
     const int* sched;   // order in which the persistent grid takes the items (nullptr: 0, 1, ...)
-    int* counter;       // next item to hand out: 0 before the matcher launch; the finish launch
-                        // resets it, so a matcher launch is always followed by its finish
+    int* counter;       // items handed out after each workgroup's first (item grid + counter):
+                        // 0 before the matcher launch; the finish launch resets it, so a
+                        // matcher launch is always followed by its finish
     uint32_t* lplane;   // every frame's left-key plane, zero at the matcher launch
     uint32_t* rplane;   // every frame's right-key plane, zero at the matcher launch
     int nframes;
