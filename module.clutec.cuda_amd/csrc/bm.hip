@@ -35,6 +35,7 @@
 //                 partial result for the tile to the right; lane 63 parks it in
 //                 LDS and the wave stores 64 steps at a time, coalesced.
 #include "common.hpp"
+#include "cnn_input.hpp"
 #include "kernels.hpp"
 
 namespace stereo {
@@ -564,6 +565,12 @@ __global__ __launch_bounds__(256) void bm_finish_planes_kernel(BmFinishParams f)
     const bool row_ok = y >= f.by && y < f.H - f.by;
     planes_row(f.disp, f.disp_l, f.disp_r, f.dispid, f.W, y, row_ok, f.bx, f.tol, f.value_base,
                f.lplane + (size_t)y * f.W, f.rplane + (size_t)y * f.W, rrow);
+    if (f.cnn.x) {
+        // the learned proposal's network input from the row just written
+        __shared__ uint64_t valid[CNN_MAX_W4 / 64];
+        __syncthreads();
+        cnn_input_row(f.cnn.L4, f.cnn.R4, f.disp, f.cnn.n, f.cnn.x, f.cnn.dfill, valid, y);
+    }
 }
 
 // Clears n words (the key planes, where a caller cannot rely on the last frame's merge
@@ -681,6 +688,9 @@ hipError_t launch_bm_finish(const BmFinishParams& f, hipStream_t s)
     const size_t lds = (size_t)f.W * sizeof(uint32_t);
     if (f.lplane) {
         if (!f.rplane) return hipErrorInvalidValue;
+        if (f.cnn.x && (!f.disp.ptr || !f.cnn.dfill || f.W > CNN_MAX_W4 || f.cnn.L4.w != f.W || f.cnn.L4.h != f.H ||
+                        f.cnn.R4.w != f.W || f.cnn.R4.h != f.H))
+            return hipErrorInvalidValue;
         hipLaunchKernelGGL(bm::bm_finish_planes_kernel, dim3(f.H), dim3(256), lds, s, f);
         return hipGetLastError();
     }

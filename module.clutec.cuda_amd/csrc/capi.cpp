@@ -615,6 +615,7 @@ static stereo_status matcher_run(stereo_driver* d, stereo_image* disp, bool disp
     f.by = d->by;
     f.tol = d->mp.lr_tolerance;
     f.value_base = d->mp.algorithm == STEREO_ALGO_AW1 ? 255 * 3 * d->mp.window_w * d->mp.window_w : 0;
+    if (!ncc && !aw) f.cnn = d->finish_cnn;
     HIP_TRY(ctx, stereo::launch_bm_finish(f, s), "finish launch");
     if (d->rkey) d->planes_dirty = false;
     return end(d, s);

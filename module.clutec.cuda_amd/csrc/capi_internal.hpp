@@ -162,6 +162,8 @@ struct stereo_driver {
     uint32_t* rpart = nullptr;
     uint32_t* rkey = nullptr;
     bool planes_dirty = false;
+    // the learned proposal's coarse matcher: its merge also writes the network input
+    stereo::CnnInputTarget finish_cnn{};
     int2* lstats = nullptr;  // NCC window statistics
     int2* rstats = nullptr;
     // crosscheck

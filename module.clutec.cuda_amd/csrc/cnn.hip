@@ -30,6 +30,7 @@
 // and the tap offsets negated.  The weight gradient uses the pixels as the K
 // dimension: D[co][ci] += dY[px][co] * X[px + off][ci].
 #include "common.hpp"
+#include "cnn_input.hpp"
 #include "kernels.hpp"
 
 namespace stereo {
@@ -38,12 +39,6 @@ namespace cnn {
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest, ties to even (finite inputs)
-    return (uint16_t)(u >> 16);
-}
 __device__ __forceinline__ float bf2f(uint32_t h) { return __uint_as_float((h & 0xFFFFu) << 16); }
 __device__ __forceinline__ bool bf_positive(uint32_t h) { return (h & 0x8000u) == 0 && (h & 0x7FFFu) != 0; }
 
@@ -335,54 +330,11 @@ __global__ __launch_bounds__(256) void adam_kernel(float* param, const float* gr
 
 // ---- data preparation -----------------------------------------------------------
 
-// Network input of frame n: ch0 = L/255, ch1 = R(x - dfill)/255 (bf16), ch2..15 = 0,
-// where dfill is the coarse disparity with holes filled by the smaller of the
-// nearest valid values to the left and right within 64 pixels (0 when none).
-// dfill is written as float for the range kernel.  One workgroup per row: each wave
-// takes 64-pixel chunks and records which pixels are valid as a ballot in LDS; the
-// nearest valid pixel within 64 then lies in the pixel's own chunk or in the
-// neighbouring one, found with a leading / trailing bit count.
-constexpr int CNN_MAX_W4 = 4096;
-
+// Network input of frame n, one workgroup per row (cnn_input_row).
 __global__ __launch_bounds__(256) void cnn_input_kernel(View L4, View R4, View dc, int n, uint16_t* x, float* dfill)
 {
     __shared__ uint64_t valid[CNN_MAX_W4 / 64];
-    const int Y = blockIdx.x;
-    const int W = L4.w, H = L4.h;
-    const int nch = (W + 63) >> 6;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int16_t* src = row_ptr<const int16_t>(dc, Y);
-    for (int c = wv; c < nch; c += 4) {
-        const int X = c * 64 + lane;
-        const uint64_t m = __ballot(X < W && src[X] >= 0);
-        if (lane == 0) valid[c] = m;
-    }
-    __syncthreads();
-    const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);  // lanes 0 .. lane
-    for (int c = wv; c < nch; c += 4) {
-        const int X = c * 64 + lane;
-        if (X >= W) continue;
-        int d = src[X];
-        if (d < 0) {
-            const uint64_t ml = valid[c] & le, mr = valid[c] & ~le;
-            int l = -1, r = W;
-            if (ml) l = c * 64 + 63 - __clzll((long long)ml);
-            else if (c > 0 && valid[c - 1]) l = (c - 1) * 64 + 63 - __clzll((long long)valid[c - 1]);
-            if (mr) r = c * 64 + __ffsll((unsigned long long)mr) - 1;
-            else if (c + 1 < nch && valid[c + 1]) r = (c + 1) * 64 + __ffsll((unsigned long long)valid[c + 1]) - 1;
-            const int a = (l >= 0 && X - l <= 64) ? src[l] : -1;
-            const int b = (r < W && r - X <= 64) ? src[r] : -1;
-            d = a < 0 ? (b < 0 ? 0 : b) : (b < 0 ? a : min(a, b));
-        }
-        const size_t p = ((size_t)n * H + Y) * W + X;
-        dfill[p] = (float)d;
-        const int xr = clampi(X - d, 0, W - 1);
-        const float l = row_ptr<const uint8_t>(L4, Y)[X] * (1.0f / 255.0f);
-        const float r = row_ptr<const uint8_t>(R4, Y)[xr] * (1.0f / 255.0f);
-        uint4* o = reinterpret_cast<uint4*>(x + p * 16);
-        o[0] = make_uint4((uint32_t)f2bf(l) | ((uint32_t)f2bf(r) << 16), 0, 0, 0);
-        o[1] = make_uint4(0, 0, 0, 0);
-    }
+    cnn_input_row(L4, R4, dc, n, x, dfill, valid, blockIdx.x);
 }
 
 // Training label of quarter pixel (X, Y) of frame n from the full-resolution checked
@@ -837,7 +789,7 @@ hipError_t launch_cnn_fused_forward(const CnnFusedArgs& a, hipStream_t s)
 hipError_t launch_cnn_input(const View& L4, const View& R4, const View& dc, int n, uint16_t* x, float* dfill,
                             hipStream_t s)
 {
-    if (L4.w > cnn::CNN_MAX_W4) return hipErrorInvalidValue;
+    if (L4.w > CNN_MAX_W4) return hipErrorInvalidValue;
     hipLaunchKernelGGL(cnn::cnn_input_kernel, dim3(L4.h), dim3(256), 0, s, L4, R4, dc, n, x, dfill);
     return hipGetLastError();
 }

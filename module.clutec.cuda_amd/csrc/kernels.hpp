@@ -47,6 +47,18 @@ struct BmParams {
     int ntx, nseg, seg_len;
 };
 
+// The learned proposal's network input, written by the merge of the coarse census
+// matcher row by row right after the row's checked map (cnn_input_row); x == nullptr:
+// not written.
+constexpr int CNN_MAX_W4 = 4096;  // widest quarter-resolution row of the network input
+
+struct CnnInputTarget {
+    View L4, R4;    // the quarter-resolution pair
+    uint16_t* x;    // bf16 network input of frame n
+    float* dfill;   // the hole-filled coarse map (float)
+    int n;
+};
+
 struct BmFinishParams {
     // the NCC and adaptive-window matchers' keys: per disparity segment
     const uint32_t* lkey;   // nseg*W*H
@@ -62,6 +74,7 @@ struct BmFinishParams {
     int nseg, seg_len;
     int bx, by, tol;
     int value_base;  // != 0: the 4 x uint16 output carries value_base - cost (AW1 similarity)
+    CnnInputTarget cnn{};  // with the key planes: the network input from the checked map `disp`
 };
 
 bool bm_supported(int kind, int ww, int wh, int D);

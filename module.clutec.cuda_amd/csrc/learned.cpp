@@ -634,6 +634,12 @@ stereo_status stereo_learned_configure(stereo_context* ctx, stereo_driver** out,
     if ((st = stereo_image_alloc(ctx, &x->L4, W4, H4, STEREO_PIXEL_LUM_U8))) return cleanup(st);
     if ((st = stereo_image_alloc(ctx, &x->R4, W4, H4, STEREO_PIXEL_LUM_U8))) return cleanup(st);
     if ((st = stereo_image_alloc(ctx, &x->dc, W4, H4, STEREO_PIXEL_LUM_I16))) return cleanup(st);
+    if (W4 > stereo::CNN_MAX_W4)
+        return cleanup(fail(ctx, STEREO_ERROR_INVALID_CONFIG, "learned matcher: quarter width %d above %d", W4,
+                            stereo::CNN_MAX_W4));
+    // the coarse matcher's merge writes the network input (the work of stereo_cnn_set_input)
+    // row by row, right after each row of its checked map
+    x->coarse->finish_cnn = stereo::CnnInputTarget{view_of(&x->L4), view_of(&x->R4), x->cnn->act[0], x->cnn->dfill, 0};
 
     d->bx = p->matcher.window_w / 2 + p->matcher.census_w / 2;
     d->by = p->matcher.window_h / 2 + p->matcher.census_h / 2;
@@ -697,7 +703,6 @@ stereo_status learned_enqueue(stereo_driver* d, LearnedExt* x, const stereo_imag
     if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[1], s), "hipEventRecord");
     if ((st = stereo_matcher_process(x->coarse, &x->dc, nullptr, nullptr, &x->L4, &x->R4, s))) return st;
     if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[2], s), "hipEventRecord");
-    if ((st = stereo_cnn_set_input(x->cnn, 0, &x->L4, &x->R4, &x->dc, s))) return st;
     if ((st = stereo_cnn_forward(x->cnn, 1, nullptr, s))) return st;
     if (timed) HIP_TRY(ctx, hipEventRecord(x->ev[3], s), "hipEventRecord");
     stereo::CnnBandArgs ba;
